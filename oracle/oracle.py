@@ -1,0 +1,123 @@
+"""ctypes front-end of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  tests/, the package's smoke() and bench.py's
+cpu_baseline leg use it as the checker or as the timed CPU baseline; the
+shipped library never imports it.
+
+State convention (the reference's, TEBDtools_module.f90:486-521): uniform chi,
+``gam`` is a complex128 array of shape (L, chi, d, chi) indexed [site, b, i, a]
+so that each site's memory is the Fortran column-major Gamma(a, i, b);
+``lam`` is float64 of shape (L+1, chi); gates are complex128 (nb, d^2, d^2)
+matrices indexed [bond, row, col] with row (a-1)d+b (left site slow).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        # liboracle links MKL's single dynamic library; sequential LAPACK per
+        # call, as the reference links mkl_sequential.
+        os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
+        L = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+        P = ctypes.c_void_p
+        I = ctypes.c_int
+        L.orc_two_site_op.argtypes = [I, I, I, P, P, I, P, P]
+        L.orc_trotter_step.argtypes = [I, I, I, P, P, P, P, I]
+        L.orc_canonical_form_all.argtypes = [I, I, I, P, P]
+        L.orc_canonical_form.argtypes = [I, I, I, P, P, I]
+        L.orc_total_energy.argtypes = [I, I, I, P, P, P]
+        L.orc_total_energy.restype = ctypes.c_double
+        L.orc_two_site_rho_all.argtypes = [I, I, I, P, P, P]
+        L.orc_one_site_expval.argtypes = [I, I, I, P, P, P, P]
+        L.orc_chain_entropy.argtypes = [I, I, I, P, P]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _gate_stack(gates, d):
+    # (nb, d2, d2) row/col indexed -> Fortran column-major per bond
+    g = np.asarray(gates, dtype=np.complex128)
+    return np.ascontiguousarray(np.transpose(g, (0, 2, 1)))
+
+
+class OracleState:
+    """Vidal-form MPS held in the reference's padded layout."""
+
+    def __init__(self, gam, lam):
+        self.gam = np.ascontiguousarray(gam, dtype=np.complex128)
+        self.lam = np.ascontiguousarray(lam, dtype=np.float64)
+        self.L, self.chi, self.d, _ = self.gam.shape
+
+    @classmethod
+    def product(cls, coeffs, chi):
+        """ProductStateMPD (Hamiltonian_tools_module.f90:89-121)."""
+        c = np.asarray(coeffs, dtype=np.complex128)
+        L, d = c.shape
+        gam = np.zeros((L, chi, d, chi), np.complex128)
+        gam[:, 0, :, 0] = c
+        lam = np.zeros((L + 1, chi))
+        lam[:, 0] = 1.0
+        return cls(gam, lam)
+
+    def copy(self):
+        return OracleState(self.gam.copy(), self.lam.copy())
+
+    def _args(self):
+        return self.L, self.d, self.chi, _p(self.gam), _p(self.lam)
+
+    def two_site_op(self, link, gate):
+        g = np.ascontiguousarray(np.asarray(gate, np.complex128).T)
+        t = ctypes.c_double()
+        info = lib().orc_two_site_op(*self._args(), int(link), _p(g), ctypes.byref(t))
+        if info != 0:
+            raise RuntimeError(f"zgesvd info={info}")
+        return t.value
+
+    def trotter_step(self, gates, nthreads=1):
+        g = _gate_stack(gates, self.d)
+        t = ctypes.c_double()
+        info = lib().orc_trotter_step(*self._args(), _p(g), ctypes.byref(t), int(nthreads))
+        if info != 0:
+            raise RuntimeError(f"zgesvd info={info}")
+        return t.value
+
+    def canonical_form_all(self):
+        info = lib().orc_canonical_form_all(*self._args())
+        if info != 0:
+            raise RuntimeError(f"zgesvd info={info}")
+
+    def total_energy(self, H):
+        h = _gate_stack(H, self.d)
+        return lib().orc_total_energy(*self._args(), _p(h))
+
+    def two_site_rho(self):
+        d2 = self.d * self.d
+        out = np.zeros((self.L - 1, d2, d2), np.complex128)
+        lib().orc_two_site_rho_all(*self._args(), _p(out))
+        # stored column-major per bond -> [bond, row, col]
+        return np.transpose(out, (0, 2, 1))
+
+    def one_site_expval(self, op):
+        o = np.ascontiguousarray(np.asarray(op, np.float64).T)
+        out = np.zeros(self.L)
+        lib().orc_one_site_expval(*self._args(), _p(o), _p(out))
+        return out
+
+    def chain_entropy(self):
+        out = np.zeros(self.L - 1)
+        lib().orc_chain_entropy(self.L, self.d, self.chi, _p(self.lam), _p(out))
+        return out

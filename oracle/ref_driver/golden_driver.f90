@@ -1,0 +1,225 @@
+! Golden-vector driver: runs the reference TEBD modules on one job file and
+! prints every quantity the parity tests compare.  It is test infrastructure:
+! it is compiled only against the reference sources (recipe: oracle/Makefile,
+! target `ref`), its binary lives in oracle/_ref/, and it never ships.
+!
+! The driver follows the reference's own mains:
+!   model setup      Case_Studies/Heisenberg_Wrapper/XXDynamics.f90:62-101
+!   state + gates    XXDynamics.f90:104-132 (AllocateGamLam, ProductStateMPD,
+!                    AllocateProp, ConstructPropagators)
+!   time loop        XXDynamics.f90:151-178 (SVDInit, TrotterStep)
+!   imaginary time   propagation_module.f90:656-662,741-743 (dt=-i*dtau,
+!                    TrotterStep then CanonicalFormAll)
+! Site-dependent fields use the reference's half-split helpers
+! (Hamiltonian_tools_module.f90:816-886), exactly as the uniform-field branch
+! of HamiltonianHeisenberg (heisenberg_module.f90:158-167) does.
+!
+! Job file (list-directed):
+!   model L param chi         model 1=Heisenberg(param=2*spin) 2=BH JVector
+!                             (param=maxFilling) 3=Fermi-Hubbard (param=maxFilling)
+!   c1 c2 c3                  Heisenberg: Jx Jy Jz ; BH: U0 mu0 V0 ; FH: t U0 mu0
+!   field(1:nf)               Heisenberg: h_i (nf=L) ; BH: J_i (nf=L-1) ; FH: none
+!   dtre dtim nsteps canon    complex dt passed to ConstructPropagators,
+!                             canon=1 runs CanonicalFormAll after each step
+!   (re,im) coefficients of the product state, d per site, L sites
+!   mode                      0 = trajectory, 1 = single TwoSiteOp on a given state
+!   [mode 1: link, then Gamma(chi,d,chi) re im for each site, Lambda(chi) for L+1 links]
+PROGRAM golden_driver
+USE system_parameters
+USE TEBDtools_module
+USE Hamiltonian_tools_module
+USE heisenberg_module
+USE bose_hubbard_module
+USE fermi_hubbard_module
+USE local_operations_module
+USE observables_module
+USE propagation_module
+IMPLICIT NONE
+TYPE(tensor), POINTER :: Gammas(:)
+TYPE(vector), POINTER :: Lambdas(:)
+TYPE(matrix), POINTER :: H(:), U(:)
+COMPLEX(KIND=rKind), ALLOCATABLE :: carray(:,:)
+REAL(KIND=rKind), ALLOCATABLE :: field(:), expl(:), expn(:), buf(:)
+REAL(KIND=rKind) :: c1, c2, c3, dtre, dtim, err, energy, totTrunc
+REAL(KIND=rKind), ALLOCATABLE :: sz(:,:), nop(:,:)
+INTEGER :: model, L, param, chi, nsteps, canon, i, j, k, step, nf, mode, link
+INTEGER :: a, b, dd
+CHARACTER(len=256) :: jobfile
+
+CALL GET_COMMAND_ARGUMENT(1, jobfile)
+OPEN(UNIT=11, FILE=TRIM(jobfile), STATUS='OLD', ACTION='READ')
+READ(11,*) model, L, param, chi
+READ(11,*) c1, c2, c3
+systemSize = L
+chiMax = chi
+print_switch = .FALSE.
+BoundaryCond = 'O'
+trotterOrder = 2
+
+IF (model == 1) THEN
+  spin = 0.5_rKind*param
+  spinSize = FLOOR(2.0_rKind*spin+1.0_rKind)
+  maxFilling = spinSize
+  localSize = heisenbergLocalDim()
+  CALL CreateHeisenbergOps()
+  nf = L
+ELSE IF (model == 2) THEN
+  maxFilling = param
+  localSize = maxFilling+1
+  CALL CreateFieldOps()
+  nf = L-1
+ELSE
+  spin = 0.5_rKind
+  spinSize = 2
+  maxFilling = param
+  localSize = spinSFermiLocalDim()
+  CALL CreateFermiSops()
+  nf = 0
+END IF
+dd = localSize
+ALLOCATE(field(MAX(nf,1)))
+field = 0.0_rKind
+IF (nf > 0) READ(11,*) (field(i), i=1,nf)
+READ(11,*) dtre, dtim, nsteps, canon
+ALLOCATE(carray(dd, L), buf(2*dd))
+DO i=1,L
+  READ(11,*) (buf(k), k=1,2*dd)
+  DO k=1,dd
+    carray(k,i) = CMPLX(buf(2*k-1), buf(2*k), KIND=rKind)
+  END DO
+END DO
+READ(11,*) mode
+
+CALL AllocateOps(H, L-1, dd*dd)
+IF (model == 1) THEN
+  CALL HamiltonianHeisenberg(H, c1, c2, c3)
+  DO i=1,L-1
+    H(i)%m = H(i)%m + field(i)*HamiLeft(Sz_opS) + field(i+1)*HamiRight(Sz_opS)
+  END DO
+  H(1)%m = H(1)%m + field(1)*HamiLeft(Sz_opS)
+  H(L-1)%m = H(L-1)%m + field(L)*HamiRight(Sz_opS)
+ELSE IF (model == 2) THEN
+  CALL HamiltonianBoseHubbardJVector(H, field, c1, c2, c3)
+ELSE
+  CALL HamiltonianHubbard(H, c1, c2, c3)
+END IF
+
+! one-site operators for the observables: Sz (spins) or n (bosons, fermions)
+ALLOCATE(sz(dd,dd), nop(dd,dd))
+sz = 0.0_rKind
+nop = 0.0_rKind
+IF (model == 1) THEN
+  sz = Sz_opS%mr
+  DO k=1,dd
+    nop(k,k) = k-1
+  END DO
+ELSE IF (model == 2) THEN
+  nop = MATMUL(TRANSPOSE(a_op%mr), a_op%mr)
+ELSE
+  sz = Sz_opS%mr
+  nop = ntot_opS%mr
+END IF
+
+PRINT '(A,3I6)', 'DIMS', L, dd, chi
+DO i=1,L-1
+  PRINT '(A,I6)', 'H', i
+  DO a=1,dd*dd
+    PRINT '(64ES26.17E3)', (REAL(H(i)%m(a,b)), AIMAG(H(i)%m(a,b)), b=1,dd*dd)
+  END DO
+END DO
+PRINT '(A)', 'SZOP'
+DO a=1,dd
+  PRINT '(16ES26.17E3)', (sz(a,b), b=1,dd)
+END DO
+PRINT '(A)', 'NOP'
+DO a=1,dd
+  PRINT '(16ES26.17E3)', (nop(a,b), b=1,dd)
+END DO
+IF (model == 3) THEN
+  DO k=1,spinSize
+    PRINT '(A,I3)', 'AOP', k
+    DO a=1,dd
+      PRINT '(16ES26.17E3)', (a_opS(k)%mr(a,b), b=1,dd)
+    END DO
+  END DO
+END IF
+
+CALL AllocateProp(U)
+CALL ConstructPropagators(H, U, CMPLX(dtre, dtim, KIND=rKind))
+DO i=1,L-1
+  PRINT '(A,I6)', 'U', i
+  DO a=1,dd*dd
+    PRINT '(64ES26.17E3)', (REAL(U(i)%m(a,b)), AIMAG(U(i)%m(a,b)), b=1,dd*dd)
+  END DO
+END DO
+
+CALL AllocateGamLam(Gammas, Lambdas, chi)
+CALL ProductStateMPD(Gammas, Lambdas, carray)
+CALL SVDInit(chi)
+
+IF (mode == 1) THEN
+  ! single two-site update on an explicit state read from the job file
+  READ(11,*) link
+  DEALLOCATE(buf)
+  ALLOCATE(buf(2*chi*dd*chi))
+  DO i=1,L
+    READ(11,*) (buf(k), k=1,2*chi*dd*chi)
+    k = 0
+    DO b=1,chi
+      DO j=1,dd
+        DO a=1,chi
+          Gammas(i)%t(a,j,b) = CMPLX(buf(2*k+1), buf(2*k+2), KIND=rKind)
+          k = k+1
+        END DO
+      END DO
+    END DO
+  END DO
+  DO i=1,L+1
+    READ(11,*) (Lambdas(i)%v(k), k=1,chi)
+  END DO
+  CALL TwoSiteOp(link, U(link)%m, Gammas, Lambdas, err)
+  PRINT '(A,ES26.17E3)', 'TRUNC', err
+  CALL PrintState()
+  STOP
+END IF
+
+ALLOCATE(expl(L), expn(L))
+totTrunc = 0.0_rKind
+CALL PrintObs(0, 0.0_rKind)
+DO step=1,nsteps
+  CALL TrotterStep(U, Gammas, Lambdas, err)
+  IF (canon == 1) CALL CanonicalFormAll(Gammas, Lambdas)
+  totTrunc = totTrunc + err
+  CALL PrintObs(step, err)
+END DO
+CALL PrintState()
+
+CONTAINS
+
+SUBROUTINE PrintObs(st, tr)
+INTEGER, INTENT(IN) :: st
+REAL(KIND=rKind), INTENT(IN) :: tr
+INTEGER :: ii
+  CALL TotalEnergy(energy, H, Gammas, Lambdas)
+  CALL OneSiteExpVal(expl, sz, Gammas, Lambdas)
+  CALL OneSiteExpVal(expn, nop, Gammas, Lambdas)
+  PRINT '(A,I6,2ES26.17E3)', 'STEP', st, energy, tr
+  PRINT '(A,256ES26.17E3)', 'SZ', (expl(ii), ii=1,L)
+  PRINT '(A,256ES26.17E3)', 'N', (expn(ii), ii=1,L)
+  PRINT '(A,256ES26.17E3)', 'S', (ChainEntropy(ii, Lambdas), ii=2,L)
+END SUBROUTINE PrintObs
+
+SUBROUTINE PrintState()
+INTEGER :: ii, aa, jj, bb
+  DO ii=1,L+1
+    PRINT '(A,I6)', 'LAMBDA', ii
+    PRINT '(1024ES26.17E3)', (Lambdas(ii)%v(aa), aa=1,chi)
+  END DO
+  DO ii=1,L
+    PRINT '(A,I6)', 'GAMMA', ii
+    PRINT '(4096ES26.17E3)', (((REAL(Gammas(ii)%t(aa,jj,bb)), AIMAG(Gammas(ii)%t(aa,jj,bb)), &
+         aa=1,chi), jj=1,dd), bb=1,chi)
+  END DO
+END SUBROUTINE PrintState
+
+END PROGRAM golden_driver

@@ -1,0 +1,346 @@
+// FP64 CPU ORACLE for the two-site Trotter update.  TEST INFRASTRUCTURE ONLY:
+// tests/, the package's smoke() and bench.py's cpu_baseline leg may load it,
+// as the checker or the timed CPU baseline; the shipped library never does.
+//
+// It restates the reference algorithm (OpenSourceTEBD v2.0, Fortran) with the
+// reference's semantics kept on purpose:
+//   * uniform, zero-padded chi for every bond (TEBDtools_module.f90:486-521),
+//   * theta = lambda Gamma lambda Gamma lambda   (local_operations_module.f90:184-211),
+//   * gate U((i-1)d+j,(k-1)d+l) on theta        (local_operations_module.f90:248-280),
+//   * Frobenius normalisation + reshape         (local_operations_module.f90:283-320),
+//   * full LAPACK ZGESVD('A','A') of the (chi d)x(chi d) matrix with the
+//     reference's minimum workspace 5 chi d     (local_operations_module.f90:84-122,322-345),
+//   * keep chi values, drop |s|<=1e-15, renormalise, truncerr=|1-norm|
+//                                               (local_operations_module.f90:347-379),
+//   * Gamma = U/lambda_l, V^T/lambda_r with 1e-15 guard (local_operations_module.f90:381-428),
+//   * 2nd-order OBC schedule odd/even/odd       (propagation_module.f90:82-114),
+//   * CanonicalForm / CanonicalFormAll          (local_operations_module.f90:496-633,
+//                                                propagation_module.f90:617-634),
+//   * observables: TotalEnergy, OneSiteExpVal, ChainEntropy
+//                                               (observables_module.f90:95-117,134-162,
+//                                                704-832,1432-1466,1583-1602).
+//
+// Storage (identical to the reference's per-site arrays, packed one after another):
+//   Gamma[s] : chi x d x chi complex(8), column-major, element (a,i,b) at a + chi*(i + d*b)
+//   lambda[l]: chi real(8), l = 0..L (reference Lambdas(1..L+1))
+//   gates    : (d^2 x d^2) complex(8) column-major per bond, row (a-1)d+b (left site slow)
+#include <complex>
+#include <vector>
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+
+typedef std::complex<double> cplx;
+
+extern "C" void zgesvd_(const char* jobu, const char* jobvt, const int* m, const int* n,
+                        cplx* a, const int* lda, double* s, cplx* u, const int* ldu,
+                        cplx* vt, const int* ldvt, cplx* work, const int* lwork,
+                        double* rwork, int* info);
+
+namespace {
+
+struct View {
+  int L, d, chi;
+  cplx* G;
+  double* lam;
+  size_t gsz() const { return (size_t)chi * d * chi; }
+  cplx* gam(int s) const { return G + (size_t)s * gsz(); }       // s = 0..L-1
+  double* lm(int l) const { return lam + (size_t)l * chi; }       // l = 0..L
+  cplx& g(int s, int a, int i, int b) const { return gam(s)[a + (size_t)chi * (i + (size_t)d * b)]; }
+};
+
+// Full SVD exactly as the reference calls it: jobu=jobvt='A', lwork = 5*n.
+int ref_zgesvd(int n, cplx* A, double* S, cplx* U, cplx* VT) {
+  int lwork = 5 * n, info = 0;
+  std::vector<cplx> work(lwork);
+  std::vector<double> rwork(5 * n);
+  const char job = 'A';
+  zgesvd_(&job, &job, &n, &n, A, &n, S, U, &n, VT, &n, work.data(), &lwork, rwork.data(), &info);
+  return info;
+}
+
+// TwoSiteOp_c (local_operations_module.f90:463-494); link is 0-based bond (sites link, link+1).
+int two_site_op(const View& v, int link, const cplx* Uop, double* truncerr) {
+  const int chi = v.chi, d = v.d, n = chi * d;
+  const double* l0 = v.lm(link);
+  const double* l1 = v.lm(link + 1);
+  const double* l2 = v.lm(link + 2);
+  // FormTheta: Theta(a,i,j,b) = sum_g l0(a) G1(a,i,g) l1(g) G2(g,j,b) l2(b)
+  std::vector<cplx> theta((size_t)chi * d * d * chi);
+  auto th = [&](int a, int i, int j, int b) -> cplx& {
+    return theta[a + (size_t)chi * (i + (size_t)d * (j + (size_t)d * b))];
+  };
+  std::vector<cplx> rowtmp(chi);
+  for (int b = 0; b < chi; ++b)
+    for (int j = 0; j < d; ++j)
+      for (int i = 0; i < d; ++i)
+        for (int a = 0; a < chi; ++a) {
+          cplx acc = 0.0;
+          for (int gm = 0; gm < chi; ++gm)
+            acc += v.g(link, a, i, gm) * l1[gm] * v.g(link + 1, gm, j, b);
+          th(a, i, j, b) = l0[a] * acc * l2[b];
+        }
+  // ThetaOperation_c: Theta'(a,i,j,b) = sum_{k,l} U((i-1)d+j,(k-1)d+l) Theta(a,k,l,b)
+  const int d2 = d * d;
+  std::vector<cplx> tmp(d2);
+  for (int b = 0; b < chi; ++b)
+    for (int a = 0; a < chi; ++a) {
+      for (int k = 0; k < d; ++k)
+        for (int l = 0; l < d; ++l) tmp[k * d + l] = th(a, k, l, b);
+      for (int i = 0; i < d; ++i)
+        for (int j = 0; j < d; ++j) {
+          cplx acc = 0.0;
+          for (int kl = 0; kl < d2; ++kl) acc += Uop[(i * d + j) + (size_t)d2 * kl] * tmp[kl];
+          th(a, i, j, b) = acc;
+        }
+    }
+  // ReshapeTheta: normalise, ThetaRS((i-1)chi+a,(j-1)chi+b) = Theta/norm
+  double nrm2 = 0.0;
+  for (auto& z : theta) nrm2 += std::norm(z);
+  const double nrm = std::sqrt(nrm2);
+  std::vector<cplx> RS((size_t)n * n);
+  for (int b = 0; b < chi; ++b)
+    for (int j = 0; j < d; ++j)
+      for (int i = 0; i < d; ++i)
+        for (int a = 0; a < chi; ++a)
+          RS[(i * chi + a) + (size_t)n * (j * chi + b)] = th(a, i, j, b) / nrm;
+  // SVDTruncation
+  std::vector<double> S(n);
+  std::vector<cplx> Um((size_t)n * n), VT((size_t)n * n);
+  int info = ref_zgesvd(n, RS.data(), S.data(), Um.data(), VT.data());
+  // FormLambda1
+  double norm = 0.0;
+  for (int a = 0; a < chi; ++a)
+    if (std::fabs(S[a]) > 1e-15) norm += S[a] * S[a];
+  *truncerr = std::fabs(1.0 - norm);
+  double* lc = v.lm(link + 1);
+  for (int a = 0; a < chi; ++a) lc[a] = std::fabs(S[a]) > 1e-15 ? S[a] / std::sqrt(norm) : 0.0;
+  // FormGamma1: G1(a,i,b) = U((i-1)chi+a,b)/l0(a)
+  for (int b = 0; b < chi; ++b)
+    for (int a = 0; a < chi; ++a)
+      for (int i = 0; i < d; ++i)
+        v.g(link, a, i, b) = std::fabs(l0[a]) > 1e-15 ? Um[(i * chi + a) + (size_t)n * b] / l0[a] : cplx(0.0);
+  // FormGamma2: G2(a,i,b) = VT(a,(i-1)chi+b)/l2(b)
+  for (int b = 0; b < chi; ++b)
+    for (int a = 0; a < chi; ++a)
+      for (int i = 0; i < d; ++i)
+        v.g(link + 1, a, i, b) =
+            std::fabs(l2[b]) > 1e-15 ? VT[a + (size_t)n * (i * chi + b)] / l2[b] : cplx(0.0);
+  return info;
+}
+
+// CanonicalForm (local_operations_module.f90:496-620); link is 0-based bond.
+int canonical_form(const View& v, int link) {
+  const int chi = v.chi, d = v.d;
+  const double* l0 = v.lm(link);
+  double* l1 = v.lm(link + 1);
+  const double* l2 = v.lm(link + 2);
+  const size_t cc = (size_t)chi * chi;
+  std::vector<cplx> M(cc), X(cc), Y(cc), U(cc), VT(cc);
+  std::vector<double> S(chi);
+  std::vector<cplx> G1t(v.gsz()), G2t(v.gsz());
+  auto idx = [&](int a, int i, int b) { return a + (size_t)chi * (i + (size_t)d * b); };
+  int info = 0;
+  for (int b = 0; b < chi; ++b)
+    for (int a = 0; a < chi; ++a) {
+      cplx acc = 0.0;
+      for (int i = 0; i < d; ++i)
+        for (int g = 0; g < chi; ++g)
+          acc += std::conj(v.g(link, g, i, a)) * l0[g] * l0[g] * v.g(link, g, i, b);
+      M[a + (size_t)chi * b] = acc;
+    }
+  info |= ref_zgesvd(chi, M.data(), S.data(), U.data(), VT.data());
+  for (int b = 0; b < chi; ++b)
+    for (int i = 0; i < d; ++i)
+      for (int a = 0; a < chi; ++a) {
+        cplx acc = 0.0;
+        if (S[b] > 1e-8)
+          for (int g = 0; g < chi; ++g) acc += v.g(link, a, i, g) * U[g + (size_t)chi * b] * (1.0 / std::sqrt(S[b]));
+        G1t[idx(a, i, b)] = acc;
+      }
+  for (int b = 0; b < chi; ++b)
+    for (int a = 0; a < chi; ++a) X[a + (size_t)chi * b] = std::sqrt(S[a]) * std::conj(U[b + (size_t)chi * a]);
+  for (int b = 0; b < chi; ++b)
+    for (int a = 0; a < chi; ++a) {
+      cplx acc = 0.0;
+      for (int i = 0; i < d; ++i)
+        for (int g = 0; g < chi; ++g)
+          acc += v.g(link + 1, a, i, g) * l2[g] * l2[g] * std::conj(v.g(link + 1, b, i, g));
+      M[a + (size_t)chi * b] = acc;
+    }
+  info |= ref_zgesvd(chi, M.data(), S.data(), U.data(), VT.data());
+  for (int b = 0; b < chi; ++b)
+    for (int i = 0; i < d; ++i)
+      for (int a = 0; a < chi; ++a) {
+        cplx acc = 0.0;
+        if (S[a] > 1e-8)
+          for (int g = 0; g < chi; ++g)
+            acc += (1.0 / std::sqrt(S[a])) * std::conj(U[g + (size_t)chi * a]) * v.g(link + 1, g, i, b);
+        G2t[idx(a, i, b)] = acc;
+      }
+  for (int b = 0; b < chi; ++b)
+    for (int a = 0; a < chi; ++a) Y[a + (size_t)chi * b] = l1[a] * U[a + (size_t)chi * b] * std::sqrt(S[b]);
+  for (int b = 0; b < chi; ++b)
+    for (int a = 0; a < chi; ++a) {
+      cplx acc = 0.0;
+      for (int g = 0; g < chi; ++g) acc += X[a + (size_t)chi * g] * Y[g + (size_t)chi * b];
+      M[a + (size_t)chi * b] = acc;
+    }
+  info |= ref_zgesvd(chi, M.data(), S.data(), U.data(), VT.data());
+  double normsq = 0.0;
+  for (int a = 0; a < chi; ++a) normsq += S[a] * S[a];
+  for (int a = 0; a < chi; ++a) l1[a] = S[a] / std::sqrt(normsq);
+  for (int b = 0; b < chi; ++b)
+    for (int i = 0; i < d; ++i)
+      for (int a = 0; a < chi; ++a) {
+        cplx acc = 0.0;
+        for (int g = 0; g < chi; ++g) acc += G1t[idx(a, i, g)] * U[g + (size_t)chi * b];
+        v.g(link, a, i, b) = acc;
+      }
+  for (int b = 0; b < chi; ++b)
+    for (int i = 0; i < d; ++i)
+      for (int a = 0; a < chi; ++a) {
+        cplx acc = 0.0;
+        for (int g = 0; g < chi; ++g) acc += VT[a + (size_t)chi * g] * G2t[idx(g, i, b)];
+        v.g(link + 1, a, i, b) = acc;
+      }
+  return info;
+}
+
+// Two-site reduced density matrix rho((a-1)d+b,(a'-1)d+b') of sites (s, s+1) in
+// canonical form: sum over outer bonds of theta * conj(theta), theta = lGlGl
+// (ThetaKernal + TwoSiteRho, observables_module.f90:704-730,780-832).
+void two_site_rho(const View& v, int s, cplx* rho) {
+  const int chi = v.chi, d = v.d, d2 = d * d;
+  const double* l0 = v.lm(s);
+  const double* l1 = v.lm(s + 1);
+  const double* l2 = v.lm(s + 2);
+  std::vector<cplx> th((size_t)chi * d2);  // theta(a, ij) for fixed b
+  std::fill(rho, rho + d2 * d2, cplx(0.0));
+  for (int b = 0; b < chi; ++b) {
+    if (l2[b] == 0.0) continue;
+    for (int a = 0; a < chi; ++a)
+      for (int i = 0; i < d; ++i)
+        for (int j = 0; j < d; ++j) {
+          cplx acc = 0.0;
+          for (int g = 0; g < chi; ++g) acc += v.g(s, a, i, g) * l1[g] * v.g(s + 1, g, j, b);
+          th[a + (size_t)chi * (i * d + j)] = l0[a] * acc * l2[b];
+        }
+    for (int c = 0; c < d2; ++c)
+      for (int r = 0; r < d2; ++r) {
+        cplx acc = 0.0;
+        for (int a = 0; a < chi; ++a) acc += th[a + (size_t)chi * r] * std::conj(th[a + (size_t)chi * c]);
+        rho[r + (size_t)d2 * c] += acc;
+      }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// One two-site update on bond `link` (1-based, reference convention).
+int orc_two_site_op(int L, int d, int chi, double* G, double* lam, int link, const double* U,
+                    double* truncerr) {
+  View v{L, d, chi, (cplx*)G, lam};
+  return two_site_op(v, link - 1, (const cplx*)U, truncerr);
+}
+
+// TrotterStep2ndOrder: odd bonds, even bonds, odd bonds (1-based bond numbering).
+// gates: (L-1) consecutive d^2 x d^2 complex matrices.  nthreads>1 runs the
+// independent bonds of one half-sweep concurrently (same arithmetic per bond).
+int orc_trotter_step(int L, int d, int chi, double* G, double* lam, const double* gates,
+                     double* total_truncerr, int nthreads) {
+  View v{L, d, chi, (cplx*)G, lam};
+  const size_t gs = (size_t)d * d * d * d;
+  const cplx* Ug = (const cplx*)gates;
+  double total = 0.0;
+  int info = 0;
+  const int starts[3] = {1, 2, 1};
+  for (int sweep = 0; sweep < 3; ++sweep) {
+    const int nb = (L - 1 - starts[sweep]) / 2 + 1;
+    std::vector<double> te(nb, 0.0);
+    std::vector<int> inf(nb, 0);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int k = 0; k < nb; ++k) {
+      const int bond = starts[sweep] + 2 * k;  // 1-based
+      inf[k] = two_site_op(v, bond - 1, Ug + gs * (bond - 1), &te[k]);
+    }
+    for (int k = 0; k < nb; ++k) { total += te[k]; info |= inf[k]; }
+  }
+  *total_truncerr = total;
+  return info;
+}
+
+int orc_canonical_form_all(int L, int d, int chi, double* G, double* lam) {
+  View v{L, d, chi, (cplx*)G, lam};
+  int info = 0;
+  for (int k = 0; k < L - 1; ++k) info |= canonical_form(v, k);
+  for (int k = L - 2; k >= 0; --k) info |= canonical_form(v, k);
+  return info;
+}
+
+int orc_canonical_form(int L, int d, int chi, double* G, double* lam, int link) {
+  View v{L, d, chi, (cplx*)G, lam};
+  return canonical_form(v, link - 1);
+}
+
+// TotalEnergy: sum over bonds of Re Tr(H_b rho_b); H: (L-1) d^2 x d^2 complex.
+double orc_total_energy(int L, int d, int chi, const double* G, const double* lam, const double* H) {
+  View v{L, d, chi, (cplx*)G, (double*)lam};
+  const int d2 = d * d;
+  std::vector<cplx> rho((size_t)d2 * d2);
+  double e = 0.0;
+  for (int s = 0; s < L - 1; ++s) {
+    two_site_rho(v, s, rho.data());
+    const cplx* Hb = (const cplx*)H + (size_t)s * d2 * d2;
+    cplx tr = 0.0;
+    for (int r = 0; r < d2; ++r)
+      for (int c = 0; c < d2; ++c) tr += Hb[r + (size_t)d2 * c] * rho[c + (size_t)d2 * r];
+    e += tr.real();
+  }
+  return e;
+}
+
+// Two-site reduced density matrices of every bond: out (L-1) x d^2 x d^2 complex.
+void orc_two_site_rho_all(int L, int d, int chi, const double* G, const double* lam, double* out) {
+  View v{L, d, chi, (cplx*)G, (double*)lam};
+  const size_t d4 = (size_t)d * d * d * d;
+  for (int s = 0; s < L - 1; ++s) two_site_rho(v, s, (cplx*)out + d4 * s);
+}
+
+// OneSiteExpVal for a real d x d operator (column-major): out[s] = Re Tr(op rho_s).
+void orc_one_site_expval(int L, int d, int chi, const double* G, const double* lam, const double* op,
+                         double* out) {
+  View v{L, d, chi, (cplx*)G, (double*)lam};
+  std::vector<cplx> rho((size_t)d * d);
+  for (int s = 0; s < L; ++s) {
+    const double* l0 = v.lm(s);
+    const double* l1 = v.lm(s + 1);
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; j < d; ++j) {
+        cplx acc = 0.0;
+        for (int a = 0; a < chi; ++a)
+          for (int b = 0; b < chi; ++b)
+            acc += l0[a] * v.g(s, a, i, b) * l1[b] * l1[b] * std::conj(v.g(s, a, j, b)) * l0[a];
+        rho[i + (size_t)d * j] = acc;
+      }
+    cplx tr = 0.0;
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; j < d; ++j) tr += op[i + (size_t)d * j] * rho[j + (size_t)d * i];
+    out[s] = tr.real();
+  }
+}
+
+// ChainEntropy(link) for link = 2..L (reference numbering), log base d: out[L-1].
+void orc_chain_entropy(int L, int d, int chi, const double* lam, double* out) {
+  for (int link = 2; link <= L; ++link) {
+    const double* l = lam + (size_t)(link - 1) * chi;
+    double t = 0.0;
+    for (int i = 0; i < chi; ++i)
+      if (l[i] != 0.0) t -= l[i] * l[i] * std::log(std::fabs(l[i] * l[i])) / std::log((double)d);
+    out[link - 2] = t;
+  }
+}
+
+}  // extern "C"

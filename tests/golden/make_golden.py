@@ -1,0 +1,203 @@
+"""Write the golden fixtures in tests/golden/ from the reference itself.
+
+Runs oracle/_ref/golden_driver (the reference's Fortran modules compiled from
+/root/reference by ``make -C oracle ref``, plus a project-written driver) on
+seeded jobs and stores its outputs as .npz data: bond Hamiltonians, Trotter
+gates, per-step observables and final states.  Inputs that need randomness
+(fields, hoppings, product states, random Vidal tensors) come from numpy with
+fixed seeds and are written to the job file, so both sides see the same bytes.
+
+Usage:  make -C oracle ref && python tests/golden/make_golden.py
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "golden_driver")
+
+
+def fmt(x):
+    return " ".join(repr(float(v)) for v in np.ravel(x))
+
+
+def write_job(path, model, L, param, chi, consts, field, dt, nsteps, canon, coeffs, mode=0, extra=None):
+    with open(path, "w") as f:
+        f.write(f"{model} {L} {param} {chi}\n")
+        f.write(fmt(consts) + "\n")
+        if field is not None and len(field):
+            f.write(fmt(field) + "\n")
+        f.write(f"{float(dt.real)!r} {float(dt.imag)!r} {nsteps} {canon}\n")
+        for s in range(L):
+            f.write(" ".join(f"{float(c.real)!r} {float(c.imag)!r}" for c in coeffs[s]) + "\n")
+        f.write(f"{mode}\n")
+        if extra:
+            f.write(extra)
+
+
+def parse(text):
+    blocks = []
+    for line in text.splitlines():
+        tok = line.split()
+        if not tok:
+            continue
+        if tok[0][0].isalpha():
+            head = tok[0]
+            nums = []
+            rest = tok[1:]
+            blocks.append((head, nums))
+            # integer tag arguments (indices) for H/U/LAMBDA/GAMMA/AOP/DIMS
+            if head in ("H", "U", "LAMBDA", "GAMMA", "AOP", "DIMS"):
+                blocks[-1] = (head, nums, [int(t) for t in rest])
+                continue
+            nums.extend(float(t) for t in rest)
+            blocks[-1] = (head, nums, [])
+        else:
+            blocks[-1][1].extend(float(t) for t in tok)
+    return blocks
+
+
+def run(job):
+    out = subprocess.run([DRIVER, job], check=True, capture_output=True, text=True)
+    return parse(out.stdout)
+
+
+def collect(blocks, L, d, chi):
+    d2 = d * d
+    res = {"H": [], "U": [], "energy": [], "trunc": [], "sz": [], "n": [], "S": [],
+           "lam": [], "gam": [], "aop": []}
+    for head, nums, args in blocks:
+        a = np.array(nums)
+        if head in ("H", "U"):
+            m = a.reshape(d2, 2 * d2)
+            res[head].append(m[:, 0::2] + 1j * m[:, 1::2])
+        elif head == "SZOP":
+            res["szop"] = a.reshape(d, d)
+        elif head == "NOP":
+            res["nop"] = a.reshape(d, d)
+        elif head == "AOP":
+            res["aop"].append(a.reshape(d, d))
+        elif head == "STEP":  # step index, energy, truncerr of that step
+            res["energy"].append(a[1])
+            res["trunc"].append(a[2])
+        elif head == "SZ":
+            res["sz"].append(a)
+        elif head == "N":
+            res["n"].append(a)
+        elif head == "S":
+            res["S"].append(a)
+        elif head == "TRUNC":
+            res["trunc"].append(a[0])
+        elif head == "LAMBDA":
+            res["lam"].append(a)
+        elif head == "GAMMA":
+            z = a[0::2] + 1j * a[1::2]
+            # Fortran (a, i, b) column-major -> [b, i, a]
+            res["gam"].append(z.reshape(chi, d, chi))
+    out = {}
+    for k, v in res.items():
+        if isinstance(v, list):
+            if len(v):
+                out[k] = np.array(v)
+        else:
+            out[k] = v
+    return out
+
+
+def neel(L, d, up, dn):
+    c = np.zeros((L, d), np.complex128)
+    for s in range(L):
+        c[s, up if s % 2 == 0 else dn] = 1.0
+    return c
+
+
+def main():
+    if not os.path.exists(DRIVER):
+        sys.exit("build the reference driver first: make -C oracle ref")
+    tmp = tempfile.mkdtemp()
+    jobs = {}
+
+    # XXZ spin-1/2 (the CPU-runnable workload): L=32, Jx=Jy=Jz=1, h_i~U[-1/2,1/2]
+    # seed 1, Neel, chi=32, dt=0.01, 100 steps.
+    L = 32
+    h = np.random.default_rng(1).uniform(-0.5, 0.5, L)
+    jobs["xxz_l32_chi32"] = dict(model=1, L=L, param=1, chi=32, consts=[1.0, 1.0, 1.0], field=h,
+                                 dt=0.01 + 0j, nsteps=100, canon=0, coeffs=neel(L, 2, 0, 1), d=2)
+    # Same chain at chi=8: truncation active from the first steps.
+    jobs["xxz_l16_chi8"] = dict(model=1, L=16, param=1, chi=8, consts=[1.0, 1.0, 0.5], field=h[:16],
+                                dt=0.01 + 0j, nsteps=60, canon=0, coeffs=neel(16, 2, 0, 1), d=2)
+    # Bose-Hubbard JVector: d=5 (maxFilling 4), U=1, J_i~U[0.05,0.15] seed 2, Mott n=1.
+    L = 12
+    J = np.random.default_rng(2).uniform(0.05, 0.15, L - 1)
+    c = np.zeros((L, 5), np.complex128)
+    c[:, 1] = 1.0
+    jobs["bh_l12_chi24"] = dict(model=2, L=L, param=4, chi=24, consts=[1.0, 0.0, 0.0], field=J,
+                                dt=0.005 + 0j, nsteps=40, canon=0, coeffs=c, d=5)
+    # Fermi-Hubbard spin-1/2, maxFilling 2 (d=4), t=1, U=4, Neel (up=index 2, down=index 3).
+    jobs["fh_l10_chi32"] = dict(model=3, L=10, param=2, chi=32, consts=[1.0, 4.0, 0.0], field=None,
+                                dt=0.01 + 0j, nsteps=30, canon=0, coeffs=neel(10, 4, 1, 2), d=4)
+    # Truncated FH run: a random complex product start (seed 3) keeps the Schmidt
+    # spectrum non-degenerate at the chi=8 cut (a Neel start truncates through
+    # exactly degenerate multiplets, where the kept vectors are not unique).
+    r = np.random.default_rng(3)
+    c = r.normal(size=(8, 4)) + 1j * r.normal(size=(8, 4))
+    c /= np.linalg.norm(c, axis=1, keepdims=True)
+    jobs["fh_l8_chi8"] = dict(model=3, L=8, param=2, chi=8, consts=[1.0, 4.0, 0.0], field=None,
+                              dt=0.02 + 0j, nsteps=25, canon=0, coeffs=c, d=4)
+    # Spin-1 Heisenberg, random normalised complex product state seed 4.
+    L = 8
+    r = np.random.default_rng(4)
+    c = r.normal(size=(L, 3)) + 1j * r.normal(size=(L, 3))
+    c /= np.linalg.norm(c, axis=1, keepdims=True)
+    jobs["s1_l8_chi12"] = dict(model=1, L=L, param=2, chi=12, consts=[1.0, 1.0, 1.0], field=np.zeros(L),
+                               dt=0.01 + 0j, nsteps=30, canon=0, coeffs=c, d=3)
+    # Imaginary time (dt = -i dtau), CanonicalFormAll after every step;
+    # spin-1/2 Heisenberg with h_i~U[-1,1] seed 5, equal-weight product start (AllStates).
+    L = 10
+    hf = np.random.default_rng(5).uniform(-1.0, 1.0, L)
+    c = np.full((L, 2), 1.0 / np.sqrt(2.0), np.complex128)
+    jobs["itp_l10_chi8"] = dict(model=1, L=L, param=1, chi=8, consts=[1.0, 1.0, 1.0], field=hf,
+                                dt=-0.05j, nsteps=40, canon=1, coeffs=c, d=2)
+
+    for name, j in jobs.items():
+        path = os.path.join(tmp, name + ".txt")
+        write_job(path, j["model"], j["L"], j["param"], j["chi"], j["consts"], j["field"], j["dt"],
+                  j["nsteps"], j["canon"], j["coeffs"])
+        res = collect(run(path), j["L"], j["d"], j["chi"])
+        meta = dict(model=j["model"], L=j["L"], param=j["param"], chi=j["chi"], d=j["d"],
+                    consts=np.array(j["consts"], float), dt=np.complex128(j["dt"]), nsteps=j["nsteps"],
+                    canon=j["canon"], coeffs=j["coeffs"],
+                    field=np.array([] if j["field"] is None else j["field"], float))
+        if j["L"] * j["chi"] * j["d"] * j["chi"] > 4096:  # keep fixtures small
+            res.pop("gam", None)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **meta, **res)
+        print(name, "energy[-1]=", res["energy"][-1], "sum trunc=", np.sum(res["trunc"]))
+
+    # Single TwoSiteOp on a random, non-canonical state (mode 1): L=4, d=2, chi=4, link 2.
+    r = np.random.default_rng(7)
+    L, d, chi, link = 4, 2, 4, 2
+    gam = r.normal(size=(L, chi, d, chi)) + 1j * r.normal(size=(L, chi, d, chi))
+    lam = np.sort(r.uniform(0.1, 1.0, size=(L + 1, chi)), axis=1)[:, ::-1].copy()
+    lam /= np.linalg.norm(lam, axis=1, keepdims=True)
+    extra = f"{link}\n"
+    for s in range(L):
+        z = gam[s].reshape(-1)  # [b, i, a] C-order == Fortran (a,i,b)
+        extra += " ".join(f"{float(v.real)!r} {float(v.imag)!r}" for v in z) + "\n"
+    for l in range(L + 1):
+        extra += fmt(lam[l]) + "\n"
+    hz = r.uniform(-0.5, 0.5, L)
+    path = os.path.join(tmp, "tso.txt")
+    write_job(path, 1, L, 1, chi, [1.0, 0.7, 1.3], hz, 0.05 + 0j, 0, 0, neel(L, 2, 0, 1), mode=1, extra=extra)
+    res = collect(run(path), L, d, chi)
+    np.savez_compressed(os.path.join(HERE, "two_site_op_l4.npz"), gam_in=gam, lam_in=lam, link=link,
+                        field=hz, consts=np.array([1.0, 0.7, 1.3]), dt=np.complex128(0.05), L=L, d=d, chi=chi,
+                        **res)
+    print("two_site_op_l4 trunc=", res["trunc"])
+
+
+if __name__ == "__main__":
+    main()
