@@ -1,0 +1,112 @@
+/* C ABI of libtebdhip.so: the two-site Trotter update of OpenSourceTEBD v2.0
+ * (vvolkl/OpenTEBD-extended) on one MI355X (gfx950).
+ *
+ * Plain pointers and sizes only.  Complex numbers are interleaved (re, im)
+ * doubles, i.e. Fortran COMPLEX(KIND=8); matrices are column-major, exactly
+ * as the reference's derived types hold them:
+ *   Gammas(i)%t(chi,d,chi)  complex(8)  (VidalCore_v_2.0/TEBDtools_module.f90:45,486-521)
+ *   Lambdas(i)%v(chi)       real(8)     (TEBDtools_module.f90:31-33,486-521)
+ *   Udt(i)%m(d*d,d*d)       complex(8), row (a-1)d+b, left site slow
+ *                                       (TEBDtools_module.f90:37-39,937-998)
+ * Every call returns 0 (TEBD_OK) or a negative code; tebd_last_error() gives
+ * the message.  The reference has no error channel (ZGESVD info is ignored,
+ * VidalCore_v_2.0/local_operations_module.f90:335-336; bad schedules STOP,
+ * propagation_module.f90:64,76); a Fortran shim maps non-zero codes to STOP.
+ *
+ * One context = one chain = one HIP stream; calls on a context are
+ * serialised, contexts are independent (no global state).  State lives on
+ * the device between calls; host arrays are authoritative only after
+ * tebd_download().
+ */
+#ifndef TEBD_HIP_H
+#define TEBD_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tebd_ctx tebd_ctx;
+
+enum tebd_status {
+  TEBD_OK = 0,
+  TEBD_ERR_ARG = -1,         /* bad argument / unsupported order or boundary */
+  TEBD_ERR_HIP = -2,         /* HIP runtime error (allocation, launch, copy) */
+  TEBD_ERR_SHAPE = -3,       /* dimensions do not fit chi_max / d / L */
+  TEBD_ERR_NAN = -4,         /* theta is zero, NaN or Inf */
+  TEBD_ERR_NOCONV = -5,      /* SVD sweeps did not converge */
+  TEBD_ERR_STATE = -6,       /* call out of order (no gates, no state) */
+  TEBD_ERR_UNSUPPORTED = -7  /* feature not available in this build */
+};
+
+/* Mode bits: they change precision and launch policy of the same kernels. */
+enum tebd_mode {
+  TEBD_MODE_FP64 = 0u,              /* plain FP64 drop-in: one bond at a time, full SVD */
+  TEBD_MODE_MIXED = 1u << 0,        /* FP32 theta/gate on MFMA, FP64 Gram/SVD/lambda */
+  TEBD_MODE_BATCHED = 1u << 1,      /* all bonds of one parity per launch */
+  TEBD_MODE_LEARNED_RANK = 1u << 2  /* randomized SVD sized by the rank model, full-SVD fallback */
+};
+
+/* Allocates the device state for a chain of L sites, local dimension d and
+ * bond dimension cap chi_max.  order must be 2 and bc 'O' (the configurations'
+ * schedule, TrotterStep -> TrotterStep2ndOrder, propagation_module.f90:42-114).
+ * Replaces AllocateGamLam (TEBDtools_module.f90:486) + SVDInit
+ * (local_operations_module.f90:84-122). */
+int tebd_create(tebd_ctx** ctx, int L, int d, int chi_max, int order, char bc, unsigned mode);
+
+/* Frees everything; replaces SVDFinish (local_operations_module.f90:124-153)
+ * and DeallocateGamLam. */
+void tebd_destroy(tebd_ctx* ctx);
+
+/* Trotter gates of every bond, nb = L-1 matrices U(i) of (d*d) x (d*d)
+ * complex, column-major, packed back to back (the Udt table built by
+ * ConstructPropagators, TEBDtools_module.f90:1079-1101). */
+int tebd_set_gates(tebd_ctx* ctx, const double* U, int nb);
+
+/* Uploads a Vidal state held in the reference's padded layout: gamma[s] points
+ * at site s's chi_in x d x chi_in complex array, lambda[l] (l = 0..L) at bond
+ * l's chi_in reals.  Bond dimensions are taken from the non-zero lambdas. */
+int tebd_upload(tebd_ctx* ctx, const double* const* gamma, const double* const* lambda, int chi_in);
+/* Same, with all sites (and all lambdas) contiguous. */
+int tebd_upload_packed(tebd_ctx* ctx, const double* gamma, const double* lambda, int chi_in);
+
+/* Downloads the state into the padded layout (zero-filled past each bond's
+ * current dimension); chi_out must be >= every bond dimension. */
+int tebd_download(tebd_ctx* ctx, double* const* gamma, double* const* lambda, int chi_out);
+int tebd_download_packed(tebd_ctx* ctx, double* gamma, double* lambda, int chi_out);
+
+/* One 2nd-order OBC Trotter step: odd bonds (gate dt/2), even bonds (dt),
+ * odd bonds (dt/2); *truncerr = sum of the bond truncation errors.
+ * Replaces TrotterStep / TrotterStep2ndOrder (propagation_module.f90:42-114). */
+int tebd_trotter_step(tebd_ctx* ctx, double* truncerr);
+
+/* One two-site update with an arbitrary (d*d)x(d*d) operator op2 on bond
+ * `link` (1-based, sites link and link+1).  Replaces TwoSiteOp_c
+ * (local_operations_module.f90:463-494). */
+int tebd_two_site_op(tebd_ctx* ctx, int link, const double* op2, double* truncerr);
+
+/* Re-orthogonalises every bipartition (left-to-right, then right-to-left
+ * sweep).  Replaces CanonicalFormAll (propagation_module.f90:617-634). */
+int tebd_canonical_form_all(tebd_ctx* ctx);
+
+/* Singular-value cutoff on the normalised weight s^2/||theta||^2.  Default
+ * 1e-30, the reference's rule (keep |s| > 1e-15 after normalisation,
+ * FormLambda1, local_operations_module.f90:360-375).  Larger cutoffs shrink
+ * the bond dimensions but perturb later observables at order sqrt(cutoff). */
+int tebd_set_cutoff(tebd_ctx* ctx, double cutoff);
+
+/* Current bond dimensions chi[l], l = 0..L (chi[0] = chi[L] = 1). */
+int tebd_get_bond_dims(tebd_ctx* ctx, int* chi);
+
+/* Counters since creation: [0] bond updates, [1] Jacobi sweeps, [2] fallbacks
+ * of the learned-rank path, [3] device milliseconds in trotter steps. */
+int tebd_get_stats(tebd_ctx* ctx, double* stats, int n);
+
+/* Blocks until the context's stream is idle. */
+int tebd_sync(tebd_ctx* ctx);
+
+const char* tebd_last_error(const tebd_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TEBD_HIP_H */

@@ -1,0 +1,72 @@
+// Device-side types shared by the gfx950 kernels of the two-site update.
+//
+// Layout in HBM (one context = one chain):
+//   Gamma of site s : packed complex(8) tensor Gamma(a,i,b), a < chi_s, i < d,
+//                     b < chi_{s+1}, element at a + chi_s*(i + d*b) -- the
+//                     reference's column-major order (TEBDtools_module.f90:496-513)
+//                     with the *current* bond dimensions as leading dimensions,
+//                     inside a slab of capacity chi_max*d*chi_max.
+//   lambda of bond l: chi_max doubles, entries >= chi_l are zero.
+//   theta/W         : R x Cpad complex column-major work matrix of one bond
+//                     (R = max(m,n), C = min(m,n), Cpad = C rounded up to a
+//                     multiple of the Jacobi pair width).
+//   V               : Cpad x Cpad complex, accumulated right rotations.
+#pragma once
+#include <hip/hip_runtime.h>
+
+struct __align__(16) zc {
+  double x, y;
+};
+
+__device__ __forceinline__ zc zmk(double x, double y) { zc r; r.x = x; r.y = y; return r; }
+__device__ __forceinline__ zc zadd(zc a, zc b) { return zmk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ zc zsub(zc a, zc b) { return zmk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ zc zmul(zc a, zc b) { return zmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ zc zscale(zc a, double s) { return zmk(a.x * s, a.y * s); }
+__device__ __forceinline__ zc zconj(zc a) { return zmk(a.x, -a.y); }
+// acc += a * b
+__device__ __forceinline__ void zfma(zc& acc, zc a, zc b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+}
+// acc += conj(a) * b
+__device__ __forceinline__ void zfmac(zc& acc, zc a, zc b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(-a.y, b.x, acc.y);
+}
+__device__ __forceinline__ double zabs2(zc a) { return a.x * a.x + a.y * a.y; }
+
+// Jacobi block geometry: pairs of JB-column blocks, i.e. JC = 2*JB columns
+// are orthogonalised together by one workgroup.
+constexpr int JB = 16;
+constexpr int JC = 2 * JB;
+
+// One bond update ("task") as the kernels see it.
+struct BondTask {
+  // inputs of the theta contraction
+  const zc* gl;      // Gamma_l  (chi_l x d x chi_c)
+  const zc* gr;      // Gamma_r  (chi_c x d x chi_r)
+  const double* ll;  // lambda_l (chi_l)
+  const double* lc;  // lambda_c (chi_c)
+  const double* lr;  // lambda_r (chi_r)
+  const zc* gate;    // d^2 x d^2 column-major, row (i*d+j)
+  int chil, chic, chir, d;
+  // SVD work
+  int transposed;    // 0: W = theta (m x n); 1: W = theta^dagger (n x m)
+  int R, C, Cpad, P; // rows, true columns, padded columns, Jacobi pairs (Cpad / JC)
+  zc* W;             // R x Cpad, ld = R
+  zc* V;             // Cpad x Cpad, ld = Cpad
+  double* s;         // Cpad: sorted singular values (descending)
+  int* perm;         // Cpad: original column of each sorted value
+  int* counter;      // rotations in the current sweep
+  double* result;    // [0]=kept k (as double), [1]=truncerr, [2]=||theta||^2, [3]=sweeps
+  // outputs
+  zc* gl_out;        // new Gamma_l (chi_l x d x k)
+  zc* gr_out;        // new Gamma_r (k x d x chi_r)
+  double* lc_out;    // new lambda_c (chi_max entries, zero-filled past k)
+  int chi_max;
+};

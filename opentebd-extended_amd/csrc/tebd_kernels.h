@@ -1,0 +1,13 @@
+// Host-side launchers of the gfx950 kernels (tebd_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "tebd_device.h"
+
+hipError_t launch_theta_gate(const BondTask* d_tasks, int ntasks, int d, int max_chil, int max_chir, hipStream_t st);
+hipError_t launch_prepare_svd(const BondTask* d_tasks, int ntasks, hipStream_t st);
+hipError_t launch_frobenius(const BondTask* d_tasks, int ntasks, int max_cpad, hipStream_t st);
+hipError_t launch_reset_counters(const BondTask* d_tasks, int ntasks, hipStream_t st);
+hipError_t launch_jacobi_round(const BondTask* d_tasks, int ntasks, int max_pairs, int round, double tol_rel,
+                               double tol_abs, hipStream_t st);
+hipError_t launch_finalize(const BondTask* d_tasks, int ntasks, int max_cpad, double cutoff, hipStream_t st);
+hipError_t launch_write_gammas(const BondTask* d_tasks, int ntasks, hipStream_t st);

@@ -1,0 +1,548 @@
+// gfx950 kernels of the two-site update (FP64).
+//
+//   k_theta_gate   FormTheta + ThetaOperation_c (local_operations_module.f90:184-211,248-280)
+//                  fused: theta'(a,i,j,b) = ll(a) lr(b) sum_kl U(ij,kl) sum_g Gl(a,k,g) lc(g) Gr(g,l,b)
+//                  written straight into the SVD work matrix W (theta, or theta^dagger when
+//                  the bond has more columns than rows).
+//   k_jacobi_round one round of a one-sided block-Jacobi SVD: every workgroup takes one pair of
+//                  JB-column blocks, forms their JC x JC Gram matrix, diagonalises it in LDS with
+//                  cyclic Jacobi, and rotates the JC columns of W and V.  Replaces ZGESVD
+//                  (local_operations_module.f90:322-345).
+//   k_col_norms    squared column norms of W.
+//   k_truncate     sort descending, choose the kept rank, lambda and truncerr
+//                  (FormLambda1, local_operations_module.f90:347-379).
+//   k_write_gammas FormGamma1 / FormGamma2 (local_operations_module.f90:381-428).
+#include "tebd_device.h"
+#include "tebd_kernels.h"
+
+#include <cfloat>
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// theta + gate
+// ---------------------------------------------------------------------------
+constexpr int KC = 16;  // gamma (contraction) chunk staged in LDS
+
+template <int D>
+struct ThetaCfg {
+  static constexpr int BA = (D <= 5) ? 16 : 8;  // alpha block == beta block
+  static constexpr int TR = BA * D;             // tile rows == tile cols: (k, alpha) pairs
+  static constexpr int MT = TR / 16;            // per-thread micro tile edge
+};
+
+template <int D>
+__global__ __launch_bounds__(256) void k_theta_gate(const BondTask* __restrict__ tasks) {
+  using Cfg = ThetaCfg<D>;
+  constexpr int BA = Cfg::BA, TR = Cfg::TR, MT = Cfg::MT, D2 = D * D;
+  const BondTask t = tasks[blockIdx.z];
+  const int a0 = blockIdx.x * BA, b0 = blockIdx.y * BA;
+  if (a0 >= t.chil || b0 >= t.chir) return;
+
+  constexpr int AB_ELEMS = 2 * KC * (TR + 1);
+  constexpr int T_ELEMS = TR * (TR + 1);
+  constexpr int SM_ELEMS = AB_ELEMS > T_ELEMS ? AB_ELEMS : T_ELEMS;
+  __shared__ zc sm[SM_ELEMS];
+  __shared__ zc sgate[D2 * D2];
+  zc* As = sm;                     // [KC][TR+1]  A(r, g) = Gl(a,k,g) lc(g), r = k*BA + a_local
+  zc* Bs = sm + KC * (TR + 1);     // [KC][TR+1]  B(g, c) = Gr(g,l,b),       c = l*BA + b_local
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+
+  for (int e = tid; e < D2 * D2; e += 256) sgate[e] = t.gate[e];
+
+  zc acc[MT][MT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < MT; ++j) acc[i][j] = zmk(0.0, 0.0);
+
+  const int chil = t.chil, chic = t.chic, chir = t.chir;
+  for (int g0 = 0; g0 < chic; g0 += KC) {
+    // A chunk: r fastest (contiguous alpha within one k)
+    for (int e = tid; e < KC * TR; e += 256) {
+      const int r = e % TR, g = e / TR;
+      const int k = r / BA, a = a0 + (r % BA), gg = g0 + g;
+      zc v = zmk(0.0, 0.0);
+      if (a < chil && gg < chic) v = zscale(t.gl[a + (size_t)chil * (k + (size_t)D * gg)], t.lc[gg]);
+      As[g * (TR + 1) + r] = v;
+    }
+    // B chunk: g fastest (contiguous gamma)
+    for (int e = tid; e < KC * TR; e += 256) {
+      const int g = e % KC, c = e / KC;
+      const int l = c / BA, b = b0 + (c % BA), gg = g0 + g;
+      zc v = zmk(0.0, 0.0);
+      if (b < chir && gg < chic) v = t.gr[gg + (size_t)chic * (l + (size_t)D * b)];
+      Bs[g * (TR + 1) + c] = v;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int g = 0; g < KC; ++g) {
+      zc av[MT], bv[MT];
+#pragma unroll
+      for (int i = 0; i < MT; ++i) av[i] = As[g * (TR + 1) + ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < MT; ++j) bv[j] = Bs[g * (TR + 1) + tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < MT; ++j) zfma(acc[i][j], av[i], bv[j]);
+    }
+    __syncthreads();
+  }
+  // stage T(r, c) in LDS for the gate
+  zc* T = sm;
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < MT; ++j) T[(ty + 16 * i) * (TR + 1) + tx + 16 * j] = acc[i][j];
+  __syncthreads();
+
+  // gate: one (alpha, beta) pair per thread
+  for (int pr = tid; pr < BA * BA; pr += 256) {
+    int al, bl;
+    if (!t.transposed) { al = pr % BA; bl = pr / BA; } else { bl = pr % BA; al = pr / BA; }
+    const int a = a0 + al, b = b0 + bl;
+    if (a >= chil || b >= chir) continue;
+    zc x[D2];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+      for (int l = 0; l < D; ++l) x[k * D + l] = T[(k * BA + al) * (TR + 1) + l * BA + bl];
+    const double sc = t.ll[a] * t.lr[b];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        zc o = zmk(0.0, 0.0);
+#pragma unroll
+        for (int kl = 0; kl < D2; ++kl) zfma(o, sgate[(i * D + j) + D2 * kl], x[kl]);
+        o = zscale(o, sc);
+        const size_t row = (size_t)i * chil + a;  // theta row (i, alpha)
+        const size_t col = (size_t)j * chir + b;  // theta col (j, beta)
+        if (!t.transposed) t.W[row + (size_t)t.R * col] = o;
+        else t.W[col + (size_t)t.R * row] = zconj(o);
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// W padding columns := 0, V := identity
+// ---------------------------------------------------------------------------
+__global__ void k_prepare_svd(const BondTask* __restrict__ tasks) {
+  const BondTask t = tasks[blockIdx.y];
+  const size_t nV = (size_t)t.Cpad * t.Cpad;
+  const size_t nWp = (size_t)t.R * (t.Cpad - t.C);
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nV + nWp; e += (size_t)gridDim.x * blockDim.x) {
+    if (e < nV) {
+      const size_t r = e % t.Cpad, c = e / t.Cpad;
+      t.V[e] = zmk(r == c ? 1.0 : 0.0, 0.0);
+    } else {
+      t.W[(size_t)t.R * t.C + (e - nV)] = zmk(0.0, 0.0);
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *t.counter = 0;
+}
+
+__global__ void k_reset_counters(const BondTask* __restrict__ tasks, int ntasks) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ntasks) *tasks[i].counter = 0;
+}
+
+// ---------------------------------------------------------------------------
+// one-sided block Jacobi
+// ---------------------------------------------------------------------------
+// Round-robin ("circle") schedule over nb = 2P blocks: block 0 stays, the others
+// rotate; over nb-1 rounds every pair of blocks meets exactly once.
+__device__ __forceinline__ void rr_pair(int nb, int round, int p, int& bi, int& bj) {
+  auto pos = [&](int k) { return k == 0 ? 0 : 1 + (k - 1 + round) % (nb - 1); };
+  bi = pos(p);
+  bj = pos(nb - 1 - p);
+}
+
+// Hermitian 2x2 Jacobi rotation J = [[c, s w], [-s conj(w), c]], w = g/|g|, that
+// zeroes g in [[a, g], [conj(g), b]] via J^dagger H J.
+__device__ __forceinline__ void jrot(double a, double b, zc g, double& c, double& s, zc& w) {
+  const double ag = sqrt(zabs2(g));
+  if (ag == 0.0) { c = 1.0; s = 0.0; w = zmk(1.0, 0.0); return; }
+  w = zmk(g.x / ag, g.y / ag);
+  const double zeta = (b - a) / (2.0 * ag);
+  const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  c = 1.0 / sqrt(1.0 + tt * tt);
+  s = tt * c;
+}
+
+constexpr int RC = 64;  // rows of W / V streamed through LDS per chunk
+
+// A pair is left alone when every off-diagonal Gram entry satisfies
+// |A_pq| <= tol_rel sqrt(A_pp A_qq)  or
+// |A_pq| <= tol_abs ||theta|| (sqrt(A_pp) + sqrt(A_qq)).
+// The second bound is the rounding floor: rotations add errors of order
+// eps * sigma_max to every column they touch, so inner products below it are
+// noise.  It scales with the two column norms, which keeps the small singular
+// values resolved to absolute accuracy ~eps * sigma_max (as LAPACK's are).
+__global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict__ tasks, int round, double tol_rel,
+                                                       double tol_abs) {
+  const BondTask t = tasks[blockIdx.y];
+  const int p = blockIdx.x;
+  if (p >= t.P || round >= 2 * t.P - 1) return;
+  int bi, bj;
+  rr_pair(2 * t.P, round, p, bi, bj);
+
+  __shared__ zc A[JC][JC + 1];
+  __shared__ zc Q[JC][JC + 1];
+  __shared__ zc buf[RC][JC + 1];
+  __shared__ double rc_c[JC / 2], rc_s[JC / 2];
+  __shared__ zc rc_w[JC / 2];
+  __shared__ int rc_p[JC / 2], rc_q[JC / 2];
+  __shared__ int s_flag;
+
+  const int tid = threadIdx.x;
+  auto colidx = [&](int c) { return (c < JB ? bi * JB + c : bj * JB + (c - JB)); };
+
+  // ---- Gram of the JC columns
+  {
+    const int gp = tid >> 3, gq0 = (tid & 7) * 4;
+    zc acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = zmk(0.0, 0.0);
+    for (int r0 = 0; r0 < t.R; r0 += RC) {
+      for (int e = tid; e < RC * JC; e += 256) {
+        const int rr = e % RC, c = e / RC;
+        const int r = r0 + rr;
+        buf[rr][c] = (r < t.R) ? t.W[r + (size_t)t.R * colidx(c)] : zmk(0.0, 0.0);
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int rr = 0; rr < RC; ++rr) {
+        const zc a = buf[rr][gp];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) zfmac(acc[q], a, buf[rr][gq0 + q]);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) A[gp][gq0 + q] = acc[q];
+  }
+  __syncthreads();
+
+  // ---- does this pair need a rotation?  |A_pq| > tol_rel sqrt(A_pp A_qq) and > tol_abs
+  {
+    const double fnorm = sqrt(t.result[2]);
+    int need = 0;
+    for (int e = tid; e < JC * JC; e += 256) {
+      const int pp = e % JC, qq = e / JC;
+      if (pp == qq) continue;
+      const double g2 = zabs2(A[pp][qq]);
+      const double app = fmax(A[pp][pp].x, 0.0), aqq = fmax(A[qq][qq].x, 0.0);
+      const double fl = tol_abs * fnorm * (sqrt(app) + sqrt(aqq));
+      if (g2 > tol_rel * tol_rel * app * aqq && g2 > fl * fl) need = 1;
+    }
+    if (tid == 0) s_flag = 0;
+    __syncthreads();
+    if (need) s_flag = 1;
+    __syncthreads();
+    if (!s_flag) return;
+  }
+  if (tid == 0) atomicAdd(t.counter, 1);
+
+  // ---- cyclic Jacobi on A (JC x JC Hermitian) in LDS, Q accumulates rotations.
+  // A Gram entry is accurate relative to sqrt(A_pp A_qq), so the inner solve
+  // uses a purely relative test (skip when |A_pq| <= 1e-15 sqrt(A_pp A_qq)),
+  // sets each annihilated entry to exactly zero, and ends after a sweep without
+  // rotations.  The pair's columns keep their block (re-sorting columns between
+  // blocks stalls the convergence of the cyclic block sweep).
+  for (int e = tid; e < JC * JC; e += 256) {
+    const int r = e % JC, c = e / JC;
+    Q[r][c] = zmk(r == c ? 1.0 : 0.0, 0.0);
+  }
+  __shared__ int s_rot;
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    if (tid == 0) s_rot = 0;
+    __syncthreads();
+    for (int rd = 0; rd < JC - 1; ++rd) {
+      if (tid < JC / 2) {
+        int pp, qq;
+        rr_pair(JC, rd, tid, pp, qq);
+        if (pp > qq) { const int tmp = pp; pp = qq; qq = tmp; }
+        double c, s;
+        zc w;
+        const zc g = A[pp][qq];
+        const double a = A[pp][pp].x, b = A[qq][qq].x;
+        const double g2 = zabs2(g);
+        if (g2 <= 1e-30 * fabs(a * b) || g2 == 0.0) { c = 1.0; s = 0.0; w = zmk(1.0, 0.0); }
+        else { jrot(a, b, g, c, s, w); s_rot = 1; }
+        rc_c[tid] = c; rc_s[tid] = s; rc_w[tid] = w; rc_p[tid] = pp; rc_q[tid] = qq;
+      }
+      __syncthreads();
+      // rows: A_p <- c A_p - s w A_q ; A_q <- s conj(w) A_p + c A_q
+      for (int e = tid; e < (JC / 2) * JC; e += 256) {
+        const int pr = e / JC, k = e % JC;
+        const double c = rc_c[pr], s = rc_s[pr];
+        if (s == 0.0) continue;
+        const zc w = rc_w[pr];
+        const int pp = rc_p[pr], qq = rc_q[pr];
+        const zc xp = A[pp][k], xq = A[qq][k];
+        const zc sw = zscale(w, s), swc = zscale(zconj(w), s);
+        A[pp][k] = zsub(zscale(xp, c), zmul(sw, xq));
+        A[qq][k] = zadd(zmul(swc, xp), zscale(xq, c));
+      }
+      __syncthreads();
+      // cols: A_:p <- c A_:p - s conj(w) A_:q ; A_:q <- s w A_:p + c A_:q  (same for Q)
+      for (int e = tid; e < (JC / 2) * JC; e += 256) {
+        const int pr = e / JC, k = e % JC;
+        const double c = rc_c[pr], s = rc_s[pr];
+        if (s == 0.0) continue;
+        const zc w = rc_w[pr];
+        const int pp = rc_p[pr], qq = rc_q[pr];
+        const zc sw = zscale(w, s), swc = zscale(zconj(w), s);
+        zc xp = A[k][pp], xq = A[k][qq];
+        A[k][pp] = zsub(zscale(xp, c), zmul(swc, xq));
+        A[k][qq] = zadd(zmul(sw, xp), zscale(xq, c));
+        xp = Q[k][pp]; xq = Q[k][qq];
+        Q[k][pp] = zsub(zscale(xp, c), zmul(swc, xq));
+        Q[k][qq] = zadd(zmul(sw, xp), zscale(xq, c));
+      }
+      __syncthreads();
+      if (tid < JC / 2 && rc_s[tid] != 0.0) {
+        A[rc_p[tid]][rc_q[tid]] = zmk(0.0, 0.0);
+        A[rc_q[tid]][rc_p[tid]] = zmk(0.0, 0.0);
+      }
+      __syncthreads();
+    }
+    if (!s_rot) break;
+    __syncthreads();
+  }
+
+  // ---- rotate the JC columns of W and of V by Q
+  auto rotate = [&](zc* M, int rows, int ld) {
+    const int orow = tid >> 2, oc0 = (tid & 3) * 8;
+    for (int r0 = 0; r0 < rows; r0 += RC) {
+      for (int e = tid; e < RC * JC; e += 256) {
+        const int rr = e % RC, c = e / RC;
+        const int r = r0 + rr;
+        buf[rr][c] = (r < rows) ? M[r + (size_t)ld * colidx(c)] : zmk(0.0, 0.0);
+      }
+      __syncthreads();
+      zc o[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) o[q] = zmk(0.0, 0.0);
+#pragma unroll 4
+      for (int k = 0; k < JC; ++k) {
+        const zc x = buf[orow][k];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) zfma(o[q], x, Q[k][oc0 + q]);
+      }
+      const int r = r0 + orow;
+      if (r < rows) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) M[r + (size_t)ld * colidx(oc0 + q)] = o[q];
+      }
+      __syncthreads();
+    }
+  };
+  rotate(t.W, t.R, t.R);
+  rotate(t.V, t.Cpad, t.Cpad);
+}
+
+// ---------------------------------------------------------------------------
+// column norms, truncation, Gamma rebuild
+// ---------------------------------------------------------------------------
+// one wavefront per column: s[c] = ||W[:, c]||^2 (not yet sorted)
+__global__ __launch_bounds__(256) void k_col_norms(const BondTask* __restrict__ tasks) {
+  const BondTask t = tasks[blockIdx.y];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + wave;
+  if (c >= t.Cpad) return;
+  double acc = 0.0;
+  if (c < t.C)
+    for (int r = lane; r < t.R; r += 64) acc += zabs2(t.W[r + (size_t)t.R * c]);
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) t.s[c] = acc;
+}
+
+// ||theta||^2 = sum of the squared column norms (one workgroup per task, fixed
+// summation order so repeated runs are bitwise identical).
+__global__ __launch_bounds__(256) void k_frob(const BondTask* __restrict__ tasks) {
+  const BondTask t = tasks[blockIdx.x];
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int c = threadIdx.x; c < t.C; c += 256) acc += t.s[c];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) t.result[2] = red[0];
+}
+
+constexpr int SORT_N = 4096;
+
+// one workgroup per task: bitonic sort of the squared norms (descending), kept rank,
+// lambda_c and truncerr.  Rule (FormLambda1): keep index < chi_max with
+// s^2/||theta||^2 > cutoff (cutoff >= 1e-30, i.e. |s| > 1e-15 after normalisation).
+__global__ __launch_bounds__(1024) void k_truncate(const BondTask* __restrict__ tasks, double cutoff) {
+  const BondTask t = tasks[blockIdx.x];
+  __shared__ double key[SORT_N];
+  __shared__ int idx[SORT_N];
+  __shared__ double red[1024];
+  __shared__ int s_k;
+  const int tid = threadIdx.x;
+  int n2 = 1;
+  while (n2 < t.Cpad) n2 <<= 1;
+  for (int e = tid; e < n2; e += 1024) {
+    key[e] = (e < t.C) ? t.s[e] : -1.0;
+    idx[e] = e;
+  }
+  __syncthreads();
+  for (int size = 2; size <= n2; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int e = tid; e < n2; e += 1024) {
+        const int partner = e ^ stride;
+        if (partner > e) {
+          const bool desc = ((e & size) == 0);
+          const double ke = key[e], kp = key[partner];
+          const bool swap = desc ? (ke < kp) : (ke > kp);
+          if (swap) {
+            key[e] = kp; key[partner] = ke;
+            const int ti = idx[e]; idx[e] = idx[partner]; idx[partner] = ti;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // total weight
+  double part = 0.0;
+  for (int e = tid; e < t.C; e += 1024) part += key[e];
+  red[tid] = part;
+  __syncthreads();
+  for (int st = 512; st > 0; st >>= 1) { if (tid < st) red[tid] += red[tid + st]; __syncthreads(); }
+  const double total = red[0];
+  __syncthreads();
+  if (tid == 0) s_k = 0;
+  __syncthreads();
+  const int kmax = t.C < t.chi_max ? t.C : t.chi_max;
+  const double thr = (cutoff > 1e-30 ? cutoff : 1e-30) * total;
+  int cnt = 0;
+  for (int e = tid; e < kmax; e += 1024) cnt += (key[e] > thr) ? 1 : 0;
+  if (cnt) atomicAdd(&s_k, cnt);
+  __syncthreads();
+  const int k = s_k;
+  part = 0.0;
+  for (int e = tid; e < k; e += 1024) part += key[e];
+  red[tid] = part;
+  __syncthreads();
+  for (int st = 512; st > 0; st >>= 1) { if (tid < st) red[tid] += red[tid + st]; __syncthreads(); }
+  const double kept = red[0];
+  const double inv = kept > 0.0 ? 1.0 / sqrt(kept) : 0.0;
+  for (int e = tid; e < t.Cpad; e += 1024) {
+    const double v = (e < t.C && key[e] > 0.0) ? key[e] : 0.0;
+    t.s[e] = sqrt(v);
+    t.perm[e] = idx[e];
+  }
+  for (int e = tid; e < t.chi_max; e += 1024) t.lc_out[e] = (e < k) ? sqrt(key[e]) * inv : 0.0;
+  if (tid == 0) {
+    t.result[0] = (double)k;
+    t.result[1] = total > 0.0 ? fabs(1.0 - kept / total) : 1.0;
+    t.result[2] = total;
+  }
+}
+
+// Gamma_l(a,i,b) = U((i,a), b)/ll(a);  Gamma_r(g,j,b) = conj(V((j,b), g))/lr(b)
+__global__ void k_write_gammas(const BondTask* __restrict__ tasks) {
+  const BondTask t = tasks[blockIdx.y];
+  const int k = (int)t.result[0];
+  const int dd = t.d;
+  const size_t nl = (size_t)t.chil * dd * k;
+  const size_t nr = (size_t)k * dd * t.chir;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nl + nr; e += (size_t)gridDim.x * blockDim.x) {
+    if (e < nl) {
+      const int a = (int)(e % t.chil);
+      const size_t rest = e / t.chil;
+      const int i = (int)(rest % dd);
+      const int b = (int)(rest / dd);
+      const size_t row = (size_t)i * t.chil + a;
+      const int col = t.perm[b];
+      zc u;
+      if (!t.transposed) u = zscale(t.W[row + (size_t)t.R * col], t.s[b] > 0.0 ? 1.0 / t.s[b] : 0.0);
+      else u = t.V[row + (size_t)t.Cpad * col];
+      const double la = t.ll[a];
+      t.gl_out[e] = fabs(la) > 1e-15 ? zscale(u, 1.0 / la) : zmk(0.0, 0.0);
+    } else {
+      const size_t f = e - nl;
+      const int g = (int)(f % k);
+      const size_t rest = f / k;
+      const int j = (int)(rest % dd);
+      const int b = (int)(rest / dd);
+      const size_t row = (size_t)j * t.chir + b;
+      const int col = t.perm[g];
+      zc v;
+      if (!t.transposed) v = t.V[row + (size_t)t.Cpad * col];
+      else v = zscale(t.W[row + (size_t)t.R * col], t.s[g] > 0.0 ? 1.0 / t.s[g] : 0.0);
+      const double lb = t.lr[b];
+      t.gr_out[f] = fabs(lb) > 1e-15 ? zscale(zconj(v), 1.0 / lb) : zmk(0.0, 0.0);
+    }
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_theta_gate(const BondTask* d_tasks, int ntasks, int d, int max_chil, int max_chir,
+                             hipStream_t st) {
+  auto go = [&](auto kern, int BA) {
+    dim3 grid((max_chil + BA - 1) / BA, (max_chir + BA - 1) / BA, ntasks);
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, d_tasks);
+    return hipGetLastError();
+  };
+  switch (d) {
+    case 2: return go(k_theta_gate<2>, ThetaCfg<2>::BA);
+    case 3: return go(k_theta_gate<3>, ThetaCfg<3>::BA);
+    case 4: return go(k_theta_gate<4>, ThetaCfg<4>::BA);
+    case 5: return go(k_theta_gate<5>, ThetaCfg<5>::BA);
+    case 6: return go(k_theta_gate<6>, ThetaCfg<6>::BA);
+    case 8: return go(k_theta_gate<8>, ThetaCfg<8>::BA);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_prepare_svd(const BondTask* d_tasks, int ntasks, hipStream_t st) {
+  hipLaunchKernelGGL(k_prepare_svd, dim3(256, ntasks), dim3(256), 0, st, d_tasks);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset_counters(const BondTask* d_tasks, int ntasks, hipStream_t st) {
+  hipLaunchKernelGGL(k_reset_counters, dim3((ntasks + 255) / 256), dim3(256), 0, st, d_tasks, ntasks);
+  return hipGetLastError();
+}
+
+hipError_t launch_frobenius(const BondTask* d_tasks, int ntasks, int max_cpad, hipStream_t st) {
+  hipLaunchKernelGGL(k_col_norms, dim3((max_cpad + 3) / 4, ntasks), dim3(256), 0, st, d_tasks);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_frob, dim3(ntasks), dim3(256), 0, st, d_tasks);
+  return hipGetLastError();
+}
+
+hipError_t launch_jacobi_round(const BondTask* d_tasks, int ntasks, int max_pairs, int round, double tol_rel,
+                               double tol_abs, hipStream_t st) {
+  hipLaunchKernelGGL(k_jacobi_round, dim3(max_pairs, ntasks), dim3(256), 0, st, d_tasks, round, tol_rel, tol_abs);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const BondTask* d_tasks, int ntasks, int max_cpad, double cutoff, hipStream_t st) {
+  hipLaunchKernelGGL(k_col_norms, dim3((max_cpad + 3) / 4, ntasks), dim3(256), 0, st, d_tasks);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_truncate, dim3(ntasks), dim3(1024), 0, st, d_tasks, cutoff);
+  return hipGetLastError();
+}
+
+hipError_t launch_write_gammas(const BondTask* d_tasks, int ntasks, hipStream_t st) {
+  hipLaunchKernelGGL(k_write_gammas, dim3(512, ntasks), dim3(256), 0, st, d_tasks);
+  return hipGetLastError();
+}

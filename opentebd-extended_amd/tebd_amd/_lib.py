@@ -1,0 +1,67 @@
+"""ctypes binding of libtebdhip.so (declarations: include/tebd_hip.h).
+
+The HIP library is the only compute path: if it is not built, importing the
+binding raises instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_PKG), "libtebdhip.so")
+
+TEBD_OK = 0
+TEBD_ERR_ARG = -1
+TEBD_ERR_HIP = -2
+TEBD_ERR_SHAPE = -3
+TEBD_ERR_NAN = -4
+TEBD_ERR_NOCONV = -5
+TEBD_ERR_STATE = -6
+TEBD_ERR_UNSUPPORTED = -7
+
+MODE_FP64 = 0
+MODE_MIXED = 1 << 0
+MODE_BATCHED = 1 << 1
+MODE_LEARNED_RANK = 1 << 2
+
+_lib = None
+
+
+class TEBDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                              "(make -C opentebd-extended_amd/csrc)")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        I = ctypes.c_int
+        D = ctypes.c_double
+        L.tebd_create.argtypes = [ctypes.POINTER(P), I, I, I, I, ctypes.c_char, ctypes.c_uint]
+        L.tebd_destroy.argtypes = [P]
+        L.tebd_destroy.restype = None
+        L.tebd_set_gates.argtypes = [P, P, I]
+        L.tebd_upload.argtypes = [P, P, P, I]
+        L.tebd_upload_packed.argtypes = [P, P, P, I]
+        L.tebd_download.argtypes = [P, P, P, I]
+        L.tebd_download_packed.argtypes = [P, P, P, I]
+        L.tebd_trotter_step.argtypes = [P, ctypes.POINTER(D)]
+        L.tebd_two_site_op.argtypes = [P, I, P, ctypes.POINTER(D)]
+        L.tebd_canonical_form_all.argtypes = [P]
+        L.tebd_set_cutoff.argtypes = [P, D]
+        L.tebd_get_bond_dims.argtypes = [P, P]
+        L.tebd_get_stats.argtypes = [P, P, I]
+        L.tebd_sync.argtypes = [P]
+        L.tebd_last_error.argtypes = [P]
+        L.tebd_last_error.restype = ctypes.c_char_p
+        for name in ("tebd_create", "tebd_set_gates", "tebd_upload", "tebd_upload_packed", "tebd_download",
+                     "tebd_download_packed", "tebd_trotter_step", "tebd_two_site_op", "tebd_canonical_form_all",
+                     "tebd_set_cutoff", "tebd_get_bond_dims", "tebd_get_stats", "tebd_sync"):
+            getattr(L, name).restype = I
+        _lib = L
+    return _lib

@@ -1,0 +1,137 @@
+"""Device-resident Vidal state and the two-site update, through libtebdhip.so.
+
+Mirrors the reference's call surface (VidalCore_v_2.0/propagation_module.f90,
+local_operations_module.f90) on a context object:
+
+=========================  ==================================================
+reference                  here
+=========================  ==================================================
+AllocateGamLam + SVDInit   ``Context(L, d, chi_max)``  (TEBDtools_module.f90:486,
+                           local_operations_module.f90:84)
+ConstructPropagators       ``models.construct_propagators`` + ``set_gates``
+TrotterStep                ``trotter_step()`` (propagation_module.f90:42-114)
+TwoSiteOp                  ``two_site_op(link, op)`` (local_operations_module.f90:463-494)
+CanonicalFormAll           ``canonical_form_all()`` (propagation_module.f90:617-634)
+SVDFinish + Deallocate     ``close()``
+=========================  ==================================================
+
+States cross the boundary in the reference's padded layout: ``gam`` complex128
+(L, chi, d, chi) indexed [site, b, i, a] (each site's memory is the Fortran
+Gamma(a, i, b)), ``lam`` float64 (L+1, chi).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import TEBDError
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+_MODES = {"fp64": _lib.MODE_FP64, "mixed": _lib.MODE_MIXED, "batched": _lib.MODE_BATCHED,
+          "learned": _lib.MODE_LEARNED_RANK}
+
+
+def parse_mode(mode):
+    if isinstance(mode, int):
+        return mode
+    m = 0
+    for part in str(mode).replace("+", ",").split(","):
+        part = part.strip()
+        if part:
+            m |= _MODES[part]
+    return m
+
+
+class Context:
+    """One chain on one GPU (one HIP stream)."""
+
+    def __init__(self, L, d, chi_max, mode="fp64", order=2, bc="O", cutoff=None):
+        self._lib = _lib.lib()
+        self.L, self.d, self.chi_max = int(L), int(d), int(chi_max)
+        self.mode = parse_mode(mode)
+        h = ctypes.c_void_p()
+        rc = self._lib.tebd_create(ctypes.byref(h), self.L, self.d, self.chi_max, int(order),
+                                   bc.encode()[:1], self.mode)
+        if rc != 0:
+            raise TEBDError(rc, "tebd_create failed")
+        self._h = h
+        if cutoff is not None:
+            self.set_cutoff(cutoff)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise TEBDError(rc, self._lib.tebd_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.tebd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- set-up -------------------------------------------------------------
+    def set_cutoff(self, cutoff):
+        self._check(self._lib.tebd_set_cutoff(self._h, float(cutoff)))
+
+    def set_gates(self, U):
+        """U: (L-1, d^2, d^2) complex gates indexed [bond, row, col]."""
+        u = np.ascontiguousarray(np.transpose(np.asarray(U, np.complex128), (0, 2, 1)))
+        self._check(self._lib.tebd_set_gates(self._h, _p(u), u.shape[0]))
+
+    def upload(self, gam, lam):
+        gam = np.ascontiguousarray(gam, np.complex128)
+        lam = np.ascontiguousarray(lam, np.float64)
+        chi = gam.shape[1]
+        if gam.shape != (self.L, chi, self.d, chi) or lam.shape != (self.L + 1, chi):
+            raise ValueError("state shape does not match the context")
+        self._check(self._lib.tebd_upload_packed(self._h, _p(gam), _p(lam), chi))
+
+    def download(self, chi_out=None):
+        chi = int(chi_out or self.chi_max)
+        gam = np.zeros((self.L, chi, self.d, chi), np.complex128)
+        lam = np.zeros((self.L + 1, chi))
+        self._check(self._lib.tebd_download_packed(self._h, _p(gam), _p(lam), chi))
+        return gam, lam
+
+    # -- the hot path ---------------------------------------------------------
+    def trotter_step(self):
+        t = ctypes.c_double()
+        self._check(self._lib.tebd_trotter_step(self._h, ctypes.byref(t)))
+        return t.value
+
+    def two_site_op(self, link, op):
+        o = np.ascontiguousarray(np.asarray(op, np.complex128).T)
+        t = ctypes.c_double()
+        self._check(self._lib.tebd_two_site_op(self._h, int(link), _p(o), ctypes.byref(t)))
+        return t.value
+
+    def canonical_form_all(self):
+        self._check(self._lib.tebd_canonical_form_all(self._h))
+
+    # -- introspection --------------------------------------------------------
+    def bond_dims(self):
+        out = np.zeros(self.L + 1, np.int32)
+        self._check(self._lib.tebd_get_bond_dims(self._h, _p(out)))
+        return out
+
+    def stats(self):
+        out = np.zeros(4)
+        self._check(self._lib.tebd_get_stats(self._h, _p(out), 4))
+        return {"bond_updates": out[0], "jacobi_sweeps": out[1], "fallbacks": out[2], "device_ms": out[3]}
+
+    def sync(self):
+        self._check(self._lib.tebd_sync(self._h))

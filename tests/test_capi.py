@@ -1,0 +1,47 @@
+"""The C-ABI library loads and exports every entry point include/tebd_hip.h declares
+(no device calls: argument errors are rejected before any HIP call)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+from tebd_amd import _lib
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "tebd_hip.h")).read()
+    return sorted(set(re.findall(r"\b(tebd_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_functions()
+    assert "tebd_trotter_step" in names and len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("args", [
+    (1, 2, 4, 2, b"O"),      # L < 2
+    (8, 7, 4, 2, b"O"),      # unsupported local dimension
+    (8, 2, 4, 5, b"O"),      # 5th-order schedule not provided
+    (8, 2, 4, 2, b"P"),      # periodic boundaries not provided
+    (8, 2, 0, 2, b"O"),      # chi_max < 1
+    (8, 4, 2048, 2, b"O"),   # d * chi_max beyond the SVD work size
+])
+def test_create_rejects_bad_arguments_without_touching_the_gpu(args):
+    lib = _lib.lib()
+    h = ctypes.c_void_p()
+    rc = lib.tebd_create(ctypes.byref(h), *args, 0)
+    assert rc == _lib.TEBD_ERR_ARG
+    assert not h.value
+
+
+def test_null_context_calls_are_errors():
+    lib = _lib.lib()
+    t = ctypes.c_double()
+    assert lib.tebd_trotter_step(None, ctypes.byref(t)) == _lib.TEBD_ERR_ARG
+    assert lib.tebd_set_cutoff(None, 1e-12) == _lib.TEBD_ERR_ARG
+    assert lib.tebd_last_error(None) == b"null context"
