@@ -98,8 +98,20 @@ int tebd_set_cutoff(tebd_ctx* ctx, double cutoff);
 int tebd_get_bond_dims(tebd_ctx* ctx, int* chi);
 
 /* Counters since creation: [0] bond updates, [1] Jacobi sweeps, [2] fallbacks
- * of the learned-rank path, [3] device milliseconds in trotter steps. */
+ * of the learned-rank path, [3] device milliseconds in trotter steps,
+ * [4..7] Jacobi sweeps of the sketched path's stages (Z0, residual block,
+ * residual re-orthogonalisation, Rayleigh-Ritz). */
 int tebd_get_stats(tebd_ctx* ctx, double* stats, int n);
+
+/* Per-kernel timing with HIP events recorded on the context's stream (no
+ * counterpart in the reference, whose only timing is CPU_TIME around whole
+ * mains, Case_Studies/Heisenberg_Wrapper/XXDynamics.f90:89,196-198).
+ * tebd_set_profiling(ctx, 1) clears and enables the counters;
+ * tebd_get_profile fills out[0..8] = {launches, ms, algorithmic flops} for
+ * the theta+gate kernel, the Jacobi round kernel and the truncation + Gamma
+ * rebuild kernels, in that order. */
+int tebd_set_profiling(tebd_ctx* ctx, int on);
+int tebd_get_profile(tebd_ctx* ctx, double* out, int n);
 
 /* Blocks until the context's stream is idle. */
 int tebd_sync(tebd_ctx* ctx);

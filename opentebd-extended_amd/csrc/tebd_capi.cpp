@@ -15,9 +15,10 @@
 
 namespace {
 
+constexpr int kStages = 5;
 constexpr int kMaxSweeps = 40;
 constexpr double kTolRel = 1e-13;   // relative orthogonality of converged column pairs
-constexpr double kTolAbs = 1e-14;   // rounding floor: |A_pq| <= kTolAbs ||theta|| (|w_p| + |w_q|)
+constexpr double kTolAbs = 1e-13;   // rounding floor: |A_pq| <= kTolAbs ||theta|| (|w_p| + |w_q|)
 
 int round_up(int x, int m) { return ((x + m - 1) / m) * m; }
 
@@ -44,13 +45,36 @@ struct tebd_ctx {
   int* perm = nullptr;
   int* counters = nullptr;
   double* results = nullptr;
-  BondTask* d_tasks = nullptr;
-  BondTask* h_tasks = nullptr;   // pinned
+  BondTask* d_tasks = nullptr;   // kStages x ntask_max
+  BondTask* h_tasks = nullptr;   // pinned, kStages x ntask_max
+  // sketched (learned-rank) path workspace
+  zc* Zb = nullptr;              // per task n x (2 k') right basis
+  zc* Bb = nullptr;              // per task m x (2 k') sketch of theta
+  double* lam_scratch = nullptr; // per task chi_max: lambda before it is committed
+  GemmTask* d_gemm = nullptr;
+  GemmTask* h_gemm = nullptr;    // pinned
+  int* d_kprime = nullptr;
+  int* h_kprime = nullptr;       // pinned
+  unsigned long long sketch_seed = 0x5DEECE66Dull;
+  double cutoff_sketch = 1e-24;  // kept weight s^2/||theta||^2 on the sketched path
+  double tol_missed = 1e-20;     // max weight of theta outside the sketch
+  double tol_trunc = 1e-10;      // max discarded weight before the full SVD judges
+  int oversample = 8;
+  long long step_index = 0;
+  std::vector<int> last_rank;    // kept rank of each bond's last update (rank model input)
   double* h_results = nullptr;   // pinned
   int* h_counters = nullptr;     // pinned
   std::vector<zc> staging;
   std::string err;
-  double stats[4] = {0, 0, 0, 0};
+  double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // [4..7]: Jacobi sweeps of the sketch stages Z0, R, R re-orth, Rayleigh-Ritz
+  // optional per-kernel timing with HIP events on the context's stream:
+  // prof[k*3 + 0] launches, [1] milliseconds, [2] algorithmic flops, for
+  // k = 0 theta+gate, 1 Jacobi round, 2 truncation + Gamma rebuild
+  bool profiling = false;
+  double prof[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  struct Pending { hipEvent_t a, b; int kind; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
 
   size_t gslab() const { return (size_t)chi_max * d * chi_max; }
   size_t d4() const { return (size_t)d * d * d * d; }
@@ -70,8 +94,76 @@ int fail(tebd_ctx* c, int code, const std::string& msg) {
       return fail((c), TEBD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+hipEvent_t pool_event(tebd_ctx* c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one launch with events when profiling is on.
+struct ProfScope {
+  tebd_ctx* c;
+  int kind;
+  hipEvent_t a = nullptr;
+  ProfScope(tebd_ctx* cc, int k) : c(cc), kind(k) {
+    if (c->profiling && (a = pool_event(c))) (void)hipEventRecord(a, c->st);
+  }
+  ~ProfScope() {
+    if (!a) return;
+    hipEvent_t b = pool_event(c);
+    if (!b) { c->event_pool.push_back(a); return; }
+    (void)hipEventRecord(b, c->st);
+    c->pending.push_back({a, b, kind});
+  }
+};
+
+// Called after a stream synchronisation: folds finished event pairs into prof[].
+void drain_profile(tebd_ctx* c) {
+  for (auto& p : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      c->prof[3 * p.kind + 0] += 1;
+      c->prof[3 * p.kind + 1] += ms;
+    }
+    c->event_pool.push_back(p.a);
+    c->event_pool.push_back(p.b);
+  }
+  c->pending.clear();
+}
+
+// Jacobi sweeps over the tasks of one stage until a sweep rotates nothing.
+int jacobi_sweeps(tebd_ctx* c, BondTask* d_t, const BondTask* h_t, int nb, int Pmax) {
+  int sweeps = 0;
+  for (; sweeps < kMaxSweeps; ++sweeps) {
+    HIPCHK(c, launch_reset_counters(d_t, nb, c->st));
+    for (int r = 0; r < 2 * Pmax - 1; ++r) {
+      ProfScope ps(c, 1);
+      HIPCHK(c, launch_jacobi_round(d_t, nb, Pmax, r, kTolRel, kTolAbs, c->st));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(int) * nb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    int rot = 0;
+    for (int i = 0; i < nb; ++i) rot += c->h_counters[i];
+    if (c->profiling)
+      for (int i = 0; i < nb; ++i) {
+        const BondTask& t = h_t[i];
+        const double jc2 = (double)JC * JC;
+        c->prof[5] += (double)t.P * (2 * t.P - 1) * 8.0 * t.R * jc2 +
+                      c->h_counters[i] * 8.0 * (t.R + (t.V ? t.NV : 0)) * jc2;
+      }
+    if (rot == 0) { ++sweeps; c->stats[1] += sweeps; return TEBD_OK; }
+  }
+  c->stats[1] += sweeps;
+  return fail(c, TEBD_ERR_NOCONV, "one-sided Jacobi did not converge");
+}
+
 // Process `nb` independent bonds (0-based links) as one batch of tasks.
-int run_bonds(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
+int run_bonds_full(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
   const int d = c->d;
   int max_chil = 1, max_chir = 1, Pmax = 1, Cpmax = 1;
   for (int i = 0; i < nb; ++i) {
@@ -98,8 +190,9 @@ int run_bonds(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs,
     t.V = c->V + (size_t)i * c->Cpmax * c->Cpmax;
     t.s = c->s + (size_t)i * c->Cpmax;
     t.perm = c->perm + (size_t)i * c->Cpmax;
+    t.NV = t.Cpad;
     t.counter = c->counters + i;
-    t.result = c->results + (size_t)4 * i;
+    t.result = c->results + (size_t)RESULT_STRIDE * i;
     t.gl_out = const_cast<zc*>(t.gl);
     t.gr_out = const_cast<zc*>(t.gr);
     t.lc_out = const_cast<double*>(t.lc);
@@ -110,30 +203,35 @@ int run_bonds(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs,
     Cpmax = std::max(Cpmax, t.Cpad);
   }
   HIPCHK(c, hipMemcpyAsync(c->d_tasks, c->h_tasks, sizeof(BondTask) * nb, hipMemcpyHostToDevice, c->st));
-  HIPCHK(c, launch_theta_gate(c->d_tasks, nb, d, max_chil, max_chir, c->st));
-  HIPCHK(c, launch_prepare_svd(c->d_tasks, nb, c->st));
-  HIPCHK(c, launch_frobenius(c->d_tasks, nb, Cpmax, c->st));
-  int sweeps = 0;
-  bool converged = false;
-  for (; sweeps < kMaxSweeps; ++sweeps) {
-    if (sweeps > 0) HIPCHK(c, launch_reset_counters(c->d_tasks, nb, c->st));
-    for (int r = 0; r < 2 * Pmax - 1; ++r)
-      HIPCHK(c, launch_jacobi_round(c->d_tasks, nb, Pmax, r, kTolRel, kTolAbs, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(int) * nb, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    int rot = 0;
-    for (int i = 0; i < nb; ++i) rot += c->h_counters[i];
-    if (rot == 0) { converged = true; ++sweeps; break; }
+  {
+    ProfScope ps(c, 0);
+    HIPCHK(c, launch_theta_gate(c->d_tasks, nb, d, max_chil, max_chir, c->st));
   }
-  c->stats[1] += sweeps;
-  if (!converged) return fail(c, TEBD_ERR_NOCONV, "one-sided Jacobi did not converge");
-  HIPCHK(c, launch_finalize(c->d_tasks, nb, Cpmax, c->cutoff, c->st));
-  HIPCHK(c, launch_write_gammas(c->d_tasks, nb, c->st));
-  HIPCHK(c, hipMemcpyAsync(c->h_results, c->results, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, c->st));
+  if (c->profiling) {
+    // theta: 8 d^2 chi_l chi_c chi_r, gate: 8 chi_l chi_r d^4 (complex MAC = 8 flops)
+    for (int i = 0; i < nb; ++i) {
+      const BondTask& t = c->h_tasks[i];
+      c->prof[2] += 8.0 * d * d * t.chil * (double)t.chic * t.chir + 8.0 * t.chil * (double)t.chir * d * d * d * d;
+    }
+  }
+  HIPCHK(c, launch_prepare_svd(c->d_tasks, nb, c->st));
+  HIPCHK(c, launch_frobenius(c->d_tasks, nb, Cpmax, c->st, 0));
+  {
+    // per visited pair: Gram 8 R JC^2; per rotated pair: W and V updates 8 (R + NV) JC^2
+    int e = jacobi_sweeps(c, c->d_tasks, c->h_tasks, nb, Pmax);
+    if (e) return e;
+  }
+  {
+    ProfScope ps(c, 2);
+    HIPCHK(c, launch_finalize(c->d_tasks, nb, Cpmax, c->cutoff, c->st));
+    HIPCHK(c, launch_write_gammas(c->d_tasks, nb, c->st));
+  }
+  HIPCHK(c, hipMemcpyAsync(c->h_results, c->results, sizeof(double) * RESULT_STRIDE * nb, hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
+  drain_profile(c);
   double tsum = 0.0;
   for (int i = 0; i < nb; ++i) {
-    const double* r = c->h_results + 4 * i;
+    const double* r = c->h_results + RESULT_STRIDE * i;
     const int k = (int)r[0];
     if (!(std::isfinite(r[2])) || r[2] <= 0.0 || k < 1) {
       char buf[160];
@@ -144,8 +242,213 @@ int run_bonds(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs,
     tsum += r[1];
   }
   c->stats[0] += nb;
+  for (int i = 0; i < nb; ++i) c->last_rank[links[i] + 1] = c->chi[links[i] + 1];
   *trunc_sum = tsum;
   return TEBD_OK;
+}
+
+// Sketch width for one bond: the predicted kept rank plus oversampling.
+int predicted_rank(tebd_ctx* c, int link) {
+  const int chic = c->chi[link + 1];
+  const int grow = std::max(4, chic / 8);
+  return std::min(c->chi_max, chic + grow);
+}
+
+// Learned-rank path (one bond = one task, all tasks of a batch per launch):
+//   Z0 = orth([ (Gamma_r lambda_r)^H | Gaussian ])          n x kpad
+//   R  = (I - Z0 Z0^H)^2 theta^H theta Z0, orthonormalised     n x kpad  (CGS2)
+//   Rayleigh-Ritz: one-sided Jacobi on theta [Z0 R] with V = [Z0 R].
+// The sketch is accepted when the weight of theta outside it is below
+// tol_missed ||theta||^2 and the discarded weight below tol_trunc; other bonds
+// are redone by the full SVD before anything is written.
+int orthonormalize(tebd_ctx* c, BondTask* D, const BondTask* H, int ns, int cmax, int P, int passes, int slot) {
+  for (int pass = 0; pass < passes; ++pass) {
+    HIPCHK(c, launch_col_norms(D, ns, cmax, c->st));
+    HIPCHK(c, launch_normalize_cols(D, ns, c->st));
+    HIPCHK(c, launch_frobenius(D, ns, cmax, c->st, 0));
+    const double before = c->stats[1];
+    int e = jacobi_sweeps(c, D, H, ns, P);
+    if (e) return e;
+    c->stats[slot] += c->stats[1] - before;
+  }
+  HIPCHK(c, launch_col_norms(D, ns, cmax, c->st));
+  HIPCHK(c, launch_normalize_cols(D, ns, c->st));
+  return TEBD_OK;
+}
+
+int run_bonds_sketch(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
+  const int d = c->d;
+  std::vector<int> sk, sk_links, full_links;
+  std::vector<const zc*> full_gates;
+  std::vector<int> kp;
+  for (int i = 0; i < nb; ++i) {
+    const int link = links[i];
+    const int m = d * c->chi[link], n = d * c->chi[link + 2];
+    const int k1 = std::min(n, predicted_rank(c, link) + c->oversample);
+    const int kpad = round_up(k1, JC);
+    if (2 * kpad >= std::min(m, n)) {
+      full_links.push_back(link);
+      full_gates.push_back(gate_ptrs[i]);
+    } else {
+      sk.push_back(i);
+      sk_links.push_back(link);
+      kp.push_back(k1);
+    }
+  }
+  double tsum = 0.0;
+  const int ns = (int)sk.size();
+  if (ns > 0) {
+    const int nt = c->ntask_max;
+    BondTask *S0 = c->h_tasks, *S1 = S0 + nt, *SA = S0 + 2 * nt, *SB = S0 + 3 * nt, *S3 = S0 + 4 * nt;
+    BondTask *D0 = c->d_tasks, *D1 = D0 + nt, *DA = D0 + 2 * nt, *DB = D0 + 3 * nt, *D3 = D0 + 4 * nt;
+    GemmTask* HG = c->h_gemm;
+    GemmTask* DG = c->d_gemm;
+    int chil_max = 1, chir_max = 1, Pk = 1, P2 = 1, c0max = 1, kmax = 1;
+    int gM[5] = {1, 1, 1, 1, 1}, gN[5] = {1, 1, 1, 1, 1};
+    for (int q = 0; q < ns; ++q) {
+      const int i = sk[q], link = sk_links[q];
+      const int chil = c->chi[link], chic = c->chi[link + 1], chir = c->chi[link + 2];
+      const int m = d * chil, n = d * chir;
+      const int kpad = round_up(kp[q], JC);
+      zc* Wt = c->W + (size_t)q * c->Rmax * c->Cpmax;
+      zc* Zt = c->Zb + (size_t)q * c->Rmax * c->Cpmax;
+      zc* Bt = c->Bb + (size_t)q * c->Rmax * c->Cpmax;
+      zc* Rt = Zt + (size_t)n * kpad;
+      zc* Mt = Bt + (size_t)m * kpad;
+      BondTask base;
+      std::memset(&base, 0, sizeof(base));
+      base.gl = c->gam + (size_t)link * c->gslab();
+      base.gr = c->gam + (size_t)(link + 1) * c->gslab();
+      base.ll = c->lam + (size_t)link * c->chi_max;
+      base.lc = c->lam + (size_t)(link + 1) * c->chi_max;
+      base.lr = c->lam + (size_t)(link + 2) * c->chi_max;
+      base.gate = gate_ptrs[i];
+      base.chil = chil; base.chic = chic; base.chir = chir; base.d = d;
+      base.s = c->s + (size_t)q * c->Cpmax;
+      base.perm = c->perm + (size_t)q * c->Cpmax;
+      base.counter = c->counters + q;
+      base.result = c->results + (size_t)RESULT_STRIDE * q;
+      base.chi_max = c->chi_max;
+      base.drop_rel = 1e-13;
+      S0[q] = base;  // theta (m x n) -> W
+      S0[q].W = Wt; S0[q].R = m; S0[q].C = n; S0[q].Cpad = round_up(n, JC);
+      S1[q] = base;  // Z0 (n x kpad)
+      S1[q].V = Zt; S1[q].NV = n; S1[q].Cpad = kpad;
+      SA[q] = base;  // orthonormalise Z0
+      SA[q].W = Zt; SA[q].R = n; SA[q].C = kpad; SA[q].Cpad = kpad; SA[q].P = kpad / JC;
+      SB[q] = base;  // orthonormalise R
+      SB[q].W = Rt; SB[q].R = n; SB[q].C = kpad; SB[q].Cpad = kpad; SB[q].P = kpad / JC;
+      S3[q] = base;  // Rayleigh-Ritz on theta [Z0 R]
+      S3[q].W = Bt; S3[q].R = m; S3[q].C = 2 * kpad; S3[q].Cpad = 2 * kpad; S3[q].P = 2 * kpad / JC;
+      S3[q].V = Zt; S3[q].NV = n;
+      S3[q].gl_out = const_cast<zc*>(base.gl);
+      S3[q].gr_out = const_cast<zc*>(base.gr);
+      S3[q].lc_out = c->lam_scratch + (size_t)q * c->chi_max;
+      c->h_kprime[q] = kp[q];
+      // GEMM slots: 0 B = theta Z0, 1 R = theta^H B, 2 M = Z0^H R, 3 R -= Z0 M, 4 W = theta [Z0 R]
+      HG[0 * nt + q] = {Wt, Zt, Bt, m, kpad, n, m, n, m, 1.0, 0};
+      HG[1 * nt + q] = {Wt, Bt, Rt, n, kpad, m, m, m, n, 1.0, 0};
+      HG[2 * nt + q] = {Zt, Rt, Mt, kpad, kpad, n, n, n, kpad, 1.0, 0};
+      HG[3 * nt + q] = {Zt, Mt, Rt, n, kpad, kpad, n, kpad, n, -1.0, 1};
+      HG[4 * nt + q] = {Wt, Zt, Bt, m, 2 * kpad, n, m, n, m, 1.0, 0};
+      const int Ms[5] = {m, n, kpad, n, m}, Ns[5] = {kpad, kpad, kpad, kpad, 2 * kpad};
+      for (int g = 0; g < 5; ++g) { gM[g] = std::max(gM[g], Ms[g]); gN[g] = std::max(gN[g], Ns[g]); }
+      chil_max = std::max(chil_max, chil); chir_max = std::max(chir_max, chir);
+      Pk = std::max(Pk, kpad / JC); P2 = std::max(P2, 2 * kpad / JC);
+      c0max = std::max(c0max, S0[q].Cpad); kmax = std::max(kmax, kpad);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_tasks, c->h_tasks, sizeof(BondTask) * nt * kStages, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(DG, HG, sizeof(GemmTask) * nt * 5, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d_kprime, c->h_kprime, sizeof(int) * ns, hipMemcpyHostToDevice, c->st));
+    auto gemm = [&](int slot, bool conj) {
+      return launch_zgemm(DG + slot * nt, ns, gM[slot], gN[slot], conj, c->st);
+    };
+    {
+      ProfScope ps(c, 0);
+      HIPCHK(c, launch_theta_gate(D0, ns, d, chil_max, chir_max, c->st));
+    }
+    HIPCHK(c, launch_frobenius(D0, ns, c0max, c->st, 1));  // result[3] = ||theta||^2
+    HIPCHK(c, launch_build_sketch(D1, c->d_kprime, ns, c->sketch_seed ^ (unsigned long long)c->step_index, c->st));
+    int e = orthonormalize(c, DA, SA, ns, kmax, Pk, 2, 4);
+    if (e) return e;
+    HIPCHK(c, gemm(0, false));
+    HIPCHK(c, gemm(1, true));
+    for (int r = 0; r < 2; ++r) {
+      HIPCHK(c, gemm(2, true));
+      HIPCHK(c, gemm(3, false));
+    }
+    e = orthonormalize(c, DB, SB, ns, kmax, Pk, 2, 5);
+    if (e) return e;
+    HIPCHK(c, gemm(2, true));
+    HIPCHK(c, gemm(3, false));
+    e = orthonormalize(c, DB, SB, ns, kmax, Pk, 1, 6);
+    if (e) return e;
+    HIPCHK(c, gemm(4, false));
+    HIPCHK(c, launch_frobenius(D3, ns, 2 * kmax, c->st, 0));
+    {
+      const double before = c->stats[1];
+      e = jacobi_sweeps(c, D3, S3, ns, P2);
+      if (e) return e;
+      c->stats[7] += c->stats[1] - before;
+    }
+    {
+      ProfScope ps(c, 2);
+      HIPCHK(c, launch_col_norms(D3, ns, 2 * kmax, c->st));
+      HIPCHK(c, launch_truncate(D3, ns, c->cutoff_sketch > c->cutoff ? c->cutoff_sketch : c->cutoff, c->st));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->h_results, c->results, sizeof(double) * RESULT_STRIDE * ns, hipMemcpyDeviceToHost,
+                             c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    drain_profile(c);
+    int na = 0;
+    for (int q = 0; q < ns; ++q) {
+      const double* r = c->h_results + RESULT_STRIDE * q;
+      const int k = (int)r[0];
+      const double total = r[3], captured = r[4];
+      const bool finite = std::isfinite(total) && total > 0.0 && std::isfinite(captured);
+      if (!finite) {
+        char buf[160];
+        std::snprintf(buf, sizeof(buf), "bond %d: theta is zero or not finite", sk_links[q] + 1);
+        return fail(c, TEBD_ERR_NAN, buf);
+      }
+      const bool ok = k >= 1 && (total - captured) <= c->tol_missed * total && r[1] <= c->tol_trunc;
+      if (ok) {
+        S3[na] = S3[q];
+        S3[na].lc_commit = const_cast<double*>(S3[q].lc);
+        c->chi[sk_links[q] + 1] = k;
+        c->last_rank[sk_links[q] + 1] = k;
+        tsum += r[1];
+        ++na;
+      } else {
+        full_links.push_back(sk_links[q]);
+        full_gates.push_back(gate_ptrs[sk[q]]);
+        c->stats[2] += 1;
+      }
+    }
+    if (na > 0) {
+      HIPCHK(c, hipMemcpyAsync(D3, S3, sizeof(BondTask) * na, hipMemcpyHostToDevice, c->st));
+      {
+        ProfScope ps(c, 2);
+        HIPCHK(c, launch_write_gammas(D3, na, c->st));
+      }
+      HIPCHK(c, hipStreamSynchronize(c->st));
+      drain_profile(c);
+      c->stats[0] += na;
+    }
+  }
+  if (!full_links.empty()) {
+    double t = 0.0;
+    int e = run_bonds_full(c, full_links.data(), (int)full_links.size(), full_gates.data(), &t);
+    if (e) return e;
+    tsum += t;
+  }
+  *trunc_sum = tsum;
+  return TEBD_OK;
+}
+
+int run_bonds(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
+  if (c->mode & TEBD_MODE_LEARNED_RANK) return run_bonds_sketch(c, links, nb, gate_ptrs, trunc_sum);
+  return run_bonds_full(c, links, nb, gate_ptrs, trunc_sum);
 }
 
 int ensure_state(tebd_ctx* c, bool need_gates) {
@@ -215,22 +518,34 @@ int download_impl(tebd_ctx* c, double* const* gamma, double* const* lambda, int 
 }
 
 void free_all(tebd_ctx* c) {
-  hipFree(c->gam);
-  hipFree(c->lam);
-  hipFree(c->gates);
-  hipFree(c->W);
-  hipFree(c->V);
-  hipFree(c->s);
-  hipFree(c->perm);
-  hipFree(c->counters);
-  hipFree(c->results);
-  hipFree(c->d_tasks);
-  hipHostFree(c->h_tasks);
-  hipHostFree(c->h_results);
-  hipHostFree(c->h_counters);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
-  if (c->st) hipStreamDestroy(c->st);
+  (void)hipFree(c->gam);
+  (void)hipFree(c->lam);
+  (void)hipFree(c->gates);
+  (void)hipFree(c->W);
+  (void)hipFree(c->V);
+  (void)hipFree(c->s);
+  (void)hipFree(c->perm);
+  (void)hipFree(c->counters);
+  (void)hipFree(c->results);
+  (void)hipFree(c->d_tasks);
+  (void)hipFree(c->Zb);
+  (void)hipFree(c->Bb);
+  (void)hipFree(c->lam_scratch);
+  (void)hipFree(c->d_gemm);
+  (void)hipHostFree(c->h_gemm);
+  (void)hipFree(c->d_kprime);
+  (void)hipHostFree(c->h_kprime);
+  (void)hipHostFree(c->h_tasks);
+  (void)hipHostFree(c->h_results);
+  (void)hipHostFree(c->h_counters);
+  for (auto& p : c->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->st) (void)hipStreamDestroy(c->st);
 }
 
 }  // namespace
@@ -272,10 +587,20 @@ int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, u
   if (bad(hipMalloc(&c->s, sizeof(double) * (size_t)c->Cpmax * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->perm, sizeof(int) * (size_t)c->Cpmax * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->counters, sizeof(int) * nt))) return TEBD_ERR_HIP;
-  if (bad(hipMalloc(&c->results, sizeof(double) * 4 * nt))) return TEBD_ERR_HIP;
-  if (bad(hipMalloc(&c->d_tasks, sizeof(BondTask) * nt))) return TEBD_ERR_HIP;
-  if (bad(hipHostMalloc(&c->h_tasks, sizeof(BondTask) * nt))) return TEBD_ERR_HIP;
-  if (bad(hipHostMalloc(&c->h_results, sizeof(double) * 4 * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->results, sizeof(double) * RESULT_STRIDE * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->d_tasks, sizeof(BondTask) * nt * kStages))) return TEBD_ERR_HIP;
+  if (bad(hipHostMalloc(&c->h_tasks, sizeof(BondTask) * nt * kStages))) return TEBD_ERR_HIP;
+  if (mode & TEBD_MODE_LEARNED_RANK) {
+    if (bad(hipMalloc(&c->Zb, sizeof(zc) * (size_t)c->Rmax * c->Cpmax * nt))) return TEBD_ERR_HIP;
+    if (bad(hipMalloc(&c->Bb, sizeof(zc) * (size_t)c->Rmax * c->Cpmax * nt))) return TEBD_ERR_HIP;
+    if (bad(hipMalloc(&c->lam_scratch, sizeof(double) * (size_t)chi_max * nt))) return TEBD_ERR_HIP;
+    if (bad(hipMalloc(&c->d_gemm, sizeof(GemmTask) * nt * 5))) return TEBD_ERR_HIP;
+    if (bad(hipHostMalloc(&c->h_gemm, sizeof(GemmTask) * nt * 5))) return TEBD_ERR_HIP;
+    if (bad(hipMalloc(&c->d_kprime, sizeof(int) * nt))) return TEBD_ERR_HIP;
+    if (bad(hipHostMalloc(&c->h_kprime, sizeof(int) * nt))) return TEBD_ERR_HIP;
+  }
+  c->last_rank.assign(L + 1, 0);
+  if (bad(hipHostMalloc(&c->h_results, sizeof(double) * RESULT_STRIDE * nt))) return TEBD_ERR_HIP;
   if (bad(hipHostMalloc(&c->h_counters, sizeof(int) * nt))) return TEBD_ERR_HIP;
   *out = c;
   return TEBD_OK;
@@ -283,7 +608,7 @@ int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, u
 
 void tebd_destroy(tebd_ctx* c) {
   if (!c) return;
-  if (c->st) hipStreamSynchronize(c->st);
+  if (c->st) (void)hipStreamSynchronize(c->st);
   free_all(c);
   delete c;
 }
@@ -329,6 +654,7 @@ int tebd_trotter_step(tebd_ctx* c, double* truncerr) {
   const int starts[3] = {0, 1, 0};  // reference bonds 1,3,5.. / 2,4,.. / 1,3,5..
   std::vector<int> links;
   std::vector<const zc*> gp;
+  c->step_index++;
   for (int half = 0; half < 3; ++half) {
     links.clear();
     gp.clear();
@@ -395,7 +721,24 @@ int tebd_get_bond_dims(tebd_ctx* c, int* chi) {
 
 int tebd_get_stats(tebd_ctx* c, double* stats, int n) {
   if (!c || !stats || n < 0) return TEBD_ERR_ARG;
-  for (int i = 0; i < n && i < 4; ++i) stats[i] = c->stats[i];
+  for (int i = 0; i < n && i < 8; ++i) stats[i] = c->stats[i];
+  return TEBD_OK;
+}
+
+int tebd_set_profiling(tebd_ctx* c, int on) {
+  if (!c) return TEBD_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  drain_profile(c);
+  c->profiling = on != 0;
+  for (double& v : c->prof) v = 0.0;
+  return TEBD_OK;
+}
+
+int tebd_get_profile(tebd_ctx* c, double* out, int n) {
+  if (!c || !out || n < 0) return TEBD_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  drain_profile(c);
+  for (int i = 0; i < n && i < 9; ++i) out[i] = c->prof[i];
   return TEBD_OK;
 }
 

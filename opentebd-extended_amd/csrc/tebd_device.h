@@ -18,6 +18,19 @@ struct __align__(16) zc {
   double x, y;
 };
 
+// Batched complex GEMM: C (M x N) = op(A) (M x K) * B (K x N); op(A) = A (stored
+// M x K, lda) or A^dagger (A stored K x M, lda).  All column-major.
+struct GemmTask {
+  const zc* A;
+  const zc* B;
+  zc* C;
+  int M, N, K, lda, ldb, ldc;
+  double alpha;    // C = alpha op(A) B (+ C when accumulate)
+  int accumulate;
+};
+
+constexpr int RESULT_STRIDE = 8;
+
 __device__ __forceinline__ zc zmk(double x, double y) { zc r; r.x = x; r.y = y; return r; }
 __device__ __forceinline__ zc zadd(zc a, zc b) { return zmk(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ zc zsub(zc a, zc b) { return zmk(a.x - b.x, a.y - b.y); }
@@ -59,14 +72,19 @@ struct BondTask {
   int transposed;    // 0: W = theta (m x n); 1: W = theta^dagger (n x m)
   int R, C, Cpad, P; // rows, true columns, padded columns, Jacobi pairs (Cpad / JC)
   zc* W;             // R x Cpad, ld = R
-  zc* V;             // Cpad x Cpad, ld = Cpad
+  zc* V;             // NV x Cpad, ld = NV: accumulated right rotations (identity start) or a
+                     // right basis to rotate along with W; nullptr = not needed
+  int NV;
   double* s;         // Cpad: sorted singular values (descending)
   int* perm;         // Cpad: original column of each sorted value
   int* counter;      // rotations in the current sweep
-  double* result;    // [0]=kept k (as double), [1]=truncerr, [2]=||theta||^2, [3]=sweeps
+  double* result;    // [0]=kept k, [1]=truncerr, [2]=max column |w|^2 (Jacobi floor), [3]=||theta||^2 when
+                     // W is a sketch (else <= 0), [4]=weight captured by W, [5]=smallest kept s^2
   // outputs
   zc* gl_out;        // new Gamma_l (chi_l x d x k)
   zc* gr_out;        // new Gamma_r (k x d x chi_r)
   double* lc_out;    // new lambda_c (chi_max entries, zero-filled past k)
+  double* lc_commit; // when set, k_write_gammas copies lc_out here (sketched path)
+  double drop_rel;   // k_normalize_cols zeroes columns with |w| <= drop_rel * max |w|
   int chi_max;
 };

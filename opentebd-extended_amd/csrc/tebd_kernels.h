@@ -5,9 +5,15 @@
 
 hipError_t launch_theta_gate(const BondTask* d_tasks, int ntasks, int d, int max_chil, int max_chir, hipStream_t st);
 hipError_t launch_prepare_svd(const BondTask* d_tasks, int ntasks, hipStream_t st);
-hipError_t launch_frobenius(const BondTask* d_tasks, int ntasks, int max_cpad, hipStream_t st);
+hipError_t launch_frobenius(const BondTask* d_tasks, int ntasks, int max_cpad, hipStream_t st, int keep_sum);
 hipError_t launch_reset_counters(const BondTask* d_tasks, int ntasks, hipStream_t st);
 hipError_t launch_jacobi_round(const BondTask* d_tasks, int ntasks, int max_pairs, int round, double tol_rel,
                                double tol_abs, hipStream_t st);
 hipError_t launch_finalize(const BondTask* d_tasks, int ntasks, int max_cpad, double cutoff, hipStream_t st);
 hipError_t launch_write_gammas(const BondTask* d_tasks, int ntasks, hipStream_t st);
+hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_n, bool conj_a, hipStream_t st);
+hipError_t launch_build_sketch(const BondTask* d_tasks, const int* d_kprime, int ntasks, unsigned long long seed,
+                               hipStream_t st);
+hipError_t launch_normalize_cols(const BondTask* d_tasks, int ntasks, hipStream_t st);
+hipError_t launch_col_norms(const BondTask* d_tasks, int ntasks, int max_cpad, hipStream_t st);
+hipError_t launch_truncate(const BondTask* d_tasks, int ntasks, double cutoff, hipStream_t st);

@@ -141,7 +141,10 @@ __global__ void k_prepare_svd(const BondTask* __restrict__ tasks) {
       t.W[(size_t)t.R * t.C + (e - nV)] = zmk(0.0, 0.0);
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) *t.counter = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *t.counter = 0;
+    t.result[3] = -1.0;
+  }
 }
 
 __global__ void k_reset_counters(const BondTask* __restrict__ tasks, int ntasks) {
@@ -176,7 +179,7 @@ constexpr int RC = 64;  // rows of W / V streamed through LDS per chunk
 
 // A pair is left alone when every off-diagonal Gram entry satisfies
 // |A_pq| <= tol_rel sqrt(A_pp A_qq)  or
-// |A_pq| <= tol_abs ||theta|| (sqrt(A_pp) + sqrt(A_qq)).
+// |A_pq| <= tol_abs max_c |w_c| (sqrt(A_pp) + sqrt(A_qq)).
 // The second bound is the rounding floor: rotations add errors of order
 // eps * sigma_max to every column they touch, so inner products below it are
 // noise.  It scales with the two column norms, which keeps the small singular
@@ -228,7 +231,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict
 
   // ---- does this pair need a rotation?  |A_pq| > tol_rel sqrt(A_pp A_qq) and > tol_abs
   {
-    const double fnorm = sqrt(t.result[2]);
+    const double fnorm = sqrt(t.result[2]);  // largest column norm
     int need = 0;
     for (int e = tid; e < JC * JC; e += 256) {
       const int pp = e % JC, qq = e / JC;
@@ -342,7 +345,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict
     }
   };
   rotate(t.W, t.R, t.R);
-  rotate(t.V, t.Cpad, t.Cpad);
+  if (t.V) rotate(t.V, t.NV, t.NV);
 }
 
 // ---------------------------------------------------------------------------
@@ -361,20 +364,29 @@ __global__ __launch_bounds__(256) void k_col_norms(const BondTask* __restrict__ 
   if (lane == 0) t.s[c] = acc;
 }
 
-// ||theta||^2 = sum of the squared column norms (one workgroup per task, fixed
-// summation order so repeated runs are bitwise identical).
-__global__ __launch_bounds__(256) void k_frob(const BondTask* __restrict__ tasks) {
+// Column-norm reductions over s (one workgroup per task, fixed order so repeated
+// runs are bitwise identical): result[2] = max_c |w_c|^2 (the Jacobi rounding
+// floor's scale), and with keep_sum also result[3] = sum_c |w_c|^2 = ||W||_F^2.
+__global__ __launch_bounds__(256) void k_frob(const BondTask* __restrict__ tasks, int keep_sum) {
   const BondTask t = tasks[blockIdx.x];
   __shared__ double red[256];
-  double acc = 0.0;
-  for (int c = threadIdx.x; c < t.C; c += 256) acc += t.s[c];
+  __shared__ double rmx[256];
+  double acc = 0.0, mx = 0.0;
+  for (int c = threadIdx.x; c < t.C; c += 256) { acc += t.s[c]; mx = fmax(mx, t.s[c]); }
   red[threadIdx.x] = acc;
+  rmx[threadIdx.x] = mx;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    if (threadIdx.x < st) {
+      red[threadIdx.x] += red[threadIdx.x + st];
+      rmx[threadIdx.x] = fmax(rmx[threadIdx.x], rmx[threadIdx.x + st]);
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) t.result[2] = red[0];
+  if (threadIdx.x == 0) {
+    t.result[2] = rmx[0];
+    if (keep_sum) t.result[3] = red[0];
+  }
 }
 
 constexpr int SORT_N = 4096;
@@ -413,13 +425,14 @@ __global__ __launch_bounds__(1024) void k_truncate(const BondTask* __restrict__ 
       __syncthreads();
     }
   }
-  // total weight
+  // weight held by W; when W is a sketch of theta the reference total is ||theta||^2
   double part = 0.0;
   for (int e = tid; e < t.C; e += 1024) part += key[e];
   red[tid] = part;
   __syncthreads();
   for (int st = 512; st > 0; st >>= 1) { if (tid < st) red[tid] += red[tid + st]; __syncthreads(); }
-  const double total = red[0];
+  const double captured = red[0];
+  const double total = t.result[3] > 0.0 ? t.result[3] : captured;
   __syncthreads();
   if (tid == 0) s_k = 0;
   __syncthreads();
@@ -447,6 +460,8 @@ __global__ __launch_bounds__(1024) void k_truncate(const BondTask* __restrict__ 
     t.result[0] = (double)k;
     t.result[1] = total > 0.0 ? fabs(1.0 - kept / total) : 1.0;
     t.result[2] = total;
+    t.result[4] = captured;
+    t.result[5] = k > 0 ? key[k - 1] : 0.0;
   }
 }
 
@@ -455,6 +470,8 @@ __global__ void k_write_gammas(const BondTask* __restrict__ tasks) {
   const BondTask t = tasks[blockIdx.y];
   const int k = (int)t.result[0];
   const int dd = t.d;
+  if (t.lc_commit && blockIdx.x == 0)
+    for (int e = threadIdx.x; e < t.chi_max; e += blockDim.x) t.lc_commit[e] = t.lc_out[e];
   const size_t nl = (size_t)t.chil * dd * k;
   const size_t nr = (size_t)k * dd * t.chir;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nl + nr; e += (size_t)gridDim.x * blockDim.x) {
@@ -467,7 +484,7 @@ __global__ void k_write_gammas(const BondTask* __restrict__ tasks) {
       const int col = t.perm[b];
       zc u;
       if (!t.transposed) u = zscale(t.W[row + (size_t)t.R * col], t.s[b] > 0.0 ? 1.0 / t.s[b] : 0.0);
-      else u = t.V[row + (size_t)t.Cpad * col];
+      else u = t.V[row + (size_t)t.NV * col];
       const double la = t.ll[a];
       t.gl_out[e] = fabs(la) > 1e-15 ? zscale(u, 1.0 / la) : zmk(0.0, 0.0);
     } else {
@@ -479,7 +496,7 @@ __global__ void k_write_gammas(const BondTask* __restrict__ tasks) {
       const size_t row = (size_t)j * t.chir + b;
       const int col = t.perm[g];
       zc v;
-      if (!t.transposed) v = t.V[row + (size_t)t.Cpad * col];
+      if (!t.transposed) v = t.V[row + (size_t)t.NV * col];
       else v = zscale(t.W[row + (size_t)t.R * col], t.s[g] > 0.0 ? 1.0 / t.s[g] : 0.0);
       const double lb = t.lr[b];
       t.gr_out[f] = fabs(lb) > 1e-15 ? zscale(zconj(v), 1.0 / lb) : zmk(0.0, 0.0);
@@ -520,11 +537,11 @@ hipError_t launch_reset_counters(const BondTask* d_tasks, int ntasks, hipStream_
   return hipGetLastError();
 }
 
-hipError_t launch_frobenius(const BondTask* d_tasks, int ntasks, int max_cpad, hipStream_t st) {
+hipError_t launch_frobenius(const BondTask* d_tasks, int ntasks, int max_cpad, hipStream_t st, int keep_sum) {
   hipLaunchKernelGGL(k_col_norms, dim3((max_cpad + 3) / 4, ntasks), dim3(256), 0, st, d_tasks);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_frob, dim3(ntasks), dim3(256), 0, st, d_tasks);
+  hipLaunchKernelGGL(k_frob, dim3(ntasks), dim3(256), 0, st, d_tasks, keep_sum);
   return hipGetLastError();
 }
 
@@ -538,6 +555,16 @@ hipError_t launch_finalize(const BondTask* d_tasks, int ntasks, int max_cpad, do
   hipLaunchKernelGGL(k_col_norms, dim3((max_cpad + 3) / 4, ntasks), dim3(256), 0, st, d_tasks);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_truncate, dim3(ntasks), dim3(1024), 0, st, d_tasks, cutoff);
+  return hipGetLastError();
+}
+
+hipError_t launch_col_norms(const BondTask* d_tasks, int ntasks, int max_cpad, hipStream_t st) {
+  hipLaunchKernelGGL(k_col_norms, dim3((max_cpad + 3) / 4, ntasks), dim3(256), 0, st, d_tasks);
+  return hipGetLastError();
+}
+
+hipError_t launch_truncate(const BondTask* d_tasks, int ntasks, double cutoff, hipStream_t st) {
   hipLaunchKernelGGL(k_truncate, dim3(ntasks), dim3(1024), 0, st, d_tasks, cutoff);
   return hipGetLastError();
 }

@@ -1,0 +1,158 @@
+// gfx950 kernels of the sketched (randomised) SVD used by the learned-rank path.
+//
+// The reference always takes the full SVD of theta (ZGESVD,
+// local_operations_module.f90:322-345) and keeps chi values.  The sketched path
+// computes only the leading singular triplets from a right basis [Z0 R]:
+//   Z0 = [ (Gamma_r lambda_r)^dagger | Gaussian ]   the right Schmidt vectors of the
+//        pre-gate theta plus random directions for rank growth
+//   R  = the part of theta^dagger theta Z0 outside Z0 (one block-Krylov step)
+// then one-sided Jacobi on theta [Z0 R] (Rayleigh-Ritz), handing U, S, V to the
+// same truncation / Gamma rebuild kernels as the full SVD.
+#include "tebd_device.h"
+#include "tebd_kernels.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// batched complex GEMM, 64 x 64 tiles, 4 x 4 complex register micro-tiles
+// ---------------------------------------------------------------------------
+constexpr int GT = 64;
+constexpr int GK = 16;
+
+template <bool CONJ_A>
+__global__ __launch_bounds__(256) void k_zgemm(const GemmTask* __restrict__ tasks) {
+  const GemmTask t = tasks[blockIdx.z];
+  const int m0 = blockIdx.x * GT, n0 = blockIdx.y * GT;
+  if (m0 >= t.M || n0 >= t.N) return;
+  __shared__ zc As[GK][GT + 1];
+  __shared__ zc Bs[GK][GT + 1];
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  zc acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = zmk(0.0, 0.0);
+  for (int k0 = 0; k0 < t.K; k0 += GK) {
+    for (int e = tid; e < GK * GT; e += 256) {
+      int kk, mm;
+      zc v = zmk(0.0, 0.0);
+      if (!CONJ_A) {  // A is M x K: contiguous along m
+        mm = e % GT; kk = e / GT;
+        const int m = m0 + mm, k = k0 + kk;
+        if (m < t.M && k < t.K) v = t.A[m + (size_t)t.lda * k];
+      } else {        // A is K x M: contiguous along k
+        kk = e % GK; mm = e / GK;
+        const int m = m0 + mm, k = k0 + kk;
+        if (m < t.M && k < t.K) v = zconj(t.A[k + (size_t)t.lda * m]);
+      }
+      As[kk][mm] = v;
+    }
+    for (int e = tid; e < GK * GT; e += 256) {
+      const int kk = e % GK, nn = e / GK;
+      const int k = k0 + kk, n = n0 + nn;
+      Bs[kk][nn] = (k < t.K && n < t.N) ? t.B[k + (size_t)t.ldb * n] : zmk(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < GK; ++kk) {
+      zc a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) zfma(acc[i][j], a[i], b[j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int m = m0 + ty + 16 * i, n = n0 + tx + 16 * j;
+      if (m < t.M && n < t.N) {
+        zc v = zscale(acc[i][j], t.alpha);
+        if (t.accumulate) v = zadd(v, t.C[m + (size_t)t.ldc * n]);
+        t.C[m + (size_t)t.ldc * n] = v;
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// sketch construction
+// ---------------------------------------------------------------------------
+// 64-bit mix (splitmix64 finaliser) -> uniform (0,1]
+__device__ __forceinline__ double u01(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return ((double)(x >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+}
+
+// Z0[(j, b), c] = conj(Gamma_r(c, j, b) lambda_r(b)) for c < chi_c, Gaussian for
+// chi_c <= c < k', zero for k' <= c < kpad.  Z0 has n = d chi_r rows (theta's
+// column index (j, b) -> j chi_r + b).
+__global__ void k_build_sketch(const BondTask* __restrict__ tasks, const int* __restrict__ kprime,
+                               unsigned long long seed) {
+  const BondTask t = tasks[blockIdx.y];
+  const int n = t.d * t.chir;
+  const int kp = kprime[blockIdx.y];
+  const size_t total = (size_t)n * t.Cpad;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int row = (int)(e % n), c = (int)(e / n);
+    const int j = row / t.chir, b = row % t.chir;
+    zc v = zmk(0.0, 0.0);
+    if (c < t.chic && c < kp) {
+      v = zconj(zscale(t.gr[c + (size_t)t.chic * (j + (size_t)t.d * b)], t.lr[b]));
+    } else if (c < kp) {
+      const unsigned long long key = seed ^ ((unsigned long long)blockIdx.y << 48) ^ ((unsigned long long)e << 1);
+      const double u1 = u01(key), u2 = u01(key + 0x632BE59BD9B4E019ull);
+      const double r = sqrt(-2.0 * log(u1));
+      v = zmk(r * cos(6.283185307179586 * u2), r * sin(6.283185307179586 * u2));
+    }
+    t.V[e] = v;
+  }
+}
+
+// Z[:, c] /= ||Z[:, c]|| (s holds squared norms); columns with norm at most
+// drop_rel times the largest are dropped (set to zero).
+__global__ void k_normalize_cols(const BondTask* __restrict__ tasks) {
+  const BondTask t = tasks[blockIdx.y];
+  __shared__ double smax;
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int c = 0; c < t.C; ++c) m = fmax(m, t.s[c]);
+    smax = m;
+  }
+  __syncthreads();
+  const size_t total = (size_t)t.R * t.Cpad;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e / t.R);
+    const double s2 = c < t.C ? t.s[c] : 0.0;
+    t.W[e] = (s2 > t.drop_rel * t.drop_rel * smax && s2 > 0.0) ? zscale(t.W[e], 1.0 / sqrt(s2)) : zmk(0.0, 0.0);
+  }
+}
+
+}  // namespace
+
+hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_n, bool conj_a, hipStream_t st) {
+  dim3 grid((max_m + GT - 1) / GT, (max_n + GT - 1) / GT, ntasks);
+  if (conj_a) hipLaunchKernelGGL(k_zgemm<true>, grid, dim3(256), 0, st, d_tasks);
+  else hipLaunchKernelGGL(k_zgemm<false>, grid, dim3(256), 0, st, d_tasks);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_sketch(const BondTask* d_tasks, const int* d_kprime, int ntasks, unsigned long long seed,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_build_sketch, dim3(256, ntasks), dim3(256), 0, st, d_tasks, d_kprime, seed);
+  return hipGetLastError();
+}
+
+hipError_t launch_normalize_cols(const BondTask* d_tasks, int ntasks, hipStream_t st) {
+  hipLaunchKernelGGL(k_normalize_cols, dim3(128, ntasks), dim3(256), 0, st, d_tasks);
+  return hipGetLastError();
+}
+

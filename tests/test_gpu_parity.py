@@ -40,7 +40,7 @@ def run_device(z, mode, check_every=10):
     return worst, dims
 
 
-@pytest.mark.parametrize("mode", ["fp64", "batched"])
+@pytest.mark.parametrize("mode", ["fp64", "batched", "learned", "batched,learned"])
 @pytest.mark.parametrize("name", RTP)
 def test_trajectory_matches_reference(name, mode):
     z = load_golden(name)
