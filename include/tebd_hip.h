@@ -73,6 +73,10 @@ int tebd_upload_packed(tebd_ctx* ctx, const double* gamma, const double* lambda,
  * current dimension); chi_out must be >= every bond dimension. */
 int tebd_download(tebd_ctx* ctx, double* const* gamma, double* const* lambda, int chi_out);
 int tebd_download_packed(tebd_ctx* ctx, double* gamma, double* lambda, int chi_out);
+/* Only the lambdas, (L+1) x chi_out reals, zero-filled: what ChainEntropy
+ * (observables_module.f90:1583-1602) and the imaginary-time convergence test
+ * (propagation_module.f90:704-738) read. */
+int tebd_download_lambdas(tebd_ctx* ctx, double* lambda, int chi_out);
 
 /* One 2nd-order OBC Trotter step: odd bonds (gate dt/2), even bonds (dt),
  * odd bonds (dt/2); *truncerr = sum of the bond truncation errors.
@@ -87,6 +91,10 @@ int tebd_two_site_op(tebd_ctx* ctx, int link, const double* op2, double* truncer
 /* Re-orthogonalises every bipartition (left-to-right, then right-to-left
  * sweep).  Replaces CanonicalFormAll (propagation_module.f90:617-634). */
 int tebd_canonical_form_all(tebd_ctx* ctx);
+
+/* Re-orthogonalises one bipartition, bond `link` (1-based).  Replaces
+ * CanonicalForm (local_operations_module.f90:496-620). */
+int tebd_canonical_form(tebd_ctx* ctx, int link);
 
 /* Singular-value cutoff on the normalised weight s^2/||theta||^2.  Default
  * 1e-30, the reference's rule (keep |s| > 1e-15 after normalisation,

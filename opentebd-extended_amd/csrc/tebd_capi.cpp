@@ -61,6 +61,11 @@ struct tebd_ctx {
   double tol_trunc = 1e-10;      // max discarded weight before the full SVD judges
   int oversample = 8;
   long long step_index = 0;
+  // CanonicalForm workspace (allocated on first use): 8 matrices, 3 (s, perm) pairs
+  zc* cf = nullptr;
+  double* cf_s = nullptr;
+  int* cf_perm = nullptr;
+  double* cf_lam = nullptr;
   std::vector<int> last_rank;    // kept rank of each bond's last update (rank model input)
   double* h_results = nullptr;   // pinned
   int* h_counters = nullptr;     // pinned
@@ -446,6 +451,96 @@ int run_bonds_sketch(tebd_ctx* c, const int* links, int nb, const zc* const* gat
   return TEBD_OK;
 }
 
+// SVD of W (R x C, ld R, R >= C) by one-sided Jacobi.  On return W holds W J
+// (columns U s, unsorted), V (Cpad x Cpad) holds J, s[0..C) the singular values
+// in descending order and perm[b] the column of W / V that carries s[b].
+int device_svd(tebd_ctx* c, zc* W, int R, int C, zc* V, double* s, int* perm, double* lam_scratch) {
+  BondTask& t = c->h_tasks[0];
+  std::memset(&t, 0, sizeof(t));
+  t.W = W; t.R = R; t.C = C; t.Cpad = round_up(C, JC); t.P = t.Cpad / JC;
+  t.V = V; t.NV = t.Cpad; t.s = s; t.perm = perm;
+  t.counter = c->counters; t.result = c->results; t.chi_max = C; t.lc_out = lam_scratch;
+  HIPCHK(c, hipMemcpyAsync(c->d_tasks, c->h_tasks, sizeof(BondTask), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, launch_prepare_svd(c->d_tasks, 1, c->st));
+  HIPCHK(c, launch_frobenius(c->d_tasks, 1, t.Cpad, c->st, 0));
+  int e = jacobi_sweeps(c, c->d_tasks, c->h_tasks, 1, t.P);
+  if (e) return e;
+  HIPCHK(c, launch_col_norms(c->d_tasks, 1, t.Cpad, c->st));
+  HIPCHK(c, launch_truncate(c->d_tasks, 1, 0.0, c->st));
+  return TEBD_OK;
+}
+
+int gemm1(tebd_ctx* c, const zc* A, const zc* B, zc* C, int M, int N, int K, int lda, int ldb, int ldc, bool conj) {
+  GemmTask* g = c->h_gemm;
+  *g = {A, B, C, M, N, K, lda, ldb, ldc, 1.0, 0};
+  HIPCHK(c, hipMemcpyAsync(c->d_gemm, g, sizeof(GemmTask), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, launch_zgemm(c->d_gemm, 1, M, N, conj, c->st));
+  // the pinned descriptor is rewritten by the next call
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return TEBD_OK;
+}
+
+// CanonicalForm on bond `l` (0-based; sites l, l+1), local_operations_module.f90:496-620.
+// The SVDs of the weighted Gram matrices M1 = B1^H B1 and M2 = C2 C2^H are taken
+// as one-sided Jacobi SVDs of B1 and C2^H: S = s^2 and the same U, without
+// squaring the condition number.  Bond dimensions are kept, as in the reference.
+int canonical_form(tebd_ctx* c, int l) {
+  const int d = c->d;
+  const int chi0 = c->chi[l], chi1 = c->chi[l + 1], chi2 = c->chi[l + 2];
+  const int m1 = d * chi0, n2 = d * chi2;
+  if (m1 < chi1 || n2 < chi1) return fail(c, TEBD_ERR_SHAPE, "canonical_form: bond dimension exceeds d x neighbour");
+  // every slab holds an R x Cpad matrix or a Cpad x Cpad Jacobi V
+  const size_t slab = (size_t)std::max(c->Rmax, c->Cpmax) * c->Cpmax;
+  zc* P[8];
+  for (int i = 0; i < 8; ++i) P[i] = c->cf + slab * i;
+  double *s1 = c->cf_s, *s2 = c->cf_s + c->Cpmax, *s3 = c->cf_s + 2 * c->Cpmax;
+  int *p1 = c->cf_perm, *p2 = c->cf_perm + c->Cpmax, *p3 = c->cf_perm + 2 * c->Cpmax;
+  const int cp1 = round_up(chi1, JC);
+  zc* G1 = c->gam + (size_t)l * c->gslab();
+  zc* G2 = c->gam + (size_t)(l + 1) * c->gslab();
+  const double* lam0 = c->lam + (size_t)l * c->chi_max;
+  double* lam1 = c->lam + (size_t)(l + 1) * c->chi_max;
+  const double* lam2 = c->lam + (size_t)(l + 2) * c->chi_max;
+  const double thr = 1e-8;  // S > 1e-8 (local_operations_module.f90:536,571)
+  int e;
+  // M1 = Gamma1^H lambda0^2 Gamma1  ->  U1, S1
+  HIPCHK(c, launch_scaled_copy(G1, m1, P[0], m1, m1, chi1, lam0, chi0, nullptr, 1, 0, c->st));
+  if ((e = device_svd(c, P[0], m1, chi1, P[1], s1, p1, c->cf_lam))) return e;
+  HIPCHK(c, launch_perm_cols(P[1], cp1, P[2], chi1, chi1, chi1, p1, s1, 0, thr, c->st));  // U1
+  HIPCHK(c, launch_perm_cols(P[1], cp1, P[3], chi1, chi1, chi1, p1, s1, 2, thr, c->st));  // U1 S1^-1/2
+  if ((e = gemm1(c, G1, P[3], P[4], m1, chi1, chi1, m1, chi1, m1, false))) return e;      // Gamma1Temp
+  // M2 = Gamma2 lambda2^2 Gamma2^H  ->  U2, S2
+  HIPCHK(c, launch_scaled_copy(G2, chi1, P[0], n2, chi1, n2, nullptr, 1, lam2, d, 1, c->st));
+  if ((e = device_svd(c, P[0], n2, chi1, P[1], s2, p2, c->cf_lam))) return e;
+  HIPCHK(c, launch_perm_cols(P[1], cp1, P[5], chi1, chi1, chi1, p2, s2, 0, thr, c->st));  // U2
+  HIPCHK(c, launch_perm_cols(P[1], cp1, P[6], chi1, chi1, chi1, p2, s2, 2, thr, c->st));  // U2 S2^-1/2
+  if ((e = gemm1(c, P[6], G2, P[7], chi1, n2, chi1, chi1, chi1, chi1, true))) return e;    // Gamma2Temp
+  // X = S1^1/2 U1^H, Y = lambda1 U2 S2^1/2, M = X Y -> U, S, V^T
+  HIPCHK(c, launch_cf_xy(P[2], s1, P[5], s2, lam1, P[3], P[6], chi1, c->st));
+  if ((e = gemm1(c, P[3], P[6], P[0], chi1, chi1, chi1, chi1, chi1, chi1, false))) return e;
+  if ((e = device_svd(c, P[0], chi1, chi1, P[1], s3, p3, c->cf_lam))) return e;
+  HIPCHK(c, launch_perm_cols(P[0], chi1, P[2], chi1, chi1, chi1, p3, s3, 1, 0.0, c->st));  // U
+  HIPCHK(c, launch_perm_cols(P[1], cp1, P[5], chi1, chi1, chi1, p3, s3, 0, 0.0, c->st));   // V (V^T = V^H)
+  HIPCHK(c, launch_cf_lambda(s3, chi1, lam1, c->chi_max, c->st));
+  if ((e = gemm1(c, P[4], P[2], G1, m1, chi1, chi1, m1, chi1, m1, false))) return e;      // Gamma1 = Temp U
+  if ((e = gemm1(c, P[5], P[7], G2, chi1, n2, chi1, chi1, chi1, chi1, true))) return e;     // Gamma2 = V^T Temp
+  return TEBD_OK;
+}
+
+int ensure_cf_workspace(tebd_ctx* c) {
+  if (c->cf) return TEBD_OK;
+  const size_t slab = (size_t)std::max(c->Rmax, c->Cpmax) * c->Cpmax;
+  HIPCHK(c, hipMalloc(&c->cf, sizeof(zc) * slab * 8));
+  HIPCHK(c, hipMalloc(&c->cf_s, sizeof(double) * 3 * c->Cpmax));
+  HIPCHK(c, hipMalloc(&c->cf_perm, sizeof(int) * 3 * c->Cpmax));
+  HIPCHK(c, hipMalloc(&c->cf_lam, sizeof(double) * c->Cpmax));
+  if (!c->h_gemm) {
+    HIPCHK(c, hipMalloc(&c->d_gemm, sizeof(GemmTask) * c->ntask_max * 5));
+    HIPCHK(c, hipHostMalloc(&c->h_gemm, sizeof(GemmTask) * c->ntask_max * 5));
+  }
+  return TEBD_OK;
+}
+
 int run_bonds(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
   if (c->mode & TEBD_MODE_LEARNED_RANK) return run_bonds_sketch(c, links, nb, gate_ptrs, trunc_sum);
   return run_bonds_full(c, links, nb, gate_ptrs, trunc_sum);
@@ -528,6 +623,10 @@ void free_all(tebd_ctx* c) {
   (void)hipFree(c->counters);
   (void)hipFree(c->results);
   (void)hipFree(c->d_tasks);
+  (void)hipFree(c->cf);
+  (void)hipFree(c->cf_s);
+  (void)hipFree(c->cf_perm);
+  (void)hipFree(c->cf_lam);
   (void)hipFree(c->Zb);
   (void)hipFree(c->Bb);
   (void)hipFree(c->lam_scratch);
@@ -645,6 +744,21 @@ int tebd_download_packed(tebd_ctx* c, double* gamma, double* lambda, int chi_out
   return download_impl(c, g.data(), l.data(), chi_out);
 }
 
+int tebd_download_lambdas(tebd_ctx* c, double* lambda, int chi_out) {
+  int e = ensure_state(c, false);
+  if (e) return e;
+  if (!lambda || chi_out < 1) return fail(c, TEBD_ERR_ARG, "download_lambdas: bad arguments");
+  for (int l = 0; l <= c->L; ++l)
+    if (c->chi[l] > chi_out) return fail(c, TEBD_ERR_SHAPE, "download_lambdas: chi_out smaller than a bond dimension");
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  std::vector<double> lam((size_t)(c->L + 1) * c->chi_max);
+  HIPCHK(c, hipMemcpy(lam.data(), c->lam, lam.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int l = 0; l <= c->L; ++l)
+    for (int a = 0; a < chi_out; ++a)
+      lambda[(size_t)l * chi_out + a] = a < c->chi[l] ? lam[(size_t)l * c->chi_max + a] : 0.0;
+  return TEBD_OK;
+}
+
 int tebd_trotter_step(tebd_ctx* c, double* truncerr) {
   int e = ensure_state(c, true);
   if (e) return e;
@@ -703,7 +817,25 @@ int tebd_two_site_op(tebd_ctx* c, int link, const double* op2, double* truncerr)
 int tebd_canonical_form_all(tebd_ctx* c) {
   int e = ensure_state(c, false);
   if (e) return e;
-  return fail(c, TEBD_ERR_UNSUPPORTED, "canonical_form_all: not available");
+  if ((e = ensure_cf_workspace(c))) return e;
+  for (int k = 0; k < c->L - 1; ++k)
+    if ((e = canonical_form(c, k))) return e;
+  for (int k = c->L - 2; k >= 0; --k)
+    if ((e = canonical_form(c, k))) return e;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  drain_profile(c);
+  return TEBD_OK;
+}
+
+int tebd_canonical_form(tebd_ctx* c, int link) {
+  int e = ensure_state(c, false);
+  if (e) return e;
+  if (link < 1 || link > c->L - 1) return fail(c, TEBD_ERR_ARG, "canonical_form: bad link");
+  if ((e = ensure_cf_workspace(c))) return e;
+  if ((e = canonical_form(c, link - 1))) return e;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  drain_profile(c);
+  return TEBD_OK;
 }
 
 int tebd_set_cutoff(tebd_ctx* c, double cutoff) {

@@ -17,3 +17,10 @@ hipError_t launch_build_sketch(const BondTask* d_tasks, const int* d_kprime, int
 hipError_t launch_normalize_cols(const BondTask* d_tasks, int ntasks, hipStream_t st);
 hipError_t launch_col_norms(const BondTask* d_tasks, int ntasks, int max_cpad, hipStream_t st);
 hipError_t launch_truncate(const BondTask* d_tasks, int ntasks, double cutoff, hipStream_t st);
+hipError_t launch_scaled_copy(const zc* in, int ldi, zc* out, int ldo, int rows, int cols, const double* rowscale,
+                              int row_mod, const double* colscale, int col_div, int conj_t, hipStream_t st);
+hipError_t launch_perm_cols(const zc* src, int lds, zc* dst, int ldd, int rows, int ncols, const int* perm,
+                            const double* s, int mode, double thr, hipStream_t st);
+hipError_t launch_cf_xy(const zc* U1, const double* s1, const zc* U2, const double* s2, const double* lam1, zc* X,
+                        zc* Y, int n, hipStream_t st);
+hipError_t launch_cf_lambda(const double* s, int n, double* lam, int chi_max, hipStream_t st);

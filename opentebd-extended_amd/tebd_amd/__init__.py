@@ -1,6 +1,6 @@
 """MI355X (gfx950) two-site Trotter update for OpenSourceTEBD-style Vidal states."""
 from . import models
 from ._lib import TEBDError, LIB_PATH
-from .engine import Context, parse_mode
+from .engine import Context, imag_time_prop, parse_mode
 
-__all__ = ["Context", "TEBDError", "models", "parse_mode", "LIB_PATH"]
+__all__ = ["Context", "TEBDError", "imag_time_prop", "models", "parse_mode", "LIB_PATH"]

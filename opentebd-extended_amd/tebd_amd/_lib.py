@@ -50,9 +50,11 @@ def lib():
         L.tebd_upload_packed.argtypes = [P, P, P, I]
         L.tebd_download.argtypes = [P, P, P, I]
         L.tebd_download_packed.argtypes = [P, P, P, I]
+        L.tebd_download_lambdas.argtypes = [P, P, I]
         L.tebd_trotter_step.argtypes = [P, ctypes.POINTER(D)]
         L.tebd_two_site_op.argtypes = [P, I, P, ctypes.POINTER(D)]
         L.tebd_canonical_form_all.argtypes = [P]
+        L.tebd_canonical_form.argtypes = [P, I]
         L.tebd_set_cutoff.argtypes = [P, D]
         L.tebd_get_bond_dims.argtypes = [P, P]
         L.tebd_get_stats.argtypes = [P, P, I]
@@ -62,7 +64,7 @@ def lib():
         L.tebd_last_error.argtypes = [P]
         L.tebd_last_error.restype = ctypes.c_char_p
         for name in ("tebd_create", "tebd_set_gates", "tebd_upload", "tebd_upload_packed", "tebd_download",
-                     "tebd_download_packed", "tebd_trotter_step", "tebd_two_site_op", "tebd_canonical_form_all",
+                     "tebd_download_packed", "tebd_download_lambdas", "tebd_trotter_step", "tebd_two_site_op", "tebd_canonical_form_all", "tebd_canonical_form",
                      "tebd_set_cutoff", "tebd_get_bond_dims", "tebd_get_stats", "tebd_sync",
                      "tebd_set_profiling", "tebd_get_profile"):
             getattr(L, name).restype = I

@@ -107,6 +107,22 @@ class Context:
         self._check(self._lib.tebd_download_packed(self._h, _p(gam), _p(lam), chi))
         return gam, lam
 
+    def lambdas(self, chi_out=None):
+        chi = int(chi_out or self.chi_max)
+        lam = np.zeros((self.L + 1, chi))
+        self._check(self._lib.tebd_download_lambdas(self._h, _p(lam), chi))
+        return lam
+
+    def chain_entropy(self):
+        """ChainEntropy for links 2..L in log base d (observables_module.f90:1583-1602)."""
+        lam = self.lambdas()
+        out = np.zeros(self.L - 1)
+        for link in range(2, self.L + 1):
+            v = lam[link - 1]
+            v = v[v != 0.0]
+            out[link - 2] = -np.sum(v * v * np.log(np.abs(v * v))) / np.log(self.d)
+        return out
+
     # -- the hot path ---------------------------------------------------------
     def trotter_step(self):
         t = ctypes.c_double()
@@ -121,6 +137,9 @@ class Context:
 
     def canonical_form_all(self):
         self._check(self._lib.tebd_canonical_form_all(self._h))
+
+    def canonical_form(self, link):
+        self._check(self._lib.tebd_canonical_form(self._h, int(link)))
 
     # -- introspection --------------------------------------------------------
     def bond_dims(self):
@@ -146,3 +165,29 @@ class Context:
 
     def sync(self):
         self._check(self._lib.tebd_sync(self._h))
+
+
+def imag_time_prop(ctx, H, dtau, max_steps, steps_for_judge=10, conv_criterion=1e-6):
+    """Imaginary-time ground-state search on the device.
+
+    Follows ImagTimeProp's inner loop (propagation_module.f90:636-760): gates
+    exp(-dtau H) from ConstructPropagators with dt = -i dtau (:656-662); every
+    ``steps_for_judge`` steps the relative change of each bond's largest lambda
+    is compared with ``conv_criterion`` (:704-738); otherwise one TrotterStep
+    followed by CanonicalFormAll (:741-743).  The reference's chi stages
+    (chiMin -> chiMax) are separate contexts here.  Returns the steps taken.
+    """
+    from . import models
+
+    ctx.set_gates(models.construct_propagators(H, -1j * dtau))
+    before = ctx.lambdas()[:, 0].copy()
+    for j in range(1, max_steps + 1):
+        if j % steps_for_judge == 0:
+            after = ctx.lambdas()[:, 0]
+            gap = np.abs((before - after) / np.where(before != 0.0, before, 1.0))
+            if gap.max() < conv_criterion:
+                return j
+            before = after.copy()
+        ctx.trotter_step()
+        ctx.canonical_form_all()
+    return max_steps

@@ -26,8 +26,11 @@ def run_device(z, mode, check_every=10):
     ctx.set_gates(z["U"])
     worst, cum, cum_ref = {}, 0.0, 0.0
     n = int(z["nsteps"])
+    canon = int(z["canon"])
     for k in range(1, n + 1):
         cum += ctx.trotter_step()
+        if canon:
+            ctx.canonical_form_all()
         cum_ref += z["trunc"][k]
         if k % check_every == 0 or k == n:
             g, l = ctx.download(chi)
@@ -48,6 +51,48 @@ def test_trajectory_matches_reference(name, mode):
     worst, dims = run_device(z, mode)
     assert max(worst.values()) < TOL, worst
     assert dims.max() <= int(z["chi"]) and dims[0] == 1 and dims[-1] == 1
+
+
+@pytest.mark.parametrize("mode", ["fp64", "batched"])
+def test_imaginary_time_with_canonical_form_matches_reference(mode):
+    """TrotterStep with exp(-dtau H) gates + CanonicalFormAll every step
+    (propagation_module.f90:741-743) against the reference's trajectory."""
+    z = load_golden("itp_l10_chi8")
+    print(f"[gpu] itp_l10_chi8 mode={mode}", flush=True)
+    worst, dims = run_device(z, mode, check_every=5)
+    assert max(worst.values()) < TOL, worst
+
+
+def test_canonical_form_restores_orthonormality():
+    """After random non-unitary updates, CanonicalFormAll leaves A = lambda_l Gamma
+    left-orthonormal and B = Gamma lambda_r right-orthonormal on every bond, and
+    keeps the state (two-site density matrices) unchanged up to normalisation."""
+    print("[gpu] canonical form invariants", flush=True)
+    rng = np.random.default_rng(5)
+    L, d, chi = 8, 2, 8
+    G = np.eye(4) + 0.4 * (rng.normal(size=(L - 1, 4, 4)) + 1j * rng.normal(size=(L - 1, 4, 4)))
+    ctx = Context(L, d, chi)
+    ctx.upload(*M.product_state(rng.normal(size=(L, d)) + 1j * rng.normal(size=(L, d)), chi))
+    ctx.set_gates(G)
+    for _ in range(3):
+        ctx.trotter_step()
+    ref = OracleState(*ctx.download(chi))
+    ref.canonical_form_all()
+    ctx.canonical_form_all()
+    g, l = ctx.download(chi)
+    dims = ctx.bond_dims()
+    for s in range(L):
+        cl, cr = dims[s], dims[s + 1]
+        A = g[s, :cr, :, :cl] * l[s, None, None, :cl]          # [b, i, a] * lambda_l(a)
+        Ml = np.einsum("bia,cia->bc", A.conj(), A)
+        B = g[s, :cr, :, :cl] * l[s + 1, :cr, None, None]      # Gamma(a,i,b) lambda_r(b)
+        Mr = np.einsum("bia,bic->ac", B, B.conj())
+        keep_l = l[s + 1, :cr] > 1e-6
+        keep_r = l[s, :cl] > 1e-6
+        np.testing.assert_allclose(Ml[np.ix_(keep_l, keep_l)], np.eye(keep_l.sum()), atol=1e-10)
+        np.testing.assert_allclose(Mr[np.ix_(keep_r, keep_r)], np.eye(keep_r.sum()), atol=1e-10)
+    np.testing.assert_allclose(l, ref.lam, atol=1e-10)
+    np.testing.assert_allclose(OracleState(g, l).two_site_rho(), ref.two_site_rho(), atol=1e-10)
 
 
 def test_single_two_site_op_on_random_state():
