@@ -42,6 +42,29 @@ def make_context(mode):
     return ctx
 
 
+class _Roctx:
+    """roctx range around the timed region, so a rocprofv3 --marker-trace run can
+    restrict its kernel statistics to exactly the launches timed here."""
+
+    def __init__(self):
+        self.lib = None
+        try:
+            import ctypes
+
+            self.lib = ctypes.CDLL("/opt/rocm/lib/librocprofiler-sdk-roctx.so.1")
+            self.lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+        except OSError:
+            self.lib = None
+
+    def push(self, name):
+        if self.lib:
+            self.lib.roctxRangePushA(name.encode())
+
+    def pop(self):
+        if self.lib:
+            self.lib.roctxRangePop()
+
+
 def run_mode(mode, steps, warmup, log):
     ctx = make_context(mode)
     tw = time.perf_counter()
@@ -51,13 +74,16 @@ def run_mode(mode, steps, warmup, log):
             f"({time.perf_counter() - tw:.1f}s)")
     ctx.sync()
     ctx.set_profiling(True)
+    rx = _Roctx()
     trunc = 0.0
+    rx.push(f"bench_timed_{mode}")
     t0 = time.perf_counter()
     for k in range(steps):
         trunc += ctx.trotter_step()
         log(f"[bench] {mode} timed step {k + 1}/{steps} ({time.perf_counter() - t0:.1f}s)")
     ctx.sync()
     elapsed = time.perf_counter() - t0
+    rx.pop()
     prof = ctx.profile()
     dims = ctx.bond_dims()
     ctx.close()

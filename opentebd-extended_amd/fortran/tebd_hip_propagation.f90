@@ -1,0 +1,156 @@
+! The reference's propagation call surface, backed by libtebdhip.so.
+!
+! Same names and argument lists as the reference:
+!   SVDInit(chi), SVDFinish()                 local_operations_module.f90:84-153
+!   TwoSiteOp(link, Op2, Gammas, Lambdas, truncerr)
+!                                             local_operations_module.f90:46-48,463-494
+!   TrotterStep(Udt, Gammas, Lambdas, totalTruncerr)
+!                                             propagation_module.f90:42-114
+!   CanonicalFormAll(Gammas, Lambdas)         propagation_module.f90:617-634
+! on the reference's own types (TYPE(tensor), TYPE(vector), TYPE(matrix),
+! TEBDtools_module.f90:31-45) and globals (systemSize, localSize, trotterOrder,
+! BoundaryCond from system_parameters).  Build it after the reference's
+! system_parameters and TEBDtools_module and link libtebdhip.so; a program that
+! also needs the reference's own procedures of the same names renames one set
+! on USE (e.g. USE tebd_hip_propagation, TrotterStepHIP => TrotterStep).
+!
+! Like the reference's module-global SVD workspace, the device context is module
+! state: SVDInit(chi) creates it for chains of systemSize sites with local
+! dimension localSize and bond dimension chi, SVDFinish() frees it.
+!
+! Host arrays stay authoritative by default: every call uploads Gammas/Lambdas,
+! runs on the GPU and downloads the result.  With tebd_hip_resident = .TRUE. the
+! state stays on the device between calls; call tebd_hip_to_host(Gammas,
+! Lambdas) before reading it on the host and tebd_hip_to_device after changing it.
+MODULE tebd_hip_propagation
+USE, INTRINSIC :: iso_c_binding
+USE system_parameters, ONLY: rKind, systemSize, localSize, trotterOrder, BoundaryCond
+USE TEBDtools_module, ONLY: tensor, vector, matrix
+USE tebd_hip
+IMPLICIT NONE
+PRIVATE
+PUBLIC :: SVDInit, SVDFinish, TwoSiteOp, TrotterStep, CanonicalFormAll
+PUBLIC :: tebd_hip_resident, tebd_hip_mode, tebd_hip_to_host, tebd_hip_to_device, tebd_hip_context
+
+TYPE(c_ptr), SAVE :: ctx = c_null_ptr
+INTEGER, SAVE :: ctx_chi = 0
+LOGICAL, SAVE :: on_device = .FALSE.
+LOGICAL, SAVE :: tebd_hip_resident = .FALSE.
+INTEGER(c_int), SAVE :: tebd_hip_mode = TEBD_MODE_FP64
+
+CONTAINS
+
+FUNCTION tebd_hip_context() RESULT(p)
+  TYPE(c_ptr) :: p
+  p = ctx
+END FUNCTION tebd_hip_context
+
+SUBROUTINE SVDInit(chi)
+  INTEGER, INTENT(IN) :: chi
+  CHARACTER(kind=c_char) :: bc
+  IF (c_associated(ctx)) CALL SVDFinish()
+  bc = BoundaryCond
+  CALL tebd_check(ctx, tebd_create(ctx, INT(systemSize, c_int), INT(localSize, c_int), INT(chi, c_int), &
+                  INT(trotterOrder, c_int), bc, tebd_hip_mode), 'SVDInit')
+  ctx_chi = chi
+  on_device = .FALSE.
+END SUBROUTINE SVDInit
+
+SUBROUTINE SVDFinish()
+  IF (c_associated(ctx)) CALL tebd_destroy(ctx)
+  ctx = c_null_ptr
+  ctx_chi = 0
+  on_device = .FALSE.
+END SUBROUTINE SVDFinish
+
+SUBROUTINE tebd_hip_to_device(Gammas, Lambdas)
+  TYPE(tensor), POINTER :: Gammas(:)
+  TYPE(vector), POINTER :: Lambdas(:)
+  TYPE(c_ptr), ALLOCATABLE, TARGET :: gp(:), lp(:)
+  INTEGER :: i
+  IF (.NOT. c_associated(ctx)) STOP 'tebd_hip: SVDInit has not been called'
+  ALLOCATE(gp(SIZE(Gammas)), lp(SIZE(Lambdas)))
+  DO i = 1, SIZE(Gammas)
+    gp(i) = c_loc(Gammas(i)%t(1, 1, 1))
+  END DO
+  DO i = 1, SIZE(Lambdas)
+    lp(i) = c_loc(Lambdas(i)%v(1))
+  END DO
+  CALL tebd_check(ctx, tebd_upload(ctx, c_loc(gp), c_loc(lp), INT(SIZE(Gammas(1)%t, 1), c_int)), 'upload')
+  on_device = .TRUE.
+END SUBROUTINE tebd_hip_to_device
+
+SUBROUTINE tebd_hip_to_host(Gammas, Lambdas)
+  TYPE(tensor), POINTER :: Gammas(:)
+  TYPE(vector), POINTER :: Lambdas(:)
+  TYPE(c_ptr), ALLOCATABLE, TARGET :: gp(:), lp(:)
+  INTEGER :: i
+  ALLOCATE(gp(SIZE(Gammas)), lp(SIZE(Lambdas)))
+  DO i = 1, SIZE(Gammas)
+    gp(i) = c_loc(Gammas(i)%t(1, 1, 1))
+  END DO
+  DO i = 1, SIZE(Lambdas)
+    lp(i) = c_loc(Lambdas(i)%v(1))
+  END DO
+  CALL tebd_check(ctx, tebd_download(ctx, c_loc(gp), c_loc(lp), INT(SIZE(Gammas(1)%t, 1), c_int)), 'download')
+END SUBROUTINE tebd_hip_to_host
+
+SUBROUTINE before_call(Gammas, Lambdas)
+  TYPE(tensor), POINTER :: Gammas(:)
+  TYPE(vector), POINTER :: Lambdas(:)
+  IF (.NOT. c_associated(ctx)) CALL SVDInit(SIZE(Gammas(1)%t, 1))
+  IF (.NOT. (tebd_hip_resident .AND. on_device)) CALL tebd_hip_to_device(Gammas, Lambdas)
+END SUBROUTINE before_call
+
+SUBROUTINE after_call(Gammas, Lambdas)
+  TYPE(tensor), POINTER :: Gammas(:)
+  TYPE(vector), POINTER :: Lambdas(:)
+  IF (.NOT. tebd_hip_resident) CALL tebd_hip_to_host(Gammas, Lambdas)
+END SUBROUTINE after_call
+
+SUBROUTINE TwoSiteOp(link, Op2, Gammas, Lambdas, truncerr)
+  INTEGER, INTENT(IN) :: link
+  COMPLEX(KIND=rKind), INTENT(IN), TARGET :: Op2(:, :)
+  TYPE(tensor), POINTER :: Gammas(:)
+  TYPE(vector), POINTER :: Lambdas(:)
+  REAL(KIND=rKind), INTENT(INOUT) :: truncerr
+  COMPLEX(c_double_complex), ALLOCATABLE, TARGET :: op(:, :)
+  REAL(c_double) :: t
+  CALL before_call(Gammas, Lambdas)
+  ALLOCATE(op(SIZE(Op2, 1), SIZE(Op2, 2)))
+  op = Op2
+  CALL tebd_check(ctx, tebd_two_site_op(ctx, INT(link, c_int), c_loc(op), t), 'TwoSiteOp')
+  truncerr = t
+  CALL after_call(Gammas, Lambdas)
+END SUBROUTINE TwoSiteOp
+
+SUBROUTINE TrotterStep(Udt, Gammas, Lambdas, totalTruncerr)
+  TYPE(matrix), POINTER :: Udt(:)
+  TYPE(tensor), POINTER :: Gammas(:)
+  TYPE(vector), POINTER :: Lambdas(:)
+  REAL(KIND=rKind), INTENT(INOUT) :: totalTruncerr
+  COMPLEX(c_double_complex), ALLOCATABLE, TARGET :: u(:, :, :)
+  REAL(c_double) :: t
+  INTEGER :: i, nb, d2
+  CALL before_call(Gammas, Lambdas)
+  nb = systemSize - 1
+  d2 = localSize * localSize
+  ALLOCATE(u(d2, d2, nb))
+  DO i = 1, nb
+    u(:, :, i) = Udt(i)%m
+  END DO
+  CALL tebd_check(ctx, tebd_set_gates(ctx, c_loc(u), INT(nb, c_int)), 'TrotterStep (gates)')
+  CALL tebd_check(ctx, tebd_trotter_step(ctx, t), 'TrotterStep')
+  totalTruncerr = t
+  CALL after_call(Gammas, Lambdas)
+END SUBROUTINE TrotterStep
+
+SUBROUTINE CanonicalFormAll(Gammas, Lambdas)
+  TYPE(tensor), POINTER :: Gammas(:)
+  TYPE(vector), POINTER :: Lambdas(:)
+  CALL before_call(Gammas, Lambdas)
+  CALL tebd_check(ctx, tebd_canonical_form_all(ctx), 'CanonicalFormAll')
+  CALL after_call(Gammas, Lambdas)
+END SUBROUTINE CanonicalFormAll
+
+END MODULE tebd_hip_propagation
