@@ -3,14 +3,16 @@
 
 Workload (the Fermi-Hubbard configuration the metric is quoted on): spin-1/2
 Hubbard chain L=128, d=4 (basis [0, up, down, up+down]), t=1, U=4, OBC,
-2nd-order Trotter with dt=0.01, chi_max=256, half-filled Neel start.  The
-first W steps of that quench are untimed warm-up: they also grow every bulk
-bond to chi=256 (the reference keeps every Schmidt value above 1e-15, so the
-bonds saturate within a few steps); the next K steps are timed at that real
-shape (191 bond updates of up to 1024 x 1024 per step).
+2nd-order Trotter with dt=0.01, chi_max=256, half-filled Neel start, 100 steps.
+The bond dimensions grow unevenly from 1 and every bulk bond reaches chi=256 by
+step ~37.  The first W=40 steps are untimed warm-up; the next K=60 steps (41-100)
+are timed, all at the real shape (191 bond updates of up to 1024 x 1024 per step).
+`--compare fp64` times the plain per-bond FP64 path on the first steps of the same
+window, started from a snapshot of the same state.
 
-Prints one JSON line.  `roofline` is the Jacobi round kernel (the dominant
-kernel): algorithmic FP64 flops per launch / average launch duration, both
+Prints one JSON line.  `roofline` is the kernel class with the most device time in
+the timed steps (pivoted-QR steps or Jacobi rounds, both HBM-bound streaming
+kernels): algorithmic HBM bytes per launch / average launch duration, both
 measured with HIP events on the context's stream inside the library.
 `cpu_baseline` times the FP64 CPU oracle (the reference algorithm: padded chi,
 full ZGESVD of every (chi d) x (chi d) matrix) on a bounded sample of bond
@@ -30,6 +32,7 @@ import numpy as np  # noqa: E402
 
 L, D, CHI, DT, T_HOP, U_INT = 128, 4, 256, 0.01, 1.0, 4.0
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0    # MI355X HBM3E
 
 
 def make_context(mode):
@@ -65,15 +68,21 @@ class _Roctx:
             self.lib.roctxRangePop()
 
 
-def run_mode(mode, steps, warmup, log):
+def run_mode(mode, steps, warmup, log, state=None, keep_state=False):
     ctx = make_context(mode)
+    if state is not None:
+        ctx.upload(*state)
     tw = time.perf_counter()
     for k in range(warmup):
         ctx.trotter_step()
-        log(f"[bench] {mode} warmup step {k + 1}/{warmup} chi_max={ctx.bond_dims().max()} "
-            f"({time.perf_counter() - tw:.1f}s)")
+        if (k + 1) % 10 == 0 or k + 1 == warmup:
+            log(f"[bench] {mode} warmup step {k + 1}/{warmup} mean chi={ctx.bond_dims().mean():.1f} "
+                f"({time.perf_counter() - tw:.1f}s)")
     ctx.sync()
+    warm_s = time.perf_counter() - tw
+    snap = ctx.download() if keep_state else None
     ctx.set_profiling(True)
+    st0 = ctx.stats()
     rx = _Roctx()
     trunc = 0.0
     rx.push(f"bench_timed_{mode}")
@@ -85,9 +94,11 @@ def run_mode(mode, steps, warmup, log):
     elapsed = time.perf_counter() - t0
     rx.pop()
     prof = ctx.profile()
+    st1 = ctx.stats()
     dims = ctx.bond_dims()
     ctx.close()
-    return dict(elapsed=elapsed, prof=prof, dims=dims, trunc=trunc)
+    delta = {k: st1[k] - st0[k] for k in st1}
+    return dict(elapsed=elapsed, prof=prof, dims=dims, trunc=trunc, stats=delta, warm_s=warm_s, snap=snap)
 
 
 def cpu_baseline(sample_bonds, log):
@@ -115,8 +126,9 @@ def cpu_baseline(sample_bonds, log):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--compare-steps", type=int, default=3)
     ap.add_argument("--mode", default="batched")
     ap.add_argument("--compare", default="", help="also time this mode (e.g. fp64) on the same workload")
     ap.add_argument("--cpu-sample-bonds", type=int, default=3)
@@ -128,16 +140,23 @@ def main():
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    res = run_mode(args.mode, args.steps, args.warmup, log)
+    res = run_mode(args.mode, args.steps, args.warmup, log, keep_state=bool(args.compare))
     sps = args.steps / res["elapsed"]
-    n_upd, jac_ms, jac_flops = res["prof"]["jacobi_round"]
-    avg_ms = jac_ms / max(n_upd, 1)
-    achieved = (jac_flops / max(n_upd, 1)) / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    prof = res["prof"]
+    kernel_of = {"jacobi_round": "k_jacobi_round", "qr_step": "k_qrp_pivot + k_qrp_update"}
+    dom = max(kernel_of, key=lambda n: prof[n][1])
+    n_launch, dom_ms, dom_flops, dom_bytes = prof[dom]
+    avg_ms = dom_ms / max(n_launch, 1)
+    gbs = (dom_bytes / max(n_launch, 1)) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    tflops = (dom_flops / max(n_launch, 1)) / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    st = res["stats"]
+    half_steps = 3 * args.steps
 
     extra = {}
     if args.compare:
-        cmp_res = run_mode(args.compare, max(1, args.steps // 3), args.warmup, log)
-        extra[f"{args.compare}_steps_per_s"] = max(1, args.steps // 3) / cmp_res["elapsed"]
+        cmp_res = run_mode(args.compare, args.compare_steps, 0, log, state=res["snap"])
+        extra[f"{args.compare}_steps_per_s"] = args.compare_steps / cmp_res["elapsed"]
+        extra[f"{args.compare}_window"] = f"steps {args.warmup + 1}-{args.warmup + args.compare_steps}"
 
     cpu = None
     if args.cpu_sample_bonds > 0:
@@ -162,12 +181,18 @@ def main():
         "dtype": "complex128 (f64)",
         "data": "synthetic (Neel product state, Fermi-Hubbard gates)",
         "config": {"workload": "Fermi-Hubbard quench L=128 d=4 t=1 U=4 dt=0.01 chi_max=256, Neel start; "
-                               "timed steps after the bonds saturate",
+                               f"timed steps {args.warmup + 1}-{args.warmup + args.steps}",
                    "mode": args.mode, "chi_max_reached": int(res["dims"].max()),
-                   "mean_bond_dim": float(res["dims"].mean())},
-        "roofline": {"bound": "mfma", "kernel": "k_jacobi_round", "achieved": achieved,
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": None, "avg_launch_ms": avg_ms, "launches": int(n_upd)},
+                   "mean_bond_dim": float(res["dims"].mean()), "warmup_s": res["warm_s"]},
+        "roofline": {"bound": "hbm", "kernel": kernel_of[dom], "achieved": gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": None, "avg_launch_ms": avg_ms, "launches": int(n_launch),
+                     "bytes_per_launch": dom_bytes / max(n_launch, 1), "fp64_tflops": tflops,
+                     "fp64_frac": tflops / FP64_PEAK_TFLOPS,
+                     "device_ms": {n: prof[n][1] for n in prof}},
+        "jacobi": {"sweeps_per_half_step": st["jacobi_sweeps"] / half_steps,
+                   "pair_visits": st["pair_visits"], "grams_formed": st["pair_grams"],
+                   "pairs_rotated": st["pair_rotations"]},
         "cpu_baseline": cpu,
     }
     out.update(extra)

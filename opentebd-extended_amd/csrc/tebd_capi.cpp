@@ -16,6 +16,11 @@
 namespace {
 
 constexpr int kStages = 5;
+constexpr int kNumStats = 11;
+constexpr int kNumProf = 16;
+constexpr double kQrTol = 1e-17;  // QR stops when every remaining column norm is <= kQrTol ||theta||_F
+constexpr int kQrCheck = 32;      // QR steps between checks whether every task has stopped
+constexpr int kQrMinCols = 64;    // bond matrices with fewer columns go straight to the Jacobi
 constexpr int kMaxSweeps = 40;
 constexpr double kTolRel = 1e-13;   // relative orthogonality of converged column pairs
 constexpr double kTolAbs = 1e-13;   // rounding floor: |A_pq| <= kTolAbs ||theta|| (|w_p| + |w_q|)
@@ -41,9 +46,21 @@ struct tebd_ctx {
   int Rmax = 0, Cpmax = 0;
   zc* W = nullptr;
   zc* V = nullptr;
+  // pivoted-QR preconditioning workspace, per task slot
+  zc* X = nullptr;               // Cpmax x Cpmax: R^dagger, the Jacobi input
+  zc* Yr = nullptr;              // Rmax x chi_max
+  zc* Zc = nullptr;              // Cpmax x chi_max
+  double* qnrm = nullptr;        // Cpmax
+  int* qpiv = nullptr;           // Cpmax
+  zc* tau = nullptr;             // Cpmax
+  double* qstate = nullptr;      // 2
+  double* h_qstate = nullptr;    // pinned, 2 per task slot
   double* s = nullptr;
   int* perm = nullptr;
-  int* counters = nullptr;
+  int* counters = nullptr;       // 4 per task slot (BondTask::counter)
+  unsigned* blkver = nullptr;    // nblk_max per task slot
+  unsigned long long* pairseen = nullptr;  // nblk_max^2 per task slot
+  int nblk_max = 0;
   double* results = nullptr;
   BondTask* d_tasks = nullptr;   // kStages x ntask_max
   BondTask* h_tasks = nullptr;   // pinned, kStages x ntask_max
@@ -68,15 +85,18 @@ struct tebd_ctx {
   double* cf_lam = nullptr;
   std::vector<int> last_rank;    // kept rank of each bond's last update (rank model input)
   double* h_results = nullptr;   // pinned
-  int* h_counters = nullptr;     // pinned
+  int* h_counters = nullptr;     // pinned, 4 per task slot
   std::vector<zc> staging;
   std::string err;
-  double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // [4..7]: Jacobi sweeps of the sketch stages Z0, R, R re-orth, Rayleigh-Ritz
+  // [0] bond updates, [1] Jacobi sweeps, [2] sketch fallbacks, [3] device ms of trotter steps,
+  // [4..7] Jacobi sweeps of the sketch stages Z0, R, R re-orth, Rayleigh-Ritz,
+  // [8] pair Gram matrices formed, [9] pairs rotated, [10] pair visits of the sweeps run
+  double stats[kNumStats] = {};
   // optional per-kernel timing with HIP events on the context's stream:
-  // prof[k*3 + 0] launches, [1] milliseconds, [2] algorithmic flops, for
-  // k = 0 theta+gate, 1 Jacobi round, 2 truncation + Gamma rebuild
+  // prof[k*4 + 0] launches, [1] milliseconds, [2] algorithmic flops, [3] algorithmic HBM bytes,
+  // for k = 0 theta+gate, 1 Jacobi round, 2 truncation + Gamma rebuild, 3 pivoted QR steps
   bool profiling = false;
-  double prof[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double prof[kNumProf] = {};
   struct Pending { hipEvent_t a, b; int kind; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
@@ -132,8 +152,8 @@ void drain_profile(tebd_ctx* c) {
   for (auto& p : c->pending) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
-      c->prof[3 * p.kind + 0] += 1;
-      c->prof[3 * p.kind + 1] += ms;
+      c->prof[4 * p.kind + 0] += 1;
+      c->prof[4 * p.kind + 1] += ms;
     }
     c->event_pool.push_back(p.a);
     c->event_pool.push_back(p.b);
@@ -141,30 +161,172 @@ void drain_profile(tebd_ctx* c) {
   c->pending.clear();
 }
 
-// Jacobi sweeps over the tasks of one stage until a sweep rotates nothing.
+// Gives task `slot` of a batch its Jacobi counters and block bookkeeping.
+void attach_jacobi(tebd_ctx* c, BondTask& t, int slot) {
+  t.counter = c->counters + 4 * slot;
+  t.blkver = c->blkver + (size_t)slot * c->nblk_max;
+  t.pairseen = c->pairseen + (size_t)slot * c->nblk_max * c->nblk_max;
+}
+
+// Jacobi sweeps over the tasks of one stage (task i uses slot i) until a sweep
+// rotates nothing.
 int jacobi_sweeps(tebd_ctx* c, BondTask* d_t, const BondTask* h_t, int nb, int Pmax) {
+  HIPCHK(c, launch_jacobi_begin(d_t, nb, c->st));
   int sweeps = 0;
-  for (; sweeps < kMaxSweeps; ++sweeps) {
-    HIPCHK(c, launch_reset_counters(d_t, nb, c->st));
+  bool converged = false;
+  while (sweeps < kMaxSweeps && !converged) {
+    HIPCHK(c, launch_jacobi_sweep_begin(d_t, nb, sweeps, c->st));
     for (int r = 0; r < 2 * Pmax - 1; ++r) {
       ProfScope ps(c, 1);
-      HIPCHK(c, launch_jacobi_round(d_t, nb, Pmax, r, kTolRel, kTolAbs, c->st));
+      HIPCHK(c, launch_jacobi_round(d_t, nb, Pmax, sweeps, r, kTolRel, kTolAbs, c->st));
     }
-    HIPCHK(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(int) * nb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(int) * 4 * nb, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     int rot = 0;
-    for (int i = 0; i < nb; ++i) rot += c->h_counters[i];
-    if (c->profiling)
-      for (int i = 0; i < nb; ++i) {
-        const BondTask& t = h_t[i];
-        const double jc2 = (double)JC * JC;
-        c->prof[5] += (double)t.P * (2 * t.P - 1) * 8.0 * t.R * jc2 +
-                      c->h_counters[i] * 8.0 * (t.R + (t.V ? t.NV : 0)) * jc2;
-      }
-    if (rot == 0) { ++sweeps; c->stats[1] += sweeps; return TEBD_OK; }
+    for (int i = 0; i < nb; ++i) rot += c->h_counters[4 * i + (sweeps & 1)];
+    converged = rot == 0;
+    ++sweeps;
   }
   c->stats[1] += sweeps;
-  return fail(c, TEBD_ERR_NOCONV, "one-sided Jacobi did not converge");
+  for (int i = 0; i < nb; ++i) {
+    const BondTask& t = h_t[i];
+    const double visits = c->h_counters[4 * i + 2], rots = c->h_counters[4 * i + 3];
+    const double rows = t.R + (t.V ? t.NV : 0);
+    c->stats[8] += visits;
+    c->stats[9] += rots;
+    c->stats[10] += (double)sweeps * t.P * (2 * t.P - 1);
+    if (c->profiling) {
+      // per formed Gram: the 16x16 cross block over R rows (8 flops per complex MAC)
+      // reading the pair's 2 JB columns; per rotation: the JC columns of W and V
+      // read and written once, JC complex MACs per element
+      c->prof[6] += visits * 8.0 * t.R * JB * JB + rots * 8.0 * rows * JC * JC;
+      c->prof[7] += visits * 16.0 * t.R * JC + rots * 32.0 * rows * JC;
+    }
+  }
+  if (!converged) return fail(c, TEBD_ERR_NOCONV, "one-sided Jacobi did not converge");
+  return TEBD_OK;
+}
+
+// Gives task `slot` its pivoted-QR buffers; Wq is the bond matrix slot W.
+void attach_qr(tebd_ctx* c, BondTask& t, int slot) {
+  t.Wq = t.W;
+  t.Rb = t.R;
+  t.Cb = t.C;
+  t.qnrm = c->qnrm + (size_t)slot * c->Cpmax;
+  t.qpiv = c->qpiv + (size_t)slot * c->Cpmax;
+  t.tau = c->tau + (size_t)slot * c->Cpmax;
+  t.qstate = c->qstate + 2 * slot;
+  t.Yr = c->Yr + (size_t)slot * c->Rmax * c->chi_max;
+  t.Zc = c->Zc + (size_t)slot * c->Cpmax * c->chi_max;
+}
+
+// Column-pivoted Householder QR of every task's bond matrix, stopped at the
+// numerical rank.  Returns the ranks in rank[].
+int qrp_factor(tebd_ctx* c, int nb, int max_cb, std::vector<int>& rank) {
+  HIPCHK(c, launch_qrp_begin(c->d_tasks, nb, kQrTol, c->st));
+  bool done = false;
+  for (int j = 0; j <= max_cb && !done; ++j) {
+    {
+      ProfScope ps(c, 3);
+      HIPCHK(c, launch_qrp_step(c->d_tasks, nb, max_cb, j, c->st));
+    }
+    if ((j + 1) % kQrCheck == 0 || j == max_cb) {
+      HIPCHK(c, hipMemcpyAsync(c->h_qstate, c->qstate, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->st));
+      HIPCHK(c, hipStreamSynchronize(c->st));
+      done = true;
+      for (int i = 0; i < nb; ++i) done = done && c->h_qstate[2 * i] >= 0.0;
+    }
+  }
+  rank.resize(nb);
+  for (int i = 0; i < nb; ++i) {
+    rank[i] = (int)c->h_qstate[2 * i];
+    if (c->profiling) {
+      // step j: the trailing (Rb - j) x (Cb - j - 1) block is read and written once,
+      // two complex MACs per element
+      const BondTask& t = c->h_tasks[i];
+      for (int j = 0; j < rank[i]; ++j) {
+        const double el = (double)(t.Rb - j) * (t.Cb - j - 1);
+        c->prof[14] += 16.0 * el;
+        c->prof[15] += 32.0 * el + 16.0 * (t.Rb - j);
+      }
+    }
+  }
+  return TEBD_OK;
+}
+
+// SVD + truncation of the tasks' bond matrices (theta already in W) and the new
+// Gammas / lambda.  Bond matrices with at least kQrMinCols columns are first
+// reduced by the pivoted QR; the one-sided Jacobi then runs on R^dagger.
+// On return h_results holds [0] kept rank and [1] truncerr of every task.
+int svd_truncate(tebd_ctx* c, const int* links, int nb, int Cpmax, int Pmax) {
+  int max_cb = 1, max_rb = 1;
+  for (int i = 0; i < nb; ++i) {
+    max_cb = std::max(max_cb, c->h_tasks[i].Cb);
+    max_rb = std::max(max_rb, c->h_tasks[i].Rb);
+  }
+  const bool use_qr = max_cb >= kQrMinCols;
+  if (use_qr) {
+    std::vector<int> rank;
+    int e = qrp_factor(c, nb, max_cb, rank);
+    if (e) return e;
+    Cpmax = 1;
+    Pmax = 1;
+    for (int i = 0; i < nb; ++i) {
+      if (rank[i] < 1) {
+        char buf[160];
+        std::snprintf(buf, sizeof(buf), "bond %d: theta is zero or not finite", links[i] + 1);
+        return fail(c, TEBD_ERR_NAN, buf);
+      }
+      BondTask& t = c->h_tasks[i];
+      t.qr = 1;
+      t.W = c->X + (size_t)i * c->Cpmax * c->Cpmax;
+      t.R = t.Cb;
+      t.C = rank[i];
+      t.Cpad = round_up(t.C, JC);
+      t.P = t.Cpad / JC;
+      t.NV = t.Cpad;
+      Cpmax = std::max(Cpmax, t.Cpad);
+      Pmax = std::max(Pmax, t.P);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_tasks, c->h_tasks, sizeof(BondTask) * nb, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_qrp_build_x(c->d_tasks, nb, c->st));
+  }
+  HIPCHK(c, launch_prepare_svd(c->d_tasks, nb, c->st));
+  HIPCHK(c, launch_frobenius(c->d_tasks, nb, Cpmax, c->st, 0));
+  {
+    int e = jacobi_sweeps(c, c->d_tasks, c->h_tasks, nb, Pmax);
+    if (e) return e;
+  }
+  {
+    ProfScope ps(c, 2);
+    HIPCHK(c, launch_finalize(c->d_tasks, nb, Cpmax, c->cutoff, c->st));
+  }
+  HIPCHK(c, hipMemcpyAsync(c->h_results, c->results, sizeof(double) * RESULT_STRIDE * nb, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  int max_k = 1;
+  for (int i = 0; i < nb; ++i) {
+    const double* r = c->h_results + RESULT_STRIDE * i;
+    const int k = (int)r[0];
+    if (!(std::isfinite(r[2])) || r[2] <= 0.0 || k < 1) {
+      char buf[160];
+      std::snprintf(buf, sizeof(buf), "bond %d: theta is zero or not finite (||theta||^2=%g)", links[i] + 1, r[2]);
+      return fail(c, TEBD_ERR_NAN, buf);
+    }
+    max_k = std::max(max_k, k);
+    if (c->profiling) {
+      // column norms read W; Gamma_l, Gamma_r written from k singular vectors
+      const BondTask& t = c->h_tasks[i];
+      c->prof[11] += 16.0 * ((double)t.R * t.C + 2.0 * k * t.d * ((double)t.chil + t.chir));
+    }
+  }
+  {
+    ProfScope ps(c, 2);
+    if (use_qr) HIPCHK(c, launch_qrp_vectors(c->d_tasks, nb, max_rb, max_k, c->st));
+    HIPCHK(c, launch_write_gammas(c->d_tasks, nb, c->st));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  drain_profile(c);
+  return TEBD_OK;
 }
 
 // Process `nb` independent bonds (0-based links) as one batch of tasks.
@@ -196,7 +358,8 @@ int run_bonds_full(tebd_ctx* c, const int* links, int nb, const zc* const* gate_
     t.s = c->s + (size_t)i * c->Cpmax;
     t.perm = c->perm + (size_t)i * c->Cpmax;
     t.NV = t.Cpad;
-    t.counter = c->counters + i;
+    attach_jacobi(c, t, i);
+    attach_qr(c, t, i);
     t.result = c->results + (size_t)RESULT_STRIDE * i;
     t.gl_out = const_cast<zc*>(t.gl);
     t.gr_out = const_cast<zc*>(t.gr);
@@ -217,33 +380,16 @@ int run_bonds_full(tebd_ctx* c, const int* links, int nb, const zc* const* gate_
     for (int i = 0; i < nb; ++i) {
       const BondTask& t = c->h_tasks[i];
       c->prof[2] += 8.0 * d * d * t.chil * (double)t.chic * t.chir + 8.0 * t.chil * (double)t.chir * d * d * d * d;
+      // Gamma_l, Gamma_r read, theta written
+      c->prof[3] += 16.0 * ((double)t.chil * d * t.chic + (double)t.chic * d * t.chir + (double)t.R * t.C);
     }
   }
-  HIPCHK(c, launch_prepare_svd(c->d_tasks, nb, c->st));
-  HIPCHK(c, launch_frobenius(c->d_tasks, nb, Cpmax, c->st, 0));
-  {
-    // per visited pair: Gram 8 R JC^2; per rotated pair: W and V updates 8 (R + NV) JC^2
-    int e = jacobi_sweeps(c, c->d_tasks, c->h_tasks, nb, Pmax);
-    if (e) return e;
-  }
-  {
-    ProfScope ps(c, 2);
-    HIPCHK(c, launch_finalize(c->d_tasks, nb, Cpmax, c->cutoff, c->st));
-    HIPCHK(c, launch_write_gammas(c->d_tasks, nb, c->st));
-  }
-  HIPCHK(c, hipMemcpyAsync(c->h_results, c->results, sizeof(double) * RESULT_STRIDE * nb, hipMemcpyDeviceToHost, c->st));
-  HIPCHK(c, hipStreamSynchronize(c->st));
-  drain_profile(c);
+  int e = svd_truncate(c, links, nb, Cpmax, Pmax);
+  if (e) return e;
   double tsum = 0.0;
   for (int i = 0; i < nb; ++i) {
     const double* r = c->h_results + RESULT_STRIDE * i;
-    const int k = (int)r[0];
-    if (!(std::isfinite(r[2])) || r[2] <= 0.0 || k < 1) {
-      char buf[160];
-      std::snprintf(buf, sizeof(buf), "bond %d: theta is zero or not finite (||theta||^2=%g)", links[i] + 1, r[2]);
-      return fail(c, TEBD_ERR_NAN, buf);
-    }
-    c->chi[links[i] + 1] = k;
+    c->chi[links[i] + 1] = (int)r[0];
     tsum += r[1];
   }
   c->stats[0] += nb;
@@ -331,7 +477,7 @@ int run_bonds_sketch(tebd_ctx* c, const int* links, int nb, const zc* const* gat
       base.chil = chil; base.chic = chic; base.chir = chir; base.d = d;
       base.s = c->s + (size_t)q * c->Cpmax;
       base.perm = c->perm + (size_t)q * c->Cpmax;
-      base.counter = c->counters + q;
+      attach_jacobi(c, base, q);
       base.result = c->results + (size_t)RESULT_STRIDE * q;
       base.chi_max = c->chi_max;
       base.drop_rel = 1e-13;
@@ -459,7 +605,8 @@ int device_svd(tebd_ctx* c, zc* W, int R, int C, zc* V, double* s, int* perm, do
   std::memset(&t, 0, sizeof(t));
   t.W = W; t.R = R; t.C = C; t.Cpad = round_up(C, JC); t.P = t.Cpad / JC;
   t.V = V; t.NV = t.Cpad; t.s = s; t.perm = perm;
-  t.counter = c->counters; t.result = c->results; t.chi_max = C; t.lc_out = lam_scratch;
+  attach_jacobi(c, t, 0);
+  t.result = c->results; t.chi_max = C; t.lc_out = lam_scratch;
   HIPCHK(c, hipMemcpyAsync(c->d_tasks, c->h_tasks, sizeof(BondTask), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, launch_prepare_svd(c->d_tasks, 1, c->st));
   HIPCHK(c, launch_frobenius(c->d_tasks, 1, t.Cpad, c->st, 0));
@@ -621,6 +768,16 @@ void free_all(tebd_ctx* c) {
   (void)hipFree(c->s);
   (void)hipFree(c->perm);
   (void)hipFree(c->counters);
+  (void)hipFree(c->blkver);
+  (void)hipFree(c->X);
+  (void)hipFree(c->Yr);
+  (void)hipFree(c->Zc);
+  (void)hipFree(c->qnrm);
+  (void)hipFree(c->qpiv);
+  (void)hipFree(c->tau);
+  (void)hipFree(c->qstate);
+  (void)hipHostFree(c->h_qstate);
+  (void)hipFree(c->pairseen);
   (void)hipFree(c->results);
   (void)hipFree(c->d_tasks);
   (void)hipFree(c->cf);
@@ -685,7 +842,18 @@ int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, u
   if (bad(hipMalloc(&c->V, sizeof(zc) * (size_t)c->Cpmax * c->Cpmax * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->s, sizeof(double) * (size_t)c->Cpmax * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->perm, sizeof(int) * (size_t)c->Cpmax * nt))) return TEBD_ERR_HIP;
-  if (bad(hipMalloc(&c->counters, sizeof(int) * nt))) return TEBD_ERR_HIP;
+  c->nblk_max = c->Cpmax / JB;
+  if (bad(hipMalloc(&c->X, sizeof(zc) * (size_t)c->Cpmax * c->Cpmax * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->Yr, sizeof(zc) * (size_t)c->Rmax * chi_max * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->Zc, sizeof(zc) * (size_t)c->Cpmax * chi_max * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->qnrm, sizeof(double) * (size_t)c->Cpmax * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->qpiv, sizeof(int) * (size_t)c->Cpmax * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->tau, sizeof(zc) * (size_t)c->Cpmax * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->qstate, sizeof(double) * 2 * nt))) return TEBD_ERR_HIP;
+  if (bad(hipHostMalloc(&c->h_qstate, sizeof(double) * 2 * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->counters, sizeof(int) * 4 * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->blkver, sizeof(unsigned) * c->nblk_max * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->pairseen, sizeof(unsigned long long) * c->nblk_max * c->nblk_max * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->results, sizeof(double) * RESULT_STRIDE * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->d_tasks, sizeof(BondTask) * nt * kStages))) return TEBD_ERR_HIP;
   if (bad(hipHostMalloc(&c->h_tasks, sizeof(BondTask) * nt * kStages))) return TEBD_ERR_HIP;
@@ -700,7 +868,7 @@ int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, u
   }
   c->last_rank.assign(L + 1, 0);
   if (bad(hipHostMalloc(&c->h_results, sizeof(double) * RESULT_STRIDE * nt))) return TEBD_ERR_HIP;
-  if (bad(hipHostMalloc(&c->h_counters, sizeof(int) * nt))) return TEBD_ERR_HIP;
+  if (bad(hipHostMalloc(&c->h_counters, sizeof(int) * 4 * nt))) return TEBD_ERR_HIP;
   *out = c;
   return TEBD_OK;
 }
@@ -853,7 +1021,7 @@ int tebd_get_bond_dims(tebd_ctx* c, int* chi) {
 
 int tebd_get_stats(tebd_ctx* c, double* stats, int n) {
   if (!c || !stats || n < 0) return TEBD_ERR_ARG;
-  for (int i = 0; i < n && i < 8; ++i) stats[i] = c->stats[i];
+  for (int i = 0; i < n && i < kNumStats; ++i) stats[i] = c->stats[i];
   return TEBD_OK;
 }
 
@@ -870,7 +1038,7 @@ int tebd_get_profile(tebd_ctx* c, double* out, int n) {
   if (!c || !out || n < 0) return TEBD_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->st));
   drain_profile(c);
-  for (int i = 0; i < n && i < 9; ++i) out[i] = c->prof[i];
+  for (int i = 0; i < n && i < kNumProf; ++i) out[i] = c->prof[i];
   return TEBD_OK;
 }
 

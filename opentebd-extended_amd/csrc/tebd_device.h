@@ -77,7 +77,9 @@ struct BondTask {
   int NV;
   double* s;         // Cpad: sorted singular values (descending)
   int* perm;         // Cpad: original column of each sorted value
-  int* counter;      // rotations in the current sweep
+  int* counter;      // Jacobi counters: [0..1] rotated pairs per sweep parity, [2] Gram visits, [3] rotations
+  unsigned* blkver;  // 2P: rotations of each column block in the current solve
+  unsigned long long* pairseen;  // (2P)^2: block versions when a pair was last verified orthogonal
   double* result;    // [0]=kept k, [1]=truncerr, [2]=max column |w|^2 (Jacobi floor), [3]=||theta||^2 when
                      // W is a sketch (else <= 0), [4]=weight captured by W, [5]=smallest kept s^2
   // outputs
@@ -87,4 +89,15 @@ struct BondTask {
   double* lc_commit; // when set, k_write_gammas copies lc_out here (sketched path)
   double drop_rel;   // k_normalize_cols zeroes columns with |w| <= drop_rel * max |w|
   int chi_max;
+  // column-pivoted QR preconditioning (qr = 1): Wq (Rb x Cb) holds the bond matrix and then its
+  // Householder QR; the Jacobi (W, R, C above) runs on X = R(0:r, :)^dagger (Cb x r)
+  int qr;
+  int Rb, Cb;
+  zc* Wq;
+  double* qnrm;      // Cb remaining column norms^2
+  int* qpiv;         // Cb column permutation
+  zc* tau;           // Cb reflector scalars
+  double* qstate;    // [0] numerical rank (-1 while running), [1] stop threshold on |col|^2
+  zc* Yr;            // Rb x chi_max: Rb-side singular vectors of the kept values, sorted
+  zc* Zc;            // Cb x chi_max: Cb-side singular vectors of the kept values, sorted
 };

@@ -1,4 +1,5 @@
-// Host-side launchers of the gfx950 kernels (tebd_kernels.hip).
+// Host-side launchers of the gfx950 kernels (tebd_kernels.hip, tebd_jacobi.hip, tebd_qrp.hip,
+// tebd_sketch.hip, tebd_canonical.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include "tebd_device.h"
@@ -6,9 +7,14 @@
 hipError_t launch_theta_gate(const BondTask* d_tasks, int ntasks, int d, int max_chil, int max_chir, hipStream_t st);
 hipError_t launch_prepare_svd(const BondTask* d_tasks, int ntasks, hipStream_t st);
 hipError_t launch_frobenius(const BondTask* d_tasks, int ntasks, int max_cpad, hipStream_t st, int keep_sum);
-hipError_t launch_reset_counters(const BondTask* d_tasks, int ntasks, hipStream_t st);
-hipError_t launch_jacobi_round(const BondTask* d_tasks, int ntasks, int max_pairs, int round, double tol_rel,
-                               double tol_abs, hipStream_t st);
+hipError_t launch_jacobi_begin(const BondTask* d_tasks, int ntasks, hipStream_t st);
+hipError_t launch_jacobi_sweep_begin(const BondTask* d_tasks, int ntasks, int sweep, hipStream_t st);
+hipError_t launch_jacobi_round(const BondTask* d_tasks, int ntasks, int max_pairs, int sweep, int round,
+                               double tol_rel, double tol_abs, hipStream_t st);
+hipError_t launch_qrp_begin(const BondTask* d_tasks, int ntasks, double tol_rel, hipStream_t st);
+hipError_t launch_qrp_step(const BondTask* d_tasks, int ntasks, int max_cb, int j, hipStream_t st);
+hipError_t launch_qrp_build_x(const BondTask* d_tasks, int ntasks, hipStream_t st);
+hipError_t launch_qrp_vectors(const BondTask* d_tasks, int ntasks, int max_rb, int max_k, hipStream_t st);
 hipError_t launch_finalize(const BondTask* d_tasks, int ntasks, int max_cpad, double cutoff, hipStream_t st);
 hipError_t launch_write_gammas(const BondTask* d_tasks, int ntasks, hipStream_t st);
 hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_n, bool conj_a, hipStream_t st);

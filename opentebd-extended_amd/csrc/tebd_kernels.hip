@@ -4,10 +4,6 @@
 //                  fused: theta'(a,i,j,b) = ll(a) lr(b) sum_kl U(ij,kl) sum_g Gl(a,k,g) lc(g) Gr(g,l,b)
 //                  written straight into the SVD work matrix W (theta, or theta^dagger when
 //                  the bond has more columns than rows).
-//   k_jacobi_round one round of a one-sided block-Jacobi SVD: every workgroup takes one pair of
-//                  JB-column blocks, forms their JC x JC Gram matrix, diagonalises it in LDS with
-//                  cyclic Jacobi, and rotates the JC columns of W and V.  Replaces ZGESVD
-//                  (local_operations_module.f90:322-345).
 //   k_col_norms    squared column norms of W.
 //   k_truncate     sort descending, choose the kept rank, lambda and truncerr
 //                  (FormLambda1, local_operations_module.f90:347-379).
@@ -127,7 +123,7 @@ __global__ __launch_bounds__(256) void k_theta_gate(const BondTask* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// W padding columns := 0, V := identity
+// W padding columns := 0, V := identity, no ||theta||^2 reference
 // ---------------------------------------------------------------------------
 __global__ void k_prepare_svd(const BondTask* __restrict__ tasks) {
   const BondTask t = tasks[blockIdx.y];
@@ -141,211 +137,7 @@ __global__ void k_prepare_svd(const BondTask* __restrict__ tasks) {
       t.W[(size_t)t.R * t.C + (e - nV)] = zmk(0.0, 0.0);
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *t.counter = 0;
-    t.result[3] = -1.0;
-  }
-}
-
-__global__ void k_reset_counters(const BondTask* __restrict__ tasks, int ntasks) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ntasks) *tasks[i].counter = 0;
-}
-
-// ---------------------------------------------------------------------------
-// one-sided block Jacobi
-// ---------------------------------------------------------------------------
-// Round-robin ("circle") schedule over nb = 2P blocks: block 0 stays, the others
-// rotate; over nb-1 rounds every pair of blocks meets exactly once.
-__device__ __forceinline__ void rr_pair(int nb, int round, int p, int& bi, int& bj) {
-  auto pos = [&](int k) { return k == 0 ? 0 : 1 + (k - 1 + round) % (nb - 1); };
-  bi = pos(p);
-  bj = pos(nb - 1 - p);
-}
-
-// Hermitian 2x2 Jacobi rotation J = [[c, s w], [-s conj(w), c]], w = g/|g|, that
-// zeroes g in [[a, g], [conj(g), b]] via J^dagger H J.
-__device__ __forceinline__ void jrot(double a, double b, zc g, double& c, double& s, zc& w) {
-  const double ag = sqrt(zabs2(g));
-  if (ag == 0.0) { c = 1.0; s = 0.0; w = zmk(1.0, 0.0); return; }
-  w = zmk(g.x / ag, g.y / ag);
-  const double zeta = (b - a) / (2.0 * ag);
-  const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-  c = 1.0 / sqrt(1.0 + tt * tt);
-  s = tt * c;
-}
-
-constexpr int RC = 64;  // rows of W / V streamed through LDS per chunk
-
-// A pair is left alone when every off-diagonal Gram entry satisfies
-// |A_pq| <= tol_rel sqrt(A_pp A_qq)  or
-// |A_pq| <= tol_abs max_c |w_c| (sqrt(A_pp) + sqrt(A_qq)).
-// The second bound is the rounding floor: rotations add errors of order
-// eps * sigma_max to every column they touch, so inner products below it are
-// noise.  It scales with the two column norms, which keeps the small singular
-// values resolved to absolute accuracy ~eps * sigma_max (as LAPACK's are).
-__global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict__ tasks, int round, double tol_rel,
-                                                       double tol_abs) {
-  const BondTask t = tasks[blockIdx.y];
-  const int p = blockIdx.x;
-  if (p >= t.P || round >= 2 * t.P - 1) return;
-  int bi, bj;
-  rr_pair(2 * t.P, round, p, bi, bj);
-
-  __shared__ zc A[JC][JC + 1];
-  __shared__ zc Q[JC][JC + 1];
-  __shared__ zc buf[RC][JC + 1];
-  __shared__ double rc_c[JC / 2], rc_s[JC / 2];
-  __shared__ zc rc_w[JC / 2];
-  __shared__ int rc_p[JC / 2], rc_q[JC / 2];
-  __shared__ int s_flag;
-
-  const int tid = threadIdx.x;
-  auto colidx = [&](int c) { return (c < JB ? bi * JB + c : bj * JB + (c - JB)); };
-
-  // ---- Gram of the JC columns
-  {
-    const int gp = tid >> 3, gq0 = (tid & 7) * 4;
-    zc acc[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] = zmk(0.0, 0.0);
-    for (int r0 = 0; r0 < t.R; r0 += RC) {
-      for (int e = tid; e < RC * JC; e += 256) {
-        const int rr = e % RC, c = e / RC;
-        const int r = r0 + rr;
-        buf[rr][c] = (r < t.R) ? t.W[r + (size_t)t.R * colidx(c)] : zmk(0.0, 0.0);
-      }
-      __syncthreads();
-#pragma unroll 4
-      for (int rr = 0; rr < RC; ++rr) {
-        const zc a = buf[rr][gp];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) zfmac(acc[q], a, buf[rr][gq0 + q]);
-      }
-      __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) A[gp][gq0 + q] = acc[q];
-  }
-  __syncthreads();
-
-  // ---- does this pair need a rotation?  |A_pq| > tol_rel sqrt(A_pp A_qq) and > tol_abs
-  {
-    const double fnorm = sqrt(t.result[2]);  // largest column norm
-    int need = 0;
-    for (int e = tid; e < JC * JC; e += 256) {
-      const int pp = e % JC, qq = e / JC;
-      if (pp == qq) continue;
-      const double g2 = zabs2(A[pp][qq]);
-      const double app = fmax(A[pp][pp].x, 0.0), aqq = fmax(A[qq][qq].x, 0.0);
-      const double fl = tol_abs * fnorm * (sqrt(app) + sqrt(aqq));
-      if (g2 > tol_rel * tol_rel * app * aqq && g2 > fl * fl) need = 1;
-    }
-    if (tid == 0) s_flag = 0;
-    __syncthreads();
-    if (need) s_flag = 1;
-    __syncthreads();
-    if (!s_flag) return;
-  }
-  if (tid == 0) atomicAdd(t.counter, 1);
-
-  // ---- cyclic Jacobi on A (JC x JC Hermitian) in LDS, Q accumulates rotations.
-  // A Gram entry is accurate relative to sqrt(A_pp A_qq), so the inner solve
-  // uses a purely relative test (skip when |A_pq| <= 1e-15 sqrt(A_pp A_qq)),
-  // sets each annihilated entry to exactly zero, and ends after a sweep without
-  // rotations.  The pair's columns keep their block (re-sorting columns between
-  // blocks stalls the convergence of the cyclic block sweep).
-  for (int e = tid; e < JC * JC; e += 256) {
-    const int r = e % JC, c = e / JC;
-    Q[r][c] = zmk(r == c ? 1.0 : 0.0, 0.0);
-  }
-  __shared__ int s_rot;
-  for (int sweep = 0; sweep < 30; ++sweep) {
-    if (tid == 0) s_rot = 0;
-    __syncthreads();
-    for (int rd = 0; rd < JC - 1; ++rd) {
-      if (tid < JC / 2) {
-        int pp, qq;
-        rr_pair(JC, rd, tid, pp, qq);
-        if (pp > qq) { const int tmp = pp; pp = qq; qq = tmp; }
-        double c, s;
-        zc w;
-        const zc g = A[pp][qq];
-        const double a = A[pp][pp].x, b = A[qq][qq].x;
-        const double g2 = zabs2(g);
-        if (g2 <= 1e-30 * fabs(a * b) || g2 == 0.0) { c = 1.0; s = 0.0; w = zmk(1.0, 0.0); }
-        else { jrot(a, b, g, c, s, w); s_rot = 1; }
-        rc_c[tid] = c; rc_s[tid] = s; rc_w[tid] = w; rc_p[tid] = pp; rc_q[tid] = qq;
-      }
-      __syncthreads();
-      // rows: A_p <- c A_p - s w A_q ; A_q <- s conj(w) A_p + c A_q
-      for (int e = tid; e < (JC / 2) * JC; e += 256) {
-        const int pr = e / JC, k = e % JC;
-        const double c = rc_c[pr], s = rc_s[pr];
-        if (s == 0.0) continue;
-        const zc w = rc_w[pr];
-        const int pp = rc_p[pr], qq = rc_q[pr];
-        const zc xp = A[pp][k], xq = A[qq][k];
-        const zc sw = zscale(w, s), swc = zscale(zconj(w), s);
-        A[pp][k] = zsub(zscale(xp, c), zmul(sw, xq));
-        A[qq][k] = zadd(zmul(swc, xp), zscale(xq, c));
-      }
-      __syncthreads();
-      // cols: A_:p <- c A_:p - s conj(w) A_:q ; A_:q <- s w A_:p + c A_:q  (same for Q)
-      for (int e = tid; e < (JC / 2) * JC; e += 256) {
-        const int pr = e / JC, k = e % JC;
-        const double c = rc_c[pr], s = rc_s[pr];
-        if (s == 0.0) continue;
-        const zc w = rc_w[pr];
-        const int pp = rc_p[pr], qq = rc_q[pr];
-        const zc sw = zscale(w, s), swc = zscale(zconj(w), s);
-        zc xp = A[k][pp], xq = A[k][qq];
-        A[k][pp] = zsub(zscale(xp, c), zmul(swc, xq));
-        A[k][qq] = zadd(zmul(sw, xp), zscale(xq, c));
-        xp = Q[k][pp]; xq = Q[k][qq];
-        Q[k][pp] = zsub(zscale(xp, c), zmul(swc, xq));
-        Q[k][qq] = zadd(zmul(sw, xp), zscale(xq, c));
-      }
-      __syncthreads();
-      if (tid < JC / 2 && rc_s[tid] != 0.0) {
-        A[rc_p[tid]][rc_q[tid]] = zmk(0.0, 0.0);
-        A[rc_q[tid]][rc_p[tid]] = zmk(0.0, 0.0);
-      }
-      __syncthreads();
-    }
-    if (!s_rot) break;
-    __syncthreads();
-  }
-
-  // ---- rotate the JC columns of W and of V by Q
-  auto rotate = [&](zc* M, int rows, int ld) {
-    const int orow = tid >> 2, oc0 = (tid & 3) * 8;
-    for (int r0 = 0; r0 < rows; r0 += RC) {
-      for (int e = tid; e < RC * JC; e += 256) {
-        const int rr = e % RC, c = e / RC;
-        const int r = r0 + rr;
-        buf[rr][c] = (r < rows) ? M[r + (size_t)ld * colidx(c)] : zmk(0.0, 0.0);
-      }
-      __syncthreads();
-      zc o[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) o[q] = zmk(0.0, 0.0);
-#pragma unroll 4
-      for (int k = 0; k < JC; ++k) {
-        const zc x = buf[orow][k];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) zfma(o[q], x, Q[k][oc0 + q]);
-      }
-      const int r = r0 + orow;
-      if (r < rows) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) M[r + (size_t)ld * colidx(oc0 + q)] = o[q];
-      }
-      __syncthreads();
-    }
-  };
-  rotate(t.W, t.R, t.R);
-  if (t.V) rotate(t.V, t.NV, t.NV);
+  if (blockIdx.x == 0 && threadIdx.x == 0) t.result[3] = -1.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -465,7 +257,9 @@ __global__ __launch_bounds__(1024) void k_truncate(const BondTask* __restrict__ 
   }
 }
 
-// Gamma_l(a,i,b) = U((i,a), b)/ll(a);  Gamma_r(g,j,b) = conj(V((j,b), g))/lr(b)
+// Gamma_l(a,i,b) = U((i,a), b)/ll(a);  Gamma_r(g,j,b) = conj(V((j,b), g))/lr(b), with U and V
+// the left / right singular vectors of theta: from W / s and the accumulated V of the
+// Jacobi, or with QR preconditioning from Yr and Zc (already sorted and normalised)
 __global__ void k_write_gammas(const BondTask* __restrict__ tasks) {
   const BondTask t = tasks[blockIdx.y];
   const int k = (int)t.result[0];
@@ -483,7 +277,8 @@ __global__ void k_write_gammas(const BondTask* __restrict__ tasks) {
       const size_t row = (size_t)i * t.chil + a;
       const int col = t.perm[b];
       zc u;
-      if (!t.transposed) u = zscale(t.W[row + (size_t)t.R * col], t.s[b] > 0.0 ? 1.0 / t.s[b] : 0.0);
+      if (t.qr) u = !t.transposed ? t.Yr[row + (size_t)t.Rb * b] : t.Zc[row + (size_t)t.Cb * b];
+      else if (!t.transposed) u = zscale(t.W[row + (size_t)t.R * col], t.s[b] > 0.0 ? 1.0 / t.s[b] : 0.0);
       else u = t.V[row + (size_t)t.NV * col];
       const double la = t.ll[a];
       t.gl_out[e] = fabs(la) > 1e-15 ? zscale(u, 1.0 / la) : zmk(0.0, 0.0);
@@ -496,7 +291,8 @@ __global__ void k_write_gammas(const BondTask* __restrict__ tasks) {
       const size_t row = (size_t)j * t.chir + b;
       const int col = t.perm[g];
       zc v;
-      if (!t.transposed) v = t.V[row + (size_t)t.NV * col];
+      if (t.qr) v = !t.transposed ? t.Zc[row + (size_t)t.Cb * g] : t.Yr[row + (size_t)t.Rb * g];
+      else if (!t.transposed) v = t.V[row + (size_t)t.NV * col];
       else v = zscale(t.W[row + (size_t)t.R * col], t.s[g] > 0.0 ? 1.0 / t.s[g] : 0.0);
       const double lb = t.lr[b];
       t.gr_out[f] = fabs(lb) > 1e-15 ? zscale(zconj(v), 1.0 / lb) : zmk(0.0, 0.0);
@@ -532,22 +328,11 @@ hipError_t launch_prepare_svd(const BondTask* d_tasks, int ntasks, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_reset_counters(const BondTask* d_tasks, int ntasks, hipStream_t st) {
-  hipLaunchKernelGGL(k_reset_counters, dim3((ntasks + 255) / 256), dim3(256), 0, st, d_tasks, ntasks);
-  return hipGetLastError();
-}
-
 hipError_t launch_frobenius(const BondTask* d_tasks, int ntasks, int max_cpad, hipStream_t st, int keep_sum) {
   hipLaunchKernelGGL(k_col_norms, dim3((max_cpad + 3) / 4, ntasks), dim3(256), 0, st, d_tasks);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_frob, dim3(ntasks), dim3(256), 0, st, d_tasks, keep_sum);
-  return hipGetLastError();
-}
-
-hipError_t launch_jacobi_round(const BondTask* d_tasks, int ntasks, int max_pairs, int round, double tol_rel,
-                               double tol_abs, hipStream_t st) {
-  hipLaunchKernelGGL(k_jacobi_round, dim3(max_pairs, ntasks), dim3(256), 0, st, d_tasks, round, tol_rel, tol_abs);
   return hipGetLastError();
 }
 

@@ -148,20 +148,22 @@ class Context:
         return out
 
     def stats(self):
-        out = np.zeros(8)
-        self._check(self._lib.tebd_get_stats(self._h, _p(out), 8))
+        out = np.zeros(11)
+        self._check(self._lib.tebd_get_stats(self._h, _p(out), 11))
         return {"bond_updates": out[0], "jacobi_sweeps": out[1], "fallbacks": out[2], "device_ms": out[3],
-                "sweeps_z0": out[4], "sweeps_resid": out[5], "sweeps_reorth": out[6], "sweeps_rr": out[7]}
+                "sweeps_z0": out[4], "sweeps_resid": out[5], "sweeps_reorth": out[6], "sweeps_rr": out[7],
+                "pair_grams": out[8], "pair_rotations": out[9], "pair_visits": out[10]}
 
     def set_profiling(self, on=True):
         self._check(self._lib.tebd_set_profiling(self._h, int(bool(on))))
 
     def profile(self):
-        """{kernel: (launches, milliseconds, algorithmic flops)} since set_profiling(True)."""
-        out = np.zeros(9)
-        self._check(self._lib.tebd_get_profile(self._h, _p(out), 9))
-        names = ("theta_gate", "jacobi_round", "truncate_gamma")
-        return {n: (out[3 * i], out[3 * i + 1], out[3 * i + 2]) for i, n in enumerate(names)}
+        """{kernel: (launches, milliseconds, algorithmic flops, algorithmic HBM bytes)} since
+        set_profiling(True)."""
+        out = np.zeros(16)
+        self._check(self._lib.tebd_get_profile(self._h, _p(out), 16))
+        names = ("theta_gate", "jacobi_round", "truncate_gamma", "qr_step")
+        return {n: tuple(out[4 * i:4 * i + 4]) for i, n in enumerate(names)}
 
     def sync(self):
         self._check(self._lib.tebd_sync(self._h))
