@@ -19,7 +19,8 @@ constexpr int kStages = 5;
 constexpr int kNumStats = 11;
 constexpr int kNumProf = 16;
 constexpr double kQrTol = 1e-17;  // QR stops when every remaining column norm is <= kQrTol ||theta||_F
-constexpr int kQrCheck = 32;      // QR steps between checks whether every task has stopped
+constexpr int kQrPanel = 32;      // panel width of the blocked pivoted QR (tebd_qrp.hip)
+constexpr int kQrCheck = 4;       // panels between checks whether every task has stopped
 constexpr int kQrMinCols = 64;    // bond matrices with fewer columns go straight to the Jacobi
 constexpr int kMaxSweeps = 40;
 constexpr double kTolRel = 1e-13;   // relative orthogonality of converged column pairs
@@ -54,6 +55,7 @@ struct tebd_ctx {
   int* qpiv = nullptr;           // Cpmax
   zc* tau = nullptr;             // Cpmax
   double* qstate = nullptr;      // 2
+  zc* Tq = nullptr;              // 32 x 32
   double* h_qstate = nullptr;    // pinned, 2 per task slot
   double* s = nullptr;
   int* perm = nullptr;
@@ -61,6 +63,10 @@ struct tebd_ctx {
   unsigned* blkver = nullptr;    // nblk_max per task slot
   unsigned long long* pairseen = nullptr;  // nblk_max^2 per task slot
   int nblk_max = 0;
+  zc* Qg = nullptr;              // per task slot: (nblk_max / 2) pair rotations of 32 x 32
+  int* wl_count = nullptr;       // 2: rotation queue lengths by round parity
+  int2* wl_items = nullptr;      // 2 x wl_cap queued (task, pair)
+  int wl_cap = 0;
   double* results = nullptr;
   BondTask* d_tasks = nullptr;   // kStages x ntask_max
   BondTask* h_tasks = nullptr;   // pinned, kStages x ntask_max
@@ -166,19 +172,26 @@ void attach_jacobi(tebd_ctx* c, BondTask& t, int slot) {
   t.counter = c->counters + 4 * slot;
   t.blkver = c->blkver + (size_t)slot * c->nblk_max;
   t.pairseen = c->pairseen + (size_t)slot * c->nblk_max * c->nblk_max;
+  t.Qg = c->Qg + (size_t)slot * (c->nblk_max / 2) * JC * JC;
 }
 
 // Jacobi sweeps over the tasks of one stage (task i uses slot i) until a sweep
 // rotates nothing.
 int jacobi_sweeps(tebd_ctx* c, BondTask* d_t, const BondTask* h_t, int nb, int Pmax) {
   HIPCHK(c, launch_jacobi_begin(d_t, nb, c->st));
+  int rows_w = 1, rows_v = 0;
+  for (int i = 0; i < nb; ++i) {
+    rows_w = std::max(rows_w, h_t[i].R);
+    if (h_t[i].V) rows_v = std::max(rows_v, h_t[i].NV);
+  }
   int sweeps = 0;
   bool converged = false;
   while (sweeps < kMaxSweeps && !converged) {
-    HIPCHK(c, launch_jacobi_sweep_begin(d_t, nb, sweeps, c->st));
+    HIPCHK(c, launch_jacobi_sweep_begin(d_t, nb, sweeps, c->wl_count, c->st));
     for (int r = 0; r < 2 * Pmax - 1; ++r) {
       ProfScope ps(c, 1);
-      HIPCHK(c, launch_jacobi_round(d_t, nb, Pmax, sweeps, r, kTolRel, kTolAbs, c->st));
+      HIPCHK(c, launch_jacobi_round(d_t, nb, Pmax, sweeps, r, kTolRel, kTolAbs, rows_w, rows_v, c->wl_count,
+                                    c->wl_items, c->wl_cap, c->st));
     }
     HIPCHK(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(int) * 4 * nb, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -216,6 +229,7 @@ void attach_qr(tebd_ctx* c, BondTask& t, int slot) {
   t.qpiv = c->qpiv + (size_t)slot * c->Cpmax;
   t.tau = c->tau + (size_t)slot * c->Cpmax;
   t.qstate = c->qstate + 2 * slot;
+  t.Tq = c->Tq + (size_t)slot * kQrPanel * kQrPanel;
   t.Yr = c->Yr + (size_t)slot * c->Rmax * c->chi_max;
   t.Zc = c->Zc + (size_t)slot * c->Cpmax * c->chi_max;
 }
@@ -224,30 +238,36 @@ void attach_qr(tebd_ctx* c, BondTask& t, int slot) {
 // numerical rank.  Returns the ranks in rank[].
 int qrp_factor(tebd_ctx* c, int nb, int max_cb, std::vector<int>& rank) {
   HIPCHK(c, launch_qrp_begin(c->d_tasks, nb, kQrTol, c->st));
+  // every task's last panel (j0 + width >= min(Rb, Cb)) records its rank, so after the
+  // panel with j0 + kQrPanel >= max_cb all tasks have finished
   bool done = false;
-  for (int j = 0; j <= max_cb && !done; ++j) {
+  for (int j0 = 0, panel = 0; !done; j0 += kQrPanel, ++panel) {
     {
       ProfScope ps(c, 3);
-      HIPCHK(c, launch_qrp_step(c->d_tasks, nb, max_cb, j, c->st));
+      HIPCHK(c, launch_qrp_panel(c->d_tasks, nb, max_cb, j0, c->st));
     }
-    if ((j + 1) % kQrCheck == 0 || j == max_cb) {
+    const bool last = j0 + kQrPanel >= max_cb;
+    if ((panel + 1) % kQrCheck == 0 || last) {
       HIPCHK(c, hipMemcpyAsync(c->h_qstate, c->qstate, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->st));
       HIPCHK(c, hipStreamSynchronize(c->st));
       done = true;
       for (int i = 0; i < nb; ++i) done = done && c->h_qstate[2 * i] >= 0.0;
     }
+    if (last && !done) return fail(c, TEBD_ERR_STATE, "pivoted QR did not finish");
   }
   rank.resize(nb);
   for (int i = 0; i < nb; ++i) {
     rank[i] = (int)c->h_qstate[2 * i];
     if (c->profiling) {
-      // step j: the trailing (Rb - j) x (Cb - j - 1) block is read and written once,
-      // two complex MACs per element
+      // per panel j0 of width w: the trailing (Rb - j0) x (Cb - j0 - w) block is read twice
+      // (Y = V^H A2, then A2 -= V Z) and written once, 2 x 32 complex MACs per element;
+      // the panel itself is read and written about w times
       const BondTask& t = c->h_tasks[i];
-      for (int j = 0; j < rank[i]; ++j) {
-        const double el = (double)(t.Rb - j) * (t.Cb - j - 1);
-        c->prof[14] += 16.0 * el;
-        c->prof[15] += 32.0 * el + 16.0 * (t.Rb - j);
+      for (int j0 = 0; j0 < rank[i]; j0 += kQrPanel) {
+        const int w = std::min(kQrPanel, rank[i] - j0);
+        const double rows = t.Rb - j0, trail = std::max(0, t.Cb - j0 - w);
+        c->prof[14] += 2.0 * 8.0 * kQrPanel * rows * trail + 8.0 * rows * w * w;
+        c->prof[15] += 48.0 * rows * trail + 32.0 * rows * w;
       }
     }
   }
@@ -776,8 +796,12 @@ void free_all(tebd_ctx* c) {
   (void)hipFree(c->qpiv);
   (void)hipFree(c->tau);
   (void)hipFree(c->qstate);
+  (void)hipFree(c->Tq);
   (void)hipHostFree(c->h_qstate);
   (void)hipFree(c->pairseen);
+  (void)hipFree(c->Qg);
+  (void)hipFree(c->wl_count);
+  (void)hipFree(c->wl_items);
   (void)hipFree(c->results);
   (void)hipFree(c->d_tasks);
   (void)hipFree(c->cf);
@@ -850,10 +874,15 @@ int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, u
   if (bad(hipMalloc(&c->qpiv, sizeof(int) * (size_t)c->Cpmax * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->tau, sizeof(zc) * (size_t)c->Cpmax * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->qstate, sizeof(double) * 2 * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->Tq, sizeof(zc) * kQrPanel * kQrPanel * nt))) return TEBD_ERR_HIP;
   if (bad(hipHostMalloc(&c->h_qstate, sizeof(double) * 2 * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->counters, sizeof(int) * 4 * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->blkver, sizeof(unsigned) * c->nblk_max * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->pairseen, sizeof(unsigned long long) * c->nblk_max * c->nblk_max * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->Qg, sizeof(zc) * (size_t)(c->nblk_max / 2) * JC * JC * nt))) return TEBD_ERR_HIP;
+  c->wl_cap = nt * (c->nblk_max / 2);
+  if (bad(hipMalloc(&c->wl_count, sizeof(int) * 2))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->wl_items, sizeof(int2) * 2 * c->wl_cap))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->results, sizeof(double) * RESULT_STRIDE * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->d_tasks, sizeof(BondTask) * nt * kStages))) return TEBD_ERR_HIP;
   if (bad(hipHostMalloc(&c->h_tasks, sizeof(BondTask) * nt * kStages))) return TEBD_ERR_HIP;

@@ -80,6 +80,7 @@ struct BondTask {
   int* counter;      // Jacobi counters: [0..1] rotated pairs per sweep parity, [2] Gram visits, [3] rotations
   unsigned* blkver;  // 2P: rotations of each column block in the current solve
   unsigned long long* pairseen;  // (2P)^2: block versions when a pair was last verified orthogonal
+  zc* Qg;            // P x 32 x 32: each pair's rotation of the current round, row-major
   double* result;    // [0]=kept k, [1]=truncerr, [2]=max column |w|^2 (Jacobi floor), [3]=||theta||^2 when
                      // W is a sketch (else <= 0), [4]=weight captured by W, [5]=smallest kept s^2
   // outputs
@@ -98,6 +99,7 @@ struct BondTask {
   int* qpiv;         // Cb column permutation
   zc* tau;           // Cb reflector scalars
   double* qstate;    // [0] numerical rank (-1 while running), [1] stop threshold on |col|^2
+  zc* Tq;            // 32 x 32 compact-WY factor of the current panel
   zc* Yr;            // Rb x chi_max: Rb-side singular vectors of the kept values, sorted
   zc* Zc;            // Cb x chi_max: Cb-side singular vectors of the kept values, sorted
 };

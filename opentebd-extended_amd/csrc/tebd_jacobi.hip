@@ -3,13 +3,14 @@
 //
 // W (R x Cpad, column-major) is orthogonalised in place and V accumulates the right
 // rotations.  The columns form 2P blocks of JB; a round pairs every block with one
-// other (round-robin "circle" order, 2P-1 rounds per sweep) and one workgroup per
-// pair
-//   1. forms the pair's Gram matrix with FP64 MFMA (v_mfma_f64_16x16x4f64),
-//   2. decides whether the pair needs a rotation,
-//   3. diagonalises the 32x32 Gram by cyclic two-sided Jacobi in LDS -> Q,
-//   4. rotates the 32 columns of W and V by Q, again on MFMA, with Q held in
-//      registers and W / V streamed through with coalesced 256-byte segments.
+// other (round-robin "circle" order, 2P-1 rounds per sweep).  A round is two kernels:
+//   k_jacobi_round   one workgroup per pair: the pair's Gram matrix on FP64 MFMA
+//                    (v_mfma_f64_16x16x4f64), the decision whether it needs a
+//                    rotation, and the 32x32 Hermitian eigenproblem by cyclic Jacobi
+//                    in LDS -> Q (32x32, to global memory); rotated pairs are queued;
+//   k_jacobi_rotate  [W; V](:, pair) <- [W; V](:, pair) Q for every queued pair, on
+//                    MFMA, spread over 64-row slabs so a rotation costs one memory
+//                    round trip of the whole GPU instead of one workgroup's stream.
 // After a block has been part of a rotated (or verified) pair its columns are
 // mutually orthogonal, so after the first round of a sweep only the 16x16 cross
 // block and the 32 column norms are formed (4x less Gram work than the full 32x32).
@@ -47,23 +48,49 @@ __device__ __forceinline__ void rr_pair(int nb, int round, int p, int& bi, int& 
   bj = pos(nb - 1 - p);
 }
 
+// 1/sqrt(x) and 1/x to full FP64 precision: the hardware estimate refined by two
+// Newton-Raphson steps (y <- y (3 - x y^2) / 2, r <- r (2 - x r)).
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * fma(-0.5 * x * y, y, 1.5);
+  return y * fma(-0.5 * x * y, y, 1.5);
+}
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = r * fma(-x, r, 2.0);
+  return r * fma(-x, r, 2.0);
+}
+
 // Hermitian 2x2 Jacobi rotation J = [[c, s w], [-s conj(w), c]], w = g/|g|, that
-// zeroes g in [[a, g], [conj(g), b]] via J^dagger H J.
+// zeroes g in [[a, g], [conj(g), b]] via J^dagger H J (Rutishauser's formulas:
+// zeta = (b - a) / 2|g|, t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), c = 1/sqrt(1+t^2)).
 __device__ __forceinline__ void jrot(double a, double b, zc g, double& c, double& s, zc& w) {
-  const double ag = sqrt(zabs2(g));
-  if (ag == 0.0) { c = 1.0; s = 0.0; w = zmk(1.0, 0.0); return; }
-  w = zmk(g.x / ag, g.y / ag);
-  const double zeta = (b - a) / (2.0 * ag);
-  const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-  c = 1.0 / sqrt(1.0 + tt * tt);
+  const double g2 = zabs2(g);
+  if (g2 == 0.0) { c = 1.0; s = 0.0; w = zmk(1.0, 0.0); return; }
+  const double inv = rsqrt_nr(g2);
+  w = zmk(g.x * inv, g.y * inv);
+  const double zeta = 0.5 * (b - a) * inv;
+  const double az = fabs(zeta);
+  double tt;
+  if (az > 1e150) {
+    tt = 0.5 / az;
+  } else {
+    const double q = fma(az, az, 1.0);
+    tt = rcp_nr(az + q * rsqrt_nr(q));
+  }
+  if (zeta < 0.0) tt = -tt;
+  c = rsqrt_nr(fma(tt, tt, 1.0));
   s = tt * c;
 }
 
 // Gram block D = X^H Y (16 x 16) of two 16-column blocks over rows [0, R), as
-// per-wave partial sums: wave w takes rows 8*(w + 4 it) .. +7 in two MFMA k-steps
-// of 4 rows.  Also accumulates the lane's partial squared norms of X and Y.
+// per-wave partial sums: wave w takes rows 8 (w + 4 it) .. +7 in two MFMA k-steps
+// of 4 rows, with the loads of four such row groups issued before their MFMAs (the
+// pass is bound by memory latency).  Also accumulates the lane's partial squared
+// norms of X and Y.
 __device__ __forceinline__ void gram_block(const zc* __restrict__ X, const zc* __restrict__ Y, int R, int wave,
                                            int lane, d4& gre, d4& gim, double& nx, double& ny) {
+  constexpr int U = 4;
   const int lr = lane & 15, lk = lane >> 4;
   gre = d4{0.0, 0.0, 0.0, 0.0};
   gim = d4{0.0, 0.0, 0.0, 0.0};
@@ -71,24 +98,26 @@ __device__ __forceinline__ void gram_block(const zc* __restrict__ X, const zc* _
   ny = 0.0;
   const zc* xc = X + (size_t)R * lr;
   const zc* yc = Y + (size_t)R * lr;
-  for (int r0 = 8 * wave; r0 < R; r0 += 32) {
+  for (int r0 = 8 * wave; r0 < R; r0 += 32 * U) {
+    zc a[2 * U], b[2 * U];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int r = r0 + 4 * h + lk;
-      zc a = zmk(0.0, 0.0), b = zmk(0.0, 0.0);
-      if (r < R) { a = xc[r]; b = yc[r]; }
+    for (int u = 0; u < 2 * U; ++u) {
+      const int r = r0 + 32 * (u >> 1) + 4 * (u & 1) + lk;
+      a[u] = r < R ? xc[r] : zmk(0.0, 0.0);
+      b[u] = r < R ? yc[r] : zmk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 2 * U; ++u) {
       // A[p][k] = conj(X[k][p]), B[k][q] = Y[k][q]
-      gre = mfma4(a.x, b.x, gre);
-      gre = mfma4(a.y, b.y, gre);
-      gim = mfma4(a.x, b.y, gim);
-      gim = mfma4(-a.y, b.x, gim);
-      nx += zabs2(a);
-      ny += zabs2(b);
+      gre = mfma4(a[u].x, b[u].x, gre);
+      gre = mfma4(a[u].y, b[u].y, gre);
+      gim = mfma4(a[u].x, b[u].y, gim);
+      gim = mfma4(-a[u].y, b[u].x, gim);
+      nx += zabs2(a[u]);
+      ny += zabs2(b[u]);
     }
   }
 }
-
-constexpr int kInnerSweeps = 30;
 
 // Sums the four waves' partial 16x16 blocks in a fixed order (bitwise reproducible)
 // and stores rows [ro, ro+16) x cols [co, co+16) of A.  With `diag` the block is a
@@ -126,9 +155,104 @@ __device__ __forceinline__ void reduce_block(zc (*A)[JC + 1], double (*red)[2][2
   __syncthreads();
 }
 
+constexpr int kInnerSweeps = 30;
+
+__device__ __forceinline__ void inner_pair(int rd, int t, int& p, int& q) {
+  rr_pair(JC, rd, t, p, q);
+  if (p > q) { const int x = p; p = q; q = x; }
+}
+
+// Cyclic Jacobi on A (JC x JC Hermitian, LDS); Q (LDS) accumulates the rotations.
+// A round rotates 16 disjoint index pairs (round-robin order): threads 0..15
+// compute the rotations, then every thread updates one 2x2 block [p_r, q_r] x
+// [p_k, q_k] of A (J^H A J acts on each such block on its own, so the update is in
+// place) and two (row, pair) groups of Q; blocks whose two pairs both stay put are
+// skipped.  A Gram entry is accurate relative to sqrt(A_pp A_qq), so the solve uses
+// a purely relative test (no rotation when |A_pq| <= 1e-15 sqrt(A_pp A_qq)), sets
+// each annihilated entry to exactly zero and ends after a sweep without rotations.
+__device__ void inner_jacobi(zc (*A)[JC + 1], zc (*Q)[JC + 1], double* rc_c, double* rc_s, zc* rc_w, int* s_round,
+                             int* s_sweep, int tid) {
+  for (int e = tid; e < JC * JC; e += 256) {
+    const int r = e % JC, c = e / JC;
+    Q[r][c] = zmk(r == c ? 1.0 : 0.0, 0.0);
+  }
+  for (int isw = 0; isw < kInnerSweeps; ++isw) {
+    if (tid == 0) s_sweep[isw & 1] = 0;
+    for (int rd = 0; rd < JC - 1; ++rd) {
+      if (tid == 0) s_round[rd & 1] = 0;
+      if (tid < JC / 2) {
+        int pp, qq;
+        inner_pair(rd, tid, pp, qq);
+        double c = 1.0, s = 0.0;
+        zc w = zmk(1.0, 0.0);
+        const zc g = A[pp][qq];
+        const double a = A[pp][pp].x, b = A[qq][qq].x;
+        const double g2 = zabs2(g);
+        if (!(g2 <= 1e-30 * fabs(a * b) || g2 == 0.0)) jrot(a, b, g, c, s, w);
+        rc_c[tid] = c; rc_s[tid] = s; rc_w[tid] = w;
+        if (s != 0.0) {
+          s_round[rd & 1] = 1;
+          s_sweep[isw & 1] = 1;
+        }
+      }
+      __syncthreads();
+      if (!s_round[rd & 1]) continue;
+      {
+        const int tr = tid >> 4, tk = tid & 15;
+        const double sr = rc_s[tr], sk = rc_s[tk];
+        if (sr != 0.0 || sk != 0.0) {
+          int pr, qr, pk, qk;
+          inner_pair(rd, tr, pr, qr);
+          inner_pair(rd, tk, pk, qk);
+          zc a00 = A[pr][pk], a01 = A[pr][qk], a10 = A[qr][pk], a11 = A[qr][qk];
+          if (sr != 0.0) {  // rows: A_p <- c A_p - s w A_q ; A_q <- s conj(w) A_p + c A_q
+            const double c = rc_c[tr];
+            const zc w = rc_w[tr];
+            const zc sw = zscale(w, sr), swc = zscale(zconj(w), sr);
+            const zc n00 = zsub(zscale(a00, c), zmul(sw, a10)), n01 = zsub(zscale(a01, c), zmul(sw, a11));
+            const zc n10 = zadd(zmul(swc, a00), zscale(a10, c)), n11 = zadd(zmul(swc, a01), zscale(a11, c));
+            a00 = n00; a01 = n01; a10 = n10; a11 = n11;
+          }
+          if (sk != 0.0) {  // cols: A_:p <- c A_:p - s conj(w) A_:q ; A_:q <- s w A_:p + c A_:q
+            const double c = rc_c[tk];
+            const zc w = rc_w[tk];
+            const zc sw = zscale(w, sk), swc = zscale(zconj(w), sk);
+            const zc n00 = zsub(zscale(a00, c), zmul(swc, a01)), n01 = zadd(zmul(sw, a00), zscale(a01, c));
+            const zc n10 = zsub(zscale(a10, c), zmul(swc, a11)), n11 = zadd(zmul(sw, a10), zscale(a11, c));
+            a00 = n00; a01 = n01; a10 = n10; a11 = n11;
+          }
+          if (tr == tk) {  // the annihilated entry of a rotated pair
+            a01 = zmk(0.0, 0.0);
+            a10 = zmk(0.0, 0.0);
+          }
+          A[pr][pk] = a00; A[pr][qk] = a01; A[qr][pk] = a10; A[qr][qk] = a11;
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {  // Q_:p <- c Q_:p - s conj(w) Q_:q ; Q_:q <- s w Q_:p + c Q_:q
+        const int grp = tid + 256 * h, k = grp >> 4, tp = grp & 15;
+        const double sn = rc_s[tp];
+        if (sn == 0.0) continue;
+        int pq, qq;
+        inner_pair(rd, tp, pq, qq);
+        const double c = rc_c[tp];
+        const zc w = rc_w[tp];
+        const zc sw = zscale(w, sn), swc = zscale(zconj(w), sn);
+        const zc xp = Q[k][pq], xq = Q[k][qq];
+        Q[k][pq] = zsub(zscale(xp, c), zmul(swc, xq));
+        Q[k][qq] = zadd(zmul(sw, xp), zscale(xq, c));
+      }
+      __syncthreads();
+    }
+    if (!s_sweep[isw & 1]) break;
+  }
+  __syncthreads();
+}
+
 template <bool FULL>
 __global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict__ tasks, int sweep, int round,
-                                                      double tol_rel, double tol_abs) {
+                                                      double tol_rel, double tol_abs, int* __restrict__ wl_count,
+                                                      int2* __restrict__ wl_items, int wl_cap) {
   const BondTask t = tasks[blockIdx.y];
   const int p = blockIdx.x;
   if (p >= t.P || round >= 2 * t.P - 1) return;
@@ -149,8 +273,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict
   __shared__ double nred[4][2][16];
   __shared__ double rc_c[JC / 2], rc_s[JC / 2];
   __shared__ zc rc_w[JC / 2];
-  __shared__ int rc_p[JC / 2], rc_q[JC / 2];
-  __shared__ int s_flag, s_rot;
+  __shared__ int s_flag, s_round[2], s_sweep[2];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const zc* Wi = t.W + (size_t)t.R * (bi * JB);
@@ -218,129 +341,85 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict
       return;
     }
   }
+
+  // ---- eigenvectors of A -> Q, to the pair's slot; queue the pair for rotation
+  inner_jacobi(A, Q, rc_c, rc_s, rc_w, s_round, s_sweep, tid);
+  zc* qg = t.Qg + (size_t)p * JC * JC;
+  for (int e = tid; e < JC * JC; e += 256) qg[e] = Q[e / JC][e % JC];  // qg[k JC + c] = Q[k][c]
   if (tid == 0) {
     atomicAdd(&cnt[sweep & 1], 1);
     atomicAdd(&cnt[2], 1);
     atomicAdd(&cnt[3], 1);
-  }
-
-  // ---- cyclic Jacobi on A (JC x JC Hermitian) in LDS, Q accumulates the rotations.
-  // A Gram entry is accurate relative to sqrt(A_pp A_qq), so the inner solve uses a
-  // purely relative test (skip when |A_pq| <= 1e-15 sqrt(A_pp A_qq)), sets each
-  // annihilated entry to exactly zero and ends after a sweep without rotations.
-  // The pair's columns keep their block (re-sorting columns between blocks stalls
-  // the convergence of the cyclic block sweep).
-  for (int e = tid; e < JC * JC; e += 256) {
-    const int r = e % JC, c = e / JC;
-    Q[r][c] = zmk(r == c ? 1.0 : 0.0, 0.0);
-  }
-  for (int isw = 0; isw < kInnerSweeps; ++isw) {
-    if (tid == 0) s_rot = 0;
-    __syncthreads();
-    for (int rd = 0; rd < JC - 1; ++rd) {
-      if (tid < JC / 2) {
-        int pp, qq;
-        rr_pair(JC, rd, tid, pp, qq);
-        if (pp > qq) { const int tmp = pp; pp = qq; qq = tmp; }
-        double c, s;
-        zc w;
-        const zc g = A[pp][qq];
-        const double a = A[pp][pp].x, b = A[qq][qq].x;
-        const double g2 = zabs2(g);
-        if (g2 <= 1e-30 * fabs(a * b) || g2 == 0.0) { c = 1.0; s = 0.0; w = zmk(1.0, 0.0); }
-        else { jrot(a, b, g, c, s, w); s_rot = 1; }
-        rc_c[tid] = c; rc_s[tid] = s; rc_w[tid] = w; rc_p[tid] = pp; rc_q[tid] = qq;
-      }
-      __syncthreads();
-      // rows: A_p <- c A_p - s w A_q ; A_q <- s conj(w) A_p + c A_q
-      for (int e = tid; e < (JC / 2) * JC; e += 256) {
-        const int pr = e / JC, k = e % JC;
-        const double c = rc_c[pr], s = rc_s[pr];
-        if (s == 0.0) continue;
-        const zc w = rc_w[pr];
-        const int pp = rc_p[pr], qq = rc_q[pr];
-        const zc xp = A[pp][k], xq = A[qq][k];
-        const zc sw = zscale(w, s), swc = zscale(zconj(w), s);
-        A[pp][k] = zsub(zscale(xp, c), zmul(sw, xq));
-        A[qq][k] = zadd(zmul(swc, xp), zscale(xq, c));
-      }
-      __syncthreads();
-      // cols: A_:p <- c A_:p - s conj(w) A_:q ; A_:q <- s w A_:p + c A_:q  (same for Q)
-      for (int e = tid; e < (JC / 2) * JC; e += 256) {
-        const int pr = e / JC, k = e % JC;
-        const double c = rc_c[pr], s = rc_s[pr];
-        if (s == 0.0) continue;
-        const zc w = rc_w[pr];
-        const int pp = rc_p[pr], qq = rc_q[pr];
-        const zc sw = zscale(w, s), swc = zscale(zconj(w), s);
-        zc xp = A[k][pp], xq = A[k][qq];
-        A[k][pp] = zsub(zscale(xp, c), zmul(swc, xq));
-        A[k][qq] = zadd(zmul(sw, xp), zscale(xq, c));
-        xp = Q[k][pp]; xq = Q[k][qq];
-        Q[k][pp] = zsub(zscale(xp, c), zmul(swc, xq));
-        Q[k][qq] = zadd(zmul(sw, xp), zscale(xq, c));
-      }
-      __syncthreads();
-      if (tid < JC / 2 && rc_s[tid] != 0.0) {
-        A[rc_p[tid]][rc_q[tid]] = zmk(0.0, 0.0);
-        A[rc_q[tid]][rc_p[tid]] = zmk(0.0, 0.0);
-      }
-      __syncthreads();
-    }
-    if (!s_rot) break;
-    __syncthreads();
-  }
-
-  // ---- [W; V](:, pair) <- [W; V](:, pair) Q on MFMA, computed as (X Q)^T = Q^T X^T:
-  // A operand Q^T (16 output columns x 4 k), B operand X^T (4 k x 16 rows), so each
-  // load and store instruction touches 4 columns x 16 consecutive rows.
-  const int lr = lane & 15, lk = lane >> 4;
-  double qre[2][8], qim[2][8];
-#pragma unroll
-  for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-    for (int ks = 0; ks < 8; ++ks) {
-      const zc q = Q[4 * ks + lk][16 * ct + lr];
-      qre[ct][ks] = q.x;
-      qim[ct][ks] = q.y;
-    }
-  auto col_of = [&](int c) { return c < JB ? bi * JB + c : bj * JB + (c - JB); };
-  auto rotate = [&](zc* M, int rows, int ld) {
-    const zc* src[8];
-#pragma unroll
-    for (int ks = 0; ks < 8; ++ks) src[ks] = M + (size_t)ld * col_of(4 * ks + lk);
-    for (int r0 = 16 * wave; r0 < rows; r0 += 64) {
-      const int r = r0 + lr;
-      const bool in = r < rows;
-      zc x[8];
-#pragma unroll
-      for (int ks = 0; ks < 8; ++ks) x[ks] = in ? src[ks][r] : zmk(0.0, 0.0);
-      d4 ore[2], oim[2];
-#pragma unroll
-      for (int ct = 0; ct < 2; ++ct) {
-        ore[ct] = d4{0.0, 0.0, 0.0, 0.0};
-        oim[ct] = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int ks = 0; ks < 8; ++ks) {
-          ore[ct] = mfma4(qre[ct][ks], x[ks].x, ore[ct]);
-          ore[ct] = mfma4(-qim[ct][ks], x[ks].y, ore[ct]);
-          oim[ct] = mfma4(qre[ct][ks], x[ks].y, oim[ct]);
-          oim[ct] = mfma4(qim[ct][ks], x[ks].x, oim[ct]);
-        }
-      }
-      if (in) {
-#pragma unroll
-        for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-          for (int g = 0; g < 4; ++g) M[(size_t)ld * col_of(16 * ct + lk + 4 * g) + r] = zmk(ore[ct][g], oim[ct][g]);
-      }
-    }
-  };
-  rotate(t.W, t.R, t.R);
-  if (t.V) rotate(t.V, t.NV, t.NV);
-  if (tid == 0) {
     t.blkver[lo] = vlo + 1;
     t.blkver[hi] = vhi + 1;
+    const int slot = atomicAdd(&wl_count[round & 1], 1);
+    if (slot < wl_cap) wl_items[(size_t)(round & 1) * wl_cap + slot] = make_int2(blockIdx.y, p);
+  }
+}
+
+constexpr int kRotRows = 64;  // rows of W or V per work item of k_jacobi_rotate
+
+// Work item w = (queued pair w / nchunk, 64-row slab w % nchunk): slabs of W first,
+// then of V.  The pair's Q comes from its slot; each wave rotates 16 rows as
+// (X Q)^T = Q^T X^T: A operand Q^T (16 output columns x 4 k), B operand X^T (4 k x
+// 16 rows), so every load and store touches 4 columns x 16 consecutive rows.
+__global__ __launch_bounds__(256) void k_jacobi_rotate(const BondTask* __restrict__ tasks, int round,
+                                                       int* __restrict__ wl_count, const int2* __restrict__ wl_items,
+                                                       int wl_cap, int nchunk_w, int nchunk) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) wl_count[(round + 1) & 1] = 0;  // the next round's queue
+  const int n = min(wl_count[round & 1], wl_cap);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int w = blockIdx.x; w < n * nchunk; w += gridDim.x) {
+    const int2 item = wl_items[(size_t)(round & 1) * wl_cap + w / nchunk];
+    const int chunk = w % nchunk;
+    const BondTask& t = tasks[item.x];
+    const int p = item.y;
+    int bi, bj;
+    rr_pair(2 * t.P, round, p, bi, bj);
+    zc* M;
+    int rows, ld, r0;
+    if (chunk < nchunk_w) {
+      M = t.W; rows = t.R; ld = t.R; r0 = chunk * kRotRows;
+    } else {
+      M = t.V; rows = t.V ? t.NV : 0; ld = t.NV; r0 = (chunk - nchunk_w) * kRotRows;
+    }
+    if (r0 >= rows) continue;
+    const zc* qg = t.Qg + (size_t)p * JC * JC;
+    double qre[2][8], qim[2][8];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+      for (int ks = 0; ks < 8; ++ks) {
+        const zc q = qg[(4 * ks + lk) * JC + 16 * ct + lr];
+        qre[ct][ks] = q.x;
+        qim[ct][ks] = q.y;
+      }
+    auto col_of = [&](int c) { return c < JB ? bi * JB + c : bj * JB + (c - JB); };
+    const int r = r0 + 16 * wave + lr;
+    const bool in = r < rows;
+    zc x[8];
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) x[ks] = in ? M[(size_t)ld * col_of(4 * ks + lk) + r] : zmk(0.0, 0.0);
+    d4 ore[2], oim[2];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+      ore[ct] = d4{0.0, 0.0, 0.0, 0.0};
+      oim[ct] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < 8; ++ks) {
+        ore[ct] = mfma4(qre[ct][ks], x[ks].x, ore[ct]);
+        ore[ct] = mfma4(-qim[ct][ks], x[ks].y, ore[ct]);
+        oim[ct] = mfma4(qre[ct][ks], x[ks].y, oim[ct]);
+        oim[ct] = mfma4(qim[ct][ks], x[ks].x, oim[ct]);
+      }
+    }
+    if (in) {
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) M[(size_t)ld * col_of(16 * ct + lk + 4 * g) + r] = zmk(ore[ct][g], oim[ct][g]);
+    }
   }
 }
 
@@ -354,11 +433,12 @@ __global__ void k_jacobi_begin(const BondTask* __restrict__ tasks) {
   for (int e = threadIdx.x; e < nblk * nblk; e += blockDim.x) t.pairseen[e] = ~0ull;
 }
 
-// Start of sweep `sweep`: clear that sweep's rotation counter (its parity slot
-// last held sweep - 2, which is no longer read).
-__global__ void k_jacobi_sweep_begin(const BondTask* __restrict__ tasks, int ntasks, int sweep) {
+// Start of sweep `sweep`: clear that sweep's rotation counter (its parity slot last
+// held sweep - 2, which is no longer read) and both rotation queues.
+__global__ void k_jacobi_sweep_begin(const BondTask* __restrict__ tasks, int ntasks, int sweep, int* wl_count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < ntasks) tasks[i].counter[sweep & 1] = 0;
+  if (i < 2) wl_count[i] = 0;
 }
 
 }  // namespace
@@ -368,18 +448,25 @@ hipError_t launch_jacobi_begin(const BondTask* d_tasks, int ntasks, hipStream_t 
   return hipGetLastError();
 }
 
-hipError_t launch_jacobi_sweep_begin(const BondTask* d_tasks, int ntasks, int sweep, hipStream_t st) {
-  hipLaunchKernelGGL(k_jacobi_sweep_begin, dim3((ntasks + 255) / 256), dim3(256), 0, st, d_tasks, ntasks, sweep);
+hipError_t launch_jacobi_sweep_begin(const BondTask* d_tasks, int ntasks, int sweep, int* wl_count, hipStream_t st) {
+  hipLaunchKernelGGL(k_jacobi_sweep_begin, dim3((ntasks + 255) / 256), dim3(256), 0, st, d_tasks, ntasks, sweep,
+                     wl_count);
   return hipGetLastError();
 }
 
 hipError_t launch_jacobi_round(const BondTask* d_tasks, int ntasks, int max_pairs, int sweep, int round,
-                               double tol_rel, double tol_abs, hipStream_t st) {
+                               double tol_rel, double tol_abs, int max_rows_w, int max_rows_v, int* wl_count,
+                               int2* wl_items, int wl_cap, hipStream_t st) {
   if (round == 0)
     hipLaunchKernelGGL(k_jacobi_round<true>, dim3(max_pairs, ntasks), dim3(256), 0, st, d_tasks, sweep, round,
-                       tol_rel, tol_abs);
+                       tol_rel, tol_abs, wl_count, wl_items, wl_cap);
   else
     hipLaunchKernelGGL(k_jacobi_round<false>, dim3(max_pairs, ntasks), dim3(256), 0, st, d_tasks, sweep, round,
-                       tol_rel, tol_abs);
+                       tol_rel, tol_abs, wl_count, wl_items, wl_cap);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int nw = (max_rows_w + kRotRows - 1) / kRotRows, nv = (max_rows_v + kRotRows - 1) / kRotRows;
+  hipLaunchKernelGGL(k_jacobi_rotate, dim3(1024), dim3(256), 0, st, d_tasks, round, wl_count, wl_items, wl_cap, nw,
+                     nw + nv);
   return hipGetLastError();
 }
