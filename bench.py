@@ -7,8 +7,13 @@ Hubbard chain L=128, d=4 (basis [0, up, down, up+down]), t=1, U=4, OBC,
 The bond dimensions grow unevenly from 1 and every bulk bond reaches chi=256 by
 step ~37.  The first W=40 steps are untimed warm-up; the next K=60 steps (41-100)
 are timed, all at the real shape (191 bond updates of up to 1024 x 1024 per step).
-`--compare fp64` times the plain per-bond FP64 path on the first steps of the same
-window, started from a snapshot of the same state.
+
+`value` is the batched learned-rank path (randomised SVD sized by the rank model,
+full SVD fallback).  The same 100 steps are also run by the exact batched FP64
+path: its steps/s are reported, and the learned path's observables at step 100
+(energy, <n_i>, <Sz_i>, entanglement entropy in log_d, cumulative truncation error)
+are compared with it (`obs_err`).  `--compare fp64` times the plain per-bond FP64
+path on the first steps of the timed window, started from the exact path's state.
 
 Prints one JSON line.  `roofline` is the kernel class with the most device time in
 the timed steps (pivoted-QR steps or Jacobi rounds, both HBM-bound streaming
@@ -68,13 +73,23 @@ class _Roctx:
             self.lib.roctxRangePop()
 
 
+def device_observables(ctx, H):
+    from tebd_amd import models as M
+
+    _, _, ntot, sz, _ = M.fermion_ops(2)
+    return dict(E=ctx.energy(H), n=ctx.one_site_expval(ntot), sz=ctx.one_site_expval(sz), S=ctx.chain_entropy())
+
+
 def run_mode(mode, steps, warmup, log, state=None, keep_state=False):
+    from tebd_amd import models as M
+
     ctx = make_context(mode)
     if state is not None:
         ctx.upload(*state)
+    trunc_warm = 0.0
     tw = time.perf_counter()
     for k in range(warmup):
-        ctx.trotter_step()
+        trunc_warm += ctx.trotter_step()
         if (k + 1) % 10 == 0 or k + 1 == warmup:
             log(f"[bench] {mode} warmup step {k + 1}/{warmup} mean chi={ctx.bond_dims().mean():.1f} "
                 f"({time.perf_counter() - tw:.1f}s)")
@@ -96,9 +111,11 @@ def run_mode(mode, steps, warmup, log, state=None, keep_state=False):
     prof = ctx.profile()
     st1 = ctx.stats()
     dims = ctx.bond_dims()
+    obs = device_observables(ctx, M.fermi_hubbard(L, T_HOP, U_INT)) if state is None else None
     ctx.close()
     delta = {k: st1[k] - st0[k] for k in st1}
-    return dict(elapsed=elapsed, prof=prof, dims=dims, trunc=trunc, stats=delta, warm_s=warm_s, snap=snap)
+    return dict(elapsed=elapsed, prof=prof, dims=dims, trunc=trunc + trunc_warm, stats=delta, warm_s=warm_s,
+                snap=snap, obs=obs)
 
 
 def cpu_baseline(sample_bonds, log):
@@ -129,7 +146,8 @@ def main():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--compare-steps", type=int, default=3)
-    ap.add_argument("--mode", default="batched")
+    ap.add_argument("--mode", default="batched,learned")
+    ap.add_argument("--reference-mode", default="batched", help="exact path for the observable check ('' = skip)")
     ap.add_argument("--compare", default="", help="also time this mode (e.g. fp64) on the same workload")
     ap.add_argument("--cpu-sample-bonds", type=int, default=3)
     ap.add_argument("--quiet", action="store_true")
@@ -140,10 +158,14 @@ def main():
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    res = run_mode(args.mode, args.steps, args.warmup, log, keep_state=bool(args.compare))
+    res = run_mode(args.mode, args.steps, args.warmup, log)
     sps = args.steps / res["elapsed"]
+    ref = None
+    if args.reference_mode:
+        ref = run_mode(args.reference_mode, args.steps, args.warmup, log, keep_state=bool(args.compare))
     prof = res["prof"]
-    kernel_of = {"jacobi_round": "k_jacobi_round", "qr_step": "k_qrp_pivot + k_qrp_update"}
+    kernel_of = {"jacobi_round": "k_jacobi_round + k_jacobi_rotate", "qr_step": "k_qrp_panel + k_qrp_trail",
+                 "sketch_gemm": "k_build_sketch + k_zgemm + k_qrp_vectors"}
     dom = max(kernel_of, key=lambda n: prof[n][1])
     n_launch, dom_ms, dom_flops, dom_bytes = prof[dom]
     avg_ms = dom_ms / max(n_launch, 1)
@@ -153,8 +175,15 @@ def main():
     half_steps = 3 * args.steps
 
     extra = {}
-    if args.compare:
-        cmp_res = run_mode(args.compare, args.compare_steps, 0, log, state=res["snap"])
+    if ref is not None:
+        errs = {k: float(np.max(np.abs(np.asarray(res["obs"][k]) - np.asarray(ref["obs"][k])))) for k in res["obs"]}
+        errs["trunc"] = abs(res["trunc"] - ref["trunc"])
+        extra["obs_err"] = max(errs.values())
+        extra["obs_err_by_observable"] = errs
+        extra["obs_reference"] = f"{args.reference_mode} (exact FP64 GPU path), same steps 1-{args.warmup + args.steps}"
+        extra[f"{args.reference_mode}_steps_per_s"] = args.steps / ref["elapsed"]
+    if args.compare and ref is not None:
+        cmp_res = run_mode(args.compare, args.compare_steps, 0, log, state=ref["snap"])
         extra[f"{args.compare}_steps_per_s"] = args.compare_steps / cmp_res["elapsed"]
         extra[f"{args.compare}_window"] = f"steps {args.warmup + 1}-{args.warmup + args.compare_steps}"
 
@@ -190,6 +219,8 @@ def main():
                      "bytes_per_launch": dom_bytes / max(n_launch, 1), "fp64_tflops": tflops,
                      "fp64_frac": tflops / FP64_PEAK_TFLOPS,
                      "device_ms": {n: prof[n][1] for n in prof}},
+        "learned_rank": {"sketch_accepted": st["sketch_accepted"], "full_svd_fallbacks": st["fallbacks"],
+                         "mean_sketch_width": st["sketch_cols"] / max(st["sketch_accepted"] + st["fallbacks"], 1)},
         "jacobi": {"sweeps_per_half_step": st["jacobi_sweeps"] / half_steps,
                    "pair_visits": st["pair_visits"], "grams_formed": st["pair_grams"],
                    "pairs_rotated": st["pair_rotations"]},

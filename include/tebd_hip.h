@@ -96,6 +96,14 @@ int tebd_canonical_form_all(tebd_ctx* ctx);
  * CanonicalForm (local_operations_module.f90:496-620). */
 int tebd_canonical_form(tebd_ctx* ctx, int link);
 
+/* <psi| O_b |psi> for every bond b = 1..L-1, with O_b the d^2 x d^2 operator ops[b-1]
+ * (column-major, row (a-1)d+b as the gates) on sites b, b+1 of the canonical state;
+ * out[2(b-1)], out[2(b-1)+1] = real and imaginary part.  Replaces the host-side
+ * TwoSiteRho / TotalEnergy / OneSiteExpVal evaluation
+ * (observables_module.f90:95-162,704-832,1432-1466): Tr(O rho) with
+ * rho = theta theta^dagger is formed on the device from the resident state. */
+int tebd_expval_two_site(tebd_ctx* ctx, const double* ops, double* out);
+
 /* Singular-value cutoff on the normalised weight s^2/||theta||^2.  Default
  * 1e-30, the reference's rule (keep |s| > 1e-15 after normalisation,
  * FormLambda1, local_operations_module.f90:360-375).  Larger cutoffs shrink

@@ -17,7 +17,7 @@ namespace {
 
 constexpr int kStages = 5;
 constexpr int kNumStats = 11;
-constexpr int kNumProf = 16;
+constexpr int kNumProf = 20;
 constexpr double kQrTol = 1e-17;  // QR stops when every remaining column norm is <= kQrTol ||theta||_F
 constexpr int kQrPanel = 32;      // panel width of the blocked pivoted QR (tebd_qrp.hip)
 constexpr int kQrCheck = 4;       // panels between checks whether every task has stopped
@@ -68,18 +68,20 @@ struct tebd_ctx {
   int2* wl_items = nullptr;      // 2 x wl_cap queued (task, pair)
   int wl_cap = 0;
   double* results = nullptr;
+  double* results2 = nullptr;    // learned-rank path: per task ||theta||^2 and column norms of theta
+  zc* obs_ops = nullptr;         // (L-1) x d^4 operators of tebd_expval_two_site, allocated on first use
+  zc* ident = nullptr;           // d^4 identity gate
   BondTask* d_tasks = nullptr;   // kStages x ntask_max
   BondTask* h_tasks = nullptr;   // pinned, kStages x ntask_max
   // sketched (learned-rank) path workspace
-  zc* Zb = nullptr;              // per task n x (2 k') right basis
-  zc* Bb = nullptr;              // per task m x (2 k') sketch of theta
+  zc* Zb = nullptr;              // per task: range basis Q of the sketch (m x l)
+  zc* Bb = nullptr;              // per task: theta Omega and its QR, then B^dagger
   double* lam_scratch = nullptr; // per task chi_max: lambda before it is committed
   GemmTask* d_gemm = nullptr;
   GemmTask* h_gemm = nullptr;    // pinned
   int* d_kprime = nullptr;
   int* h_kprime = nullptr;       // pinned
   unsigned long long sketch_seed = 0x5DEECE66Dull;
-  double cutoff_sketch = 1e-24;  // kept weight s^2/||theta||^2 on the sketched path
   double tol_missed = 1e-20;     // max weight of theta outside the sketch
   double tol_trunc = 1e-10;      // max discarded weight before the full SVD judges
   int oversample = 8;
@@ -95,12 +97,13 @@ struct tebd_ctx {
   std::vector<zc> staging;
   std::string err;
   // [0] bond updates, [1] Jacobi sweeps, [2] sketch fallbacks, [3] device ms of trotter steps,
-  // [4..7] Jacobi sweeps of the sketch stages Z0, R, R re-orth, Rayleigh-Ritz,
+  // [4] sketch columns requested, [5] sketches accepted, [6] sketch ranks (sum of r_y), [7] unused,
   // [8] pair Gram matrices formed, [9] pairs rotated, [10] pair visits of the sweeps run
   double stats[kNumStats] = {};
   // optional per-kernel timing with HIP events on the context's stream:
   // prof[k*4 + 0] launches, [1] milliseconds, [2] algorithmic flops, [3] algorithmic HBM bytes,
-  // for k = 0 theta+gate, 1 Jacobi round, 2 truncation + Gamma rebuild, 3 pivoted QR steps
+  // for k = 0 theta+gate, 1 Jacobi round, 2 truncation + Gamma rebuild, 3 pivoted QR steps,
+  // 4 sketch construction and GEMMs of the learned-rank path
   bool profiling = false;
   double prof[kNumProf] = {};
   struct Pending { hipEvent_t a, b; int kind; };
@@ -236,15 +239,15 @@ void attach_qr(tebd_ctx* c, BondTask& t, int slot) {
 
 // Column-pivoted Householder QR of every task's bond matrix, stopped at the
 // numerical rank.  Returns the ranks in rank[].
-int qrp_factor(tebd_ctx* c, int nb, int max_cb, std::vector<int>& rank) {
-  HIPCHK(c, launch_qrp_begin(c->d_tasks, nb, kQrTol, c->st));
+int qrp_factor(tebd_ctx* c, BondTask* D, const BondTask* H, int nb, int max_cb, std::vector<int>& rank) {
+  HIPCHK(c, launch_qrp_begin(D, nb, kQrTol, c->st));
   // every task's last panel (j0 + width >= min(Rb, Cb)) records its rank, so after the
   // panel with j0 + kQrPanel >= max_cb all tasks have finished
   bool done = false;
   for (int j0 = 0, panel = 0; !done; j0 += kQrPanel, ++panel) {
     {
       ProfScope ps(c, 3);
-      HIPCHK(c, launch_qrp_panel(c->d_tasks, nb, max_cb, j0, c->st));
+      HIPCHK(c, launch_qrp_panel(D, nb, max_cb, j0, c->st));
     }
     const bool last = j0 + kQrPanel >= max_cb;
     if ((panel + 1) % kQrCheck == 0 || last) {
@@ -262,7 +265,7 @@ int qrp_factor(tebd_ctx* c, int nb, int max_cb, std::vector<int>& rank) {
       // per panel j0 of width w: the trailing (Rb - j0) x (Cb - j0 - w) block is read twice
       // (Y = V^H A2, then A2 -= V Z) and written once, 2 x 32 complex MACs per element;
       // the panel itself is read and written about w times
-      const BondTask& t = c->h_tasks[i];
+      const BondTask& t = H[i];
       for (int j0 = 0; j0 < rank[i]; j0 += kQrPanel) {
         const int w = std::min(kQrPanel, rank[i] - j0);
         const double rows = t.Rb - j0, trail = std::max(0, t.Cb - j0 - w);
@@ -274,20 +277,23 @@ int qrp_factor(tebd_ctx* c, int nb, int max_cb, std::vector<int>& rank) {
   return TEBD_OK;
 }
 
-// SVD + truncation of the tasks' bond matrices (theta already in W) and the new
-// Gammas / lambda.  Bond matrices with at least kQrMinCols columns are first
-// reduced by the pivoted QR; the one-sided Jacobi then runs on R^dagger.
-// On return h_results holds [0] kept rank and [1] truncerr of every task.
-int svd_truncate(tebd_ctx* c, const int* links, int nb, int Cpmax, int Pmax) {
+// SVD + truncation of the tasks' bond matrices (in W; tasks D / H, task i uses slot
+// i and results[i]): singular values, kept rank, lambda (into lc_out) and truncerr.
+// Bond matrices with at least kQrMinCols columns are first reduced by the pivoted
+// QR; the one-sided Jacobi then runs on R^dagger and the kept singular vectors are
+// formed into Yr / Zc.  On return h_results holds every task's results and *max_k
+// the largest kept rank.
+int svd_factor(tebd_ctx* c, BondTask* D, BondTask* H, const int* links, int nb, int Cpmax, int Pmax, bool force_qr,
+               bool* used_qr, int* max_k) {
   int max_cb = 1, max_rb = 1;
   for (int i = 0; i < nb; ++i) {
-    max_cb = std::max(max_cb, c->h_tasks[i].Cb);
-    max_rb = std::max(max_rb, c->h_tasks[i].Rb);
+    max_cb = std::max(max_cb, H[i].Cb);
+    max_rb = std::max(max_rb, H[i].Rb);
   }
-  const bool use_qr = max_cb >= kQrMinCols;
+  const bool use_qr = force_qr || max_cb >= kQrMinCols;
   if (use_qr) {
     std::vector<int> rank;
-    int e = qrp_factor(c, nb, max_cb, rank);
+    int e = qrp_factor(c, D, H, nb, max_cb, rank);
     if (e) return e;
     Cpmax = 1;
     Pmax = 1;
@@ -297,7 +303,7 @@ int svd_truncate(tebd_ctx* c, const int* links, int nb, int Cpmax, int Pmax) {
         std::snprintf(buf, sizeof(buf), "bond %d: theta is zero or not finite", links[i] + 1);
         return fail(c, TEBD_ERR_NAN, buf);
       }
-      BondTask& t = c->h_tasks[i];
+      BondTask& t = H[i];
       t.qr = 1;
       t.W = c->X + (size_t)i * c->Cpmax * c->Cpmax;
       t.R = t.Cb;
@@ -308,22 +314,22 @@ int svd_truncate(tebd_ctx* c, const int* links, int nb, int Cpmax, int Pmax) {
       Cpmax = std::max(Cpmax, t.Cpad);
       Pmax = std::max(Pmax, t.P);
     }
-    HIPCHK(c, hipMemcpyAsync(c->d_tasks, c->h_tasks, sizeof(BondTask) * nb, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, launch_qrp_build_x(c->d_tasks, nb, c->st));
+    HIPCHK(c, hipMemcpyAsync(D, H, sizeof(BondTask) * nb, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_qrp_build_x(D, nb, c->st));
   }
-  HIPCHK(c, launch_prepare_svd(c->d_tasks, nb, c->st));
-  HIPCHK(c, launch_frobenius(c->d_tasks, nb, Cpmax, c->st, 0));
+  HIPCHK(c, launch_prepare_svd(D, nb, c->st));
+  HIPCHK(c, launch_frobenius(D, nb, Cpmax, c->st, 0));
   {
-    int e = jacobi_sweeps(c, c->d_tasks, c->h_tasks, nb, Pmax);
+    int e = jacobi_sweeps(c, D, H, nb, Pmax);
     if (e) return e;
   }
   {
     ProfScope ps(c, 2);
-    HIPCHK(c, launch_finalize(c->d_tasks, nb, Cpmax, c->cutoff, c->st));
+    HIPCHK(c, launch_finalize(D, nb, Cpmax, c->cutoff, c->st));
   }
   HIPCHK(c, hipMemcpyAsync(c->h_results, c->results, sizeof(double) * RESULT_STRIDE * nb, hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
-  int max_k = 1;
+  *max_k = 1;
   for (int i = 0; i < nb; ++i) {
     const double* r = c->h_results + RESULT_STRIDE * i;
     const int k = (int)r[0];
@@ -332,16 +338,29 @@ int svd_truncate(tebd_ctx* c, const int* links, int nb, int Cpmax, int Pmax) {
       std::snprintf(buf, sizeof(buf), "bond %d: theta is zero or not finite (||theta||^2=%g)", links[i] + 1, r[2]);
       return fail(c, TEBD_ERR_NAN, buf);
     }
-    max_k = std::max(max_k, k);
+    *max_k = std::max(*max_k, k);
     if (c->profiling) {
       // column norms read W; Gamma_l, Gamma_r written from k singular vectors
-      const BondTask& t = c->h_tasks[i];
+      const BondTask& t = H[i];
       c->prof[11] += 16.0 * ((double)t.R * t.C + 2.0 * k * t.d * ((double)t.chil + t.chir));
     }
   }
+  if (use_qr) {
+    ProfScope ps(c, 2);
+    HIPCHK(c, launch_qrp_vectors(D, nb, max_rb, *max_k, c->st));
+  }
+  *used_qr = use_qr;
+  return TEBD_OK;
+}
+
+// svd_factor of the batch (stage-0 tasks) followed by the new Gammas and lambda.
+int svd_truncate(tebd_ctx* c, const int* links, int nb, int Cpmax, int Pmax) {
+  bool used_qr = false;
+  int max_k = 1;
+  int e = svd_factor(c, c->d_tasks, c->h_tasks, links, nb, Cpmax, Pmax, false, &used_qr, &max_k);
+  if (e) return e;
   {
     ProfScope ps(c, 2);
-    if (use_qr) HIPCHK(c, launch_qrp_vectors(c->d_tasks, nb, max_rb, max_k, c->st));
     HIPCHK(c, launch_write_gammas(c->d_tasks, nb, c->st));
   }
   HIPCHK(c, hipStreamSynchronize(c->st));
@@ -419,73 +438,47 @@ int run_bonds_full(tebd_ctx* c, const int* links, int nb, const zc* const* gate_
 }
 
 // Sketch width for one bond: the predicted kept rank plus oversampling.
-int predicted_rank(tebd_ctx* c, int link) {
+int sketch_width(tebd_ctx* c, int link) {
   const int chic = c->chi[link + 1];
   const int grow = std::max(4, chic / 8);
-  return std::min(c->chi_max, chic + grow);
+  return std::min(c->chi_max, chic + grow) + c->oversample;
 }
 
-// Learned-rank path (one bond = one task, all tasks of a batch per launch):
-//   Z0 = orth([ (Gamma_r lambda_r)^H | Gaussian ])          n x kpad
-//   R  = (I - Z0 Z0^H)^2 theta^H theta Z0, orthonormalised     n x kpad  (CGS2)
-//   Rayleigh-Ritz: one-sided Jacobi on theta [Z0 R] with V = [Z0 R].
-// The sketch is accepted when the weight of theta outside it is below
-// tol_missed ||theta||^2 and the discarded weight below tol_trunc; other bonds
-// are redone by the full SVD before anything is written.
-int orthonormalize(tebd_ctx* c, BondTask* D, const BondTask* H, int ns, int cmax, int P, int passes, int slot) {
-  for (int pass = 0; pass < passes; ++pass) {
-    HIPCHK(c, launch_col_norms(D, ns, cmax, c->st));
-    HIPCHK(c, launch_normalize_cols(D, ns, c->st));
-    HIPCHK(c, launch_frobenius(D, ns, cmax, c->st, 0));
-    const double before = c->stats[1];
-    int e = jacobi_sweeps(c, D, H, ns, P);
-    if (e) return e;
-    c->stats[slot] += c->stats[1] - before;
-  }
-  HIPCHK(c, launch_col_norms(D, ns, cmax, c->st));
-  HIPCHK(c, launch_normalize_cols(D, ns, c->st));
-  return TEBD_OK;
-}
-
-int run_bonds_sketch(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
-  const int d = c->d;
-  std::vector<int> sk, sk_links, full_links;
-  std::vector<const zc*> full_gates;
-  std::vector<int> kp;
+int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
+  const int d = c->d, nt = c->ntask_max;
+  std::vector<int> sk, full;
+  std::vector<int> ell;
   for (int i = 0; i < nb; ++i) {
     const int link = links[i];
     const int m = d * c->chi[link], n = d * c->chi[link + 2];
-    const int k1 = std::min(n, predicted_rank(c, link) + c->oversample);
-    const int kpad = round_up(k1, JC);
-    if (2 * kpad >= std::min(m, n)) {
-      full_links.push_back(link);
-      full_gates.push_back(gate_ptrs[i]);
+    const int l = round_up(std::min(std::min(m, n), sketch_width(c, link)), JC);
+    if (2 * l > std::min(m, n)) {
+      full.push_back(i);
     } else {
       sk.push_back(i);
-      sk_links.push_back(link);
-      kp.push_back(k1);
+      ell.push_back(l);
     }
   }
   double tsum = 0.0;
+  std::vector<int> fb_links;
+  std::vector<const zc*> fb_gates;
+  for (int i : full) {
+    fb_links.push_back(links[i]);
+    fb_gates.push_back(gate_ptrs[i]);
+  }
   const int ns = (int)sk.size();
   if (ns > 0) {
-    const int nt = c->ntask_max;
-    BondTask *S0 = c->h_tasks, *S1 = S0 + nt, *SA = S0 + 2 * nt, *SB = S0 + 3 * nt, *S3 = S0 + 4 * nt;
-    BondTask *D0 = c->d_tasks, *D1 = D0 + nt, *DA = D0 + 2 * nt, *DB = D0 + 3 * nt, *D3 = D0 + 4 * nt;
+    BondTask *S0 = c->h_tasks, *S1 = S0 + nt, *S2 = S0 + 2 * nt, *S3 = S0 + 3 * nt, *S4 = S0 + 4 * nt;
+    BondTask *D0 = c->d_tasks, *D1 = D0 + nt, *D2 = D0 + 2 * nt, *D3 = D0 + 3 * nt, *D4 = D0 + 4 * nt;
     GemmTask* HG = c->h_gemm;
     GemmTask* DG = c->d_gemm;
-    int chil_max = 1, chir_max = 1, Pk = 1, P2 = 1, c0max = 1, kmax = 1;
-    int gM[5] = {1, 1, 1, 1, 1}, gN[5] = {1, 1, 1, 1, 1};
+    int chil_max = 1, chir_max = 1, max_m = 1, max_n = 1, max_l = 1;
+    std::vector<int> sk_links(ns);
     for (int q = 0; q < ns; ++q) {
-      const int i = sk[q], link = sk_links[q];
+      const int i = sk[q], link = links[i];
+      sk_links[q] = link;
       const int chil = c->chi[link], chic = c->chi[link + 1], chir = c->chi[link + 2];
-      const int m = d * chil, n = d * chir;
-      const int kpad = round_up(kp[q], JC);
-      zc* Wt = c->W + (size_t)q * c->Rmax * c->Cpmax;
-      zc* Zt = c->Zb + (size_t)q * c->Rmax * c->Cpmax;
-      zc* Bt = c->Bb + (size_t)q * c->Rmax * c->Cpmax;
-      zc* Rt = Zt + (size_t)n * kpad;
-      zc* Mt = Bt + (size_t)m * kpad;
+      const int m = d * chil, n = d * chir, l = ell[q];
       BondTask base;
       std::memset(&base, 0, sizeof(base));
       base.gl = c->gam + (size_t)link * c->gslab();
@@ -495,121 +488,141 @@ int run_bonds_sketch(tebd_ctx* c, const int* links, int nb, const zc* const* gat
       base.lr = c->lam + (size_t)(link + 2) * c->chi_max;
       base.gate = gate_ptrs[i];
       base.chil = chil; base.chic = chic; base.chir = chir; base.d = d;
+      base.chi_max = c->chi_max;
       base.s = c->s + (size_t)q * c->Cpmax;
       base.perm = c->perm + (size_t)q * c->Cpmax;
-      attach_jacobi(c, base, q);
-      base.result = c->results + (size_t)RESULT_STRIDE * q;
-      base.chi_max = c->chi_max;
-      base.drop_rel = 1e-13;
-      S0[q] = base;  // theta (m x n) -> W
+      zc* Wt = c->W + (size_t)q * c->Rmax * c->Cpmax;   // theta, later the left vectors
+      zc* Vt = c->V + (size_t)q * c->Cpmax * c->Cpmax;  // Omega, later the Jacobi V
+      zc* Bt = c->Bb + (size_t)q * c->Rmax * c->Cpmax;  // Y and its QR, later B^dagger
+      zc* Qt = c->Zb + (size_t)q * c->Rmax * c->Cpmax;  // Q
+      // stage 0: theta (m x n) and ||theta||^2 into results2[q][3]
+      S0[q] = base;
       S0[q].W = Wt; S0[q].R = m; S0[q].C = n; S0[q].Cpad = round_up(n, JC);
-      S1[q] = base;  // Z0 (n x kpad)
-      S1[q].V = Zt; S1[q].NV = n; S1[q].Cpad = kpad;
-      SA[q] = base;  // orthonormalise Z0
-      SA[q].W = Zt; SA[q].R = n; SA[q].C = kpad; SA[q].Cpad = kpad; SA[q].P = kpad / JC;
-      SB[q] = base;  // orthonormalise R
-      SB[q].W = Rt; SB[q].R = n; SB[q].C = kpad; SB[q].Cpad = kpad; SB[q].P = kpad / JC;
-      S3[q] = base;  // Rayleigh-Ritz on theta [Z0 R]
-      S3[q].W = Bt; S3[q].R = m; S3[q].C = 2 * kpad; S3[q].Cpad = 2 * kpad; S3[q].P = 2 * kpad / JC;
-      S3[q].V = Zt; S3[q].NV = n;
-      S3[q].gl_out = const_cast<zc*>(base.gl);
-      S3[q].gr_out = const_cast<zc*>(base.gr);
-      S3[q].lc_out = c->lam_scratch + (size_t)q * c->chi_max;
-      c->h_kprime[q] = kp[q];
-      // GEMM slots: 0 B = theta Z0, 1 R = theta^H B, 2 M = Z0^H R, 3 R -= Z0 M, 4 W = theta [Z0 R]
-      HG[0 * nt + q] = {Wt, Zt, Bt, m, kpad, n, m, n, m, 1.0, 0};
-      HG[1 * nt + q] = {Wt, Bt, Rt, n, kpad, m, m, m, n, 1.0, 0};
-      HG[2 * nt + q] = {Zt, Rt, Mt, kpad, kpad, n, n, n, kpad, 1.0, 0};
-      HG[3 * nt + q] = {Zt, Mt, Rt, n, kpad, kpad, n, kpad, n, -1.0, 1};
-      HG[4 * nt + q] = {Wt, Zt, Bt, m, 2 * kpad, n, m, n, m, 1.0, 0};
-      const int Ms[5] = {m, n, kpad, n, m}, Ns[5] = {kpad, kpad, kpad, kpad, 2 * kpad};
-      for (int g = 0; g < 5; ++g) { gM[g] = std::max(gM[g], Ms[g]); gN[g] = std::max(gN[g], Ns[g]); }
+      S0[q].result = c->results2 + (size_t)RESULT_STRIDE * q;
+      // stage 1: Omega (n x l)
+      S1[q] = base;
+      S1[q].V = Vt; S1[q].Cpad = l;
+      c->h_kprime[q] = l;
+      // stage 2: pivoted QR of Y (m x l), explicit Q
+      S2[q] = base;
+      S2[q].W = Bt; S2[q].R = m; S2[q].C = l;
+      attach_qr(c, S2[q], q);
+      S2[q].Qs = Qt;
+      S2[q].result = c->results + (size_t)RESULT_STRIDE * q;
+      // GEMM 0: Y = theta Omega
+      HG[0 * nt + q] = {Wt, Vt, Bt, m, l, n, m, n, m, 1.0, 0};
+      max_m = std::max(max_m, m); max_n = std::max(max_n, n); max_l = std::max(max_l, l);
       chil_max = std::max(chil_max, chil); chir_max = std::max(chir_max, chir);
-      Pk = std::max(Pk, kpad / JC); P2 = std::max(P2, 2 * kpad / JC);
-      c0max = std::max(c0max, S0[q].Cpad); kmax = std::max(kmax, kpad);
     }
     HIPCHK(c, hipMemcpyAsync(c->d_tasks, c->h_tasks, sizeof(BondTask) * nt * kStages, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(DG, HG, sizeof(GemmTask) * nt * 5, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->d_kprime, c->h_kprime, sizeof(int) * ns, hipMemcpyHostToDevice, c->st));
-    auto gemm = [&](int slot, bool conj) {
-      return launch_zgemm(DG + slot * nt, ns, gM[slot], gN[slot], conj, c->st);
-    };
+    HIPCHK(c, hipMemcpyAsync(DG, HG, sizeof(GemmTask) * nt, hipMemcpyHostToDevice, c->st));
     {
       ProfScope ps(c, 0);
       HIPCHK(c, launch_theta_gate(D0, ns, d, chil_max, chir_max, c->st));
     }
-    HIPCHK(c, launch_frobenius(D0, ns, c0max, c->st, 1));  // result[3] = ||theta||^2
-    HIPCHK(c, launch_build_sketch(D1, c->d_kprime, ns, c->sketch_seed ^ (unsigned long long)c->step_index, c->st));
-    int e = orthonormalize(c, DA, SA, ns, kmax, Pk, 2, 4);
-    if (e) return e;
-    HIPCHK(c, gemm(0, false));
-    HIPCHK(c, gemm(1, true));
-    for (int r = 0; r < 2; ++r) {
-      HIPCHK(c, gemm(2, true));
-      HIPCHK(c, gemm(3, false));
-    }
-    e = orthonormalize(c, DB, SB, ns, kmax, Pk, 2, 5);
-    if (e) return e;
-    HIPCHK(c, gemm(2, true));
-    HIPCHK(c, gemm(3, false));
-    e = orthonormalize(c, DB, SB, ns, kmax, Pk, 1, 6);
-    if (e) return e;
-    HIPCHK(c, gemm(4, false));
-    HIPCHK(c, launch_frobenius(D3, ns, 2 * kmax, c->st, 0));
+    HIPCHK(c, launch_frobenius(D0, ns, round_up(max_n, JC), c->st, 1));
     {
-      const double before = c->stats[1];
-      e = jacobi_sweeps(c, D3, S3, ns, P2);
-      if (e) return e;
-      c->stats[7] += c->stats[1] - before;
+      ProfScope ps(c, 4);
+      HIPCHK(c, launch_build_sketch(D1, c->d_kprime, ns, c->sketch_seed ^ (unsigned long long)c->step_index, c->st));
+      HIPCHK(c, launch_zgemm(DG, ns, max_m, max_l, false, c->st));
     }
+    std::vector<int> ry;
+    int e = qrp_factor(c, D2, S2, ns, max_l, ry);
+    if (e) return e;
+    int max_ry = 1;
+    for (int q = 0; q < ns; ++q) {
+      S2[q].C = std::max(ry[q], 1);
+      max_ry = std::max(max_ry, S2[q].C);
+    }
+    HIPCHK(c, hipMemcpyAsync(D2, S2, sizeof(BondTask) * ns, hipMemcpyHostToDevice, c->st));
+    for (int q = 0; q < ns; ++q) {
+      const BondTask& b0 = S0[q];
+      const int m = b0.R, n = b0.C, r = S2[q].C;
+      zc* Bt = c->Bb + (size_t)q * c->Rmax * c->Cpmax;
+      // GEMM 1: B^dagger (n x r) = theta^dagger Q
+      HG[1 * nt + q] = {b0.W, S2[q].Qs, Bt, n, r, m, m, m, n, 1.0, 0};
+      // stage 3: SVD of B^dagger (rows n, columns r)
+      BondTask& t = S3[q];
+      t = S1[q];
+      t.V = c->V + (size_t)q * c->Cpmax * c->Cpmax;
+      t.W = Bt; t.R = n; t.C = r; t.Cpad = round_up(r, JC); t.P = t.Cpad / JC; t.NV = t.Cpad;
+      attach_jacobi(c, t, q);
+      attach_qr(c, t, q);
+      t.result = c->results + (size_t)RESULT_STRIDE * q;
+      t.lc_out = c->lam_scratch + (size_t)q * c->chi_max;
+      t.wtot = c->results2 + (size_t)RESULT_STRIDE * q + 3;
+    }
+    HIPCHK(c, hipMemcpyAsync(DG + nt, HG + nt, sizeof(GemmTask) * ns, hipMemcpyHostToDevice, c->st));
     {
-      ProfScope ps(c, 2);
-      HIPCHK(c, launch_col_norms(D3, ns, 2 * kmax, c->st));
-      HIPCHK(c, launch_truncate(D3, ns, c->cutoff_sketch > c->cutoff ? c->cutoff_sketch : c->cutoff, c->st));
+      ProfScope ps(c, 4);
+      HIPCHK(c, launch_qrp_form_q(D2, ns, max_m, max_ry, c->st));
+      HIPCHK(c, launch_zgemm(DG + nt, ns, max_n, max_ry, true, c->st));
     }
-    HIPCHK(c, hipMemcpyAsync(c->h_results, c->results, sizeof(double) * RESULT_STRIDE * ns, hipMemcpyDeviceToHost,
-                             c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    drain_profile(c);
+    HIPCHK(c, hipMemcpyAsync(D3, S3, sizeof(BondTask) * ns, hipMemcpyHostToDevice, c->st));
+    int max_k = 1;
+    bool used_qr = false;
+    e = svd_factor(c, D3, S3, sk_links.data(), ns, round_up(max_ry, JC), round_up(max_ry, JC) / JC, true, &used_qr,
+                   &max_k);
+    if (e) return e;
+    // left vectors of theta: Q U_B (m x k), U_B = the factor's Cb-side output (r x k)
     int na = 0;
+    std::vector<int> acc;
     for (int q = 0; q < ns; ++q) {
       const double* r = c->h_results + RESULT_STRIDE * q;
-      const int k = (int)r[0];
-      const double total = r[3], captured = r[4];
-      const bool finite = std::isfinite(total) && total > 0.0 && std::isfinite(captured);
-      if (!finite) {
-        char buf[160];
-        std::snprintf(buf, sizeof(buf), "bond %d: theta is zero or not finite", sk_links[q] + 1);
-        return fail(c, TEBD_ERR_NAN, buf);
-      }
-      const bool ok = k >= 1 && (total - captured) <= c->tol_missed * total && r[1] <= c->tol_trunc;
+      const double total = r[2], captured = r[4];
+      const bool ok = used_qr && (total - captured) <= c->tol_missed * total && r[1] <= c->tol_trunc;
+      c->stats[4] += ell[q];
+      c->stats[6] += S3[q].C;
       if (ok) {
-        S3[na] = S3[q];
-        S3[na].lc_commit = const_cast<double*>(S3[q].lc);
+        acc.push_back(q);
+        const int k = (int)r[0];
+        const BondTask& b0 = S0[q];
+        HG[2 * nt + na] = {S2[q].Qs, S3[q].Zc, b0.W, b0.R, k, S3[q].C, b0.R, S3[q].C, b0.R, 1.0, 0};
+        BondTask& w = S4[na];
+        w = S1[q];
+        w.qr = 1;
+        w.transposed = 0;
+        w.Rb = b0.R; w.Cb = b0.C;
+        w.Yr = b0.W;            // left vectors of theta (m x k)
+        w.Zc = S3[q].Yr;        // right vectors of theta (n x k)
+        w.perm = S3[q].perm;
+        w.s = S3[q].s;
+        w.result = S3[q].result;
+        w.gl_out = const_cast<zc*>(w.gl);
+        w.gr_out = const_cast<zc*>(w.gr);
+        w.lc_out = S3[q].lc_out;
+        w.lc_commit = const_cast<double*>(w.lc);
         c->chi[sk_links[q] + 1] = k;
         c->last_rank[sk_links[q] + 1] = k;
         tsum += r[1];
         ++na;
       } else {
-        full_links.push_back(sk_links[q]);
-        full_gates.push_back(gate_ptrs[sk[q]]);
+        fb_links.push_back(sk_links[q]);
+        fb_gates.push_back(gate_ptrs[sk[q]]);
         c->stats[2] += 1;
       }
     }
+    c->stats[5] += na;
     if (na > 0) {
-      HIPCHK(c, hipMemcpyAsync(D3, S3, sizeof(BondTask) * na, hipMemcpyHostToDevice, c->st));
+      HIPCHK(c, hipMemcpyAsync(DG + 2 * nt, HG + 2 * nt, sizeof(GemmTask) * na, hipMemcpyHostToDevice, c->st));
+      HIPCHK(c, hipMemcpyAsync(D4, S4, sizeof(BondTask) * na, hipMemcpyHostToDevice, c->st));
+      {
+        ProfScope ps(c, 4);
+        HIPCHK(c, launch_zgemm(DG + 2 * nt, na, max_m, max_k, false, c->st));
+      }
       {
         ProfScope ps(c, 2);
-        HIPCHK(c, launch_write_gammas(D3, na, c->st));
+        HIPCHK(c, launch_write_gammas(D4, na, c->st));
       }
       HIPCHK(c, hipStreamSynchronize(c->st));
-      drain_profile(c);
       c->stats[0] += na;
     }
+    drain_profile(c);
   }
-  if (!full_links.empty()) {
+  if (!fb_links.empty()) {
     double t = 0.0;
-    int e = run_bonds_full(c, full_links.data(), (int)full_links.size(), full_gates.data(), &t);
+    int e = run_bonds_full(c, fb_links.data(), (int)fb_links.size(), fb_gates.data(), &t);
     if (e) return e;
     tsum += t;
   }
@@ -709,7 +722,7 @@ int ensure_cf_workspace(tebd_ctx* c) {
 }
 
 int run_bonds(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
-  if (c->mode & TEBD_MODE_LEARNED_RANK) return run_bonds_sketch(c, links, nb, gate_ptrs, trunc_sum);
+  if (c->mode & TEBD_MODE_LEARNED_RANK) return run_bonds_learned(c, links, nb, gate_ptrs, trunc_sum);
   return run_bonds_full(c, links, nb, gate_ptrs, trunc_sum);
 }
 
@@ -803,6 +816,9 @@ void free_all(tebd_ctx* c) {
   (void)hipFree(c->wl_count);
   (void)hipFree(c->wl_items);
   (void)hipFree(c->results);
+  (void)hipFree(c->results2);
+  (void)hipFree(c->obs_ops);
+  (void)hipFree(c->ident);
   (void)hipFree(c->d_tasks);
   (void)hipFree(c->cf);
   (void)hipFree(c->cf_s);
@@ -884,6 +900,7 @@ int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, u
   if (bad(hipMalloc(&c->wl_count, sizeof(int) * 2))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->wl_items, sizeof(int2) * 2 * c->wl_cap))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->results, sizeof(double) * RESULT_STRIDE * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->results2, sizeof(double) * RESULT_STRIDE * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->d_tasks, sizeof(BondTask) * nt * kStages))) return TEBD_ERR_HIP;
   if (bad(hipHostMalloc(&c->h_tasks, sizeof(BondTask) * nt * kStages))) return TEBD_ERR_HIP;
   if (mode & TEBD_MODE_LEARNED_RANK) {
@@ -1032,6 +1049,56 @@ int tebd_canonical_form(tebd_ctx* c, int link) {
   if ((e = canonical_form(c, link - 1))) return e;
   HIPCHK(c, hipStreamSynchronize(c->st));
   drain_profile(c);
+  return TEBD_OK;
+}
+
+int tebd_expval_two_site(tebd_ctx* c, const double* ops, double* out) {
+  int e = ensure_state(c, false);
+  if (e) return e;
+  if (!ops || !out) return fail(c, TEBD_ERR_ARG, "expval_two_site: null operator or output");
+  const int L = c->L, d = c->d, nt = c->ntask_max, d2 = d * d;
+  if (!c->obs_ops) {
+    HIPCHK(c, hipMalloc(&c->obs_ops, sizeof(zc) * c->d4() * (L - 1)));
+    HIPCHK(c, hipMalloc(&c->ident, sizeof(zc) * c->d4()));
+    std::vector<zc> id(c->d4());
+    for (int r = 0; r < d2; ++r)
+      for (int q = 0; q < d2; ++q) id[r + (size_t)d2 * q] = zc{r == q ? 1.0 : 0.0, 0.0};
+    HIPCHK(c, hipMemcpy(c->ident, id.data(), sizeof(zc) * c->d4(), hipMemcpyHostToDevice));
+  }
+  HIPCHK(c, hipMemcpyAsync(c->obs_ops, ops, sizeof(zc) * c->d4() * (L - 1), hipMemcpyHostToDevice, c->st));
+  BondTask *S0 = c->h_tasks, *S1 = S0 + nt;
+  BondTask *D0 = c->d_tasks, *D1 = D0 + nt;
+  for (int b0 = 0; b0 < L - 1; b0 += nt) {
+    const int nb = std::min(nt, L - 1 - b0);
+    int max_chil = 1, max_chir = 1;
+    for (int i = 0; i < nb; ++i) {
+      const int link = b0 + i;
+      BondTask& t = S0[i];
+      std::memset(&t, 0, sizeof(t));
+      t.chil = c->chi[link]; t.chic = c->chi[link + 1]; t.chir = c->chi[link + 2]; t.d = d;
+      t.gl = c->gam + (size_t)link * c->gslab();
+      t.gr = c->gam + (size_t)(link + 1) * c->gslab();
+      t.ll = c->lam + (size_t)link * c->chi_max;
+      t.lc = c->lam + (size_t)(link + 1) * c->chi_max;
+      t.lr = c->lam + (size_t)(link + 2) * c->chi_max;
+      t.R = d * t.chil;
+      t.C = d * t.chir;
+      t.gate = c->ident;
+      t.W = c->W + (size_t)i * c->Rmax * c->Cpmax;
+      S1[i] = t;
+      S1[i].gate = c->obs_ops + (size_t)link * c->d4();
+      S1[i].W = c->V + (size_t)i * c->Cpmax * c->Cpmax;
+      max_chil = std::max(max_chil, t.chil);
+      max_chir = std::max(max_chir, t.chir);
+    }
+    HIPCHK(c, hipMemcpyAsync(D0, S0, sizeof(BondTask) * nb, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(D1, S1, sizeof(BondTask) * nb, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_theta_gate(D0, nb, d, max_chil, max_chir, c->st));
+    HIPCHK(c, launch_theta_gate(D1, nb, d, max_chil, max_chir, c->st));
+    HIPCHK(c, launch_zdotc(D0, D1, nb, c->results, c->st));
+    HIPCHK(c, hipMemcpyAsync(out + 2 * b0, c->results, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+  }
   return TEBD_OK;
 }
 

@@ -88,7 +88,6 @@ struct BondTask {
   zc* gr_out;        // new Gamma_r (k x d x chi_r)
   double* lc_out;    // new lambda_c (chi_max entries, zero-filled past k)
   double* lc_commit; // when set, k_write_gammas copies lc_out here (sketched path)
-  double drop_rel;   // k_normalize_cols zeroes columns with |w| <= drop_rel * max |w|
   int chi_max;
   // column-pivoted QR preconditioning (qr = 1): Wq (Rb x Cb) holds the bond matrix and then its
   // Householder QR; the Jacobi (W, R, C above) runs on X = R(0:r, :)^dagger (Cb x r)
@@ -102,4 +101,6 @@ struct BondTask {
   zc* Tq;            // 32 x 32 compact-WY factor of the current panel
   zc* Yr;            // Rb x chi_max: Rb-side singular vectors of the kept values, sorted
   zc* Zc;            // Cb x chi_max: Cb-side singular vectors of the kept values, sorted
+  zc* Qs;            // Rb x r: explicit orthonormal factor (sketched SVD's range basis)
+  const double* wtot;  // when set, the reference total weight (||theta||^2) for the truncation
 };

@@ -123,7 +123,7 @@ __global__ __launch_bounds__(256) void k_theta_gate(const BondTask* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// W padding columns := 0, V := identity, no ||theta||^2 reference
+// W padding columns := 0, V := identity, the reference total weight (||theta||^2 or none)
 // ---------------------------------------------------------------------------
 __global__ void k_prepare_svd(const BondTask* __restrict__ tasks) {
   const BondTask t = tasks[blockIdx.y];
@@ -137,7 +137,7 @@ __global__ void k_prepare_svd(const BondTask* __restrict__ tasks) {
       t.W[(size_t)t.R * t.C + (e - nV)] = zmk(0.0, 0.0);
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) t.result[3] = -1.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) t.result[3] = t.wtot ? *t.wtot : -1.0;
 }
 
 // ---------------------------------------------------------------------------

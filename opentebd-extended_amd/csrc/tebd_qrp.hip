@@ -20,7 +20,8 @@
 //   k_qrp_trail    A2 <- Q_panel^H A2 for the columns right of the panel (MFMA), exact
 //                  remaining norms for the next panel's choice
 //   k_qrp_build_x  X = R(0:r, :)^dagger, zero padding columns
-//   k_qrp_vectors  Rb-side vectors Q V(:, kept) and Cb-side vectors P X(:, kept)/s
+//   k_qrp_vectors  Rb-side vectors Q V(:, kept) and Cb-side vectors P X(:, kept)/s, or the
+//                  explicit Q of a factorisation (the sketched SVD's range basis)
 #include "tebd_device.h"
 #include "tebd_kernels.h"
 
@@ -411,23 +412,26 @@ __global__ void k_qrp_build_x(const BondTask* __restrict__ tasks) {
 }
 
 // Rb-side vectors Y(:, b) = Q [V(:, perm[b]); 0] and Cb-side vectors
-// Z(qpiv[i], b) = X(i, perm[b]) / s[b] for the kept b < k.  One workgroup per
-// NB columns b; thread t owns rows t + 256 q (q < QR) of its columns in registers
-// while the reflectors H_{r-1} .. H_0 are applied.
-template <int QR, int NB>
+// Z(qpiv[i], b) = X(i, perm[b]) / s[b] for the kept b < k (FORM_Q = false), or the
+// explicit orthonormal factor Qs(:, b) = Q e_b, b < r (FORM_Q = true).  One workgroup
+// per NB columns b; thread t owns rows t + 256 q (q < QR) of its columns in
+// registers while the reflectors H_{r-1} .. H_0 are applied.
+template <int QR, int NB, bool FORM_Q>
 __global__ __launch_bounds__(256) void k_qrp_vectors(const BondTask* __restrict__ tasks) {
   const BondTask t = tasks[blockIdx.y];
-  const int k = (int)t.result[0];
+  const int r = t.C;
+  const int k = FORM_Q ? r : (int)t.result[0];
   const int b0 = blockIdx.x * NB;
   if (b0 >= k || t.Rb > 256 * QR) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int r = t.C;
-  // Cb-side vectors
-  for (int bb = 0; bb < NB && b0 + bb < k; ++bb) {
-    const int b = b0 + bb, col = t.perm[b];
-    const double inv = t.s[b] > 0.0 ? 1.0 / t.s[b] : 0.0;
-    for (int i = tid; i < t.Cb; i += 256)
-      t.Zc[t.qpiv[i] + (size_t)t.Cb * b] = zscale(t.W[i + (size_t)t.Cb * col], inv);
+  if (!FORM_Q) {
+    // Cb-side vectors
+    for (int bb = 0; bb < NB && b0 + bb < k; ++bb) {
+      const int b = b0 + bb, col = t.perm[b];
+      const double inv = t.s[b] > 0.0 ? 1.0 / t.s[b] : 0.0;
+      for (int i = tid; i < t.Cb; i += 256)
+        t.Zc[t.qpiv[i] + (size_t)t.Cb * b] = zscale(t.W[i + (size_t)t.Cb * col], inv);
+    }
   }
   // Rb-side vectors
   zc y[QR][NB];
@@ -436,7 +440,8 @@ __global__ __launch_bounds__(256) void k_qrp_vectors(const BondTask* __restrict_
 #pragma unroll
     for (int bb = 0; bb < NB; ++bb) {
       const int i = tid + 256 * q, b = b0 + bb;
-      y[q][bb] = (i < r && b < k) ? t.V[i + (size_t)t.NV * t.perm[b]] : zmk(0.0, 0.0);
+      if (FORM_Q) y[q][bb] = zmk((i == b && b < k) ? 1.0 : 0.0, 0.0);
+      else y[q][bb] = (i < r && b < k) ? t.V[i + (size_t)t.NV * t.perm[b]] : zmk(0.0, 0.0);
     }
   __shared__ double part[2][4][NB][2];
   for (int jj = r - 1; jj >= 0; --jj) {
@@ -486,7 +491,7 @@ __global__ __launch_bounds__(256) void k_qrp_vectors(const BondTask* __restrict_
 #pragma unroll
     for (int bb = 0; bb < NB; ++bb) {
       const int i = tid + 256 * q, b = b0 + bb;
-      if (i < t.Rb && b < k) t.Yr[i + (size_t)t.Rb * b] = y[q][bb];
+      if (i < t.Rb && b < k) (FORM_Q ? t.Qs : t.Yr)[i + (size_t)t.Rb * b] = y[q][bb];
     }
 }
 
@@ -510,15 +515,24 @@ hipError_t launch_qrp_build_x(const BondTask* d_tasks, int ntasks, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_qrp_vectors(const BondTask* d_tasks, int ntasks, int max_rb, int max_k, hipStream_t st) {
+template <bool FORM_Q>
+hipError_t launch_vectors(const BondTask* d_tasks, int ntasks, int max_rb, int max_k, hipStream_t st) {
   auto go = [&](auto kern, int nb) {
     hipLaunchKernelGGL(kern, dim3((max_k + nb - 1) / nb, ntasks), dim3(256), 0, st, d_tasks);
     return hipGetLastError();
   };
-  if (max_rb <= 256) return go(k_qrp_vectors<1, 16>, 16);
-  if (max_rb <= 512) return go(k_qrp_vectors<2, 16>, 16);
-  if (max_rb <= 1024) return go(k_qrp_vectors<4, 8>, 8);
-  if (max_rb <= 2048) return go(k_qrp_vectors<8, 4>, 4);
-  if (max_rb <= 4096) return go(k_qrp_vectors<16, 2>, 2);
+  if (max_rb <= 256) return go(k_qrp_vectors<1, 16, FORM_Q>, 16);
+  if (max_rb <= 512) return go(k_qrp_vectors<2, 16, FORM_Q>, 16);
+  if (max_rb <= 1024) return go(k_qrp_vectors<4, 8, FORM_Q>, 8);
+  if (max_rb <= 2048) return go(k_qrp_vectors<8, 4, FORM_Q>, 4);
+  if (max_rb <= 4096) return go(k_qrp_vectors<16, 2, FORM_Q>, 2);
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_qrp_vectors(const BondTask* d_tasks, int ntasks, int max_rb, int max_k, hipStream_t st) {
+  return launch_vectors<false>(d_tasks, ntasks, max_rb, max_k, st);
+}
+
+hipError_t launch_qrp_form_q(const BondTask* d_tasks, int ntasks, int max_rb, int max_r, hipStream_t st) {
+  return launch_vectors<true>(d_tasks, ntasks, max_rb, max_r, st);
 }

@@ -1,13 +1,12 @@
-// gfx950 kernels of the sketched (randomised) SVD used by the learned-rank path.
+// gfx950 kernels of the sketched (randomised) SVD used by the learned-rank path, and
+// the batched complex GEMM it and CanonicalForm use.
 //
 // The reference always takes the full SVD of theta (ZGESVD,
 // local_operations_module.f90:322-345) and keeps chi values.  The sketched path
-// computes only the leading singular triplets from a right basis [Z0 R]:
-//   Z0 = [ (Gamma_r lambda_r)^dagger | Gaussian ]   the right Schmidt vectors of the
-//        pre-gate theta plus random directions for rank growth
-//   R  = the part of theta^dagger theta Z0 outside Z0 (one block-Krylov step)
-// then one-sided Jacobi on theta [Z0 R] (Rayleigh-Ritz), handing U, S, V to the
-// same truncation / Gamma rebuild kernels as the full SVD.
+// computes only the leading singular triplets from the range of theta Omega, with
+//   Omega = [ (Gamma_r lambda_r)^dagger | Gaussian ]   the right Schmidt vectors of
+//           the pre-gate theta plus random directions for rank growth,
+// whose width is the predicted rank plus oversampling.
 #include "tebd_device.h"
 #include "tebd_kernels.h"
 
@@ -117,25 +116,6 @@ __global__ void k_build_sketch(const BondTask* __restrict__ tasks, const int* __
   }
 }
 
-// Z[:, c] /= ||Z[:, c]|| (s holds squared norms); columns with norm at most
-// drop_rel times the largest are dropped (set to zero).
-__global__ void k_normalize_cols(const BondTask* __restrict__ tasks) {
-  const BondTask t = tasks[blockIdx.y];
-  __shared__ double smax;
-  if (threadIdx.x == 0) {
-    double m = 0.0;
-    for (int c = 0; c < t.C; ++c) m = fmax(m, t.s[c]);
-    smax = m;
-  }
-  __syncthreads();
-  const size_t total = (size_t)t.R * t.Cpad;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(e / t.R);
-    const double s2 = c < t.C ? t.s[c] : 0.0;
-    t.W[e] = (s2 > t.drop_rel * t.drop_rel * smax && s2 > 0.0) ? zscale(t.W[e], 1.0 / sqrt(s2)) : zmk(0.0, 0.0);
-  }
-}
-
 }  // namespace
 
 hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_n, bool conj_a, hipStream_t st) {
@@ -151,8 +131,4 @@ hipError_t launch_build_sketch(const BondTask* d_tasks, const int* d_kprime, int
   return hipGetLastError();
 }
 
-hipError_t launch_normalize_cols(const BondTask* d_tasks, int ntasks, hipStream_t st) {
-  hipLaunchKernelGGL(k_normalize_cols, dim3(128, ntasks), dim3(256), 0, st, d_tasks);
-  return hipGetLastError();
-}
 

@@ -61,12 +61,13 @@ def lib():
         L.tebd_sync.argtypes = [P]
         L.tebd_set_profiling.argtypes = [P, I]
         L.tebd_get_profile.argtypes = [P, P, I]
+        L.tebd_expval_two_site.argtypes = [P, P, P]
         L.tebd_last_error.argtypes = [P]
         L.tebd_last_error.restype = ctypes.c_char_p
         for name in ("tebd_create", "tebd_set_gates", "tebd_upload", "tebd_upload_packed", "tebd_download",
                      "tebd_download_packed", "tebd_download_lambdas", "tebd_trotter_step", "tebd_two_site_op", "tebd_canonical_form_all", "tebd_canonical_form",
                      "tebd_set_cutoff", "tebd_get_bond_dims", "tebd_get_stats", "tebd_sync",
-                     "tebd_set_profiling", "tebd_get_profile"):
+                     "tebd_set_profiling", "tebd_get_profile", "tebd_expval_two_site"):
             getattr(L, name).restype = I
         _lib = L
     return _lib

@@ -123,6 +123,34 @@ class Context:
             out[link - 2] = -np.sum(v * v * np.log(np.abs(v * v))) / np.log(self.d)
         return out
 
+    # -- device observables ---------------------------------------------------
+    def expval_two_site(self, ops):
+        """<psi|O_b|psi> for every bond b = 1..L-1 (ops: (L-1, d^2, d^2) complex, row (a-1)d+b),
+        formed on the device from the resident canonical state (TwoSiteRho,
+        observables_module.f90:704-832)."""
+        ops = np.asarray(ops, np.complex128)
+        if ops.shape != (self.L - 1, self.d ** 2, self.d ** 2):
+            raise ValueError("need one d^2 x d^2 operator per bond")
+        buf = np.ascontiguousarray(np.transpose(ops, (0, 2, 1)))
+        out = np.zeros(2 * (self.L - 1))
+        self._check(self._lib.tebd_expval_two_site(self._h, _p(buf), _p(out)))
+        return out[0::2] + 1j * out[1::2]
+
+    def energy(self, H):
+        """TotalEnergy: sum over bonds of Tr(H_b rho_b) (observables_module.f90:1432-1466)."""
+        return float(np.sum(self.expval_two_site(H).real))
+
+    def one_site_expval(self, op):
+        """<op_i> for every site (OneSiteExpVal, observables_module.f90:134-252): op x 1 on
+        bond i, and 1 x op on the last bond for the last site."""
+        op = np.asarray(op, np.complex128)
+        one = np.eye(self.d)
+        left = np.repeat(np.kron(op, one)[None], self.L - 1, axis=0)
+        vals = self.expval_two_site(left).real
+        right = np.repeat(np.kron(one, op)[None], self.L - 1, axis=0)
+        last = self.expval_two_site(right).real[-1]
+        return np.concatenate([vals, [last]])
+
     # -- the hot path ---------------------------------------------------------
     def trotter_step(self):
         t = ctypes.c_double()
@@ -151,7 +179,7 @@ class Context:
         out = np.zeros(11)
         self._check(self._lib.tebd_get_stats(self._h, _p(out), 11))
         return {"bond_updates": out[0], "jacobi_sweeps": out[1], "fallbacks": out[2], "device_ms": out[3],
-                "sweeps_z0": out[4], "sweeps_resid": out[5], "sweeps_reorth": out[6], "sweeps_rr": out[7],
+                "sketch_cols": out[4], "sketch_accepted": out[5], "sketch_rank": out[6],
                 "pair_grams": out[8], "pair_rotations": out[9], "pair_visits": out[10]}
 
     def set_profiling(self, on=True):
@@ -160,9 +188,9 @@ class Context:
     def profile(self):
         """{kernel: (launches, milliseconds, algorithmic flops, algorithmic HBM bytes)} since
         set_profiling(True)."""
-        out = np.zeros(16)
-        self._check(self._lib.tebd_get_profile(self._h, _p(out), 16))
-        names = ("theta_gate", "jacobi_round", "truncate_gamma", "qr_step")
+        out = np.zeros(20)
+        self._check(self._lib.tebd_get_profile(self._h, _p(out), 20))
+        names = ("theta_gate", "jacobi_round", "truncate_gamma", "qr_step", "sketch_gemm")
         return {n: tuple(out[4 * i:4 * i + 4]) for i, n in enumerate(names)}
 
     def sync(self):
