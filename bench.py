@@ -8,8 +8,9 @@ The bond dimensions grow unevenly from 1 and every bulk bond reaches chi=256 by
 step ~37.  The first W=40 steps are untimed warm-up; the next K=60 steps (41-100)
 are timed, all at the real shape (191 bond updates of up to 1024 x 1024 per step).
 
-`value` is the batched learned-rank path (randomised SVD sized by the rank model,
-full SVD fallback).  The same 100 steps are also run by the exact batched FP64
+`value` is the batched learned-rank path (randomised SVD of chi_c + chi_c/8 + 8
+columns, full SVD fallback; the shipped rank model, trained on a different quench
+(L=64, chi=128), sends the bonds it predicts to be rejected to the full SVD at once).  The same 100 steps are also run by the exact batched FP64
 path: its steps/s are reported, and the learned path's observables at step 100
 (energy, <n_i>, <Sz_i>, entanglement entropy in log_d, cumulative truncation error)
 are compared with it (`obs_err`).  `--compare fp64` times the plain per-bond FP64
@@ -40,11 +41,16 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0    # MI355X HBM3E
 
 
-def make_context(mode):
+RANK_MODEL = os.path.join(ROOT, "opentebd-extended_amd", "tebd_amd", "rank_model_fh.npy")
+
+
+def make_context(mode, rank_model=None):
     from tebd_amd import Context, models as M
 
     H = M.fermi_hubbard(L, T_HOP, U_INT)
     ctx = Context(L, D, CHI, mode=mode)
+    if rank_model is not None and "learned" in mode:
+        ctx.set_rank_model(rank_model)
     ctx.upload(*M.product_state(M.neel(L, D, 1, 2), CHI))
     ctx.set_gates(M.construct_propagators(H, DT))
     return ctx
@@ -80,10 +86,10 @@ def device_observables(ctx, H):
     return dict(E=ctx.energy(H), n=ctx.one_site_expval(ntot), sz=ctx.one_site_expval(sz), S=ctx.chain_entropy())
 
 
-def run_mode(mode, steps, warmup, log, state=None, keep_state=False):
+def run_mode(mode, steps, warmup, log, state=None, keep_state=False, rank_model=None):
     from tebd_amd import models as M
 
-    ctx = make_context(mode)
+    ctx = make_context(mode, rank_model)
     if state is not None:
         ctx.upload(*state)
     trunc_warm = 0.0
@@ -148,6 +154,8 @@ def main():
     ap.add_argument("--compare-steps", type=int, default=3)
     ap.add_argument("--mode", default="batched,learned")
     ap.add_argument("--reference-mode", default="batched", help="exact path for the observable check ('' = skip)")
+    ap.add_argument("--rank-model", default=RANK_MODEL if os.path.exists(RANK_MODEL) else "",
+                    help="rank model parameters (.npy) for the learned-rank path ('' = none)")
     ap.add_argument("--compare", default="", help="also time this mode (e.g. fp64) on the same workload")
     ap.add_argument("--cpu-sample-bonds", type=int, default=3)
     ap.add_argument("--quiet", action="store_true")
@@ -158,7 +166,8 @@ def main():
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    res = run_mode(args.mode, args.steps, args.warmup, log)
+    model = np.load(args.rank_model) if args.rank_model else None
+    res = run_mode(args.mode, args.steps, args.warmup, log, rank_model=model)
     sps = args.steps / res["elapsed"]
     ref = None
     if args.reference_mode:
@@ -219,7 +228,9 @@ def main():
                      "bytes_per_launch": dom_bytes / max(n_launch, 1), "fp64_tflops": tflops,
                      "fp64_frac": tflops / FP64_PEAK_TFLOPS,
                      "device_ms": {n: prof[n][1] for n in prof}},
-        "learned_rank": {"sketch_accepted": st["sketch_accepted"], "full_svd_fallbacks": st["fallbacks"],
+        "learned_rank": {"rank_model": os.path.basename(args.rank_model) if args.rank_model else None,
+                         "sketches_skipped_by_model": st["sketches_skipped"],
+                         "sketch_accepted": st["sketch_accepted"], "full_svd_fallbacks": st["fallbacks"],
                          "mean_sketch_width": st["sketch_cols"] / max(st["sketch_accepted"] + st["fallbacks"], 1)},
         "jacobi": {"sweeps_per_half_step": st["jacobi_sweeps"] / half_steps,
                    "pair_visits": st["pair_visits"], "grams_formed": st["pair_grams"],

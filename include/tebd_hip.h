@@ -134,6 +134,46 @@ int tebd_sync(tebd_ctx* ctx);
 
 const char* tebd_last_error(const tebd_ctx* ctx);
 
+/* ---- rank model of the learned-rank path -------------------------------------
+ * No counterpart in the reference, which always takes the full SVD of every bond
+ * (local_operations_module.f90:322-345).  The learned-rank path sketches theta with
+ * chi_c + max(4, chi_c / 8) + 8 columns (chi_c: the bond's dimension) and redoes the
+ * bond by the full SVD when the sketch misses more than tol_missed of the weight.  A
+ * 66 -> 128 -> 128 -> 1 FP32 perceptron, trained on the path's own record of accepted
+ * (1) and rejected (0) sketches, predicts the acceptance from the last (smallest) 64
+ * Schmidt values of the bond (log10 / 10), the bond index / L and log(1 + step) / 5;
+ * bonds predicted below the threshold go to the full SVD at once, without the wasted
+ * sketch.  Parameters: W1[128][66], b1[128], W2[128][128], b2[128], W3[128], b3
+ * (row-major floats). */
+#define TEBD_MLP_INPUTS 66
+#define TEBD_MLP_HIDDEN 128
+#define TEBD_MLP_PARAMS (TEBD_MLP_HIDDEN * TEBD_MLP_INPUTS + TEBD_MLP_HIDDEN + TEBD_MLP_HIDDEN * TEBD_MLP_HIDDEN + \
+                         TEBD_MLP_HIDDEN + TEBD_MLP_HIDDEN + 1)
+
+/* Learned-rank path settings: a sketch is accepted when it misses at most tol_missed
+ * of ||theta||^2 and discards at most tol_trunc (defaults 1e-20, 1e-10); with a rank
+ * model, bonds whose predicted acceptance is below accept_threshold (0.5) skip it. */
+int tebd_set_learned_params(tebd_ctx* ctx, double tol_missed, double tol_trunc, double accept_threshold);
+/* Install (n = TEBD_MLP_PARAMS) or remove (params = NULL or n = 0) the context's rank
+ * model; without one every bond whose sketch fits (2 x width <= its smaller side) is
+ * sketched. */
+int tebd_rank_model_set(tebd_ctx* ctx, const float* params, int n);
+int tebd_rank_model_get(tebd_ctx* ctx, float* params, int n);
+/* Record (features, accepted) of every sketch of the learned-rank path while on;
+ * switching on clears the record.  tebd_rank_log_get copies up to max_n samples (X: max_n x 66,
+ * y: max_n) and returns the number recorded (or a negative status). */
+int tebd_rank_log(tebd_ctx* ctx, int on);
+int tebd_rank_log_get(tebd_ctx* ctx, float* X, float* y, int max_n);
+/* Stand-alone model kernels on the current device (no context):
+ * mean loss sum w (model(X_s) - y_s)^2 / n, w = w_under where the model is below y;
+ * its gradient; Adam training for `epochs` full-batch steps (loss_history: epochs
+ * values or NULL); inference. */
+int tebd_mlp_gradient(const float* params, const float* X, const float* y, int n, float w_under, float* grad,
+                      float* loss);
+int tebd_mlp_train(float* params, const float* X, const float* y, int n, int epochs, float lr, float w_under,
+                   float* loss_history);
+int tebd_mlp_predict(const float* params, const float* X, int n, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,7 +16,7 @@
 namespace {
 
 constexpr int kStages = 5;
-constexpr int kNumStats = 11;
+constexpr int kNumStats = 13;
 constexpr int kNumProf = 20;
 constexpr double kQrTol = 1e-17;  // QR stops when every remaining column norm is <= kQrTol ||theta||_F
 constexpr int kQrPanel = 32;      // panel width of the blocked pivoted QR (tebd_qrp.hip)
@@ -71,6 +71,15 @@ struct tebd_ctx {
   double* results2 = nullptr;    // learned-rank path: per task ||theta||^2 and column norms of theta
   zc* obs_ops = nullptr;         // (L-1) x d^4 operators of tebd_expval_two_site, allocated on first use
   zc* ident = nullptr;           // d^4 identity gate
+  // rank model (learned-rank path) and the training record of the full path
+  float* mlp = nullptr;          // TEBD_MLP_PARAMS on the device
+  bool have_model = false;
+  float* d_feat = nullptr;       // ntask_max x TEBD_MLP_INPUTS
+  float* d_pred = nullptr;       // ntask_max
+  float* h_feat = nullptr;       // pinned copies
+  float* h_pred = nullptr;
+  bool rank_log = false;
+  std::vector<float> log_X, log_y;
   BondTask* d_tasks = nullptr;   // kStages x ntask_max
   BondTask* h_tasks = nullptr;   // pinned, kStages x ntask_max
   // sketched (learned-rank) path workspace
@@ -84,7 +93,8 @@ struct tebd_ctx {
   unsigned long long sketch_seed = 0x5DEECE66Dull;
   double tol_missed = 1e-20;     // max weight of theta outside the sketch
   double tol_trunc = 1e-10;      // max discarded weight before the full SVD judges
-  int oversample = 8;
+  int oversample = 8;             // heuristic sketch width: chi_c + max(4, chi_c / 8) + oversample
+  float accept_threshold = 0.5f; // with a model: sketch only bonds whose predicted acceptance >= this
   long long step_index = 0;
   // CanonicalForm workspace (allocated on first use): 8 matrices, 3 (s, perm) pairs
   zc* cf = nullptr;
@@ -97,8 +107,10 @@ struct tebd_ctx {
   std::vector<zc> staging;
   std::string err;
   // [0] bond updates, [1] Jacobi sweeps, [2] sketch fallbacks, [3] device ms of trotter steps,
-  // [4] sketch columns requested, [5] sketches accepted, [6] sketch ranks (sum of r_y), [7] unused,
-  // [8] pair Gram matrices formed, [9] pairs rotated, [10] pair visits of the sweeps run
+  // [4] sketch columns requested, [5] sketches accepted, [6] sketch ranks (sum of r_y), [7] rank-model predictions,
+  // [8] pair Gram matrices formed, [9] pairs rotated, [10] pair visits of the sweeps run,
+  // [11] sketches skipped on the rank model's prediction, [12] sketches rejected for discarded weight
+  // (the others of [2] for missed weight)
   double stats[kNumStats] = {};
   // optional per-kernel timing with HIP events on the context's stream:
   // prof[k*4 + 0] launches, [1] milliseconds, [2] algorithmic flops, [3] algorithmic HBM bytes,
@@ -368,6 +380,14 @@ int svd_truncate(tebd_ctx* c, const int* links, int nb, int Cpmax, int Pmax) {
   return TEBD_OK;
 }
 
+// Sketch width for one bond without a rank model: the bond's current dimension with
+// 1/8 growth, plus oversampling.
+int sketch_width(tebd_ctx* c, int link) {
+  const int chic = c->chi[link + 1];
+  const int grow = std::max(4, chic / 8);
+  return std::min(c->chi_max, chic + grow) + c->oversample;
+}
+
 // Process `nb` independent bonds (0-based links) as one batch of tasks.
 int run_bonds_full(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
   const int d = c->d;
@@ -380,6 +400,7 @@ int run_bonds_full(tebd_ctx* c, const int* links, int nb, const zc* const* gate_
     t.chic = c->chi[link + 1];
     t.chir = c->chi[link + 2];
     t.d = d;
+    t.link = link;
     const int m = d * t.chil, n = d * t.chir;
     t.transposed = n > m ? 1 : 0;
     t.R = std::max(m, n);
@@ -437,23 +458,44 @@ int run_bonds_full(tebd_ctx* c, const int* links, int nb, const zc* const* gate_
   return TEBD_OK;
 }
 
-// Sketch width for one bond: the predicted kept rank plus oversampling.
-int sketch_width(tebd_ctx* c, int link) {
-  const int chic = c->chi[link + 1];
-  const int grow = std::max(4, chic / 8);
-  return std::min(c->chi_max, chic + grow) + c->oversample;
-}
 
 int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
   const int d = c->d, nt = c->ntask_max;
   std::vector<int> sk, full;
   std::vector<int> ell;
+  // rank model: features of every bond of the batch (for the prediction and / or the
+  // record), one launch; bonds predicted to be rejected skip the sketch
+  std::vector<char> skip(nb, 0);
+  if (c->have_model || c->rank_log) {
+    BondTask* F = c->h_tasks + 4 * nt;
+    for (int i = 0; i < nb; ++i) {
+      std::memset(&F[i], 0, sizeof(BondTask));
+      F[i].link = links[i];
+      F[i].chic = c->chi[links[i] + 1];
+      F[i].lc = c->lam + (size_t)(links[i] + 1) * c->chi_max;
+    }
+    BondTask* DF = c->d_tasks + 4 * nt;
+    HIPCHK(c, hipMemcpyAsync(DF, F, sizeof(BondTask) * nb, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_mlp_features(DF, nb, c->L, c->step_index, c->d_feat, c->st));
+    if (c->have_model) HIPCHK(c, launch_mlp_predict(c->mlp, c->d_feat, nb, c->d_pred, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->h_feat, c->d_feat, sizeof(float) * TEBD_MLP_INPUTS * nb, hipMemcpyDeviceToHost, c->st));
+    if (c->have_model)
+      HIPCHK(c, hipMemcpyAsync(c->h_pred, c->d_pred, sizeof(float) * nb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (c->have_model) {
+      for (int i = 0; i < nb; ++i) skip[i] = c->h_pred[i] < c->accept_threshold;
+      c->stats[7] += nb;
+    }
+  }
   for (int i = 0; i < nb; ++i) {
     const int link = links[i];
     const int m = d * c->chi[link], n = d * c->chi[link + 2];
     const int l = round_up(std::min(std::min(m, n), sketch_width(c, link)), JC);
     if (2 * l > std::min(m, n)) {
       full.push_back(i);
+    } else if (skip[i]) {
+      full.push_back(i);
+      c->stats[11] += 1;
     } else {
       sk.push_back(i);
       ell.push_back(l);
@@ -488,6 +530,7 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
       base.lr = c->lam + (size_t)(link + 2) * c->chi_max;
       base.gate = gate_ptrs[i];
       base.chil = chil; base.chic = chic; base.chir = chir; base.d = d;
+      base.link = link;
       base.chi_max = c->chi_max;
       base.s = c->s + (size_t)q * c->Cpmax;
       base.perm = c->perm + (size_t)q * c->Cpmax;
@@ -572,6 +615,11 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
       const double* r = c->h_results + RESULT_STRIDE * q;
       const double total = r[2], captured = r[4];
       const bool ok = used_qr && (total - captured) <= c->tol_missed * total && r[1] <= c->tol_trunc;
+      if (c->rank_log) {
+        const float* f = c->h_feat + (size_t)TEBD_MLP_INPUTS * sk[q];
+        c->log_X.insert(c->log_X.end(), f, f + TEBD_MLP_INPUTS);
+        c->log_y.push_back(ok ? 1.0f : 0.0f);
+      }
       c->stats[4] += ell[q];
       c->stats[6] += S3[q].C;
       if (ok) {
@@ -601,6 +649,7 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
         fb_links.push_back(sk_links[q]);
         fb_gates.push_back(gate_ptrs[sk[q]]);
         c->stats[2] += 1;
+        if ((total - captured) <= c->tol_missed * total) c->stats[12] += 1;
       }
     }
     c->stats[5] += na;
@@ -818,6 +867,11 @@ void free_all(tebd_ctx* c) {
   (void)hipFree(c->results);
   (void)hipFree(c->results2);
   (void)hipFree(c->obs_ops);
+  (void)hipFree(c->mlp);
+  (void)hipFree(c->d_feat);
+  (void)hipFree(c->d_pred);
+  (void)hipHostFree(c->h_feat);
+  (void)hipHostFree(c->h_pred);
   (void)hipFree(c->ident);
   (void)hipFree(c->d_tasks);
   (void)hipFree(c->cf);
@@ -901,6 +955,11 @@ int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, u
   if (bad(hipMalloc(&c->wl_items, sizeof(int2) * 2 * c->wl_cap))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->results, sizeof(double) * RESULT_STRIDE * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->results2, sizeof(double) * RESULT_STRIDE * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->mlp, sizeof(float) * TEBD_MLP_PARAMS))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->d_feat, sizeof(float) * TEBD_MLP_INPUTS * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->d_pred, sizeof(float) * nt))) return TEBD_ERR_HIP;
+  if (bad(hipHostMalloc(&c->h_feat, sizeof(float) * TEBD_MLP_INPUTS * nt))) return TEBD_ERR_HIP;
+  if (bad(hipHostMalloc(&c->h_pred, sizeof(float) * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->d_tasks, sizeof(BondTask) * nt * kStages))) return TEBD_ERR_HIP;
   if (bad(hipHostMalloc(&c->h_tasks, sizeof(BondTask) * nt * kStages))) return TEBD_ERR_HIP;
   if (mode & TEBD_MODE_LEARNED_RANK) {
@@ -1100,6 +1159,58 @@ int tebd_expval_two_site(tebd_ctx* c, const double* ops, double* out) {
     HIPCHK(c, hipStreamSynchronize(c->st));
   }
   return TEBD_OK;
+}
+
+int tebd_set_learned_params(tebd_ctx* c, double tol_missed, double tol_trunc, double accept_threshold) {
+  if (!c) return TEBD_ERR_ARG;
+  if (!(tol_missed > 0.0 && tol_missed < 1.0) || !(tol_trunc > 0.0 && tol_trunc < 1.0) ||
+      !(accept_threshold >= 0.0 && accept_threshold <= 1.0))
+    return fail(c, TEBD_ERR_ARG, "learned params: tolerances in (0, 1), threshold in [0, 1]");
+  c->tol_missed = tol_missed;
+  c->tol_trunc = tol_trunc;
+  c->accept_threshold = (float)accept_threshold;
+  return TEBD_OK;
+}
+
+int tebd_rank_model_set(tebd_ctx* c, const float* params, int n) {
+  if (!c) return TEBD_ERR_ARG;
+  if (!params || n == 0) {
+    c->have_model = false;
+    return TEBD_OK;
+  }
+  if (n != TEBD_MLP_PARAMS) return fail(c, TEBD_ERR_ARG, "rank model: wrong parameter count");
+  HIPCHK(c, hipMemcpy(c->mlp, params, sizeof(float) * n, hipMemcpyHostToDevice));
+  c->have_model = true;
+  return TEBD_OK;
+}
+
+int tebd_rank_model_get(tebd_ctx* c, float* params, int n) {
+  if (!c || !params || n != TEBD_MLP_PARAMS) return TEBD_ERR_ARG;
+  if (!c->have_model) return fail(c, TEBD_ERR_STATE, "no rank model installed");
+  HIPCHK(c, hipMemcpy(params, c->mlp, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return TEBD_OK;
+}
+
+int tebd_rank_log(tebd_ctx* c, int on) {
+  if (!c) return TEBD_ERR_ARG;
+  c->rank_log = on != 0;
+  if (c->rank_log) {
+    c->log_X.clear();
+    c->log_y.clear();
+  }
+  return TEBD_OK;
+}
+
+int tebd_rank_log_get(tebd_ctx* c, float* X, float* y, int max_n) {
+  if (!c || max_n < 0) return TEBD_ERR_ARG;
+  const int n = (int)c->log_y.size();
+  const int m = std::min(n, max_n);
+  if (m > 0 && (!X || !y)) return fail(c, TEBD_ERR_ARG, "rank_log_get: null output");
+  if (m > 0) {
+    std::memcpy(X, c->log_X.data(), sizeof(float) * TEBD_MLP_INPUTS * m);
+    std::memcpy(y, c->log_y.data(), sizeof(float) * m);
+  }
+  return n;
 }
 
 int tebd_set_cutoff(tebd_ctx* c, double cutoff) {

@@ -68,6 +68,7 @@ struct BondTask {
   const double* lr;  // lambda_r (chi_r)
   const zc* gate;    // d^2 x d^2 column-major, row (i*d+j)
   int chil, chic, chir, d;
+  int link;          // 0-based bond (sites link, link+1)
   // SVD work
   int transposed;    // 0: W = theta (m x n); 1: W = theta^dagger (n x m)
   int R, C, Cpad, P; // rows, true columns, padded columns, Jacobi pairs (Cpad / JC)

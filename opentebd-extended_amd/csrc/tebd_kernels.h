@@ -1,5 +1,5 @@
 // Host-side launchers of the gfx950 kernels (tebd_kernels.hip, tebd_jacobi.hip, tebd_qrp.hip,
-// tebd_sketch.hip, tebd_canonical.hip, tebd_observables.hip).
+// tebd_sketch.hip, tebd_canonical.hip, tebd_observables.hip, tebd_mlp.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include "tebd_device.h"
@@ -19,6 +19,8 @@ hipError_t launch_qrp_vectors(const BondTask* d_tasks, int ntasks, int max_rb, i
 hipError_t launch_qrp_form_q(const BondTask* d_tasks, int ntasks, int max_rb, int max_r, hipStream_t st);
 hipError_t launch_zdotc(const BondTask* d_a, const BondTask* d_b, int ntasks, double* d_out, hipStream_t st);
 hipError_t launch_finalize(const BondTask* d_tasks, int ntasks, int max_cpad, double cutoff, hipStream_t st);
+hipError_t launch_mlp_features(const BondTask* d_tasks, int ntasks, int L, long long step, float* d_X, hipStream_t st);
+hipError_t launch_mlp_predict(const float* d_params, const float* d_X, int n, float* d_out, hipStream_t st);
 hipError_t launch_write_gammas(const BondTask* d_tasks, int ntasks, hipStream_t st);
 hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_n, bool conj_a, hipStream_t st);
 hipError_t launch_build_sketch(const BondTask* d_tasks, const int* d_kprime, int ntasks, unsigned long long seed,

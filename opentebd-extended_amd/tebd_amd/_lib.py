@@ -23,6 +23,10 @@ MODE_MIXED = 1 << 0
 MODE_BATCHED = 1 << 1
 MODE_LEARNED_RANK = 1 << 2
 
+MLP_INPUTS = 66
+MLP_HIDDEN = 128
+MLP_PARAMS = MLP_HIDDEN * MLP_INPUTS + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN + 1
+
 _lib = None
 
 
@@ -62,12 +66,23 @@ def lib():
         L.tebd_set_profiling.argtypes = [P, I]
         L.tebd_get_profile.argtypes = [P, P, I]
         L.tebd_expval_two_site.argtypes = [P, P, P]
+        L.tebd_rank_model_set.argtypes = [P, P, I]
+        L.tebd_set_learned_params.argtypes = [P, D, D, D]
+        L.tebd_rank_model_get.argtypes = [P, P, I]
+        L.tebd_rank_log.argtypes = [P, I]
+        L.tebd_rank_log_get.argtypes = [P, P, P, I]
+        F = ctypes.c_float
+        L.tebd_mlp_gradient.argtypes = [P, P, P, I, F, P, P]
+        L.tebd_mlp_train.argtypes = [P, P, P, I, I, F, F, P]
+        L.tebd_mlp_predict.argtypes = [P, P, I, P]
         L.tebd_last_error.argtypes = [P]
         L.tebd_last_error.restype = ctypes.c_char_p
         for name in ("tebd_create", "tebd_set_gates", "tebd_upload", "tebd_upload_packed", "tebd_download",
                      "tebd_download_packed", "tebd_download_lambdas", "tebd_trotter_step", "tebd_two_site_op", "tebd_canonical_form_all", "tebd_canonical_form",
                      "tebd_set_cutoff", "tebd_get_bond_dims", "tebd_get_stats", "tebd_sync",
-                     "tebd_set_profiling", "tebd_get_profile", "tebd_expval_two_site"):
+                     "tebd_set_profiling", "tebd_get_profile", "tebd_expval_two_site", "tebd_rank_model_set",
+                     "tebd_set_learned_params", "tebd_rank_model_get", "tebd_rank_log", "tebd_rank_log_get", "tebd_mlp_gradient",
+                     "tebd_mlp_train", "tebd_mlp_predict"):
             getattr(L, name).restype = I
         _lib = L
     return _lib

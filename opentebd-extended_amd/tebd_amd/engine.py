@@ -151,6 +151,36 @@ class Context:
         last = self.expval_two_site(right).real[-1]
         return np.concatenate([vals, [last]])
 
+    # -- rank model of the learned-rank path -----------------------------------
+    def set_learned_params(self, tol_missed=1e-20, tol_trunc=1e-10, accept_threshold=0.5):
+        """Acceptance tolerances of the sketched SVD; with a rank model, bonds predicted
+        below accept_threshold skip the sketch."""
+        self._check(self._lib.tebd_set_learned_params(self._h, tol_missed, tol_trunc, accept_threshold))
+
+    def set_rank_model(self, params):
+        """Decide with this model which bonds the learned-rank path sketches (None: all that fit)."""
+        if params is None:
+            self._check(self._lib.tebd_rank_model_set(self._h, None, 0))
+            return
+        p = np.ascontiguousarray(params, np.float32)
+        self._check(self._lib.tebd_rank_model_set(self._h, _p(p), p.size))
+
+    def rank_log(self, on=True):
+        """Record (features, accepted) of every sketch of the learned-rank path while on."""
+        self._check(self._lib.tebd_rank_log(self._h, int(bool(on))))
+
+    def rank_log_data(self):
+        n = self._lib.tebd_rank_log_get(self._h, None, None, 0)
+        if n < 0:
+            self._check(n)
+        X = np.zeros((n, 66), np.float32)
+        y = np.zeros(n, np.float32)
+        if n:
+            m = self._lib.tebd_rank_log_get(self._h, _p(X), _p(y), n)
+            if m < 0:
+                self._check(m)
+        return X, y
+
     # -- the hot path ---------------------------------------------------------
     def trotter_step(self):
         t = ctypes.c_double()
@@ -176,11 +206,12 @@ class Context:
         return out
 
     def stats(self):
-        out = np.zeros(11)
-        self._check(self._lib.tebd_get_stats(self._h, _p(out), 11))
+        out = np.zeros(13)
+        self._check(self._lib.tebd_get_stats(self._h, _p(out), 13))
         return {"bond_updates": out[0], "jacobi_sweeps": out[1], "fallbacks": out[2], "device_ms": out[3],
-                "sketch_cols": out[4], "sketch_accepted": out[5], "sketch_rank": out[6],
-                "pair_grams": out[8], "pair_rotations": out[9], "pair_visits": out[10]}
+                "sketch_cols": out[4], "sketch_accepted": out[5], "sketch_rank": out[6], "model_predictions": out[7],
+                "pair_grams": out[8], "pair_rotations": out[9], "pair_visits": out[10],
+                "sketches_skipped": out[11], "rejected_discarded": out[12]}
 
     def set_profiling(self, on=True):
         self._check(self._lib.tebd_set_profiling(self._h, int(bool(on))))
