@@ -13,8 +13,9 @@ columns, full SVD fallback; the shipped rank model, trained on a different quenc
 (L=64, chi=128), sends the bonds it predicts to be rejected to the full SVD at once).  The same 100 steps are also run by the exact batched FP64
 path: its steps/s are reported, and the learned path's observables at step 100
 (energy, <n_i>, <Sz_i>, entanglement entropy in log_d, cumulative truncation error)
-are compared with it (`obs_err`).  `--compare fp64` times the plain per-bond FP64
-path on the first steps of the timed window, started from the exact path's state.
+are compared with it (`obs_err`).  The plain per-bond FP64 path (`--compare fp64`,
+the default) is timed on the first steps of the timed window, started from the
+exact path's state; `speedup_vs_fp64` is value over its steps/s.
 
 Prints one JSON line.  `roofline` is the kernel class with the most device time in
 the timed steps (pivoted-QR steps or Jacobi rounds, both HBM-bound streaming
@@ -124,8 +125,30 @@ def run_mode(mode, steps, warmup, log, state=None, keep_state=False, rank_model=
                 snap=snap, obs=obs)
 
 
+def copy_calibration(mib):
+    """Device-to-device copies of a known size: the HBM byte counters (FETCH_SIZE,
+    WRITE_SIZE) of this kernel against the bytes it must move give their scale."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 5}), flush=True)
+
+
 def cpu_baseline(sample_bonds, log):
-    """Reference algorithm on the host: every bond is a padded (chi d)^2 = 1024^2 update."""
+    """Reference algorithm on the host: every bond is a padded (chi d)^2 = 1024^2 update,
+    on one thread (the reference is a sequential program)."""
+    from threadpoolctl import threadpool_limits
+
+    with threadpool_limits(limits=1):
+        return _cpu_baseline(sample_bonds, log)
+
+
+def _cpu_baseline(sample_bonds, log):
     from oracle.oracle import OracleState
     from tebd_amd import models as M
 
@@ -156,15 +179,21 @@ def main():
     ap.add_argument("--reference-mode", default="batched", help="exact path for the observable check ('' = skip)")
     ap.add_argument("--rank-model", default=RANK_MODEL if os.path.exists(RANK_MODEL) else "",
                     help="rank model parameters (.npy) for the learned-rank path ('' = none)")
-    ap.add_argument("--compare", default="", help="also time this mode (e.g. fp64) on the same workload")
+    ap.add_argument("--compare", default="fp64", help="also time this mode on the same workload ('' = skip)")
     ap.add_argument("--cpu-sample-bonds", type=int, default=3)
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only time a device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     log = (lambda s: None) if args.quiet else (lambda s: print(s, file=sys.stderr, flush=True))
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     model = np.load(args.rank_model) if args.rank_model else None
     res = run_mode(args.mode, args.steps, args.warmup, log, rank_model=model)
@@ -195,6 +224,7 @@ def main():
         cmp_res = run_mode(args.compare, args.compare_steps, 0, log, state=ref["snap"])
         extra[f"{args.compare}_steps_per_s"] = args.compare_steps / cmp_res["elapsed"]
         extra[f"{args.compare}_window"] = f"steps {args.warmup + 1}-{args.warmup + args.compare_steps}"
+        extra[f"speedup_vs_{args.compare}"] = sps / extra[f"{args.compare}_steps_per_s"]
 
     cpu = None
     if args.cpu_sample_bonds > 0:
@@ -202,7 +232,7 @@ def main():
         n_updates = 2 * (L // 2) + (L - 1) // 2  # odd, even, odd half-sweeps: 64 + 63 + 64
         cpu = {"value": 1.0 / (t_bond * n_updates), "unit": "Trotter steps/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_sample_bonds} padded 1024x1024 bond updates of the FP64 oracle "
-                         f"(reference algorithm, sequential MKL zgesvd), {t_bond:.2f}s each, "
+                         f"(reference algorithm, single-threaded OpenBLAS zgesvd), {t_bond:.2f}s each, "
                          f"extrapolated x191 bond updates per step"}
 
     out = {

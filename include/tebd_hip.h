@@ -152,7 +152,7 @@ const char* tebd_last_error(const tebd_ctx* ctx);
 
 /* Learned-rank path settings: a sketch is accepted when it misses at most tol_missed
  * of ||theta||^2 and discards at most tol_trunc (defaults 1e-20, 1e-10); with a rank
- * model, bonds whose predicted acceptance is below accept_threshold (0.5) skip it. */
+ * model, bonds whose predicted acceptance is below accept_threshold (default 0.2) skip it. */
 int tebd_set_learned_params(tebd_ctx* ctx, double tol_missed, double tol_trunc, double accept_threshold);
 /* Install (n = TEBD_MLP_PARAMS) or remove (params = NULL or n = 0) the context's rank
  * model; without one every bond whose sketch fits (2 x width <= its smaller side) is

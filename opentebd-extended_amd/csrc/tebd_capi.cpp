@@ -19,8 +19,8 @@ constexpr int kStages = 5;
 constexpr int kNumStats = 13;
 constexpr int kNumProf = 20;
 constexpr double kQrTol = 1e-17;  // QR stops when every remaining column norm is <= kQrTol ||theta||_F
-constexpr int kQrPanel = 32;      // panel width of the blocked pivoted QR (tebd_qrp.hip)
-constexpr int kQrCheck = 4;       // panels between checks whether every task has stopped
+constexpr int kQrPanel = 16;      // panel width of the blocked pivoted QR (tebd_qrp.hip)
+constexpr int kQrCheck = 8;       // panels between checks whether every task has stopped
 constexpr int kQrMinCols = 64;    // bond matrices with fewer columns go straight to the Jacobi
 constexpr int kMaxSweeps = 40;
 constexpr double kTolRel = 1e-13;   // relative orthogonality of converged column pairs
@@ -55,7 +55,7 @@ struct tebd_ctx {
   int* qpiv = nullptr;           // Cpmax
   zc* tau = nullptr;             // Cpmax
   double* qstate = nullptr;      // 2
-  zc* Tq = nullptr;              // 32 x 32
+  zc* Tq = nullptr;              // per slot: the compact-WY factors of up to Cpmax / 16 + 1 panels
   double* h_qstate = nullptr;    // pinned, 2 per task slot
   double* s = nullptr;
   int* perm = nullptr;
@@ -94,7 +94,9 @@ struct tebd_ctx {
   double tol_missed = 1e-20;     // max weight of theta outside the sketch
   double tol_trunc = 1e-10;      // max discarded weight before the full SVD judges
   int oversample = 8;             // heuristic sketch width: chi_c + max(4, chi_c / 8) + oversample
-  float accept_threshold = 0.5f; // with a model: sketch only bonds whose predicted acceptance >= this
+  // with a model: sketch only bonds whose predicted acceptance >= this.  A skipped bond costs a
+  // full SVD (several sketches), so only confident rejections are skipped.
+  float accept_threshold = 0.2f;
   long long step_index = 0;
   // CanonicalForm workspace (allocated on first use): 8 matrices, 3 (s, perm) pairs
   zc* cf = nullptr;
@@ -235,6 +237,9 @@ int jacobi_sweeps(tebd_ctx* c, BondTask* d_t, const BondTask* h_t, int nb, int P
   return TEBD_OK;
 }
 
+// Compact-WY factors kept per slot: one kQrPanel^2 block per panel, min(Rb, Cb) <= Cpmax.
+size_t tq_stride(const tebd_ctx* c) { return (size_t)(c->Cpmax / kQrPanel + 1) * kQrPanel * kQrPanel; }
+
 // Gives task `slot` its pivoted-QR buffers; Wq is the bond matrix slot W.
 void attach_qr(tebd_ctx* c, BondTask& t, int slot) {
   t.Wq = t.W;
@@ -244,7 +249,7 @@ void attach_qr(tebd_ctx* c, BondTask& t, int slot) {
   t.qpiv = c->qpiv + (size_t)slot * c->Cpmax;
   t.tau = c->tau + (size_t)slot * c->Cpmax;
   t.qstate = c->qstate + 2 * slot;
-  t.Tq = c->Tq + (size_t)slot * kQrPanel * kQrPanel;
+  t.Tq = c->Tq + (size_t)slot * tq_stride(c);
   t.Yr = c->Yr + (size_t)slot * c->Rmax * c->chi_max;
   t.Zc = c->Zc + (size_t)slot * c->Cpmax * c->chi_max;
 }
@@ -252,6 +257,8 @@ void attach_qr(tebd_ctx* c, BondTask& t, int slot) {
 // Column-pivoted Householder QR of every task's bond matrix, stopped at the
 // numerical rank.  Returns the ranks in rank[].
 int qrp_factor(tebd_ctx* c, BondTask* D, const BondTask* H, int nb, int max_cb, std::vector<int>& rank) {
+  int max_rb = 1;
+  for (int i = 0; i < nb; ++i) max_rb = std::max(max_rb, H[i].Rb);
   HIPCHK(c, launch_qrp_begin(D, nb, kQrTol, c->st));
   // every task's last panel (j0 + width >= min(Rb, Cb)) records its rank, so after the
   // panel with j0 + kQrPanel >= max_cb all tasks have finished
@@ -259,7 +266,7 @@ int qrp_factor(tebd_ctx* c, BondTask* D, const BondTask* H, int nb, int max_cb, 
   for (int j0 = 0, panel = 0; !done; j0 += kQrPanel, ++panel) {
     {
       ProfScope ps(c, 3);
-      HIPCHK(c, launch_qrp_panel(D, nb, max_cb, j0, c->st));
+      HIPCHK(c, launch_qrp_panel(D, nb, max_rb, max_cb, j0, c->st));
     }
     const bool last = j0 + kQrPanel >= max_cb;
     if ((panel + 1) % kQrCheck == 0 || last) {
@@ -303,6 +310,7 @@ int svd_factor(tebd_ctx* c, BondTask* D, BondTask* H, const int* links, int nb, 
     max_rb = std::max(max_rb, H[i].Rb);
   }
   const bool use_qr = force_qr || max_cb >= kQrMinCols;
+  int max_rank = 1;
   if (use_qr) {
     std::vector<int> rank;
     int e = qrp_factor(c, D, H, nb, max_cb, rank);
@@ -315,6 +323,7 @@ int svd_factor(tebd_ctx* c, BondTask* D, BondTask* H, const int* links, int nb, 
         std::snprintf(buf, sizeof(buf), "bond %d: theta is zero or not finite", links[i] + 1);
         return fail(c, TEBD_ERR_NAN, buf);
       }
+      max_rank = std::max(max_rank, rank[i]);
       BondTask& t = H[i];
       t.qr = 1;
       t.W = c->X + (size_t)i * c->Cpmax * c->Cpmax;
@@ -359,7 +368,7 @@ int svd_factor(tebd_ctx* c, BondTask* D, BondTask* H, const int* links, int nb, 
   }
   if (use_qr) {
     ProfScope ps(c, 2);
-    HIPCHK(c, launch_qrp_vectors(D, nb, max_rb, *max_k, c->st));
+    HIPCHK(c, launch_qrp_vectors(D, nb, *max_k, max_rank, c->st));
   }
   *used_qr = use_qr;
   return TEBD_OK;
@@ -599,7 +608,7 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
     HIPCHK(c, hipMemcpyAsync(DG + nt, HG + nt, sizeof(GemmTask) * ns, hipMemcpyHostToDevice, c->st));
     {
       ProfScope ps(c, 4);
-      HIPCHK(c, launch_qrp_form_q(D2, ns, max_m, max_ry, c->st));
+      HIPCHK(c, launch_qrp_form_q(D2, ns, max_ry, c->st));
       HIPCHK(c, launch_zgemm(DG + nt, ns, max_n, max_ry, true, c->st));
     }
     HIPCHK(c, hipMemcpyAsync(D3, S3, sizeof(BondTask) * ns, hipMemcpyHostToDevice, c->st));
@@ -626,7 +635,8 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
         acc.push_back(q);
         const int k = (int)r[0];
         const BondTask& b0 = S0[q];
-        HG[2 * nt + na] = {S2[q].Qs, S3[q].Zc, b0.W, b0.R, k, S3[q].C, b0.R, S3[q].C, b0.R, 1.0, 0};
+        // U_B has Cb = r rows (the factor's C is B^dagger's own rank, which may be smaller)
+        HG[2 * nt + na] = {S2[q].Qs, S3[q].Zc, b0.W, b0.R, k, S3[q].Cb, b0.R, S3[q].Cb, b0.R, 1.0, 0};
         BondTask& w = S4[na];
         w = S1[q];
         w.qr = 1;
@@ -944,7 +954,7 @@ int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, u
   if (bad(hipMalloc(&c->qpiv, sizeof(int) * (size_t)c->Cpmax * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->tau, sizeof(zc) * (size_t)c->Cpmax * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->qstate, sizeof(double) * 2 * nt))) return TEBD_ERR_HIP;
-  if (bad(hipMalloc(&c->Tq, sizeof(zc) * kQrPanel * kQrPanel * nt))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->Tq, sizeof(zc) * tq_stride(c) * nt))) return TEBD_ERR_HIP;
   if (bad(hipHostMalloc(&c->h_qstate, sizeof(double) * 2 * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->counters, sizeof(int) * 4 * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->blkver, sizeof(unsigned) * c->nblk_max * nt))) return TEBD_ERR_HIP;

@@ -99,7 +99,7 @@ struct BondTask {
   int* qpiv;         // Cb column permutation
   zc* tau;           // Cb reflector scalars
   double* qstate;    // [0] numerical rank (-1 while running), [1] stop threshold on |col|^2
-  zc* Tq;            // 32 x 32 compact-WY factor of the current panel
+  zc* Tq;            // QB x QB compact-WY factors T of the panels (panel p at Tq + p QB^2, QB = 16)
   zc* Yr;            // Rb x chi_max: Rb-side singular vectors of the kept values, sorted
   zc* Zc;            // Cb x chi_max: Cb-side singular vectors of the kept values, sorted
   zc* Qs;            // Rb x r: explicit orthonormal factor (sketched SVD's range basis)

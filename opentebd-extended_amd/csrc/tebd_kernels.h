@@ -13,10 +13,10 @@ hipError_t launch_jacobi_round(const BondTask* d_tasks, int ntasks, int max_pair
                                double tol_rel, double tol_abs, int max_rows_w, int max_rows_v, int* wl_count,
                                int2* wl_items, int wl_cap, hipStream_t st);
 hipError_t launch_qrp_begin(const BondTask* d_tasks, int ntasks, double tol_rel, hipStream_t st);
-hipError_t launch_qrp_panel(const BondTask* d_tasks, int ntasks, int max_cb, int j0, hipStream_t st);
+hipError_t launch_qrp_panel(const BondTask* d_tasks, int ntasks, int max_rb, int max_cb, int j0, hipStream_t st);
 hipError_t launch_qrp_build_x(const BondTask* d_tasks, int ntasks, hipStream_t st);
-hipError_t launch_qrp_vectors(const BondTask* d_tasks, int ntasks, int max_rb, int max_k, hipStream_t st);
-hipError_t launch_qrp_form_q(const BondTask* d_tasks, int ntasks, int max_rb, int max_r, hipStream_t st);
+hipError_t launch_qrp_vectors(const BondTask* d_tasks, int ntasks, int max_k, int max_r, hipStream_t st);
+hipError_t launch_qrp_form_q(const BondTask* d_tasks, int ntasks, int max_r, hipStream_t st);
 hipError_t launch_zdotc(const BondTask* d_a, const BondTask* d_b, int ntasks, double* d_out, hipStream_t st);
 hipError_t launch_finalize(const BondTask* d_tasks, int ntasks, int max_cpad, double cutoff, hipStream_t st);
 hipError_t launch_mlp_features(const BondTask* d_tasks, int ntasks, int L, long long step, float* d_X, hipStream_t st);

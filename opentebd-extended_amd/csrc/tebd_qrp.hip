@@ -2,33 +2,43 @@
 //
 // The bond matrix W (Rb x Cb, Rb >= Cb: theta, or theta^dagger) is factorised as
 // W P = Q R with block column pivoting and stopped at the numerical rank r, where
-// every remaining column norm is below tol_rel * ||W||_F.  Each panel of QB columns
-// takes the QB columns of largest remaining norm (exact norms, recomputed by every
-// trailing update) in descending order; the panel itself is factorised without
+// every remaining column norm is below tol_rel * ||W||_F.  Each panel of QB = 16
+// columns takes the QB columns of largest remaining norm (exact norms, recomputed by
+// every trailing update) in descending order; the panel itself is factorised without
 // further pivoting and applied to the rest as one compact-WY block (BLAS3).  On
 // late-time Fermi-Hubbard bonds this ordering gives the Jacobi the same few sweeps
-// as column-by-column pivoting (xGEQP3) at a 32-fold smaller trailing traffic.  The one-sided Jacobi then runs on
+// as column-by-column pivoting (xGEQP3).  The one-sided Jacobi then runs on
 // X = R(0:r, :)^dagger (Cb x r), whose columns are graded and nearly orthogonal:
 // on late-time Fermi-Hubbard bonds (1024 x 1024, rank ~830) it converges in 4
 // sweeps instead of ~22 for W itself, and the sweeps cover r instead of Cb columns.
 // With X V = U_x S:  W = (Q V) S (P U_x)^dagger, so the Rb-side singular vectors
-// are Q V (k_qrp_vectors applies the reflectors) and the Cb-side ones P U_x.
+// are Q V and the Cb-side ones P U_x.
 //
-//   k_qrp_begin    column norms, ||W||_F, identity pivot, rank := running
-//   k_qrp_panel    panel j0: the QB largest remaining columns, descending, swapped into
-//                  place; their unpivoted Householder QR; the compact-WY factor T
-//   k_qrp_trail    A2 <- Q_panel^H A2 for the columns right of the panel (MFMA), exact
-//                  remaining norms for the next panel's choice
-//   k_qrp_build_x  X = R(0:r, :)^dagger, zero padding columns
-//   k_qrp_vectors  Rb-side vectors Q V(:, kept) and Cb-side vectors P X(:, kept)/s, or the
-//                  explicit Q of a factorisation (the sketched SVD's range basis)
+//   k_qrp_begin        column norms, ||W||_F, identity pivot, rank := running
+//   k_qrp_panel_reg    panel j0 for Rb <= 1024: the QB largest remaining columns are
+//                      gathered into registers (512 threads, one or two rows each),
+//                      factorised there (one 32-value reduce-scatter per reflector
+//                      gives the reflector's products with the later panel columns
+//                      and the earlier reflectors at once) and stored in place, with
+//                      the displaced columns moved to the vacated slots; the
+//                      compact-WY factor T of the panel is kept (panel p at Tq + p QB^2)
+//   k_qrp_panel        the same for any Rb, through global memory (256 threads)
+//   k_qrp_reflect<0>   trailing update A2 <- Q_panel^H A2 on FP64 MFMA and the exact
+//                      remaining norms for the next panel's choice
+//   k_qrp_build_x      X = R(0:r, :)^dagger, zero padding columns
+//   k_qrp_vectors_init Cb-side vectors P X(:, kept)/s and the start [V(:, kept); 0] of the
+//                      Rb-side ones, or the identity for the explicit Q
+//   k_qrp_reflect<1,2> Q applied to those columns as the stored panels' block
+//                      reflectors, last panel first: Y <- Y - V_p (T_p (V_p^H Y))
 #include "tebd_device.h"
 #include "tebd_kernels.h"
 
 namespace {
 
-constexpr int QB = 32;          // panel width
+constexpr int QB = 16;          // panel width
 constexpr int kMaxCols = 4096;  // largest Cb (d chi_max <= 4096)
+constexpr int PT = 512;         // threads of the register-resident panel kernel
+constexpr int PW = PT / 64;     // its waves
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -75,30 +85,33 @@ __device__ __forceinline__ d4 mfma4(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// Per-wave partial sums of D = L^H R (32 x 32 as four 16x16 blocks) over rows
-// r < K: wave w takes rows 8 (w + 4 it) .. +7 in two MFMA k-steps of 4 rows; lload(i, c)
-// and rload(i, c) return element (i, c) of L and R (zero outside).
-template <class LL, class RL>
-__device__ __forceinline__ void gram32_partial(LL lload, RL rload, int K, int wave, int lane, d4 (&dre)[2][2],
-                                               d4 (&dim)[2][2]) {
+// Per-wave partial sums of D = L^H R ((16 NP) x (16 NQ) as NP x NQ blocks of 16x16)
+// over rows r < K: wave w of nw takes rows 8 (w + nw it) .. +7 in two MFMA k-steps of
+// 4 rows; lload(i, c) and rload(i, c) return element (i, c) of L and R (zero outside).
+template <int NP, int NQ, class LL, class RL>
+__device__ __forceinline__ void gram_partial(LL lload, RL rload, int K, int wave, int nw, int lane,
+                                             d4 (&dre)[NP][NQ], d4 (&dim)[NP][NQ]) {
   const int lr = lane & 15, lk = lane >> 4;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < NP; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < NQ; ++b) {
       dre[a][b] = d4{0.0, 0.0, 0.0, 0.0};
       dim[a][b] = d4{0.0, 0.0, 0.0, 0.0};
     }
-  for (int r0 = 8 * wave; r0 < K; r0 += 32) {
+  for (int r0 = 8 * wave; r0 < K; r0 += 8 * nw) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int i = r0 + 4 * h + lk;
-      const zc l[2] = {lload(i, lr), lload(i, 16 + lr)};
-      const zc r[2] = {rload(i, lr), rload(i, 16 + lr)};
+      zc l[NP], r[NQ];
 #pragma unroll
-      for (int pb = 0; pb < 2; ++pb)
+      for (int pb = 0; pb < NP; ++pb) l[pb] = lload(i, 16 * pb + lr);
 #pragma unroll
-        for (int qb = 0; qb < 2; ++qb) {
+      for (int qb = 0; qb < NQ; ++qb) r[qb] = rload(i, 16 * qb + lr);
+#pragma unroll
+      for (int pb = 0; pb < NP; ++pb)
+#pragma unroll
+        for (int qb = 0; qb < NQ; ++qb) {
           dre[pb][qb] = mfma4(l[pb].x, r[qb].x, dre[pb][qb]);
           dre[pb][qb] = mfma4(l[pb].y, r[qb].y, dre[pb][qb]);
           dim[pb][qb] = mfma4(l[pb].x, r[qb].y, dim[pb][qb]);
@@ -108,14 +121,15 @@ __device__ __forceinline__ void gram32_partial(LL lload, RL rload, int K, int wa
   }
 }
 
-// Fixed-order sum of the four waves' partials into S[32][33] (bitwise reproducible).
-__device__ __forceinline__ void gram32_reduce(const d4 (&dre)[2][2], const d4 (&dim)[2][2], double (*red)[2][256],
-                                              zc (*S)[QB + 1], int wave, int lane) {
+// Fixed-order sum of the 4 waves' partials into S (row stride lds; bitwise reproducible).
+template <int NP, int NQ>
+__device__ __forceinline__ void gram_reduce(const d4 (&dre)[NP][NQ], const d4 (&dim)[NP][NQ],
+                                            double (*red)[2][256], zc* S, int lds, int wave, int lane) {
   const int lr = lane & 15, lk = lane >> 4;
 #pragma unroll
-  for (int pb = 0; pb < 2; ++pb)
+  for (int pb = 0; pb < NP; ++pb)
 #pragma unroll
-    for (int qb = 0; qb < 2; ++qb) {
+    for (int qb = 0; qb < NQ; ++qb) {
       if (wave > 0) {
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
@@ -132,146 +146,18 @@ __device__ __forceinline__ void gram32_reduce(const d4 (&dre)[2][2], const d4 (&
             xr += red[w][0][g * 64 + lane];
             xi += red[w][1][g * 64 + lane];
           }
-          S[16 * pb + lk + 4 * g][16 * qb + lr] = zmk(xr, xi);
+          S[(16 * pb + lk + 4 * g) * lds + 16 * qb + lr] = zmk(xr, xi);
         }
       }
       __syncthreads();
     }
 }
 
-// Panel j0 (one workgroup per task):
-//  1. the nbp = min(QB, kmax - j0) columns of largest remaining norm, in descending
-//     order, are swapped into columns j0 .. j0+nbp-1 (if even the largest is below the
-//     stop threshold the numerical rank is j0 and the factorisation ends);
-//  2. unpivoted Householder QR of the panel rows j0.. (xLARFG convention);
-//  3. T (nbp x nbp, upper) of the compact WY form H_0 .. H_{nbp-1} = I - V T V^H.
-__global__ __launch_bounds__(256) void k_qrp_panel(const BondTask* __restrict__ tasks, int j0) {
-  const BondTask t = tasks[blockIdx.x];
-  if (t.qstate[0] >= 0.0) return;
-  const int kmax = t.Rb < t.Cb ? t.Rb : t.Cb;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (j0 >= kmax) {
-    if (tid == 0) t.qstate[0] = (double)kmax;
-    return;
-  }
-  const int nbp = (kmax - j0) < QB ? (kmax - j0) : QB;
-  __shared__ double rv[256];
-  __shared__ int ri[256];
-  __shared__ double red[4];
-  __shared__ int sel[QB];
-  __shared__ unsigned char taken[kMaxCols];
-  __shared__ zc G[QB][QB + 1];
-  __shared__ zc T[QB][QB + 1];
-  __shared__ zc stau[QB];
-  __shared__ double gred[3][2][256];
-  for (int k = tid; k < t.Cb; k += 256) taken[k] = 0;
-  __syncthreads();
-  // 1. selection: nbp rounds of argmax (ties: lowest index)
-  for (int q = 0; q < nbp; ++q) {
-    double bv = -1.0;
-    int bi = -1;
-    for (int k = j0 + tid; k < t.Cb; k += 256) {
-      const double v = t.qnrm[k];
-      if (!taken[k] && v > bv) { bv = v; bi = k; }
-    }
-    rv[tid] = bv;
-    ri[tid] = bi;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-      if (tid < st) {
-        const double o = rv[tid + st];
-        const int oi = ri[tid + st];
-        if (oi >= 0 && (o > rv[tid] || ri[tid] < 0 || (o == rv[tid] && oi < ri[tid]))) {
-          rv[tid] = o;
-          ri[tid] = oi;
-        }
-      }
-      __syncthreads();
-    }
-    if (q == 0 && !(rv[0] > t.qstate[1])) {
-      if (tid == 0) t.qstate[0] = (double)j0;  // numerical rank reached
-      return;
-    }
-    if (tid == 0) {
-      sel[q] = ri[0];
-      taken[ri[0]] = 1;
-    }
-    __syncthreads();
-  }
-  // swap the selected columns into place, in order
-  for (int q = 0; q < nbp; ++q) {
-    const int p = sel[q], dst = j0 + q;
-    if (p != dst) {
-      zc* a = t.Wq + (size_t)t.Rb * dst;
-      zc* b = t.Wq + (size_t)t.Rb * p;
-      for (int i = tid; i < t.Rb; i += 256) {
-        const zc x = a[i];
-        a[i] = b[i];
-        b[i] = x;
-      }
-    }
-    __syncthreads();
-    if (tid == 0 && p != dst) {
-      const double tn = t.qnrm[dst]; t.qnrm[dst] = t.qnrm[p]; t.qnrm[p] = tn;
-      const int tp = t.qpiv[dst]; t.qpiv[dst] = t.qpiv[p]; t.qpiv[p] = tp;
-      for (int q2 = q + 1; q2 < nbp; ++q2)
-        if (sel[q2] == dst) sel[q2] = p;
-    }
-    __syncthreads();
-  }
-  // 2. Householder QR of the panel
-  for (int q = 0; q < nbp; ++q) {
-    const int j = j0 + q;
-    zc* cj = t.Wq + (size_t)t.Rb * j;
-    double part = 0.0;
-    for (int i = j + 1 + tid; i < t.Rb; i += 256) part += zabs2(cj[i]);
-    const double xn2 = block_sum(part, red);
-    const zc alpha = cj[j];
-    zc tau = zmk(0.0, 0.0);
-    if (!(xn2 == 0.0 && alpha.y == 0.0)) {
-      const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-      const double beta = alpha.x >= 0.0 ? -nrm : nrm;
-      tau = zmk((beta - alpha.x) / beta, -alpha.y / beta);
-      const zc am = zmk(alpha.x - beta, alpha.y);
-      const double den = zabs2(am);
-      const zc scale = zmk(am.x / den, -am.y / den);
-      __syncthreads();  // every thread has read alpha
-      for (int i = j + 1 + tid; i < t.Rb; i += 256) cj[i] = zmul(cj[i], scale);
-      if (tid == 0) cj[j] = zmk(beta, 0.0);
-    }
-    if (tid == 0) {
-      t.tau[j] = tau;
-      stau[q] = tau;
-    }
-    __syncthreads();
-    if (tau.x != 0.0 || tau.y != 0.0) {
-      const zc ctau = zconj(tau);
-      for (int k = j + 1 + wave; k < j0 + nbp; k += 4) {
-        zc* a = t.Wq + (size_t)t.Rb * k;
-        zc w = zmk(0.0, 0.0);
-        for (int i = j + lane; i < t.Rb; i += 64) zfmac(w, (i == j) ? zmk(1.0, 0.0) : cj[i], a[i]);
-        w.x = wave_sum(w.x);
-        w.y = wave_sum(w.y);
-        const zc f = zmul(ctau, w);
-        for (int i = j + lane; i < t.Rb; i += 64) a[i] = zsub(a[i], zmul(f, (i == j) ? zmk(1.0, 0.0) : cj[i]));
-      }
-    }
-    __syncthreads();
-  }
-  // 3. T: T[q][q] = tau_q, T[0:q, q] = -tau_q T[0:q, 0:q] (V^H v_q)[0:q]
-  {
-    const int K = t.Rb - j0;
-    const zc* P = t.Wq + (size_t)t.Rb * j0 + j0;
-    auto vload = [&](int i, int q) -> zc {
-      if (q >= nbp || i < q || i >= K) return zmk(0.0, 0.0);
-      if (i == q) return zmk(1.0, 0.0);
-      return P[i + (size_t)t.Rb * q];
-    };
-    d4 gre[2][2], gim[2][2];
-    gram32_partial(vload, vload, K, wave, lane, gre, gim);
-    gram32_reduce(gre, gim, gred, G, wave, lane);
-  }
-  if (tid == 0) {
+// T (nbp x nbp, upper) of the compact WY form H_0 .. H_{nbp-1} = I - V T V^H from the
+// reflector scalars and G = V^H V:  T[q][q] = tau_q, T[0:q, q] = -tau_q T[0:q, 0:q] G[0:q, q]
+// (one thread); written to the panel's slot of Tq, zero-padded to QB x QB.
+__device__ void form_t(const zc (*G)[QB + 1], const zc* stau, zc (*T)[QB + 1], int nbp, zc* Tp) {
+  if (threadIdx.x == 0) {
     for (int q = 0; q < nbp; ++q) {
       for (int pp = 0; pp < q; ++pp) {
         zc acc = zmk(0.0, 0.0);
@@ -283,31 +169,442 @@ __global__ __launch_bounds__(256) void k_qrp_panel(const BondTask* __restrict__ 
     }
   }
   __syncthreads();
-  for (int e = tid; e < QB * QB; e += 256) {
+  for (int e = threadIdx.x; e < QB * QB; e += blockDim.x) {
     const int pp = e % QB, qq = e / QB;
-    t.Tq[e] = (pp < nbp && qq < nbp) ? T[pp][qq] : zmk(0.0, 0.0);
+    Tp[e] = (pp < nbp && qq < nbp) ? T[pp][qq] : zmk(0.0, 0.0);
   }
+}
+
+// Householder scalars of the column (alpha; x) with |x|^2 = xn2 (xLARFG convention):
+// H^H (alpha; x) = (beta; 0) with H = I - tau v v^H, v = (1; x * scale).  Returns false
+// (tau = 0, H = I) when the column is already (real alpha; 0).
+__device__ __forceinline__ bool householder(zc alpha, double xn2, zc& tau, double& beta, zc& scale) {
+  tau = zmk(0.0, 0.0);
+  if (xn2 == 0.0 && alpha.y == 0.0) return false;
+  const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+  beta = alpha.x >= 0.0 ? -nrm : nrm;
+  tau = zmk((beta - alpha.x) / beta, -alpha.y / beta);
+  const zc am = zmk(alpha.x - beta, alpha.y);
+  const double den = zabs2(am);
+  scale = zmk(am.x / den, -am.y / den);
+  return true;
+}
+
+// Column moves of a panel: sel[0..nbp) (distinct columns >= j0) go to j0..j0+nbp-1 in
+// that order; the columns of j0..j0+nbp-1 not selected go to the selected slots beyond
+// the panel (both lists ascending), so every column lands in exactly one place.  One
+// thread; updates qpiv and qnrm and returns the moves in mv_src/mv_dst.
+__device__ int panel_moves(const BondTask& t, int j0, int nbp, const int* sel, int* mv_src, int* mv_dst,
+                           int* piv_tmp, double* nrm_tmp) {
+  int nmv = 0, nvac = 0;
+  for (int d = j0; d < j0 + nbp; ++d) {
+    bool in = false;
+    for (int q = 0; q < nbp; ++q) in = in || sel[q] == d;
+    if (!in) mv_src[nmv++] = d;
+  }
+  for (int q = 0; q < nbp; ++q)
+    if (sel[q] >= j0 + nbp) {
+      int m = nvac++;  // insertion into the ascending list
+      for (; m > 0 && mv_dst[m - 1] > sel[q]; --m) mv_dst[m] = mv_dst[m - 1];
+      mv_dst[m] = sel[q];
+    }
+  for (int q = 0; q < nbp; ++q) piv_tmp[q] = t.qpiv[sel[q]];
+  for (int m = 0; m < nmv; ++m) {
+    piv_tmp[QB + m] = t.qpiv[mv_src[m]];
+    nrm_tmp[m] = t.qnrm[mv_src[m]];
+  }
+  for (int q = 0; q < nbp; ++q) t.qpiv[j0 + q] = piv_tmp[q];
+  for (int m = 0; m < nmv; ++m) {
+    t.qpiv[mv_dst[m]] = piv_tmp[QB + m];
+    t.qnrm[mv_dst[m]] = nrm_tmp[m];
+  }
+  return nmv;
+}
+
+// Panel j0 with the panel in registers (Rb <= RPT * PT; one workgroup of PT threads per
+// task, thread t owns rows t + PT s, s < RPT):
+//  1. wave 0 picks the nbp = min(QB, kmax - j0) columns of largest remaining norm in
+//     descending order (ties: lowest index); if even the largest is below the stop
+//     threshold the numerical rank is j0 and the factorisation ends;
+//  2. the picked columns are loaded into registers; the displaced ones are moved;
+//  3. unpivoted Householder QR of the panel rows j0.. (xLARFG convention).  For
+//     reflector q one fixed-order reduction yields w_k = v_q^H a_k (k > q, for the
+//     update) and G[p][q] = v_p^H v_q (p < q, for T);
+//  4. the panel is stored back and T formed.
+template <int RPT>
+__global__ __launch_bounds__(PT) void k_qrp_panel_reg(const BondTask* __restrict__ tasks, int j0) {
+  const BondTask t = tasks[blockIdx.x];
+  if (t.qstate[0] >= 0.0) return;
+  const int Rb = t.Rb, Cb = t.Cb;
+  const int kmax = Rb < Cb ? Rb : Cb;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (j0 >= kmax) {
+    if (tid == 0) t.qstate[0] = (double)kmax;
+    return;
+  }
+  const int nbp = (kmax - j0) < QB ? (kmax - j0) : QB;
+  const int ncand = Cb - j0;
+  const size_t ld = Rb;
+  __shared__ double snrm[kMaxCols];
+  __shared__ int sel[QB], mv_src[QB], mv_dst[QB], piv_tmp[2 * QB];
+  __shared__ double nrm_tmp[QB];
+  __shared__ int snmv, sstop;
+  __shared__ double red[PW][32];
+  __shared__ double wsum[32];
+  __shared__ double rnorm[PW];
+  __shared__ zc salpha;
+  __shared__ zc G[QB][QB + 1];
+  __shared__ zc T[QB][QB + 1];
+  __shared__ zc stau[QB];
+
+  // 1. selection
+  for (int k = tid; k < ncand; k += PT) snrm[k] = t.qnrm[j0 + k];
+  if (tid == 0) sstop = 0;
+  if (tid < QB) sel[tid] = j0;
+  __syncthreads();
+  if (wave == 0) {
+    for (int q = 0; q < nbp; ++q) {
+      double bv = -1.0;
+      int bi = -1;
+      for (int k = lane; k < ncand; k += 64) {
+        const double v = snrm[k];
+        if (v > bv) { bv = v; bi = k; }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(bv, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (oi >= 0 && (bi < 0 || ov > bv || (ov == bv && oi < bi))) { bv = ov; bi = oi; }
+      }
+      // every lane holds the same winner; a lane only ever reads the entries it clears
+      if (q == 0 && !(bv > t.qstate[1])) {
+        if (lane == 0) sstop = 1;  // numerical rank reached (or the norms are not finite)
+        break;
+      }
+      if (bi < 0) {
+        if (lane == 0) sstop = 2;  // no finite candidate left
+        break;
+      }
+      if (lane == 0) sel[q] = j0 + bi;
+      if (lane == (bi & 63)) snrm[bi] = -1.0;
+    }
+  }
+  __syncthreads();
+  if (sstop) {
+    if (tid == 0) t.qstate[0] = sstop == 1 ? (double)j0 : 0.0;
+    return;
+  }
+  if (tid == 0) snmv = panel_moves(t, j0, nbp, sel, mv_src, mv_dst, piv_tmp, nrm_tmp);
+  __syncthreads();
+
+  // 2. gather the panel; each thread moves its own rows of the displaced columns
+  zc a[RPT][QB];
+  const int nmv = snmv;
+#pragma unroll
+  for (int s = 0; s < RPT; ++s) {
+    const int i = tid + PT * s;
+#pragma unroll
+    for (int q = 0; q < QB; ++q) a[s][q] = (i < Rb && q < nbp) ? t.Wq[i + ld * sel[q]] : zmk(0.0, 0.0);
+    if (i < Rb)
+      for (int m = 0; m < nmv; ++m) t.Wq[i + ld * mv_dst[m]] = t.Wq[i + ld * mv_src[m]];
+  }
+
+  // 3. Householder QR of the panel.  The columns rotate through a[]: at step q the
+  // current column is a[.][0], the later columns k = q + m sit at m = 1 .. 15 - q and
+  // the earlier reflectors p at m = 16 - q + p, so every register index is static.
+  for (int q = 0; q < nbp; ++q) {
+    const int j = j0 + q;
+    const int nlater = QB - 1 - q;
+    double part = 0.0;
+#pragma unroll
+    for (int s = 0; s < RPT; ++s) {
+      const int i = tid + PT * s;
+      if (i > j && i < Rb) part += zabs2(a[s][0]);
+      if (i == j) salpha = a[s][0];
+    }
+    part = wave_sum(part);
+    if (lane == 0) rnorm[wave] = part;
+    __syncthreads();
+    double xn2 = 0.0;
+#pragma unroll
+    for (int w = 0; w < PW; ++w) xn2 += rnorm[w];
+    zc tau, scale = zmk(1.0, 0.0);
+    double beta = 0.0;
+    const bool refl = householder(salpha, xn2, tau, beta, scale);
+    // v_q (rows > j; v_j = 1 implicitly) and R(j, j)
+    if (refl) {
+#pragma unroll
+      for (int s = 0; s < RPT; ++s) {
+        const int i = tid + PT * s;
+        if (i > j && i < Rb) a[s][0] = zmul(a[s][0], scale);
+        if (i == j) a[s][0] = zmk(beta, 0.0);
+      }
+    }
+    // slot m <= nlater: conj(v_q) a_(q+m);  slot m > nlater: conj(v_p) v_q
+    double x[2 * QB];
+#pragma unroll
+    for (int e = 0; e < 2 * QB; ++e) x[e] = 0.0;
+#pragma unroll
+    for (int s = 0; s < RPT; ++s) {
+      const int i = tid + PT * s;
+      if (i >= j && i < Rb) {
+        const zc vq = (i == j) ? zmk(1.0, 0.0) : a[s][0];
+#pragma unroll
+        for (int m = 1; m < QB; ++m) {
+          const bool later = m <= nlater;  // earlier reflectors: i >= j > j0 + p
+          const zc l = later ? vq : a[s][m];
+          const zc r = later ? a[s][m] : vq;
+          x[2 * m] = fma(l.x, r.x, fma(l.y, r.y, x[2 * m]));
+          x[2 * m + 1] = fma(l.x, r.y, fma(-l.y, r.x, x[2 * m + 1]));
+        }
+      }
+    }
+    // reduce-scatter of the 32 values over the wave: lane l ends with value
+    // 16 b5 + 8 b4 + 4 b3 + 2 b2 + b1 (b = bits of l), summed over all 64 lanes
+    {
+      double y16[16], y8[8], y4[4], y2[2];
+      const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8, h2 = lane & 4, h1 = lane & 2;
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        const double lo = x[m], hi = x[m + 16];
+        y16[m] = (h5 ? hi : lo) + __shfl_xor(h5 ? lo : hi, 32, 64);
+      }
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const double lo = y16[m], hi = y16[m + 8];
+        y8[m] = (h4 ? hi : lo) + __shfl_xor(h4 ? lo : hi, 16, 64);
+      }
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const double lo = y8[m], hi = y8[m + 4];
+        y4[m] = (h3 ? hi : lo) + __shfl_xor(h3 ? lo : hi, 8, 64);
+      }
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        const double lo = y4[m], hi = y4[m + 2];
+        y2[m] = (h2 ? hi : lo) + __shfl_xor(h2 ? lo : hi, 4, 64);
+      }
+      double y1 = (h1 ? y2[1] : y2[0]) + __shfl_xor(h1 ? y2[0] : y2[1], 2, 64);
+      y1 += __shfl_xor(y1, 1, 64);
+      const int idx = 16 * ((lane >> 5) & 1) + 8 * ((lane >> 4) & 1) + 4 * ((lane >> 3) & 1) +
+                      2 * ((lane >> 2) & 1) + ((lane >> 1) & 1);
+      if ((lane & 1) == 0) red[wave][idx] = y1;
+    }
+    __syncthreads();
+    if (tid < 32) {
+      double sum = 0.0;
+#pragma unroll
+      for (int w = 0; w < PW; ++w) sum += red[w][tid];
+      wsum[tid] = sum;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int m = nlater + 1; m < QB; ++m) G[m - QB + q][q] = zmk(wsum[2 * m], wsum[2 * m + 1]);
+      stau[q] = tau;
+      t.tau[j] = tau;
+    }
+    if (refl) {
+      const zc ctau = zconj(tau);
+#pragma unroll
+      for (int m = 1; m < QB; ++m) {
+        if (m <= nlater) {
+          const zc f = zmul(ctau, zmk(wsum[2 * m], wsum[2 * m + 1]));
+#pragma unroll
+          for (int s = 0; s < RPT; ++s) {
+            const int i = tid + PT * s;
+            if (i >= j && i < Rb) a[s][m] = zsub(a[s][m], (i == j) ? f : zmul(f, a[s][0]));
+          }
+        }
+      }
+    }
+    // rotate: the finished reflector goes to the end
+#pragma unroll
+    for (int s = 0; s < RPT; ++s) {
+      const zc c0 = a[s][0];
+#pragma unroll
+      for (int m = 0; m < QB - 1; ++m) a[s][m] = a[s][m + 1];
+      a[s][QB - 1] = c0;
+    }
+  }
+
+  // 4. store the panel (R above and on the diagonal, reflectors below; column c now sits
+  // at m = (c - nbp) mod QB) and T
+#pragma unroll
+  for (int s = 0; s < RPT; ++s) {
+    const int i = tid + PT * s;
+    if (i < Rb) {
+#pragma unroll
+      for (int m = 0; m < QB; ++m) {
+        const int c = (m + nbp) & (QB - 1);
+        if (c < nbp) t.Wq[i + ld * (j0 + c)] = a[s][m];
+      }
+    }
+  }
+  form_t(G, stau, T, nbp, t.Tq + (size_t)(j0 / QB) * QB * QB);
   if (tid == 0 && j0 + nbp >= kmax) t.qstate[0] = (double)kmax;
 }
 
-// Trailing update of panel j0 on 32 columns per workgroup:
-//   A2 <- Q_panel^H A2 = A2 - V (T^H (V^H A2))
-// with Y = V^H A2 and A2 -= V Z on FP64 MFMA (V: rows j0.., implicit unit diagonal),
-// and the exact remaining norms |A2(j0+nbp:, c)|^2 into qnrm.
-__global__ __launch_bounds__(256) void k_qrp_trail(const BondTask* __restrict__ tasks, int j0) {
-  const BondTask t = tasks[blockIdx.y];
-  const double rank = t.qstate[0];
-  if (rank >= 0.0 && rank <= (double)j0) return;
+// Panel j0 through global memory (any Rb; one workgroup of 256 threads per task): the
+// same selection, column moves, factorisation and T as k_qrp_panel_reg, with the
+// reflector applied column by column (one wave per column).
+__global__ __launch_bounds__(256) void k_qrp_panel(const BondTask* __restrict__ tasks, int j0) {
+  const BondTask t = tasks[blockIdx.x];
+  if (t.qstate[0] >= 0.0) return;
   const int kmax = t.Rb < t.Cb ? t.Rb : t.Cb;
-  if (j0 >= kmax) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (j0 >= kmax) {
+    if (tid == 0) t.qstate[0] = (double)kmax;
+    return;
+  }
   const int nbp = (kmax - j0) < QB ? (kmax - j0) : QB;
-  const int kb = j0 + nbp + 32 * blockIdx.x;
-  if (kb >= t.Cb) return;
-  const int nc = (t.Cb - kb) < 32 ? (t.Cb - kb) : 32;
-  const int K = t.Rb - j0;
+  const int ncand = t.Cb - j0;
   const size_t ld = t.Rb;
+  __shared__ double snrm[kMaxCols];
+  __shared__ int sel[QB], mv_src[QB], mv_dst[QB], piv_tmp[2 * QB];
+  __shared__ double nrm_tmp[QB];
+  __shared__ int snmv, sstop;
+  __shared__ double red[4];
+  __shared__ zc G[QB][QB + 1];
+  __shared__ zc T[QB][QB + 1];
+  __shared__ zc stau[QB];
+  __shared__ double gred[3][2][256];
+  for (int k = tid; k < ncand; k += 256) snrm[k] = t.qnrm[j0 + k];
+  if (tid == 0) sstop = 0;
+  if (tid < QB) sel[tid] = j0;
+  __syncthreads();
+  if (wave == 0) {
+    for (int q = 0; q < nbp; ++q) {
+      double bv = -1.0;
+      int bi = -1;
+      for (int k = lane; k < ncand; k += 64) {
+        const double v = snrm[k];
+        if (v > bv) { bv = v; bi = k; }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(bv, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (oi >= 0 && (bi < 0 || ov > bv || (ov == bv && oi < bi))) { bv = ov; bi = oi; }
+      }
+      if (q == 0 && !(bv > t.qstate[1])) {
+        if (lane == 0) sstop = 1;
+        break;
+      }
+      if (bi < 0) {
+        if (lane == 0) sstop = 2;
+        break;
+      }
+      if (lane == 0) sel[q] = j0 + bi;
+      if (lane == (bi & 63)) snrm[bi] = -1.0;
+    }
+  }
+  __syncthreads();
+  if (sstop) {
+    if (tid == 0) t.qstate[0] = sstop == 1 ? (double)j0 : 0.0;
+    return;
+  }
+  if (tid == 0) snmv = panel_moves(t, j0, nbp, sel, mv_src, mv_dst, piv_tmp, nrm_tmp);
+  __syncthreads();
+  // row by row: gather the selected columns, move the displaced ones to the vacated
+  // slots, then write the selected ones into the panel
+  for (int i = tid; i < t.Rb; i += 256) {
+    zc v[QB];
+#pragma unroll
+    for (int q = 0; q < QB; ++q) v[q] = q < nbp ? t.Wq[i + ld * sel[q]] : zmk(0.0, 0.0);
+    for (int m = 0; m < snmv; ++m) t.Wq[i + ld * mv_dst[m]] = t.Wq[i + ld * mv_src[m]];
+#pragma unroll
+    for (int q = 0; q < QB; ++q)
+      if (q < nbp) t.Wq[i + ld * (j0 + q)] = v[q];
+  }
+  __syncthreads();
+  for (int q = 0; q < nbp; ++q) {
+    const int j = j0 + q;
+    zc* cj = t.Wq + ld * j;
+    double part = 0.0;
+    for (int i = j + 1 + tid; i < t.Rb; i += 256) part += zabs2(cj[i]);
+    const double xn2 = block_sum(part, red);
+    const zc alpha = cj[j];
+    zc tau, scale = zmk(1.0, 0.0);
+    double beta = 0.0;
+    const bool refl = householder(alpha, xn2, tau, beta, scale);
+    __syncthreads();  // every thread has read alpha
+    if (refl) {
+      for (int i = j + 1 + tid; i < t.Rb; i += 256) cj[i] = zmul(cj[i], scale);
+      if (tid == 0) cj[j] = zmk(beta, 0.0);
+    }
+    if (tid == 0) {
+      t.tau[j] = tau;
+      stau[q] = tau;
+    }
+    __syncthreads();
+    if (refl) {
+      const zc ctau = zconj(tau);
+      for (int k = j + 1 + wave; k < j0 + nbp; k += 4) {
+        zc* a = t.Wq + ld * k;
+        zc w = zmk(0.0, 0.0);
+        for (int i = j + lane; i < t.Rb; i += 64) zfmac(w, (i == j) ? zmk(1.0, 0.0) : cj[i], a[i]);
+        w.x = wave_sum(w.x);
+        w.y = wave_sum(w.y);
+        const zc f = zmul(ctau, w);
+        for (int i = j + lane; i < t.Rb; i += 64) a[i] = zsub(a[i], zmul(f, (i == j) ? zmk(1.0, 0.0) : cj[i]));
+      }
+    }
+    __syncthreads();
+  }
+  {
+    const int K = t.Rb - j0;
+    const zc* P = t.Wq + ld * j0 + j0;
+    auto vload = [&](int i, int q) -> zc {
+      if (q >= nbp || i < q || i >= K) return zmk(0.0, 0.0);
+      if (i == q) return zmk(1.0, 0.0);
+      return P[i + ld * q];
+    };
+    d4 gre[1][1], gim[1][1];
+    gram_partial<1, 1>(vload, vload, K, wave, 4, lane, gre, gim);
+    gram_reduce<1, 1>(gre, gim, gred, &G[0][0], QB + 1, wave, lane);
+  }
+  form_t(G, stau, T, nbp, t.Tq + (size_t)(j0 / QB) * QB * QB);
+  if (tid == 0 && j0 + nbp >= kmax) t.qstate[0] = (double)kmax;
+}
+
+// Block reflector of one panel on 32 columns per workgroup (FP64 MFMA):
+//   MODE 0  trailing update of panel j0 = arg: A2 <- Q_p^H A2 = A2 - V (T^H (V^H A2)) on the
+//           columns right of the panel, and the exact remaining norms |A2(j0+nbp:, c)|^2
+//           into qnrm;
+//   MODE 1  panel p = arg of the factor applied to the Rb-side vectors Yr (k columns):
+//           Y <- Q_p Y = Y - V (T (V^H Y));
+//   MODE 2  the same on the explicit factor Qs (r columns, identity start: column
+//           blocks entirely left of the panel are still unit vectors and are skipped).
+// V has rows j0.. with an implicit unit diagonal; only rows j0.. of the target change.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_qrp_reflect(const BondTask* __restrict__ tasks, int arg) {
+  const BondTask t = tasks[blockIdx.y];
+  int j0, nbp, kb, nc;
+  const size_t ld = t.Rb;
+  zc* A2;
+  if (MODE == 0) {
+    j0 = arg;
+    const double rank = t.qstate[0];
+    if (rank >= 0.0 && rank <= (double)j0) return;
+    const int kmax = t.Rb < t.Cb ? t.Rb : t.Cb;
+    if (j0 >= kmax) return;
+    nbp = (kmax - j0) < QB ? (kmax - j0) : QB;
+    kb = j0 + nbp + 32 * blockIdx.x;
+    if (kb >= t.Cb) return;
+    nc = (t.Cb - kb) < 32 ? (t.Cb - kb) : 32;
+    A2 = t.Wq + ld * kb + j0;
+  } else {
+    const int r = t.C;
+    j0 = QB * arg;
+    if (j0 >= r) return;
+    nbp = (r - j0) < QB ? (r - j0) : QB;
+    const int ncols = MODE == 2 ? r : (int)t.result[0];
+    kb = 32 * blockIdx.x;
+    if (kb >= ncols || (MODE == 2 && kb + 31 < j0)) return;
+    nc = (ncols - kb) < 32 ? (ncols - kb) : 32;
+    A2 = (MODE == 2 ? t.Qs : t.Yr) + ld * kb + j0;
+  }
+  const int K = t.Rb - j0;
   const zc* P = t.Wq + ld * j0 + j0;  // P[i + ld q] = A(j0 + i, j0 + q)
-  zc* A2 = t.Wq + ld * kb + j0;       // A2[i + ld c] = A(j0 + i, kb + c)
+  const zc* Tp = t.Tq + (size_t)(j0 / QB) * QB * QB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   auto vload = [&](int i, int q) -> zc {
@@ -320,25 +617,29 @@ __global__ __launch_bounds__(256) void k_qrp_trail(const BondTask* __restrict__ 
     return A2[i + ld * c];
   };
   __shared__ double red[3][2][256];
-  __shared__ zc Y[QB][QB + 1];
+  __shared__ zc Y[QB][33];
   __shared__ zc Ts[QB][QB + 1];
   __shared__ double nsum[4][32];
-  for (int e = tid; e < QB * QB; e += 256) Ts[e % QB][e / QB] = t.Tq[e];
-  // ---- Y = V^H A2 (32 x 32), rows split over the waves
+  for (int e = tid; e < QB * QB; e += 256) Ts[e % QB][e / QB] = Tp[e];
+  // ---- Y = V^H A2 (16 x 32), rows split over the waves
   {
-    d4 yre[2][2], yim[2][2];
-    gram32_partial(vload, aload, K, wave, lane, yre, yim);
-    gram32_reduce(yre, yim, red, Y, wave, lane);
+    d4 yre[1][2], yim[1][2];
+    gram_partial<1, 2>(vload, aload, K, wave, 4, lane, yre, yim);
+    gram_reduce<1, 2>(yre, yim, red, &Y[0][0], 33, wave, lane);
   }
-  // ---- Z = T^H Y, held as MFMA A operands Z[4ks + lk][16ct + lr]
-  double zre[2][8], zim[2][8];
+  // ---- Z = T^H Y (MODE 0) or T Y, held as MFMA A operands Z[4ks + lk][16ct + lr]
+  double zre[2][4], zim[2][4];
 #pragma unroll
   for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-    for (int ks = 0; ks < 8; ++ks) {
+    for (int ks = 0; ks < 4; ++ks) {
       const int pr = 4 * ks + lk, qc = 16 * ct + lr;
       zc acc = zmk(0.0, 0.0);
-      for (int kk = 0; kk <= pr; ++kk) zfmac(acc, Ts[kk][pr], Y[kk][qc]);
+      if (MODE == 0) {
+        for (int kk = 0; kk <= pr; ++kk) zfmac(acc, Ts[kk][pr], Y[kk][qc]);
+      } else {
+        for (int kk = pr; kk < QB; ++kk) zfma(acc, Ts[pr][kk], Y[kk][qc]);
+      }
       zre[ct][ks] = acc.x;
       zim[ct][ks] = acc.y;
     }
@@ -350,16 +651,16 @@ __global__ __launch_bounds__(256) void k_qrp_trail(const BondTask* __restrict__ 
     for (int g = 0; g < 4; ++g) ns[ct][g] = 0.0;
   for (int r0 = 16 * wave; r0 < K; r0 += 64) {
     const int i = r0 + lr;
-    zc x[8];
+    zc x[4];
 #pragma unroll
-    for (int ks = 0; ks < 8; ++ks) x[ks] = vload(i, 4 * ks + lk);
+    for (int ks = 0; ks < 4; ++ks) x[ks] = vload(i, 4 * ks + lk);
     d4 ore[2], oim[2];
 #pragma unroll
     for (int ct = 0; ct < 2; ++ct) {
       ore[ct] = d4{0.0, 0.0, 0.0, 0.0};
       oim[ct] = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int ks = 0; ks < 8; ++ks) {
+      for (int ks = 0; ks < 4; ++ks) {
         ore[ct] = mfma4(zre[ct][ks], x[ks].x, ore[ct]);
         ore[ct] = mfma4(-zim[ct][ks], x[ks].y, ore[ct]);
         oim[ct] = mfma4(zre[ct][ks], x[ks].y, oim[ct]);
@@ -376,11 +677,12 @@ __global__ __launch_bounds__(256) void k_qrp_trail(const BondTask* __restrict__ 
             zc* ap = A2 + i + ld * c;
             const zc a = zsub(*ap, zmk(ore[ct][g], oim[ct][g]));
             *ap = a;
-            if (i >= nbp) ns[ct][g] += zabs2(a);
+            if (MODE == 0 && i >= nbp) ns[ct][g] += zabs2(a);
           }
         }
     }
   }
+  if (MODE != 0) return;
   // column norms: reduce over the 16 lanes of equal lk, then over the waves
 #pragma unroll
   for (int ct = 0; ct < 2; ++ct)
@@ -411,88 +713,26 @@ __global__ void k_qrp_build_x(const BondTask* __restrict__ tasks) {
   }
 }
 
-// Rb-side vectors Y(:, b) = Q [V(:, perm[b]); 0] and Cb-side vectors
-// Z(qpiv[i], b) = X(i, perm[b]) / s[b] for the kept b < k (FORM_Q = false), or the
-// explicit orthonormal factor Qs(:, b) = Q e_b, b < r (FORM_Q = true).  One workgroup
-// per NB columns b; thread t owns rows t + 256 q (q < QR) of its columns in
-// registers while the reflectors H_{r-1} .. H_0 are applied.
-template <int QR, int NB, bool FORM_Q>
-__global__ __launch_bounds__(256) void k_qrp_vectors(const BondTask* __restrict__ tasks) {
+// Column b (one workgroup) of the start of the singular vectors: for the kept b < k the
+// Cb-side vector Z(qpiv[i], b) = X(i, perm[b]) / s[b] and the Rb-side start
+// Y(:, b) = [V(0:r, perm[b]); 0] (FORM_Q = false), or Qs(:, b) = e_b, b < r (FORM_Q = true).
+template <bool FORM_Q>
+__global__ __launch_bounds__(256) void k_qrp_vectors_init(const BondTask* __restrict__ tasks) {
   const BondTask t = tasks[blockIdx.y];
   const int r = t.C;
   const int k = FORM_Q ? r : (int)t.result[0];
-  const int b0 = blockIdx.x * NB;
-  if (b0 >= k || t.Rb > 256 * QR) return;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (!FORM_Q) {
-    // Cb-side vectors
-    for (int bb = 0; bb < NB && b0 + bb < k; ++bb) {
-      const int b = b0 + bb, col = t.perm[b];
-      const double inv = t.s[b] > 0.0 ? 1.0 / t.s[b] : 0.0;
-      for (int i = tid; i < t.Cb; i += 256)
-        t.Zc[t.qpiv[i] + (size_t)t.Cb * b] = zscale(t.W[i + (size_t)t.Cb * col], inv);
-    }
+  const int b = blockIdx.x;
+  if (b >= k) return;
+  if (FORM_Q) {
+    for (int i = threadIdx.x; i < t.Rb; i += 256) t.Qs[i + (size_t)t.Rb * b] = zmk(i == b ? 1.0 : 0.0, 0.0);
+    return;
   }
-  // Rb-side vectors
-  zc y[QR][NB];
-#pragma unroll
-  for (int q = 0; q < QR; ++q)
-#pragma unroll
-    for (int bb = 0; bb < NB; ++bb) {
-      const int i = tid + 256 * q, b = b0 + bb;
-      if (FORM_Q) y[q][bb] = zmk((i == b && b < k) ? 1.0 : 0.0, 0.0);
-      else y[q][bb] = (i < r && b < k) ? t.V[i + (size_t)t.NV * t.perm[b]] : zmk(0.0, 0.0);
-    }
-  __shared__ double part[2][4][NB][2];
-  for (int jj = r - 1; jj >= 0; --jj) {
-    const zc tau = t.tau[jj];
-    const zc* v = t.Wq + (size_t)t.Rb * jj;
-    zc w[NB];
-#pragma unroll
-    for (int bb = 0; bb < NB; ++bb) w[bb] = zmk(0.0, 0.0);
-#pragma unroll
-    for (int q = 0; q < QR; ++q) {
-      const int i = tid + 256 * q;
-      if (i >= jj && i < t.Rb) {
-        const zc vi = (i == jj) ? zmk(1.0, 0.0) : v[i];
-#pragma unroll
-        for (int bb = 0; bb < NB; ++bb) zfmac(w[bb], vi, y[q][bb]);
-      }
-    }
-    const int buf = jj & 1;
-#pragma unroll
-    for (int bb = 0; bb < NB; ++bb) {
-      const double wr = wave_sum(w[bb].x), wi = wave_sum(w[bb].y);
-      if (lane == 0) {
-        part[buf][wave][bb][0] = wr;
-        part[buf][wave][bb][1] = wi;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int bb = 0; bb < NB; ++bb) {
-      const zc ws = zmk(part[buf][0][bb][0] + part[buf][1][bb][0] + part[buf][2][bb][0] + part[buf][3][bb][0],
-                        part[buf][0][bb][1] + part[buf][1][bb][1] + part[buf][2][bb][1] + part[buf][3][bb][1]);
-      w[bb] = zmul(tau, ws);
-    }
-#pragma unroll
-    for (int q = 0; q < QR; ++q) {
-      const int i = tid + 256 * q;
-      if (i >= jj && i < t.Rb) {
-        const zc vi = (i == jj) ? zmk(1.0, 0.0) : v[i];
-#pragma unroll
-        for (int bb = 0; bb < NB; ++bb) y[q][bb] = zsub(y[q][bb], zmul(w[bb], vi));
-      }
-    }
-    // the other buffer is rewritten two reflectors later, after the next barrier
-  }
-#pragma unroll
-  for (int q = 0; q < QR; ++q)
-#pragma unroll
-    for (int bb = 0; bb < NB; ++bb) {
-      const int i = tid + 256 * q, b = b0 + bb;
-      if (i < t.Rb && b < k) (FORM_Q ? t.Qs : t.Yr)[i + (size_t)t.Rb * b] = y[q][bb];
-    }
+  const int col = t.perm[b];
+  const double inv = t.s[b] > 0.0 ? 1.0 / t.s[b] : 0.0;
+  for (int i = threadIdx.x; i < t.Cb; i += 256)
+    t.Zc[t.qpiv[i] + (size_t)t.Cb * b] = zscale(t.W[i + (size_t)t.Cb * col], inv);
+  for (int i = threadIdx.x; i < t.Rb; i += 256)
+    t.Yr[i + (size_t)t.Rb * b] = i < r ? t.V[i + (size_t)t.NV * col] : zmk(0.0, 0.0);
 }
 
 }  // namespace
@@ -502,11 +742,14 @@ hipError_t launch_qrp_begin(const BondTask* d_tasks, int ntasks, double tol_rel,
   return hipGetLastError();
 }
 
-hipError_t launch_qrp_panel(const BondTask* d_tasks, int ntasks, int max_cb, int j0, hipStream_t st) {
-  hipLaunchKernelGGL(k_qrp_panel, dim3(ntasks), dim3(256), 0, st, d_tasks, j0);
+hipError_t launch_qrp_panel(const BondTask* d_tasks, int ntasks, int max_rb, int max_cb, int j0, hipStream_t st) {
+  if (max_cb > kMaxCols) return hipErrorInvalidValue;
+  if (max_rb <= PT) hipLaunchKernelGGL(k_qrp_panel_reg<1>, dim3(ntasks), dim3(PT), 0, st, d_tasks, j0);
+  else if (max_rb <= 2 * PT) hipLaunchKernelGGL(k_qrp_panel_reg<2>, dim3(ntasks), dim3(PT), 0, st, d_tasks, j0);
+  else hipLaunchKernelGGL(k_qrp_panel, dim3(ntasks), dim3(256), 0, st, d_tasks, j0);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || j0 + QB >= max_cb) return e;
-  hipLaunchKernelGGL(k_qrp_trail, dim3((max_cb - j0 - 1) / 32 + 1, ntasks), dim3(256), 0, st, d_tasks, j0);
+  hipLaunchKernelGGL(k_qrp_reflect<0>, dim3((max_cb - j0 - 1) / 32 + 1, ntasks), dim3(256), 0, st, d_tasks, j0);
   return hipGetLastError();
 }
 
@@ -515,24 +758,22 @@ hipError_t launch_qrp_build_x(const BondTask* d_tasks, int ntasks, hipStream_t s
   return hipGetLastError();
 }
 
-template <bool FORM_Q>
-hipError_t launch_vectors(const BondTask* d_tasks, int ntasks, int max_rb, int max_k, hipStream_t st) {
-  auto go = [&](auto kern, int nb) {
-    hipLaunchKernelGGL(kern, dim3((max_k + nb - 1) / nb, ntasks), dim3(256), 0, st, d_tasks);
-    return hipGetLastError();
-  };
-  if (max_rb <= 256) return go(k_qrp_vectors<1, 16, FORM_Q>, 16);
-  if (max_rb <= 512) return go(k_qrp_vectors<2, 16, FORM_Q>, 16);
-  if (max_rb <= 1024) return go(k_qrp_vectors<4, 8, FORM_Q>, 8);
-  if (max_rb <= 2048) return go(k_qrp_vectors<8, 4, FORM_Q>, 4);
-  if (max_rb <= 4096) return go(k_qrp_vectors<16, 2, FORM_Q>, 2);
-  return hipErrorInvalidValue;
+hipError_t launch_qrp_vectors(const BondTask* d_tasks, int ntasks, int max_k, int max_r, hipStream_t st) {
+  hipLaunchKernelGGL(k_qrp_vectors_init<false>, dim3(max_k, ntasks), dim3(256), 0, st, d_tasks);
+  hipError_t e = hipGetLastError();
+  for (int p = (max_r + QB - 1) / QB - 1; p >= 0 && e == hipSuccess; --p) {
+    hipLaunchKernelGGL(k_qrp_reflect<1>, dim3((max_k + 31) / 32, ntasks), dim3(256), 0, st, d_tasks, p);
+    e = hipGetLastError();
+  }
+  return e;
 }
 
-hipError_t launch_qrp_vectors(const BondTask* d_tasks, int ntasks, int max_rb, int max_k, hipStream_t st) {
-  return launch_vectors<false>(d_tasks, ntasks, max_rb, max_k, st);
-}
-
-hipError_t launch_qrp_form_q(const BondTask* d_tasks, int ntasks, int max_rb, int max_r, hipStream_t st) {
-  return launch_vectors<true>(d_tasks, ntasks, max_rb, max_r, st);
+hipError_t launch_qrp_form_q(const BondTask* d_tasks, int ntasks, int max_r, hipStream_t st) {
+  hipLaunchKernelGGL(k_qrp_vectors_init<true>, dim3(max_r, ntasks), dim3(256), 0, st, d_tasks);
+  hipError_t e = hipGetLastError();
+  for (int p = (max_r + QB - 1) / QB - 1; p >= 0 && e == hipSuccess; --p) {
+    hipLaunchKernelGGL(k_qrp_reflect<2>, dim3((max_r + 31) / 32, ntasks), dim3(256), 0, st, d_tasks, p);
+    e = hipGetLastError();
+  }
+  return e;
 }

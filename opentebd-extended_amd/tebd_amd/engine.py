@@ -152,7 +152,7 @@ class Context:
         return np.concatenate([vals, [last]])
 
     # -- rank model of the learned-rank path -----------------------------------
-    def set_learned_params(self, tol_missed=1e-20, tol_trunc=1e-10, accept_threshold=0.5):
+    def set_learned_params(self, tol_missed=1e-20, tol_trunc=1e-10, accept_threshold=0.2):
         """Acceptance tolerances of the sketched SVD; with a rank model, bonds predicted
         below accept_threshold skip the sketch."""
         self._check(self._lib.tebd_set_learned_params(self._h, tol_missed, tol_trunc, accept_threshold))
