@@ -25,6 +25,10 @@ constexpr int kQrMinCols = 64;    // bond matrices with fewer columns go straigh
 constexpr int kMaxSweeps = 40;
 constexpr double kTolRel = 1e-13;   // relative orthogonality of converged column pairs
 constexpr double kTolAbs = 1e-13;   // rounding floor: |A_pq| <= kTolAbs ||theta|| (|w_p| + |w_q|)
+// The first kEarlySweeps outer sweeps solve each pair's 32 x 32 problem with at most
+// kEarlyInnerSweeps inner sweeps (the pair is revisited anyway); later ones to convergence.
+constexpr int kEarlySweeps = 4;
+constexpr int kEarlyInnerSweeps = 2;
 
 int round_up(int x, int m) { return ((x + m - 1) / m) * m; }
 
@@ -204,11 +208,12 @@ int jacobi_sweeps(tebd_ctx* c, BondTask* d_t, const BondTask* h_t, int nb, int P
   int sweeps = 0;
   bool converged = false;
   while (sweeps < kMaxSweeps && !converged) {
+    const int inner = sweeps < kEarlySweeps ? kEarlyInnerSweeps : 0;  // 0: to convergence
     HIPCHK(c, launch_jacobi_sweep_begin(d_t, nb, sweeps, c->wl_count, c->st));
     for (int r = 0; r < 2 * Pmax - 1; ++r) {
       ProfScope ps(c, 1);
       HIPCHK(c, launch_jacobi_round(d_t, nb, Pmax, sweeps, r, kTolRel, kTolAbs, rows_w, rows_v, c->wl_count,
-                                    c->wl_items, c->wl_cap, c->st));
+                                    c->wl_items, c->wl_cap, inner, c->st));
     }
     HIPCHK(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(int) * 4 * nb, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));

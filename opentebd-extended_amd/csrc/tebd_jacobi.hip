@@ -171,12 +171,12 @@ __device__ __forceinline__ void inner_pair(int rd, int t, int& p, int& q) {
 // a purely relative test (no rotation when |A_pq| <= 1e-15 sqrt(A_pp A_qq)), sets
 // each annihilated entry to exactly zero and ends after a sweep without rotations.
 __device__ void inner_jacobi(zc (*A)[JC + 1], zc (*Q)[JC + 1], double* rc_c, double* rc_s, zc* rc_w, int* s_round,
-                             int* s_sweep, int tid) {
+                             int* s_sweep, int tid, int max_sweeps) {
   for (int e = tid; e < JC * JC; e += 256) {
     const int r = e % JC, c = e / JC;
     Q[r][c] = zmk(r == c ? 1.0 : 0.0, 0.0);
   }
-  for (int isw = 0; isw < kInnerSweeps; ++isw) {
+  for (int isw = 0; isw < max_sweeps; ++isw) {
     if (tid == 0) s_sweep[isw & 1] = 0;
     for (int rd = 0; rd < JC - 1; ++rd) {
       if (tid == 0) s_round[rd & 1] = 0;
@@ -252,7 +252,7 @@ __device__ void inner_jacobi(zc (*A)[JC + 1], zc (*Q)[JC + 1], double* rc_c, dou
 template <bool FULL>
 __global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict__ tasks, int sweep, int round,
                                                       double tol_rel, double tol_abs, int* __restrict__ wl_count,
-                                                      int2* __restrict__ wl_items, int wl_cap) {
+                                                      int2* __restrict__ wl_items, int wl_cap, int inner_sweeps) {
   const BondTask t = tasks[blockIdx.y];
   const int p = blockIdx.x;
   if (p >= t.P || round >= 2 * t.P - 1) return;
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict
   }
 
   // ---- eigenvectors of A -> Q, to the pair's slot; queue the pair for rotation
-  inner_jacobi(A, Q, rc_c, rc_s, rc_w, s_round, s_sweep, tid);
+  inner_jacobi(A, Q, rc_c, rc_s, rc_w, s_round, s_sweep, tid, inner_sweeps);
   zc* qg = t.Qg + (size_t)p * JC * JC;
   for (int e = tid; e < JC * JC; e += 256) qg[e] = Q[e / JC][e % JC];  // qg[k JC + c] = Q[k][c]
   if (tid == 0) {
@@ -456,13 +456,14 @@ hipError_t launch_jacobi_sweep_begin(const BondTask* d_tasks, int ntasks, int sw
 
 hipError_t launch_jacobi_round(const BondTask* d_tasks, int ntasks, int max_pairs, int sweep, int round,
                                double tol_rel, double tol_abs, int max_rows_w, int max_rows_v, int* wl_count,
-                               int2* wl_items, int wl_cap, hipStream_t st) {
+                               int2* wl_items, int wl_cap, int inner_sweeps, hipStream_t st) {
+  if (inner_sweeps < 1 || inner_sweeps > kInnerSweeps) inner_sweeps = kInnerSweeps;
   if (round == 0)
     hipLaunchKernelGGL(k_jacobi_round<true>, dim3(max_pairs, ntasks), dim3(256), 0, st, d_tasks, sweep, round,
-                       tol_rel, tol_abs, wl_count, wl_items, wl_cap);
+                       tol_rel, tol_abs, wl_count, wl_items, wl_cap, inner_sweeps);
   else
     hipLaunchKernelGGL(k_jacobi_round<false>, dim3(max_pairs, ntasks), dim3(256), 0, st, d_tasks, sweep, round,
-                       tol_rel, tol_abs, wl_count, wl_items, wl_cap);
+                       tol_rel, tol_abs, wl_count, wl_items, wl_cap, inner_sweeps);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int nw = (max_rows_w + kRotRows - 1) / kRotRows, nv = (max_rows_v + kRotRows - 1) / kRotRows;
