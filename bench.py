@@ -17,10 +17,18 @@ are compared with it (`obs_err`).  The plain per-bond FP64 path (`--compare fp64
 the default) is timed on the first steps of the timed window, started from the
 exact path's state; `speedup_vs_fp64` is value over its steps/s.
 
-Prints one JSON line.  `roofline` is the kernel class with the most device time in
-the timed steps (pivoted-QR steps or Jacobi rounds, both HBM-bound streaming
-kernels): algorithmic HBM bytes per launch / average launch duration, both
-measured with HIP events on the context's stream inside the library.
+The mixed-precision form of the same path (theta and gate on the FP32 matrix
+cores, `--mixed-mode`) runs the same 100 steps; its steps/s and its observable
+errors against the exact batched path are reported under `mixed`.
+
+Prints one JSON line.  `roofline` is the single kernel with the most device time in
+the timed steps: its algorithmic work per launch (HBM bytes for the streaming
+kernels -- Jacobi rounds and rotations, pivoted-QR panels and trailing updates --,
+FP64 flops for the GEMM-shaped ones) / its average launch duration, both measured
+with HIP events around each launch on the context's stream inside the library.
+`traffic` is the kernel's measured HBM bytes per launch over the same timed steps,
+read from the committed rocprofv3 counter summary (profiles/*_traffic.json) when
+one exists for that kernel and workload.
 `cpu_baseline` times the FP64 CPU oracle (the reference algorithm: padded chi,
 full ZGESVD of every (chi d) x (chi d) matrix) on a bounded sample of bond
 updates on this host and extrapolates to one step.
@@ -43,6 +51,23 @@ HBM_PEAK_GBS = 8000.0    # MI355X HBM3E
 
 
 RANK_MODEL = os.path.join(ROOT, "opentebd-extended_amd", "tebd_amd", "rank_model_fh.npy")
+WORKLOAD_TAG = "fh_l128_chi256"
+# launch classes timed by the library that are one kernel each: (kernel, roofline bound)
+SINGLE_KERNEL = {"theta_gate": ("k_theta_gate<4>", "mfma"), "jacobi_round": ("k_jacobi_round", "hbm"),
+                 "qr_panel": ("k_qrp_panel_reg<2>", "hbm"), "sketch_gemm": ("k_zgemm", "mfma"),
+                 "jacobi_rotate": ("k_jacobi_rotate", "hbm"), "qr_trailing": ("k_qrp_reflect<0>", "hbm")}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` in the timed steps, from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE passes, scaled by a copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", f"{WORKLOAD_TAG}_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)["kernels"][kernel]
+        return rec["hbm_bytes_per_launch"], os.path.relpath(path, ROOT)
+    except (OSError, KeyError, ValueError):
+        return None, None
 
 
 def make_context(mode, rank_model=None):
@@ -177,6 +202,8 @@ def main():
     ap.add_argument("--compare-steps", type=int, default=3)
     ap.add_argument("--mode", default="batched,learned")
     ap.add_argument("--reference-mode", default="batched", help="exact path for the observable check ('' = skip)")
+    ap.add_argument("--mixed-mode", default="batched,learned,mixed",
+                    help="mixed-precision path timed and checked against the exact path ('' = skip)")
     ap.add_argument("--rank-model", default=RANK_MODEL if os.path.exists(RANK_MODEL) else "",
                     help="rank model parameters (.npy) for the learned-rank path ('' = none)")
     ap.add_argument("--compare", default="fp64", help="also time this mode on the same workload ('' = skip)")
@@ -201,25 +228,51 @@ def main():
     ref = None
     if args.reference_mode:
         ref = run_mode(args.reference_mode, args.steps, args.warmup, log, keep_state=bool(args.compare))
+    mixed = None
+    if args.mixed_mode:
+        mixed = run_mode(args.mixed_mode, args.steps, args.warmup, log, rank_model=model)
     prof = res["prof"]
-    kernel_of = {"jacobi_round": "k_jacobi_round + k_jacobi_rotate", "qr_step": "k_qrp_panel + k_qrp_trail",
-                 "sketch_gemm": "k_build_sketch + k_zgemm + k_qrp_vectors"}
-    dom = max(kernel_of, key=lambda n: prof[n][1])
+    dom = max(SINGLE_KERNEL, key=lambda n: prof[n][1])
+    kernel, bound = SINGLE_KERNEL[dom]
     n_launch, dom_ms, dom_flops, dom_bytes = prof[dom]
     avg_ms = dom_ms / max(n_launch, 1)
     gbs = (dom_bytes / max(n_launch, 1)) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     tflops = (dom_flops / max(n_launch, 1)) / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    traffic, traffic_src = measured_traffic(kernel)
+    if bound == "hbm":
+        roof = {"bound": "hbm", "kernel": kernel, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": gbs / HBM_PEAK_GBS}
+    else:
+        roof = {"bound": "mfma", "kernel": kernel, "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tflops / FP64_PEAK_TFLOPS}
+    roof.update({"traffic": traffic, "traffic_source": traffic_src, "avg_launch_ms": avg_ms,
+                 "launches": int(n_launch), "bytes_per_launch": dom_bytes / max(n_launch, 1),
+                 "flops_per_launch": dom_flops / max(n_launch, 1), "fp64_tflops": tflops,
+                 "fp64_frac": tflops / FP64_PEAK_TFLOPS, "hbm_gbs": gbs,
+                 "device_ms": {n: prof[n][1] for n in prof}})
     st = res["stats"]
     half_steps = 3 * args.steps
 
     extra = {}
+    def obs_errors(run):
+        errs = {k: float(np.max(np.abs(np.asarray(run["obs"][k]) - np.asarray(ref["obs"][k])))) for k in run["obs"]}
+        errs["trunc"] = abs(run["trunc"] - ref["trunc"])
+        return errs
+
     if ref is not None:
-        errs = {k: float(np.max(np.abs(np.asarray(res["obs"][k]) - np.asarray(ref["obs"][k])))) for k in res["obs"]}
-        errs["trunc"] = abs(res["trunc"] - ref["trunc"])
+        errs = obs_errors(res)
         extra["obs_err"] = max(errs.values())
         extra["obs_err_by_observable"] = errs
         extra["obs_reference"] = f"{args.reference_mode} (exact FP64 GPU path), same steps 1-{args.warmup + args.steps}"
         extra[f"{args.reference_mode}_steps_per_s"] = args.steps / ref["elapsed"]
+    if mixed is not None:
+        extra["mixed"] = {"mode": args.mixed_mode, "steps_per_s": args.steps / mixed["elapsed"],
+                          "theta_gate_ms_per_step": mixed["prof"]["theta_gate"][1] / args.steps,
+                          "fp64_theta_gate_ms_per_step": prof["theta_gate"][1] / args.steps}
+        if ref is not None:
+            merr = obs_errors(mixed)
+            extra["mixed"]["obs_err"] = max(merr.values())
+            extra["mixed"]["obs_err_by_observable"] = merr
     if args.compare and ref is not None:
         cmp_res = run_mode(args.compare, args.compare_steps, 0, log, state=ref["snap"])
         extra[f"{args.compare}_steps_per_s"] = args.compare_steps / cmp_res["elapsed"]
@@ -252,12 +305,7 @@ def main():
                                f"timed steps {args.warmup + 1}-{args.warmup + args.steps}",
                    "mode": args.mode, "chi_max_reached": int(res["dims"].max()),
                    "mean_bond_dim": float(res["dims"].mean()), "warmup_s": res["warm_s"]},
-        "roofline": {"bound": "hbm", "kernel": kernel_of[dom], "achieved": gbs,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                     "traffic": None, "avg_launch_ms": avg_ms, "launches": int(n_launch),
-                     "bytes_per_launch": dom_bytes / max(n_launch, 1), "fp64_tflops": tflops,
-                     "fp64_frac": tflops / FP64_PEAK_TFLOPS,
-                     "device_ms": {n: prof[n][1] for n in prof}},
+        "roofline": roof,
         "learned_rank": {"rank_model": os.path.basename(args.rank_model) if args.rank_model else None,
                          "sketches_skipped_by_model": st["sketches_skipped"],
                          "sketch_accepted": st["sketch_accepted"], "full_svd_fallbacks": st["fallbacks"],

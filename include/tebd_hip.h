@@ -53,6 +53,21 @@ enum tebd_mode {
  * (local_operations_module.f90:84-122). */
 int tebd_create(tebd_ctx** ctx, int L, int d, int chi_max, int order, char bc, unsigned mode);
 
+/* An ensemble: nchain independent chains of L sites each (e.g. disorder
+ * realisations) in one context, so that every bond of one parity of every chain
+ * is updated by the same kernel launches.  No counterpart in the reference, which
+ * runs one chain per program (realisations are separate runs of a main, e.g.
+ * Case_Studies/Bose_Hubbard_Wrapper/BoseHubbard_Quench.f90).  The context then
+ * behaves as one chain of nchain * L sites whose nchain - 1 bonds between chains
+ * (bond r*L, r = 1..nchain-1, 1-based left site) are never updated: upload and
+ * download take nchain * L Gammas and nchain * L + 1 lambdas, and the lambda on each
+ * such bond must be (1, 0, ...); tebd_set_gates takes nchain * (L - 1) gates, chain
+ * after chain; the Trotter schedule restarts its odd/even bond numbering in every
+ * chain; tebd_two_site_op and tebd_canonical_form refuse the bonds between chains;
+ * tebd_expval_two_site still takes and returns nchain * L - 1 entries (those of
+ * the bonds between chains evaluate the operator on the product of two chains). */
+int tebd_create_ensemble(tebd_ctx** ctx, int nchain, int L, int d, int chi_max, int order, char bc, unsigned mode);
+
 /* Frees everything; replaces SVDFinish (local_operations_module.f90:124-153)
  * and DeallocateGamLam. */
 void tebd_destroy(tebd_ctx* ctx);
@@ -113,21 +128,46 @@ int tebd_set_cutoff(tebd_ctx* ctx, double cutoff);
 /* Current bond dimensions chi[l], l = 0..L (chi[0] = chi[L] = 1). */
 int tebd_get_bond_dims(tebd_ctx* ctx, int* chi);
 
-/* Counters since creation: [0] bond updates, [1] Jacobi sweeps, [2] fallbacks
- * of the learned-rank path, [3] device milliseconds in trotter steps,
- * [4..7] Jacobi sweeps of the sketched path's stages (Z0, residual block,
- * residual re-orthogonalisation, Rayleigh-Ritz). */
+/* Counters since creation (n <= 13): [0] bond updates, [1] Jacobi sweeps,
+ * [2] sketches of the learned-rank path redone by the full SVD, [3] device
+ * milliseconds in trotter steps, [4] sketch columns requested, [5] sketches
+ * accepted, [6] sum of the sketches' numerical ranks, [7] rank-model predictions,
+ * [8] pair Gram matrices formed, [9] pairs rotated, [10] pair visits of the sweeps
+ * run, [11] sketches skipped on the rank model's prediction, [12] sketches
+ * rejected for their discarded weight (the rest of [2] for missed weight). */
 int tebd_get_stats(tebd_ctx* ctx, double* stats, int n);
 
 /* Per-kernel timing with HIP events recorded on the context's stream (no
  * counterpart in the reference, whose only timing is CPU_TIME around whole
  * mains, Case_Studies/Heisenberg_Wrapper/XXDynamics.f90:89,196-198).
  * tebd_set_profiling(ctx, 1) clears and enables the counters;
- * tebd_get_profile fills out[0..8] = {launches, ms, algorithmic flops} for
- * the theta+gate kernel, the Jacobi round kernel and the truncation + Gamma
- * rebuild kernels, in that order. */
+ * tebd_get_profile fills out[4k .. 4k+3] = {launches, milliseconds, algorithmic
+ * flops, algorithmic HBM bytes} of launch class k (n <= 28):
+ *   0 theta + gate (k_theta_gate, or k_theta_gate_mixed in TEBD_MODE_MIXED)
+ *   1 Jacobi round: pair Grams and 32x32 eigenproblems (k_jacobi_round)
+ *   2 truncation, Gamma rebuild, singular-vector and sketch formation
+ *   3 pivoted-QR panel factorisation (k_qrp_panel_reg / k_qrp_panel)
+ *   4 batched complex GEMMs of the learned-rank path (k_zgemm)
+ *   5 Jacobi rotation of W and V (k_jacobi_rotate)
+ *   6 pivoted-QR trailing update (k_qrp_reflect<0>)
+ * Each launch of classes 1, 3, 4, 5, 6 is one kernel, so its event duration
+ * is that kernel's duration. */
 int tebd_set_profiling(tebd_ctx* ctx, int on);
 int tebd_get_profile(tebd_ctx* ctx, double* out, int n);
+
+/* Autotuner of launch shapes (no counterpart in the reference).  The batched
+ * complex GEMMs of the learned-rank path and of CanonicalForm have several tile
+ * variants; with autotuning on (the default) the first launch of every shape class
+ * (M and N rounded up to 64, batch size to a power of two) times each candidate on
+ * the real operands with HIP events and keeps the fastest for that class.  The
+ * candidates take the same floating-point operations in the same order per element,
+ * so tuning changes the speed, never the result.  on = 0 switches tuning off (new
+ * classes take the 64 x 64 MFMA tiles), on = 1 on, on = -1 on and forgets what was
+ * tuned.  tebd_autotune_info: out[0] shape classes tuned, out[1] timing launches,
+ * out[2 + v] classes that chose variant v (0 VALU 64x64, 1 MFMA 64x64, 2 MFMA
+ * 128x64); n <= 5. */
+int tebd_set_autotune(tebd_ctx* ctx, int on);
+int tebd_autotune_info(tebd_ctx* ctx, double* out, int n);
 
 /* Blocks until the context's stream is idle. */
 int tebd_sync(tebd_ctx* ctx);

@@ -8,16 +8,18 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
 namespace {
 
-constexpr int kStages = 5;
+constexpr int kStages = 6;  // task arrays per batch: 0-4 the learned path's stages, 5 the QR subset
 constexpr int kNumStats = 13;
-constexpr int kNumProf = 20;
+constexpr int kNumProf = 28;
 constexpr double kQrTol = 1e-17;  // QR stops when every remaining column norm is <= kQrTol ||theta||_F
 constexpr int kQrPanel = 16;      // panel width of the blocked pivoted QR (tebd_qrp.hip)
 constexpr int kQrCheck = 8;       // panels between checks whether every task has stopped
@@ -36,6 +38,9 @@ int round_up(int x, int m) { return ((x + m - 1) / m) * m; }
 
 struct tebd_ctx {
   int L = 0, d = 0, chi_max = 0;
+  // an ensemble holds nchain independent chains of chain_len sites as one chain of
+  // L = nchain * chain_len sites whose bonds between chains are never updated
+  int chain_len = 0, nchain = 1;
   unsigned mode = 0;
   double cutoff = 1e-30;
   hipStream_t st = nullptr;
@@ -120,10 +125,17 @@ struct tebd_ctx {
   double stats[kNumStats] = {};
   // optional per-kernel timing with HIP events on the context's stream:
   // prof[k*4 + 0] launches, [1] milliseconds, [2] algorithmic flops, [3] algorithmic HBM bytes,
-  // for k = 0 theta+gate, 1 Jacobi round, 2 truncation + Gamma rebuild, 3 pivoted QR steps,
-  // 4 sketch construction and GEMMs of the learned-rank path
+  // for k = 0 theta+gate, 1 Jacobi round (Gram + 32x32 eigenproblem), 2 truncation + Gamma
+  // rebuild, 3 pivoted-QR panel factorisation, 4 sketch construction and GEMMs of the
+  // learned-rank path, 5 Jacobi rotation of W and V, 6 pivoted-QR trailing update
   bool profiling = false;
   double prof[kNumProf] = {};
+  // autotuner: launch variant of each tunable kernel per shape class (key), chosen by
+  // timing every candidate on the first launch of that class
+  bool autotune = true;
+  std::map<uint64_t, int> tuned;
+  hipEvent_t tev0 = nullptr, tev1 = nullptr;
+  double tune_trials = 0;
   struct Pending { hipEvent_t a, b; int kind; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
@@ -188,6 +200,15 @@ void drain_profile(tebd_ctx* c) {
   c->pending.clear();
 }
 
+// Bond `link` (0-based, sites link and link + 1) joins two chains of an ensemble.
+bool is_cut(const tebd_ctx* c, int link) { return (link + 1) % c->chain_len == 0; }
+
+// theta + gate of a batch: FP32 matrix cores in the mixed mode, FP64 matrix cores otherwise.
+hipError_t theta_gate(tebd_ctx* c, const BondTask* d_t, int nb, int max_chil, int max_chir) {
+  if (c->mode & TEBD_MODE_MIXED) return launch_theta_gate_mixed(d_t, nb, c->d, max_chil, max_chir, c->st);
+  return launch_theta_gate_mfma(d_t, nb, c->d, max_chil, max_chir, c->st);
+}
+
 // Gives task `slot` of a batch its Jacobi counters and block bookkeeping.
 void attach_jacobi(tebd_ctx* c, BondTask& t, int slot) {
   t.counter = c->counters + 4 * slot;
@@ -211,9 +232,13 @@ int jacobi_sweeps(tebd_ctx* c, BondTask* d_t, const BondTask* h_t, int nb, int P
     const int inner = sweeps < kEarlySweeps ? kEarlyInnerSweeps : 0;  // 0: to convergence
     HIPCHK(c, launch_jacobi_sweep_begin(d_t, nb, sweeps, c->wl_count, c->st));
     for (int r = 0; r < 2 * Pmax - 1; ++r) {
-      ProfScope ps(c, 1);
-      HIPCHK(c, launch_jacobi_round(d_t, nb, Pmax, sweeps, r, kTolRel, kTolAbs, rows_w, rows_v, c->wl_count,
-                                    c->wl_items, c->wl_cap, inner, c->st));
+      {
+        ProfScope ps(c, 1);
+        HIPCHK(c, launch_jacobi_round(d_t, nb, Pmax, sweeps, r, kTolRel, kTolAbs, rows_w, rows_v, c->wl_count,
+                                      c->wl_items, c->wl_cap, inner, c->st));
+      }
+      ProfScope ps(c, 5);
+      HIPCHK(c, launch_jacobi_rotate(d_t, r, rows_w, rows_v, c->wl_count, c->wl_items, c->wl_cap, c->st));
     }
     HIPCHK(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(int) * 4 * nb, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -231,11 +256,14 @@ int jacobi_sweeps(tebd_ctx* c, BondTask* d_t, const BondTask* h_t, int nb, int P
     c->stats[9] += rots;
     c->stats[10] += (double)sweeps * t.P * (2 * t.P - 1);
     if (c->profiling) {
-      // per formed Gram: the 16x16 cross block over R rows (8 flops per complex MAC)
-      // reading the pair's 2 JB columns; per rotation: the JC columns of W and V
+      // k_jacobi_round, per formed Gram: the 16x16 cross block over R rows (8 flops per
+      // complex MAC) reading the pair's JC columns once, and per rotated pair its JC x JC
+      // rotation written; k_jacobi_rotate, per rotated pair: the JC columns of W and V
       // read and written once, JC complex MACs per element
-      c->prof[6] += visits * 8.0 * t.R * JB * JB + rots * 8.0 * rows * JC * JC;
-      c->prof[7] += visits * 16.0 * t.R * JC + rots * 32.0 * rows * JC;
+      c->prof[6] += visits * 8.0 * t.R * JB * JB;
+      c->prof[7] += visits * 16.0 * t.R * JC + rots * 16.0 * JC * JC;
+      c->prof[22] += rots * 8.0 * rows * JC * JC;
+      c->prof[23] += rots * 32.0 * rows * JC;
     }
   }
   if (!converged) return fail(c, TEBD_ERR_NOCONV, "one-sided Jacobi did not converge");
@@ -259,6 +287,9 @@ void attach_qr(tebd_ctx* c, BondTask& t, int slot) {
   t.Zc = c->Zc + (size_t)slot * c->Cpmax * c->chi_max;
 }
 
+// Workspace slot of a task (its QR state lives at qstate + 2 slot).
+int slot_of(const tebd_ctx* c, const BondTask& t) { return (int)((t.qstate - c->qstate) / 2); }
+
 // Column-pivoted Householder QR of every task's bond matrix, stopped at the
 // numerical rank.  Returns the ranks in rank[].
 int qrp_factor(tebd_ctx* c, BondTask* D, const BondTask* H, int nb, int max_cb, std::vector<int>& rank) {
@@ -273,28 +304,35 @@ int qrp_factor(tebd_ctx* c, BondTask* D, const BondTask* H, int nb, int max_cb, 
       ProfScope ps(c, 3);
       HIPCHK(c, launch_qrp_panel(D, nb, max_rb, max_cb, j0, c->st));
     }
+    {
+      ProfScope ps(c, 6);
+      HIPCHK(c, launch_qrp_trailing(D, nb, max_cb, j0, c->st));
+    }
     const bool last = j0 + kQrPanel >= max_cb;
     if ((panel + 1) % kQrCheck == 0 || last) {
-      HIPCHK(c, hipMemcpyAsync(c->h_qstate, c->qstate, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->st));
+      HIPCHK(c, hipMemcpyAsync(c->h_qstate, c->qstate, sizeof(double) * 2 * c->ntask_max, hipMemcpyDeviceToHost,
+                               c->st));
       HIPCHK(c, hipStreamSynchronize(c->st));
       done = true;
-      for (int i = 0; i < nb; ++i) done = done && c->h_qstate[2 * i] >= 0.0;
+      for (int i = 0; i < nb; ++i) done = done && c->h_qstate[2 * slot_of(c, H[i])] >= 0.0;
     }
     if (last && !done) return fail(c, TEBD_ERR_STATE, "pivoted QR did not finish");
   }
   rank.resize(nb);
   for (int i = 0; i < nb; ++i) {
-    rank[i] = (int)c->h_qstate[2 * i];
+    rank[i] = (int)c->h_qstate[2 * slot_of(c, H[i])];
     if (c->profiling) {
-      // per panel j0 of width w: the trailing (Rb - j0) x (Cb - j0 - w) block is read twice
-      // (Y = V^H A2, then A2 -= V Z) and written once, 2 x 32 complex MACs per element;
-      // the panel itself is read and written about w times
+      // per panel j0 of width w: the panel kernel reads and writes the (Rb - j0) x w panel
+      // and forms w reflectors of it (~8 rows w^2 flops); the trailing update reads and
+      // writes the (Rb - j0) x (Cb - j0 - w) block once, 2 w complex MACs per element
       const BondTask& t = H[i];
       for (int j0 = 0; j0 < rank[i]; j0 += kQrPanel) {
         const int w = std::min(kQrPanel, rank[i] - j0);
         const double rows = t.Rb - j0, trail = std::max(0, t.Cb - j0 - w);
-        c->prof[14] += 2.0 * 8.0 * kQrPanel * rows * trail + 8.0 * rows * w * w;
-        c->prof[15] += 48.0 * rows * trail + 32.0 * rows * w;
+        c->prof[14] += 8.0 * rows * w * w;
+        c->prof[15] += 32.0 * rows * w;
+        c->prof[26] += 2.0 * 8.0 * w * rows * trail;
+        c->prof[27] += 32.0 * rows * trail;
       }
     }
   }
@@ -314,31 +352,51 @@ int svd_factor(tebd_ctx* c, BondTask* D, BondTask* H, const int* links, int nb, 
     max_cb = std::max(max_cb, H[i].Cb);
     max_rb = std::max(max_rb, H[i].Rb);
   }
-  const bool use_qr = force_qr || max_cb >= kQrMinCols;
+  // The QR preconditioning is decided per task (bond matrices with at least
+  // kQrMinCols columns, or all with force_qr), so a bond's result does not depend on
+  // which other bonds share its batch.
+  std::vector<int> qidx;
+  for (int i = 0; i < nb; ++i)
+    if (force_qr || H[i].Cb >= kQrMinCols) qidx.push_back(i);
+  const int nq = (int)qidx.size();
+  const bool use_qr = nq > 0;
   int max_rank = 1;
   if (use_qr) {
+    BondTask* Hq = H;
+    BondTask* Dq = D;
+    int max_cb_q = 1;
+    if (nq < nb) {  // the QR tasks as a compact batch of their own
+      Hq = c->h_tasks + 5 * c->ntask_max;
+      Dq = c->d_tasks + 5 * c->ntask_max;
+      for (int j = 0; j < nq; ++j) Hq[j] = H[qidx[j]];
+      HIPCHK(c, hipMemcpyAsync(Dq, Hq, sizeof(BondTask) * nq, hipMemcpyHostToDevice, c->st));
+    }
+    for (int j = 0; j < nq; ++j) max_cb_q = std::max(max_cb_q, Hq[j].Cb);
     std::vector<int> rank;
-    int e = qrp_factor(c, D, H, nb, max_cb, rank);
+    int e = qrp_factor(c, Dq, Hq, nq, max_cb_q, rank);
     if (e) return e;
-    Cpmax = 1;
-    Pmax = 1;
-    for (int i = 0; i < nb; ++i) {
-      if (rank[i] < 1) {
+    for (int j = 0; j < nq; ++j) {
+      const int i = qidx[j];
+      if (rank[j] < 1) {
         char buf[160];
         std::snprintf(buf, sizeof(buf), "bond %d: theta is zero or not finite", links[i] + 1);
         return fail(c, TEBD_ERR_NAN, buf);
       }
-      max_rank = std::max(max_rank, rank[i]);
+      max_rank = std::max(max_rank, rank[j]);
       BondTask& t = H[i];
       t.qr = 1;
-      t.W = c->X + (size_t)i * c->Cpmax * c->Cpmax;
+      t.W = c->X + (size_t)slot_of(c, t) * c->Cpmax * c->Cpmax;
       t.R = t.Cb;
-      t.C = rank[i];
+      t.C = rank[j];
       t.Cpad = round_up(t.C, JC);
       t.P = t.Cpad / JC;
       t.NV = t.Cpad;
-      Cpmax = std::max(Cpmax, t.Cpad);
-      Pmax = std::max(Pmax, t.P);
+    }
+    Cpmax = 1;
+    Pmax = 1;
+    for (int i = 0; i < nb; ++i) {
+      Cpmax = std::max(Cpmax, H[i].Cpad);
+      Pmax = std::max(Pmax, H[i].P);
     }
     HIPCHK(c, hipMemcpyAsync(D, H, sizeof(BondTask) * nb, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, launch_qrp_build_x(D, nb, c->st));
@@ -394,6 +452,54 @@ int svd_truncate(tebd_ctx* c, const int* links, int nb, int Cpmax, int Pmax) {
   return TEBD_OK;
 }
 
+// Algorithmic work of one batched complex GEMM launch (k_zgemm): 8 M N K flops, and
+// op(A), B read and C written once.
+void account_gemm(tebd_ctx* c, const GemmTask* g, int n) {
+  if (!c->profiling) return;
+  for (int i = 0; i < n; ++i) {
+    c->prof[18] += 8.0 * g[i].M * (double)g[i].N * g[i].K;
+    c->prof[19] += 16.0 * ((double)g[i].M * g[i].K + (double)g[i].K * g[i].N + (double)g[i].M * g[i].N);
+  }
+}
+
+// Shape class of a batched GEMM launch: conj flag, M and N rounded up to 64, the batch
+// size rounded up to a power of two.
+uint64_t gemm_key(int max_m, int max_n, int ntasks, bool conj) {
+  int nb = 1;
+  while (nb < ntasks) nb <<= 1;
+  return (1ull << 62) | ((uint64_t)conj << 61) | ((uint64_t)((max_m + 63) / 64) << 40) |
+         ((uint64_t)((max_n + 63) / 64) << 20) | (uint64_t)nb;
+}
+
+// Batched complex GEMM with the tile variant the autotuner chose for this shape class.
+// The candidates (MFMA 64x64 and 128x64 tiles) take the same floating-point operations
+// in the same order for every element, so the choice changes the speed, never the
+// result; C = op(A) B is overwritten, so the timing runs leave the same C behind.
+hipError_t tuned_zgemm(tebd_ctx* c, const GemmTask* d_t, int nt, int max_m, int max_n, bool conj) {
+  static const int cand[] = {ZGEMM_MFMA_64x64, ZGEMM_MFMA_128x64};
+  const uint64_t key = gemm_key(max_m, max_n, nt, conj);
+  auto it = c->tuned.find(key);
+  if (it != c->tuned.end()) return launch_zgemm(d_t, nt, max_m, max_n, conj, c->st, it->second);
+  if (!c->autotune) return launch_zgemm(d_t, nt, max_m, max_n, conj, c->st, ZGEMM_MFMA_64x64);
+  if (!c->tev0 && (hipEventCreate(&c->tev0) != hipSuccess || hipEventCreate(&c->tev1) != hipSuccess))
+    return hipErrorOutOfMemory;
+  int best = cand[0];
+  float best_ms = 1e30f;
+  for (int v : cand) {
+    hipError_t e = hipEventRecord(c->tev0, c->st);
+    if (e == hipSuccess) e = launch_zgemm(d_t, nt, max_m, max_n, conj, c->st, v);
+    if (e == hipSuccess) e = hipEventRecord(c->tev1, c->st);
+    if (e == hipSuccess) e = hipEventSynchronize(c->tev1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->tev0, c->tev1);
+    if (e != hipSuccess) return e;
+    c->tune_trials += 1;
+    if (ms < best_ms) { best_ms = ms; best = v; }
+  }
+  c->tuned[key] = best;
+  return hipSuccess;
+}
+
 // Sketch width for one bond without a rank model: the bond's current dimension with
 // 1/8 growth, plus oversampling.
 int sketch_width(tebd_ctx* c, int link) {
@@ -403,42 +509,61 @@ int sketch_width(tebd_ctx* c, int link) {
 }
 
 // Process `nb` independent bonds (0-based links) as one batch of tasks.
-int run_bonds_full(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
+// Task of a full-SVD bond update of `link` in workspace slot `slot`: theta (or
+// theta^dagger when it has more columns than rows) into W, the new Gammas and lambda
+// written in place.
+void make_full_task(tebd_ctx* c, BondTask& t, int link, const zc* gate, int slot) {
   const int d = c->d;
+  std::memset(&t, 0, sizeof(t));
+  t.chil = c->chi[link];
+  t.chic = c->chi[link + 1];
+  t.chir = c->chi[link + 2];
+  t.d = d;
+  t.link = link;
+  const int m = d * t.chil, n = d * t.chir;
+  t.transposed = n > m ? 1 : 0;
+  t.R = std::max(m, n);
+  t.C = std::min(m, n);
+  t.Cpad = round_up(t.C, JC);
+  t.P = t.Cpad / JC;
+  t.gl = c->gam + (size_t)link * c->gslab();
+  t.gr = c->gam + (size_t)(link + 1) * c->gslab();
+  t.ll = c->lam + (size_t)link * c->chi_max;
+  t.lc = c->lam + (size_t)(link + 1) * c->chi_max;
+  t.lr = c->lam + (size_t)(link + 2) * c->chi_max;
+  t.gate = gate;
+  t.W = c->W + (size_t)slot * c->Rmax * c->Cpmax;
+  t.V = c->V + (size_t)slot * c->Cpmax * c->Cpmax;
+  t.s = c->s + (size_t)slot * c->Cpmax;
+  t.perm = c->perm + (size_t)slot * c->Cpmax;
+  t.NV = t.Cpad;
+  attach_jacobi(c, t, slot);
+  attach_qr(c, t, slot);
+  t.result = c->results + (size_t)RESULT_STRIDE * slot;
+  t.gl_out = const_cast<zc*>(t.gl);
+  t.gr_out = const_cast<zc*>(t.gr);
+  t.lc_out = const_cast<double*>(t.lc);
+  t.chi_max = c->chi_max;
+}
+
+// Algorithmic work of the theta + gate kernel on the tasks (profiling only).
+void account_theta(tebd_ctx* c, const BondTask* h, int nb) {
+  if (!c->profiling) return;
+  const double d = c->d;
+  for (int i = 0; i < nb; ++i) {
+    const BondTask& t = h[i];
+    // theta: 8 d^2 chi_l chi_c chi_r, gate: 8 chi_l chi_r d^4 (complex MAC = 8 flops)
+    c->prof[2] += 8.0 * d * d * t.chil * (double)t.chic * t.chir + 8.0 * t.chil * (double)t.chir * d * d * d * d;
+    // Gamma_l, Gamma_r read, theta written
+    c->prof[3] += 16.0 * ((double)t.chil * d * t.chic + (double)t.chic * d * t.chir + (double)t.chil * d * t.chir * d);
+  }
+}
+
+int run_bonds_full(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
   int max_chil = 1, max_chir = 1, Pmax = 1, Cpmax = 1;
   for (int i = 0; i < nb; ++i) {
-    const int link = links[i];
     BondTask& t = c->h_tasks[i];
-    std::memset(&t, 0, sizeof(t));
-    t.chil = c->chi[link];
-    t.chic = c->chi[link + 1];
-    t.chir = c->chi[link + 2];
-    t.d = d;
-    t.link = link;
-    const int m = d * t.chil, n = d * t.chir;
-    t.transposed = n > m ? 1 : 0;
-    t.R = std::max(m, n);
-    t.C = std::min(m, n);
-    t.Cpad = round_up(t.C, JC);
-    t.P = t.Cpad / JC;
-    t.gl = c->gam + (size_t)link * c->gslab();
-    t.gr = c->gam + (size_t)(link + 1) * c->gslab();
-    t.ll = c->lam + (size_t)link * c->chi_max;
-    t.lc = c->lam + (size_t)(link + 1) * c->chi_max;
-    t.lr = c->lam + (size_t)(link + 2) * c->chi_max;
-    t.gate = gate_ptrs[i];
-    t.W = c->W + (size_t)i * c->Rmax * c->Cpmax;
-    t.V = c->V + (size_t)i * c->Cpmax * c->Cpmax;
-    t.s = c->s + (size_t)i * c->Cpmax;
-    t.perm = c->perm + (size_t)i * c->Cpmax;
-    t.NV = t.Cpad;
-    attach_jacobi(c, t, i);
-    attach_qr(c, t, i);
-    t.result = c->results + (size_t)RESULT_STRIDE * i;
-    t.gl_out = const_cast<zc*>(t.gl);
-    t.gr_out = const_cast<zc*>(t.gr);
-    t.lc_out = const_cast<double*>(t.lc);
-    t.chi_max = c->chi_max;
+    make_full_task(c, t, links[i], gate_ptrs[i], i);
     max_chil = std::max(max_chil, t.chil);
     max_chir = std::max(max_chir, t.chir);
     Pmax = std::max(Pmax, t.P);
@@ -447,17 +572,9 @@ int run_bonds_full(tebd_ctx* c, const int* links, int nb, const zc* const* gate_
   HIPCHK(c, hipMemcpyAsync(c->d_tasks, c->h_tasks, sizeof(BondTask) * nb, hipMemcpyHostToDevice, c->st));
   {
     ProfScope ps(c, 0);
-    HIPCHK(c, launch_theta_gate(c->d_tasks, nb, d, max_chil, max_chir, c->st));
+    HIPCHK(c, theta_gate(c, c->d_tasks, nb, max_chil, max_chir));
   }
-  if (c->profiling) {
-    // theta: 8 d^2 chi_l chi_c chi_r, gate: 8 chi_l chi_r d^4 (complex MAC = 8 flops)
-    for (int i = 0; i < nb; ++i) {
-      const BondTask& t = c->h_tasks[i];
-      c->prof[2] += 8.0 * d * d * t.chil * (double)t.chic * t.chir + 8.0 * t.chil * (double)t.chir * d * d * d * d;
-      // Gamma_l, Gamma_r read, theta written
-      c->prof[3] += 16.0 * ((double)t.chil * d * t.chic + (double)t.chic * d * t.chir + (double)t.R * t.C);
-    }
-  }
+  account_theta(c, c->h_tasks, nb);
   int e = svd_truncate(c, links, nb, Cpmax, Pmax);
   if (e) return e;
   double tsum = 0.0;
@@ -484,13 +601,13 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
     BondTask* F = c->h_tasks + 4 * nt;
     for (int i = 0; i < nb; ++i) {
       std::memset(&F[i], 0, sizeof(BondTask));
-      F[i].link = links[i];
+      F[i].link = links[i] % c->chain_len;  // position within its chain
       F[i].chic = c->chi[links[i] + 1];
       F[i].lc = c->lam + (size_t)(links[i] + 1) * c->chi_max;
     }
     BondTask* DF = c->d_tasks + 4 * nt;
     HIPCHK(c, hipMemcpyAsync(DF, F, sizeof(BondTask) * nb, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, launch_mlp_features(DF, nb, c->L, c->step_index, c->d_feat, c->st));
+    HIPCHK(c, launch_mlp_features(DF, nb, c->chain_len, c->step_index, c->d_feat, c->st));
     if (c->have_model) HIPCHK(c, launch_mlp_predict(c->mlp, c->d_feat, nb, c->d_pred, c->st));
     HIPCHK(c, hipMemcpyAsync(c->h_feat, c->d_feat, sizeof(float) * TEBD_MLP_INPUTS * nb, hipMemcpyDeviceToHost, c->st));
     if (c->have_model)
@@ -516,13 +633,24 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
     }
   }
   double tsum = 0.0;
+  const int ns = (int)sk.size();
+  // Bonds that need the full SVD and are large enough for its QR preconditioning join
+  // the sketched bonds' final SVD stage (same QR and Jacobi launches); the others, and
+  // sketches that are rejected, go through run_bonds_full afterwards.
+  std::vector<int> merged;
   std::vector<int> fb_links;
   std::vector<const zc*> fb_gates;
   for (int i : full) {
-    fb_links.push_back(links[i]);
-    fb_gates.push_back(gate_ptrs[i]);
+    const int link = links[i];
+    const int C = d * std::min(c->chi[link], c->chi[link + 2]);
+    if (ns > 0 && C >= kQrMinCols) {
+      merged.push_back(i);
+    } else {
+      fb_links.push_back(link);
+      fb_gates.push_back(gate_ptrs[i]);
+    }
   }
-  const int ns = (int)sk.size();
+  const int nf = (int)merged.size();
   if (ns > 0) {
     BondTask *S0 = c->h_tasks, *S1 = S0 + nt, *S2 = S0 + 2 * nt, *S3 = S0 + 3 * nt, *S4 = S0 + 4 * nt;
     BondTask *D0 = c->d_tasks, *D1 = D0 + nt, *D2 = D0 + 2 * nt, *D3 = D0 + 3 * nt, *D4 = D0 + 4 * nt;
@@ -545,6 +673,7 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
       base.gate = gate_ptrs[i];
       base.chil = chil; base.chic = chic; base.chir = chir; base.d = d;
       base.link = link;
+      base.rng_key = link % c->chain_len;
       base.chi_max = c->chi_max;
       base.s = c->s + (size_t)q * c->Cpmax;
       base.perm = c->perm + (size_t)q * c->Cpmax;
@@ -576,14 +705,19 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
     HIPCHK(c, hipMemcpyAsync(DG, HG, sizeof(GemmTask) * nt, hipMemcpyHostToDevice, c->st));
     {
       ProfScope ps(c, 0);
-      HIPCHK(c, launch_theta_gate(D0, ns, d, chil_max, chir_max, c->st));
+      HIPCHK(c, theta_gate(c, D0, ns, chil_max, chir_max));
     }
+    account_theta(c, S0, ns);
     HIPCHK(c, launch_frobenius(D0, ns, round_up(max_n, JC), c->st, 1));
     {
-      ProfScope ps(c, 4);
+      ProfScope ps(c, 2);
       HIPCHK(c, launch_build_sketch(D1, c->d_kprime, ns, c->sketch_seed ^ (unsigned long long)c->step_index, c->st));
-      HIPCHK(c, launch_zgemm(DG, ns, max_m, max_l, false, c->st));
     }
+    {
+      ProfScope ps(c, 4);
+      HIPCHK(c, tuned_zgemm(c, DG, ns, max_m, max_l, false));
+    }
+    account_gemm(c, HG, ns);
     std::vector<int> ry;
     int e = qrp_factor(c, D2, S2, ns, max_l, ry);
     if (e) return e;
@@ -612,16 +746,42 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
     }
     HIPCHK(c, hipMemcpyAsync(DG + nt, HG + nt, sizeof(GemmTask) * ns, hipMemcpyHostToDevice, c->st));
     {
-      ProfScope ps(c, 4);
+      ProfScope ps(c, 2);
       HIPCHK(c, launch_qrp_form_q(D2, ns, max_ry, c->st));
-      HIPCHK(c, launch_zgemm(DG + nt, ns, max_n, max_ry, true, c->st));
     }
-    HIPCHK(c, hipMemcpyAsync(D3, S3, sizeof(BondTask) * ns, hipMemcpyHostToDevice, c->st));
+    {
+      ProfScope ps(c, 4);
+      HIPCHK(c, tuned_zgemm(c, DG + nt, ns, max_n, max_ry, true));
+    }
+    account_gemm(c, HG + nt, ns);
+    std::vector<int> all_links = sk_links;
+    int fl_max = 1, fr_max = 1;
+    for (int j = 0; j < nf; ++j) {
+      const int i = merged[j];
+      make_full_task(c, S3[ns + j], links[i], gate_ptrs[i], ns + j);
+      all_links.push_back(links[i]);
+      fl_max = std::max(fl_max, S3[ns + j].chil);
+      fr_max = std::max(fr_max, S3[ns + j].chir);
+    }
+    HIPCHK(c, hipMemcpyAsync(D3, S3, sizeof(BondTask) * (ns + nf), hipMemcpyHostToDevice, c->st));
+    if (nf > 0) {
+      ProfScope ps(c, 0);
+      HIPCHK(c, theta_gate(c, D3 + ns, nf, fl_max, fr_max));
+    }
+    account_theta(c, S3 + ns, nf);
     int max_k = 1;
     bool used_qr = false;
-    e = svd_factor(c, D3, S3, sk_links.data(), ns, round_up(max_ry, JC), round_up(max_ry, JC) / JC, true, &used_qr,
-                   &max_k);
+    e = svd_factor(c, D3, S3, all_links.data(), ns + nf, round_up(max_ry, JC), round_up(max_ry, JC) / JC, true,
+                   &used_qr, &max_k);
     if (e) return e;
+    for (int j = 0; j < nf; ++j) {
+      const double* r = c->h_results + RESULT_STRIDE * (ns + j);
+      const int link = links[merged[j]];
+      c->chi[link + 1] = (int)r[0];
+      c->last_rank[link + 1] = (int)r[0];
+      tsum += r[1];
+    }
+    c->stats[0] += nf;
     // left vectors of theta: Q U_B (m x k), U_B = the factor's Cb-side output (r x k)
     int na = 0;
     std::vector<int> acc;
@@ -673,15 +833,20 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
       HIPCHK(c, hipMemcpyAsync(D4, S4, sizeof(BondTask) * na, hipMemcpyHostToDevice, c->st));
       {
         ProfScope ps(c, 4);
-        HIPCHK(c, launch_zgemm(DG + 2 * nt, na, max_m, max_k, false, c->st));
+        HIPCHK(c, tuned_zgemm(c, DG + 2 * nt, na, max_m, max_k, false));
       }
+      account_gemm(c, HG + 2 * nt, na);
       {
         ProfScope ps(c, 2);
         HIPCHK(c, launch_write_gammas(D4, na, c->st));
       }
-      HIPCHK(c, hipStreamSynchronize(c->st));
       c->stats[0] += na;
     }
+    if (nf > 0) {
+      ProfScope ps(c, 2);
+      HIPCHK(c, launch_write_gammas(D3 + ns, nf, c->st));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
     drain_profile(c);
   }
   if (!fb_links.empty()) {
@@ -718,7 +883,7 @@ int gemm1(tebd_ctx* c, const zc* A, const zc* B, zc* C, int M, int N, int K, int
   GemmTask* g = c->h_gemm;
   *g = {A, B, C, M, N, K, lda, ldb, ldc, 1.0, 0};
   HIPCHK(c, hipMemcpyAsync(c->d_gemm, g, sizeof(GemmTask), hipMemcpyHostToDevice, c->st));
-  HIPCHK(c, launch_zgemm(c->d_gemm, 1, M, N, conj, c->st));
+  HIPCHK(c, tuned_zgemm(c, c->d_gemm, 1, M, N, conj));
   // the pinned descriptor is rewritten by the next call
   HIPCHK(c, hipStreamSynchronize(c->st));
   return TEBD_OK;
@@ -809,6 +974,9 @@ int upload_impl(tebd_ctx* c, const double* const* gamma, const double* const* la
     if (k > c->chi_max) return fail(c, TEBD_ERR_SHAPE, "upload: a bond dimension exceeds chi_max");
     chi[l] = k;
   }
+  for (int r = 1; r < c->nchain; ++r)
+    if (chi[r * c->chain_len] != 1)
+      return fail(c, TEBD_ERR_SHAPE, "upload: the bond between two chains of an ensemble must have dimension 1");
   std::vector<double> lam((size_t)(L + 1) * c->chi_max, 0.0);
   for (int l = 0; l <= L; ++l)
     for (int a = 0; a < chi[l]; ++a) lam[(size_t)l * c->chi_max + a] = lambda[l][a];
@@ -857,6 +1025,8 @@ int download_impl(tebd_ctx* c, double* const* gamma, double* const* lambda, int 
 }
 
 void free_all(tebd_ctx* c) {
+  if (c->tev0) (void)hipEventDestroy(c->tev0);
+  if (c->tev1) (void)hipEventDestroy(c->tev1);
   (void)hipFree(c->gam);
   (void)hipFree(c->lam);
   (void)hipFree(c->gates);
@@ -928,6 +1098,7 @@ int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, u
   c->L = L;
   c->d = d;
   c->chi_max = chi_max;
+  c->chain_len = L;
   c->mode = mode;
   c->chi.assign(L + 1, 1);
   c->Rmax = d * chi_max;
@@ -993,6 +1164,17 @@ int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, u
   return TEBD_OK;
 }
 
+int tebd_create_ensemble(tebd_ctx** out, int nchain, int L, int d, int chi_max, int order, char bc, unsigned mode) {
+  if (!out) return TEBD_ERR_ARG;
+  *out = nullptr;
+  if (nchain < 1 || L < 2 || (long long)nchain * L > (1 << 24)) return TEBD_ERR_ARG;
+  int e = tebd_create(out, nchain * L, d, chi_max, order, bc, mode);
+  if (e) return e;
+  (*out)->chain_len = L;
+  (*out)->nchain = nchain;
+  return TEBD_OK;
+}
+
 void tebd_destroy(tebd_ctx* c) {
   if (!c) return;
   if (c->st) (void)hipStreamSynchronize(c->st);
@@ -1002,8 +1184,27 @@ void tebd_destroy(tebd_ctx* c) {
 
 int tebd_set_gates(tebd_ctx* c, const double* U, int nb) {
   if (!c) return TEBD_ERR_ARG;
-  if (!U || nb != c->L - 1) return fail(c, TEBD_ERR_ARG, "set_gates: need L-1 gates");
-  HIPCHK(c, hipMemcpy(c->gates, U, sizeof(zc) * c->d4() * nb, hipMemcpyHostToDevice));
+  const int per = c->chain_len - 1;
+  if (!U || nb != c->nchain * per) return fail(c, TEBD_ERR_ARG, "set_gates: need L-1 gates per chain");
+  if (c->nchain == 1) {
+    HIPCHK(c, hipMemcpy(c->gates, U, sizeof(zc) * c->d4() * nb, hipMemcpyHostToDevice));
+  } else {
+    // gate b of chain r goes to bond r * chain_len + b; the bonds between chains get the
+    // identity (never applied)
+    const size_t d4 = c->d4();
+    const int d2 = c->d * c->d;
+    std::vector<zc> all(d4 * (c->L - 1), zc{0.0, 0.0});
+    for (int link = 0; link < c->L - 1; ++link) {
+      zc* dst = all.data() + d4 * link;
+      if (is_cut(c, link)) {
+        for (int q = 0; q < d2; ++q) dst[q + (size_t)d2 * q] = zc{1.0, 0.0};
+      } else {
+        const int r = link / c->chain_len, b = link % c->chain_len;
+        std::memcpy(dst, U + 2 * d4 * ((size_t)r * per + b), sizeof(zc) * d4);
+      }
+    }
+    HIPCHK(c, hipMemcpy(c->gates, all.data(), sizeof(zc) * all.size(), hipMemcpyHostToDevice));
+  }
   c->have_gates = true;
   return TEBD_OK;
 }
@@ -1060,7 +1261,9 @@ int tebd_trotter_step(tebd_ctx* c, double* truncerr) {
   for (int half = 0; half < 3; ++half) {
     links.clear();
     gp.clear();
-    for (int b = starts[half]; b < L - 1; b += 2) {
+    for (int b = 0; b < L - 1; ++b) {
+      // reference bond numbering restarts in every chain of an ensemble
+      if (is_cut(c, b) || (b % c->chain_len) % 2 != starts[half]) continue;
       links.push_back(b);
       gp.push_back(c->gates + (size_t)b * c->d4());
     }
@@ -1091,6 +1294,7 @@ int tebd_two_site_op(tebd_ctx* c, int link, const double* op2, double* truncerr)
   int e = ensure_state(c, false);
   if (e) return e;
   if (link < 1 || link > c->L - 1 || !op2) return fail(c, TEBD_ERR_ARG, "two_site_op: bad link or operator");
+  if (is_cut(c, link - 1)) return fail(c, TEBD_ERR_ARG, "two_site_op: bond joins two chains of the ensemble");
   zc* slot = c->gates + (size_t)(c->L - 1) * c->d4();
   HIPCHK(c, hipMemcpyAsync(slot, op2, sizeof(zc) * c->d4(), hipMemcpyHostToDevice, c->st));
   const int l0 = link - 1;
@@ -1107,9 +1311,9 @@ int tebd_canonical_form_all(tebd_ctx* c) {
   if (e) return e;
   if ((e = ensure_cf_workspace(c))) return e;
   for (int k = 0; k < c->L - 1; ++k)
-    if ((e = canonical_form(c, k))) return e;
+    if (!is_cut(c, k) && (e = canonical_form(c, k))) return e;
   for (int k = c->L - 2; k >= 0; --k)
-    if ((e = canonical_form(c, k))) return e;
+    if (!is_cut(c, k) && (e = canonical_form(c, k))) return e;
   HIPCHK(c, hipStreamSynchronize(c->st));
   drain_profile(c);
   return TEBD_OK;
@@ -1119,6 +1323,7 @@ int tebd_canonical_form(tebd_ctx* c, int link) {
   int e = ensure_state(c, false);
   if (e) return e;
   if (link < 1 || link > c->L - 1) return fail(c, TEBD_ERR_ARG, "canonical_form: bad link");
+  if (is_cut(c, link - 1)) return fail(c, TEBD_ERR_ARG, "canonical_form: bond joins two chains of the ensemble");
   if ((e = ensure_cf_workspace(c))) return e;
   if ((e = canonical_form(c, link - 1))) return e;
   HIPCHK(c, hipStreamSynchronize(c->st));
@@ -1261,6 +1466,23 @@ int tebd_get_profile(tebd_ctx* c, double* out, int n) {
   HIPCHK(c, hipStreamSynchronize(c->st));
   drain_profile(c);
   for (int i = 0; i < n && i < kNumProf; ++i) out[i] = c->prof[i];
+  return TEBD_OK;
+}
+
+int tebd_set_autotune(tebd_ctx* c, int on) {
+  if (!c) return TEBD_ERR_ARG;
+  c->autotune = on != 0;
+  if (on < 0) c->tuned.clear();
+  return TEBD_OK;
+}
+
+int tebd_autotune_info(tebd_ctx* c, double* out, int n) {
+  if (!c || !out || n < 0) return TEBD_ERR_ARG;
+  std::vector<double> v(2 + ZGEMM_VARIANTS, 0.0);
+  v[0] = (double)c->tuned.size();
+  v[1] = c->tune_trials;
+  for (const auto& kv : c->tuned) v[2 + kv.second] += 1;
+  for (int i = 0; i < n && i < (int)v.size(); ++i) out[i] = v[i];
   return TEBD_OK;
 }
 

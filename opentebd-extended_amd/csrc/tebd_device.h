@@ -104,4 +104,5 @@ struct BondTask {
   zc* Zc;            // Cb x chi_max: Cb-side singular vectors of the kept values, sorted
   zc* Qs;            // Rb x r: explicit orthonormal factor (sketched SVD's range basis)
   const double* wtot;  // when set, the reference total weight (||theta||^2) for the truncation
+  int rng_key;       // sketch stream of the bond: its position in its chain (not its batch slot)
 };

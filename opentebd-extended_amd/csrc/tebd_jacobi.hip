@@ -464,8 +464,11 @@ hipError_t launch_jacobi_round(const BondTask* d_tasks, int ntasks, int max_pair
   else
     hipLaunchKernelGGL(k_jacobi_round<false>, dim3(max_pairs, ntasks), dim3(256), 0, st, d_tasks, sweep, round,
                        tol_rel, tol_abs, wl_count, wl_items, wl_cap, inner_sweeps);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t launch_jacobi_rotate(const BondTask* d_tasks, int round, int max_rows_w, int max_rows_v, int* wl_count,
+                                int2* wl_items, int wl_cap, hipStream_t st) {
   const int nw = (max_rows_w + kRotRows - 1) / kRotRows, nv = (max_rows_v + kRotRows - 1) / kRotRows;
   hipLaunchKernelGGL(k_jacobi_rotate, dim3(1024), dim3(256), 0, st, d_tasks, round, wl_count, wl_items, wl_cap, nw,
                      nw + nv);

@@ -1,10 +1,14 @@
-// Host-side launchers of the gfx950 kernels (tebd_kernels.hip, tebd_jacobi.hip, tebd_qrp.hip,
+// Host-side launchers of the gfx950 kernels (tebd_kernels.hip, tebd_theta_mixed.hip, tebd_jacobi.hip, tebd_qrp.hip,
 // tebd_sketch.hip, tebd_canonical.hip, tebd_observables.hip, tebd_mlp.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include "tebd_device.h"
 
 hipError_t launch_theta_gate(const BondTask* d_tasks, int ntasks, int d, int max_chil, int max_chir, hipStream_t st);
+hipError_t launch_theta_gate_mfma(const BondTask* d_tasks, int ntasks, int d, int max_chil, int max_chir,
+                                  hipStream_t st);
+hipError_t launch_theta_gate_mixed(const BondTask* d_tasks, int ntasks, int d, int max_chil, int max_chir,
+                                   hipStream_t st);
 hipError_t launch_prepare_svd(const BondTask* d_tasks, int ntasks, hipStream_t st);
 hipError_t launch_frobenius(const BondTask* d_tasks, int ntasks, int max_cpad, hipStream_t st, int keep_sum);
 hipError_t launch_jacobi_begin(const BondTask* d_tasks, int ntasks, hipStream_t st);
@@ -12,6 +16,9 @@ hipError_t launch_jacobi_sweep_begin(const BondTask* d_tasks, int ntasks, int sw
 hipError_t launch_jacobi_round(const BondTask* d_tasks, int ntasks, int max_pairs, int sweep, int round,
                                double tol_rel, double tol_abs, int max_rows_w, int max_rows_v, int* wl_count,
                                int2* wl_items, int wl_cap, int inner_sweeps, hipStream_t st);
+hipError_t launch_jacobi_rotate(const BondTask* d_tasks, int round, int max_rows_w, int max_rows_v, int* wl_count,
+                                int2* wl_items, int wl_cap, hipStream_t st);
+hipError_t launch_qrp_trailing(const BondTask* d_tasks, int ntasks, int max_cb, int j0, hipStream_t st);
 hipError_t launch_qrp_begin(const BondTask* d_tasks, int ntasks, double tol_rel, hipStream_t st);
 hipError_t launch_qrp_panel(const BondTask* d_tasks, int ntasks, int max_rb, int max_cb, int j0, hipStream_t st);
 hipError_t launch_qrp_build_x(const BondTask* d_tasks, int ntasks, hipStream_t st);
@@ -22,7 +29,10 @@ hipError_t launch_finalize(const BondTask* d_tasks, int ntasks, int max_cpad, do
 hipError_t launch_mlp_features(const BondTask* d_tasks, int ntasks, int L, long long step, float* d_X, hipStream_t st);
 hipError_t launch_mlp_predict(const float* d_params, const float* d_X, int n, float* d_out, hipStream_t st);
 hipError_t launch_write_gammas(const BondTask* d_tasks, int ntasks, hipStream_t st);
-hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_n, bool conj_a, hipStream_t st);
+// tile variants of the batched complex GEMM (chosen per shape by the autotuner)
+enum ZgemmVariant { ZGEMM_VALU_64x64 = 0, ZGEMM_MFMA_64x64 = 1, ZGEMM_MFMA_128x64 = 2, ZGEMM_VARIANTS = 3 };
+hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_n, bool conj_a, hipStream_t st,
+                        int variant);
 hipError_t launch_build_sketch(const BondTask* d_tasks, const int* d_kprime, int ntasks, unsigned long long seed,
                                hipStream_t st);
 hipError_t launch_col_norms(const BondTask* d_tasks, int ntasks, int max_cpad, hipStream_t st);

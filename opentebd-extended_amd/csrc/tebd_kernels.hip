@@ -123,6 +123,148 @@ __global__ __launch_bounds__(256) void k_theta_gate(const BondTask* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// theta + gate on the FP64 matrix cores (v_mfma_f64_16x16x4f64).  Same tile as
+// k_theta_gate: BA alphas x BA betas, rows (k, alpha) and columns (l, beta) of the
+// pre-gate theta, TR = BA d.  Stage 1 accumulates the (TR/16)^2 blocks of 16 x 16 over
+// the middle bond (4 waves, blocks dealt round-robin); stage 2 applies the gate to the
+// BA^2 (alpha, beta) pairs as one (d^2 x d^2) x (d^2 x BA^2) complex product.
+// ---------------------------------------------------------------------------
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// D(16x16) += A(16x4) B(4x16): lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// element g of the result in lane l is D[l/16 + 4g][l%16].
+__device__ __forceinline__ d4 mfma4(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ void zmfma(zc a, zc b, d4& re, d4& im) {
+  re = mfma4(a.x, b.x, re);
+  re = mfma4(-a.y, b.y, re);
+  im = mfma4(a.x, b.y, im);
+  im = mfma4(a.y, b.x, im);
+}
+
+template <int D>
+struct MfmaThetaCfg {
+  static constexpr int BA = (D <= 5) ? 16 : 8;
+  static constexpr int TR = BA * D;
+  static constexpr int NB1 = TR / 16;
+  static constexpr int NBW = (NB1 * NB1 + 3) / 4;
+  static constexpr int D2 = D * D;
+  static constexpr int MP = (D2 + 15) / 16 * 16;
+  static constexpr int KP = (D2 + 3) / 4 * 4;
+  static constexpr int NP = BA * BA;
+  static constexpr int LDT = TR + 1;
+  static constexpr int AB = 2 * KC * LDT;
+  static constexpr int SM = AB > TR * LDT ? AB : TR * LDT;
+};
+
+template <int D>
+__global__ __launch_bounds__(256) void k_theta_gate_mfma(const BondTask* __restrict__ tasks) {
+  using C = MfmaThetaCfg<D>;
+  constexpr int BA = C::BA, TR = C::TR, NB1 = C::NB1, NBW = C::NBW, D2 = C::D2;
+  constexpr int MP = C::MP, KP = C::KP, NP = C::NP, LDT = C::LDT;
+  const BondTask t = tasks[blockIdx.z];
+  const int a0 = blockIdx.x * BA, b0 = blockIdx.y * BA;
+  if (a0 >= t.chil || b0 >= t.chir) return;
+
+  __shared__ zc sm[C::SM];     // As, Bs during stage 1; the pre-gate tile T after it
+  __shared__ zc sU[MP * KP];   // gate, row-major [ij][kl], zero padded
+  zc* As = sm;                 // [KC][LDT]
+  zc* Bs = sm + KC * LDT;      // [KC][LDT]
+  zc* T = sm;                  // [TR][LDT]
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  for (int e = tid; e < MP * KP; e += 256) {
+    const int ij = e / KP, kl = e % KP;
+    sU[e] = (ij < D2 && kl < D2) ? t.gate[ij + D2 * kl] : zmk(0.0, 0.0);
+  }
+
+  d4 re[NBW], im[NBW];
+#pragma unroll
+  for (int s = 0; s < NBW; ++s) {
+    re[s] = d4{0.0, 0.0, 0.0, 0.0};
+    im[s] = d4{0.0, 0.0, 0.0, 0.0};
+  }
+
+  const int chil = t.chil, chic = t.chic, chir = t.chir;
+  for (int g0 = 0; g0 < chic; g0 += KC) {
+    for (int e = tid; e < KC * TR; e += 256) {
+      const int r = e % TR, g = e / TR;
+      const int k = r / BA, a = a0 + (r % BA), gg = g0 + g;
+      zc v = zmk(0.0, 0.0);
+      if (a < chil && gg < chic) v = zscale(t.gl[a + (size_t)chil * (k + (size_t)D * gg)], t.lc[gg]);
+      As[g * LDT + r] = v;
+    }
+    for (int e = tid; e < KC * TR; e += 256) {
+      const int g = e % KC, c = e / KC;
+      const int l = c / BA, b = b0 + (c % BA), gg = g0 + g;
+      Bs[g * LDT + c] = (b < chir && gg < chic) ? t.gr[gg + (size_t)chic * (l + (size_t)D * b)] : zmk(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ++ks) {
+      const int kk = ks * 4 + lk;
+#pragma unroll
+      for (int s = 0; s < NBW; ++s) {
+        const int q = wave + 4 * s;
+        if (q < NB1 * NB1) {
+          const int rb = q / NB1, cb = q % NB1;
+          zmfma(As[kk * LDT + rb * 16 + lr], Bs[kk * LDT + cb * 16 + lr], re[s], im[s]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // pre-gate tile into LDS (over As / Bs, which are no longer read)
+#pragma unroll
+  for (int s = 0; s < NBW; ++s) {
+    const int q = wave + 4 * s;
+    if (q < NB1 * NB1) {
+      const int rb = q / NB1, cb = q % NB1;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) T[(rb * 16 + lk + 4 * g) * LDT + cb * 16 + lr] = zmk(re[s][g], im[s][g]);
+    }
+  }
+  __syncthreads();
+
+  const bool tr = t.transposed != 0;
+  for (int pb = wave; pb < NP / 16; pb += 4) {
+    const int p = pb * 16 + lr;
+    const int al = tr ? p / BA : p % BA;
+    const int bl = tr ? p % BA : p / BA;
+    const int a = a0 + al, b = b0 + bl;
+    const bool in = a < chil && b < chir;
+    const double sc = in ? t.ll[a] * t.lr[b] : 0.0;
+#pragma unroll
+    for (int mb = 0; mb < MP / 16; ++mb) {
+      d4 ore = {0.0, 0.0, 0.0, 0.0}, oim = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < KP / 4; ++ks) {
+        const int kl = ks * 4 + lk;
+        const zc x = kl < D2 ? T[((kl / D) * BA + al) * LDT + (kl % D) * BA + bl] : zmk(0.0, 0.0);
+        zmfma(sU[(mb * 16 + lr) * KP + kl], x, ore, oim);
+      }
+      if (in) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int ij = mb * 16 + lk + 4 * g;
+          if (ij < D2) {
+            const int i = ij / D, j = ij % D;
+            const size_t row = (size_t)i * chil + a;
+            const size_t col = (size_t)j * chir + b;
+            if (!tr) t.W[row + (size_t)t.R * col] = zmk(ore[g] * sc, oim[g] * sc);
+            else t.W[col + (size_t)t.R * row] = zmk(ore[g] * sc, -(oim[g] * sc));
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // W padding columns := 0, V := identity, the reference total weight (||theta||^2 or none)
 // ---------------------------------------------------------------------------
 __global__ void k_prepare_svd(const BondTask* __restrict__ tasks) {
@@ -319,6 +461,24 @@ hipError_t launch_theta_gate(const BondTask* d_tasks, int ntasks, int d, int max
     case 5: return go(k_theta_gate<5>, ThetaCfg<5>::BA);
     case 6: return go(k_theta_gate<6>, ThetaCfg<6>::BA);
     case 8: return go(k_theta_gate<8>, ThetaCfg<8>::BA);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_theta_gate_mfma(const BondTask* d_tasks, int ntasks, int d, int max_chil, int max_chir,
+                                  hipStream_t st) {
+  auto go = [&](auto kern, int BA) {
+    dim3 grid((max_chil + BA - 1) / BA, (max_chir + BA - 1) / BA, ntasks);
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, d_tasks);
+    return hipGetLastError();
+  };
+  switch (d) {
+    case 2: return go(k_theta_gate_mfma<2>, MfmaThetaCfg<2>::BA);
+    case 3: return go(k_theta_gate_mfma<3>, MfmaThetaCfg<3>::BA);
+    case 4: return go(k_theta_gate_mfma<4>, MfmaThetaCfg<4>::BA);
+    case 5: return go(k_theta_gate_mfma<5>, MfmaThetaCfg<5>::BA);
+    case 6: return go(k_theta_gate_mfma<6>, MfmaThetaCfg<6>::BA);
+    case 8: return go(k_theta_gate_mfma<8>, MfmaThetaCfg<8>::BA);
     default: return hipErrorInvalidValue;
   }
 }

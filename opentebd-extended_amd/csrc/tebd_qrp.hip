@@ -154,25 +154,32 @@ __device__ __forceinline__ void gram_reduce(const d4 (&dre)[NP][NQ], const d4 (&
 }
 
 // T (nbp x nbp, upper) of the compact WY form H_0 .. H_{nbp-1} = I - V T V^H from the
-// reflector scalars and G = V^H V:  T[q][q] = tau_q, T[0:q, q] = -tau_q T[0:q, 0:q] G[0:q, q]
-// (one thread); written to the panel's slot of Tq, zero-padded to QB x QB.
-__device__ void form_t(const zc (*G)[QB + 1], const zc* stau, zc (*T)[QB + 1], int nbp, zc* Tp) {
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < nbp; ++q) {
-      for (int pp = 0; pp < q; ++pp) {
+// reflector scalars and G = V^H V:  T[q][q] = tau_q, T[0:q, q] = -tau_q T[0:q, 0:q] G[0:q, q].
+// Row p of T only involves row p's own earlier entries (T is upper triangular), so
+// thread p < QB forms row p in registers; written to the panel's slot Tp (column-major,
+// zero-padded to QB x QB).  Called by every thread of the workgroup.
+__device__ void form_t(const zc (*G)[QB + 1], const zc* stau, int nbp, zc* Tp) {
+  __syncthreads();  // G and stau are complete
+  if (threadIdx.x >= QB) return;
+  const int pp = threadIdx.x;
+  zc row[QB];
+#pragma unroll
+  for (int q = 0; q < QB; ++q) row[q] = zmk(0.0, 0.0);
+  if (pp < nbp) {
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      if (q == pp) row[q] = stau[q];
+      if (q > pp && q < nbp) {
         zc acc = zmk(0.0, 0.0);
-        for (int kk = pp; kk < q; ++kk) zfma(acc, T[pp][kk], G[kk][q]);
-        T[pp][q] = zmul(zmk(-stau[q].x, -stau[q].y), acc);
+#pragma unroll
+        for (int kk = 0; kk < QB; ++kk)
+          if (kk >= pp && kk < q) zfma(acc, row[kk], G[kk][q]);
+        row[q] = zmul(zmk(-stau[q].x, -stau[q].y), acc);
       }
-      T[q][q] = stau[q];
-      for (int pp = q + 1; pp < QB; ++pp) T[pp][q] = zmk(0.0, 0.0);
     }
   }
-  __syncthreads();
-  for (int e = threadIdx.x; e < QB * QB; e += blockDim.x) {
-    const int pp = e % QB, qq = e / QB;
-    Tp[e] = (pp < nbp && qq < nbp) ? T[pp][qq] : zmk(0.0, 0.0);
-  }
+#pragma unroll
+  for (int q = 0; q < QB; ++q) Tp[pp + QB * q] = row[q];
 }
 
 // Householder scalars of the column (alpha; x) with |x|^2 = xn2 (xLARFG convention):
@@ -254,7 +261,6 @@ __global__ __launch_bounds__(PT) void k_qrp_panel_reg(const BondTask* __restrict
   __shared__ double rnorm[PW];
   __shared__ zc salpha;
   __shared__ zc G[QB][QB + 1];
-  __shared__ zc T[QB][QB + 1];
   __shared__ zc stau[QB];
 
   // 1. selection
@@ -439,7 +445,7 @@ __global__ __launch_bounds__(PT) void k_qrp_panel_reg(const BondTask* __restrict
       }
     }
   }
-  form_t(G, stau, T, nbp, t.Tq + (size_t)(j0 / QB) * QB * QB);
+  form_t(G, stau, nbp, t.Tq + (size_t)(j0 / QB) * QB * QB);
   if (tid == 0 && j0 + nbp >= kmax) t.qstate[0] = (double)kmax;
 }
 
@@ -464,7 +470,6 @@ __global__ __launch_bounds__(256) void k_qrp_panel(const BondTask* __restrict__ 
   __shared__ int snmv, sstop;
   __shared__ double red[4];
   __shared__ zc G[QB][QB + 1];
-  __shared__ zc T[QB][QB + 1];
   __shared__ zc stau[QB];
   __shared__ double gred[3][2][256];
   for (int k = tid; k < ncand; k += 256) snrm[k] = t.qnrm[j0 + k];
@@ -561,7 +566,7 @@ __global__ __launch_bounds__(256) void k_qrp_panel(const BondTask* __restrict__ 
     gram_partial<1, 1>(vload, vload, K, wave, 4, lane, gre, gim);
     gram_reduce<1, 1>(gre, gim, gred, &G[0][0], QB + 1, wave, lane);
   }
-  form_t(G, stau, T, nbp, t.Tq + (size_t)(j0 / QB) * QB * QB);
+  form_t(G, stau, nbp, t.Tq + (size_t)(j0 / QB) * QB * QB);
   if (tid == 0 && j0 + nbp >= kmax) t.qstate[0] = (double)kmax;
 }
 
@@ -592,6 +597,7 @@ __global__ __launch_bounds__(256) void k_qrp_reflect(const BondTask* __restrict_
     nc = (t.Cb - kb) < 32 ? (t.Cb - kb) : 32;
     A2 = t.Wq + ld * kb + j0;
   } else {
+    if (MODE == 1 && !t.qr) return;
     const int r = t.C;
     j0 = QB * arg;
     if (j0 >= r) return;
@@ -703,6 +709,7 @@ __global__ __launch_bounds__(256) void k_qrp_reflect(const BondTask* __restrict_
 // trapezoid of Wq.
 __global__ void k_qrp_build_x(const BondTask* __restrict__ tasks) {
   const BondTask t = tasks[blockIdx.y];
+  if (!t.qr) return;  // a task of the batch that skips the QR preconditioning
   const int r = t.C;  // the Jacobi task's column count is the numerical rank
   const size_t n = (size_t)t.Cb * t.Cpad;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
@@ -719,6 +726,7 @@ __global__ void k_qrp_build_x(const BondTask* __restrict__ tasks) {
 template <bool FORM_Q>
 __global__ __launch_bounds__(256) void k_qrp_vectors_init(const BondTask* __restrict__ tasks) {
   const BondTask t = tasks[blockIdx.y];
+  if (!FORM_Q && !t.qr) return;  // its vectors come from the Jacobi's W and V
   const int r = t.C;
   const int k = FORM_Q ? r : (int)t.result[0];
   const int b = blockIdx.x;
@@ -747,8 +755,11 @@ hipError_t launch_qrp_panel(const BondTask* d_tasks, int ntasks, int max_rb, int
   if (max_rb <= PT) hipLaunchKernelGGL(k_qrp_panel_reg<1>, dim3(ntasks), dim3(PT), 0, st, d_tasks, j0);
   else if (max_rb <= 2 * PT) hipLaunchKernelGGL(k_qrp_panel_reg<2>, dim3(ntasks), dim3(PT), 0, st, d_tasks, j0);
   else hipLaunchKernelGGL(k_qrp_panel, dim3(ntasks), dim3(256), 0, st, d_tasks, j0);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess || j0 + QB >= max_cb) return e;
+  return hipGetLastError();
+}
+
+hipError_t launch_qrp_trailing(const BondTask* d_tasks, int ntasks, int max_cb, int j0, hipStream_t st) {
+  if (j0 + QB >= max_cb) return hipSuccess;
   hipLaunchKernelGGL(k_qrp_reflect<0>, dim3((max_cb - j0 - 1) / 32 + 1, ntasks), dim3(256), 0, st, d_tasks, j0);
   return hipGetLastError();
 }

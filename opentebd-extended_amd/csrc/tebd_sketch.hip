@@ -80,6 +80,96 @@ __global__ __launch_bounds__(256) void k_zgemm(const GemmTask* __restrict__ task
 }
 
 // ---------------------------------------------------------------------------
+// batched complex GEMM on the FP64 matrix cores: TM x 64 tiles, 4 waves as 2 x 2,
+// each wave (TM/2) x 32 = (TM/32) x 2 blocks of 16 x 16, complex products as four
+// real v_mfma_f64_16x16x4f64 per k-step, operands staged through LDS in chunks of 16.
+// ---------------------------------------------------------------------------
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// D(16x16) += A(16x4) B(4x16): lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// element g of the result in lane l is D[l/16 + 4g][l%16].
+__device__ __forceinline__ d4 mfma4(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int MK = 16;  // k chunk staged in LDS
+
+template <bool CONJ_A, int TM>
+__global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__ tasks) {
+  constexpr int TN = 64, RB = TM / 32;
+  const GemmTask t = tasks[blockIdx.z];
+  const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
+  if (m0 >= t.M || n0 >= t.N) return;
+  __shared__ zc As[MK][TM + 1];
+  __shared__ zc Bs[MK][TN + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int wr = wave >> 1, wc = wave & 1;
+  d4 re[RB][2], im[RB][2];
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      re[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      im[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+  for (int k0 = 0; k0 < t.K; k0 += MK) {
+    for (int e = tid; e < MK * TM; e += 256) {
+      int kk, mm;
+      zc v = zmk(0.0, 0.0);
+      if (!CONJ_A) {  // A is M x K: contiguous along m
+        mm = e % TM; kk = e / TM;
+        const int m = m0 + mm, k = k0 + kk;
+        if (m < t.M && k < t.K) v = t.A[m + (size_t)t.lda * k];
+      } else {        // A is K x M: contiguous along k
+        kk = e % MK; mm = e / MK;
+        const int m = m0 + mm, k = k0 + kk;
+        if (m < t.M && k < t.K) v = zconj(t.A[k + (size_t)t.lda * m]);
+      }
+      As[kk][mm] = v;
+    }
+    for (int e = tid; e < MK * TN; e += 256) {
+      const int kk = e % MK, nn = e / MK;
+      const int k = k0 + kk, n = n0 + nn;
+      Bs[kk][nn] = (k < t.K && n < t.N) ? t.B[k + (size_t)t.ldb * n] : zmk(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < MK / 4; ++ks) {
+      const int kk = 4 * ks + lk;
+      zc a[RB], b[2];
+#pragma unroll
+      for (int i = 0; i < RB; ++i) a[i] = As[kk][wr * (TM / 2) + 16 * i + lr];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) b[j] = Bs[kk][wc * 32 + 16 * j + lr];
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          re[i][j] = mfma4(a[i].x, b[j].x, re[i][j]);
+          re[i][j] = mfma4(-a[i].y, b[j].y, re[i][j]);
+          im[i][j] = mfma4(a[i].x, b[j].y, im[i][j]);
+          im[i][j] = mfma4(a[i].y, b[j].x, im[i][j]);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int m = m0 + wr * (TM / 2) + 16 * i + lk + 4 * g, n = n0 + wc * 32 + 16 * j + lr;
+        if (m < t.M && n < t.N) {
+          zc v = zmk(re[i][j][g] * t.alpha, im[i][j][g] * t.alpha);
+          if (t.accumulate) v = zadd(v, t.C[m + (size_t)t.ldc * n]);
+          t.C[m + (size_t)t.ldc * n] = v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
 // sketch construction
 // ---------------------------------------------------------------------------
 // 64-bit mix (splitmix64 finaliser) -> uniform (0,1]
@@ -107,7 +197,7 @@ __global__ void k_build_sketch(const BondTask* __restrict__ tasks, const int* __
     if (c < t.chic && c < kp) {
       v = zconj(zscale(t.gr[c + (size_t)t.chic * (j + (size_t)t.d * b)], t.lr[b]));
     } else if (c < kp) {
-      const unsigned long long key = seed ^ ((unsigned long long)blockIdx.y << 48) ^ ((unsigned long long)e << 1);
+      const unsigned long long key = seed ^ ((unsigned long long)t.rng_key << 48) ^ ((unsigned long long)e << 1);
       const double u1 = u01(key), u2 = u01(key + 0x632BE59BD9B4E019ull);
       const double r = sqrt(-2.0 * log(u1));
       v = zmk(r * cos(6.283185307179586 * u2), r * sin(6.283185307179586 * u2));
@@ -118,11 +208,23 @@ __global__ void k_build_sketch(const BondTask* __restrict__ tasks, const int* __
 
 }  // namespace
 
-hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_n, bool conj_a, hipStream_t st) {
-  dim3 grid((max_m + GT - 1) / GT, (max_n + GT - 1) / GT, ntasks);
-  if (conj_a) hipLaunchKernelGGL(k_zgemm<true>, grid, dim3(256), 0, st, d_tasks);
-  else hipLaunchKernelGGL(k_zgemm<false>, grid, dim3(256), 0, st, d_tasks);
-  return hipGetLastError();
+hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_n, bool conj_a, hipStream_t st,
+                        int variant) {
+  auto go = [&](auto kern, int tm, int tn) {
+    dim3 grid((max_m + tm - 1) / tm, (max_n + tn - 1) / tn, ntasks);
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, d_tasks);
+    return hipGetLastError();
+  };
+  switch (variant) {
+    case ZGEMM_VALU_64x64:
+      return conj_a ? go(k_zgemm<true>, GT, GT) : go(k_zgemm<false>, GT, GT);
+    case ZGEMM_MFMA_64x64:
+      return conj_a ? go(k_zgemm_mfma<true, 64>, 64, 64) : go(k_zgemm_mfma<false, 64>, 64, 64);
+    case ZGEMM_MFMA_128x64:
+      return conj_a ? go(k_zgemm_mfma<true, 128>, 128, 64) : go(k_zgemm_mfma<false, 128>, 128, 64);
+    default:
+      return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_build_sketch(const BondTask* d_tasks, const int* d_kprime, int ntasks, unsigned long long seed,

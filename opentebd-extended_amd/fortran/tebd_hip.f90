@@ -22,6 +22,16 @@ INTERFACE
     INTEGER(c_int), VALUE :: mode
   END FUNCTION tebd_create
 
+  ! nchain independent chains of L sites in one context (disorder realisations)
+  INTEGER(c_int) FUNCTION tebd_create_ensemble(ctx, nchain, L, d, chi_max, order, bc, mode) &
+      BIND(C, name="tebd_create_ensemble")
+    IMPORT :: c_ptr, c_int, c_char
+    TYPE(c_ptr) :: ctx
+    INTEGER(c_int), VALUE :: nchain, L, d, chi_max, order
+    CHARACTER(kind=c_char), VALUE :: bc
+    INTEGER(c_int), VALUE :: mode
+  END FUNCTION tebd_create_ensemble
+
   SUBROUTINE tebd_destroy(ctx) BIND(C, name="tebd_destroy")
     IMPORT :: c_ptr
     TYPE(c_ptr), VALUE :: ctx

@@ -47,6 +47,7 @@ def lib():
         I = ctypes.c_int
         D = ctypes.c_double
         L.tebd_create.argtypes = [ctypes.POINTER(P), I, I, I, I, ctypes.c_char, ctypes.c_uint]
+        L.tebd_create_ensemble.argtypes = [ctypes.POINTER(P), I, I, I, I, I, ctypes.c_char, ctypes.c_uint]
         L.tebd_destroy.argtypes = [P]
         L.tebd_destroy.restype = None
         L.tebd_set_gates.argtypes = [P, P, I]
@@ -63,6 +64,8 @@ def lib():
         L.tebd_get_bond_dims.argtypes = [P, P]
         L.tebd_get_stats.argtypes = [P, P, I]
         L.tebd_sync.argtypes = [P]
+        L.tebd_set_autotune.argtypes = [P, I]
+        L.tebd_autotune_info.argtypes = [P, P, I]
         L.tebd_set_profiling.argtypes = [P, I]
         L.tebd_get_profile.argtypes = [P, P, I]
         L.tebd_expval_two_site.argtypes = [P, P, P]
@@ -77,9 +80,10 @@ def lib():
         L.tebd_mlp_predict.argtypes = [P, P, I, P]
         L.tebd_last_error.argtypes = [P]
         L.tebd_last_error.restype = ctypes.c_char_p
-        for name in ("tebd_create", "tebd_set_gates", "tebd_upload", "tebd_upload_packed", "tebd_download",
+        for name in ("tebd_create", "tebd_create_ensemble", "tebd_set_gates", "tebd_upload", "tebd_upload_packed", "tebd_download",
                      "tebd_download_packed", "tebd_download_lambdas", "tebd_trotter_step", "tebd_two_site_op", "tebd_canonical_form_all", "tebd_canonical_form",
-                     "tebd_set_cutoff", "tebd_get_bond_dims", "tebd_get_stats", "tebd_sync",
+                     "tebd_set_cutoff", "tebd_get_bond_dims", "tebd_get_stats", "tebd_sync", "tebd_set_autotune",
+                     "tebd_autotune_info",
                      "tebd_set_profiling", "tebd_get_profile", "tebd_expval_two_site", "tebd_rank_model_set",
                      "tebd_set_learned_params", "tebd_rank_model_get", "tebd_rank_log", "tebd_rank_log_get", "tebd_mlp_gradient",
                      "tebd_mlp_train", "tebd_mlp_predict"):

@@ -46,6 +46,18 @@ def parse_mode(mode):
     return m
 
 
+# profiled launch classes of tebd_get_profile, in order, with the kernels each covers
+PROFILE_KINDS = {
+    "theta_gate": "k_theta_gate / k_theta_gate_mixed",
+    "jacobi_round": "k_jacobi_round",
+    "truncate_gamma": "k_truncate, k_write_gammas, k_qrp_vectors_init, k_qrp_reflect<1,2>, k_build_sketch",
+    "qr_panel": "k_qrp_panel_reg",
+    "sketch_gemm": "k_zgemm",
+    "jacobi_rotate": "k_jacobi_rotate",
+    "qr_trailing": "k_qrp_reflect<0>",
+}
+
+
 class Context:
     """One chain on one GPU (one HIP stream)."""
 
@@ -196,6 +208,7 @@ class Context:
     def canonical_form_all(self):
         self._check(self._lib.tebd_canonical_form_all(self._h))
 
+
     def canonical_form(self, link):
         self._check(self._lib.tebd_canonical_form(self._h, int(link)))
 
@@ -219,13 +232,107 @@ class Context:
     def profile(self):
         """{kernel: (launches, milliseconds, algorithmic flops, algorithmic HBM bytes)} since
         set_profiling(True)."""
-        out = np.zeros(20)
-        self._check(self._lib.tebd_get_profile(self._h, _p(out), 20))
-        names = ("theta_gate", "jacobi_round", "truncate_gamma", "qr_step", "sketch_gemm")
+        out = np.zeros(4 * len(PROFILE_KINDS))
+        self._check(self._lib.tebd_get_profile(self._h, _p(out), out.size))
+        names = tuple(PROFILE_KINDS)
         return {n: tuple(out[4 * i:4 * i + 4]) for i, n in enumerate(names)}
 
     def sync(self):
         self._check(self._lib.tebd_sync(self._h))
+
+    def set_autotune(self, on=True):
+        """Autotuning of launch shapes: True on, False off, -1 on and forget the choices."""
+        self._check(self._lib.tebd_set_autotune(self._h, int(on)))
+
+    def autotune_info(self):
+        out = np.zeros(5)
+        self._check(self._lib.tebd_autotune_info(self._h, _p(out), 5))
+        return {"shape_classes": int(out[0]), "timing_launches": int(out[1]),
+                "chosen": {"valu_64x64": int(out[2]), "mfma_64x64": int(out[3]), "mfma_128x64": int(out[4])}}
+
+
+class Ensemble(Context):
+    """`nchain` independent chains of `L` sites (e.g. disorder realisations) in one
+    context: every bond of one parity of every chain is updated by the same kernel
+    launches (tebd_create_ensemble).  The Context methods act on the joined chain of
+    nchain * L sites; the *_chains methods take and return per-chain arrays."""
+
+    def __init__(self, nchain, L, d, chi_max, mode="batched", order=2, bc="O", cutoff=None):
+        self._lib = _lib.lib()
+        self.nchain, self.chain_len = int(nchain), int(L)
+        self.L, self.d, self.chi_max = self.nchain * self.chain_len, int(d), int(chi_max)
+        self.mode = parse_mode(mode)
+        h = ctypes.c_void_p()
+        rc = self._lib.tebd_create_ensemble(ctypes.byref(h), self.nchain, self.chain_len, self.d, self.chi_max,
+                                            int(order), bc.encode()[:1], self.mode)
+        if rc != 0:
+            raise TEBDError(rc, "tebd_create_ensemble failed")
+        self._h = h
+        if cutoff is not None:
+            self.set_cutoff(cutoff)
+
+    def set_gates(self, U):
+        """U: (nchain, L-1, d^2, d^2) complex gates, chain by chain."""
+        U = np.asarray(U, np.complex128)
+        n = self.nchain * (self.chain_len - 1)
+        u = np.ascontiguousarray(np.transpose(U.reshape(n, self.d ** 2, self.d ** 2), (0, 2, 1)))
+        self._check(self._lib.tebd_set_gates(self._h, _p(u), n))
+
+    def upload_chains(self, gams, lams):
+        """gams: (nchain, L, chi, d, chi), lams: (nchain, L+1, chi); the outer lambdas of
+        every chain are (1, 0, ...)."""
+        gams = np.asarray(gams, np.complex128)
+        lams = np.asarray(lams, np.float64)
+        n, Lc = self.nchain, self.chain_len
+        if gams.shape[:2] != (n, Lc) or lams.shape[:2] != (n, Lc + 1):
+            raise ValueError("need nchain states of L sites")
+        lam = np.concatenate([lams[:, :Lc].reshape(n * Lc, -1), lams[-1:, Lc]], axis=0)
+        self.upload(gams.reshape((n * Lc,) + gams.shape[2:]), lam)
+
+    def _split_lam(self, lam):
+        Lc = self.chain_len
+        return np.stack([lam[r * Lc:r * Lc + Lc + 1] for r in range(self.nchain)])
+
+    def download_chains(self, chi_out=None):
+        gam, lam = self.download(chi_out)
+        return gam.reshape((self.nchain, self.chain_len) + gam.shape[1:]), self._split_lam(lam)
+
+    def lambdas_chains(self, chi_out=None):
+        return self._split_lam(self.lambdas(chi_out))
+
+    def expval_two_site_chains(self, ops):
+        """ops: (nchain, L-1, d^2, d^2) -> (nchain, L-1) complex expectation values."""
+        ops = np.asarray(ops, np.complex128)
+        Lc, d2 = self.chain_len, self.d ** 2
+        full = np.zeros((self.L - 1, d2, d2), np.complex128)
+        for r in range(self.nchain):
+            full[r * Lc:r * Lc + Lc - 1] = ops[r]
+        vals = self.expval_two_site(full)
+        return np.stack([vals[r * Lc:r * Lc + Lc - 1] for r in range(self.nchain)])
+
+    def energy_chains(self, H):
+        """TotalEnergy of every chain; H: (nchain, L-1, d^2, d^2)."""
+        return self.expval_two_site_chains(H).real.sum(axis=1)
+
+    def one_site_expval_chains(self, op):
+        op = np.asarray(op, np.complex128)
+        one = np.eye(self.d)
+        n, Lc = self.nchain, self.chain_len
+        left = np.broadcast_to(np.kron(op, one), (n, Lc - 1, self.d ** 2, self.d ** 2))
+        right = np.broadcast_to(np.kron(one, op), (n, Lc - 1, self.d ** 2, self.d ** 2))
+        vl = self.expval_two_site_chains(left).real
+        vr = self.expval_two_site_chains(right).real
+        return np.concatenate([vl, vr[:, -1:]], axis=1)
+
+    def chain_entropy_chains(self):
+        lam = self.lambdas_chains()
+        out = np.zeros((self.nchain, self.chain_len - 1))
+        for r in range(self.nchain):
+            for link in range(2, self.chain_len + 1):
+                v = lam[r, link - 1]
+                v = v[v != 0.0]
+                out[r, link - 2] = -np.sum(v * v * np.log(np.abs(v * v))) / np.log(self.d)
+        return out
 
 
 def imag_time_prop(ctx, H, dtau, max_steps, steps_for_judge=10, conv_criterion=1e-6):

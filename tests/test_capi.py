@@ -45,3 +45,32 @@ def test_null_context_calls_are_errors():
     assert lib.tebd_trotter_step(None, ctypes.byref(t)) == _lib.TEBD_ERR_ARG
     assert lib.tebd_set_cutoff(None, 1e-12) == _lib.TEBD_ERR_ARG
     assert lib.tebd_last_error(None) == b"null context"
+
+
+@pytest.mark.parametrize("args", [
+    (0, 8, 2, 4, 2, b"O"),   # no chain
+    (3, 1, 2, 4, 2, b"O"),   # chains of one site
+    (3, 8, 7, 4, 2, b"O"),   # unsupported local dimension
+    (3, 8, 2, 4, 2, b"P"),   # periodic boundaries not provided
+])
+def test_create_ensemble_rejects_bad_arguments_without_touching_the_gpu(args):
+    lib = _lib.lib()
+    h = ctypes.c_void_p()
+    rc = lib.tebd_create_ensemble(ctypes.byref(h), *args, 0)
+    assert rc == _lib.TEBD_ERR_ARG
+    assert not h.value
+
+
+def test_python_contexts_expose_the_library_surface():
+    """Every method the bench, the tests and the docs use is on the Python classes
+    (checked without a device)."""
+    from tebd_amd import Context, Ensemble
+    for name in ("upload", "download", "set_gates", "trotter_step", "two_site_op", "canonical_form_all",
+                 "canonical_form", "bond_dims", "stats", "set_profiling", "profile", "sync", "energy",
+                 "one_site_expval", "chain_entropy", "expval_two_site", "set_rank_model", "rank_log",
+                 "set_autotune", "autotune_info", "set_cutoff", "lambdas"):
+        assert callable(getattr(Context, name, None)), name
+    for name in ("upload_chains", "download_chains", "energy_chains", "one_site_expval_chains",
+                 "chain_entropy_chains", "expval_two_site_chains"):
+        assert callable(getattr(Ensemble, name, None)), name
+    assert issubclass(Ensemble, Context)

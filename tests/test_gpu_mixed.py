@@ -1,0 +1,58 @@
+"""Mixed-precision mode: theta and the gate on the FP32 matrix cores, everything
+after theta (Gram / orthogonalisation, cutoff, lambda renormalisation) in FP64.
+
+Tolerance: stated per test.  FP32 rounding of theta (unit roundoff 6e-8, FP32
+accumulation over the middle bond) perturbs each bond update by ~1e-7 relative; over
+a trajectory the observables drift from the reference by the amounts recorded in
+MIXED_TOL, about 15x the largest error measured on MI355X.  The exact
+(FP64) modes are held to 1e-9 in test_gpu_parity.py.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from test_gpu_parity import RTP, run_device
+from tebd_amd import Context
+
+pytestmark = pytest.mark.gpu
+
+# max |observable - reference| over the trajectories of tests/golden, FP32 theta; the
+# largest error measured on MI355X is 6.0e-6 (spin-1 Heisenberg L=8 energy)
+MIXED_TOL = 1e-4
+
+
+@pytest.mark.parametrize("mode", ["mixed", "batched,mixed", "batched,learned,mixed"])
+@pytest.mark.parametrize("name", RTP)
+def test_mixed_trajectory_error_is_bounded(name, mode):
+    z = load_golden(name)
+    print(f"[gpu] mixed trajectory {name} mode={mode}", flush=True)
+    worst, dims = run_device(z, mode)
+    print(f"[gpu] {name} {mode} errors {worst}", flush=True)
+    assert max(worst.values()) < MIXED_TOL, worst
+    assert dims.max() <= int(z["chi"])
+
+
+def test_mixed_two_site_op_matches_fp64_to_fp32_rounding():
+    """One update with a random non-unitary operator on a random state: the mixed
+    theta differs from the FP64 one only by FP32 rounding, so lambda and the
+    gauge-invariant block lambda Gamma lambda Gamma lambda agree to ~1e-6."""
+    z = load_golden("two_site_op_l4")
+    print("[gpu] mixed two_site_op_l4", flush=True)
+    L, d, chi, link = int(z["L"]), int(z["d"]), int(z["chi"]), int(z["link"])
+    out = {}
+    for mode in ("fp64", "mixed"):
+        ctx = Context(L, d, chi, mode=mode, cutoff=0.0)
+        ctx.upload(z["gam_in"], z["lam_in"])
+        ctx.two_site_op(link, z["U"][link - 1])
+        out[mode] = ctx.download(chi)
+        ctx.close()
+    (g0, l0), (g1, l1) = out["fp64"], out["mixed"]
+    rec = lambda G, lm: np.einsum("a,bia,b,cjb,c->aijc", lm[link - 1], G[link - 1], lm[link], G[link], lm[link + 1])
+    r0, r1 = rec(g0, l0), rec(g1, l1)
+    scale = np.abs(r0).max()
+    err_l = np.abs(l1 - l0).max()
+    err_r = np.abs(r1 - r0).max() / scale
+    print(f"[gpu] mixed two-site op: lambda err {err_l:.2e}, block err {err_r:.2e}", flush=True)
+    assert err_l < 1e-5 and err_r < 1e-5
+    # and it is not bit-identical: the FP32 path really ran
+    assert err_l > 0.0 or err_r > 0.0
