@@ -295,7 +295,7 @@ int slot_of(const tebd_ctx* c, const BondTask& t) { return (int)((t.qstate - c->
 int qrp_factor(tebd_ctx* c, BondTask* D, const BondTask* H, int nb, int max_cb, std::vector<int>& rank) {
   int max_rb = 1;
   for (int i = 0; i < nb; ++i) max_rb = std::max(max_rb, H[i].Rb);
-  HIPCHK(c, launch_qrp_begin(D, nb, kQrTol, c->st));
+  HIPCHK(c, launch_qrp_begin(D, nb, max_cb, kQrTol, c->st));
   // every task's last panel (j0 + width >= min(Rb, Cb)) records its rank, so after the
   // panel with j0 + kQrPanel >= max_cb all tasks have finished
   bool done = false;

@@ -169,9 +169,16 @@ __device__ __forceinline__ void inner_pair(int rd, int t, int& p, int& q) {
 // place) and two (row, pair) groups of Q; blocks whose two pairs both stay put are
 // skipped.  A Gram entry is accurate relative to sqrt(A_pp A_qq), so the solve uses
 // a purely relative test (no rotation when |A_pq| <= 1e-15 sqrt(A_pp A_qq)), sets
-// each annihilated entry to exactly zero and ends after a sweep without rotations.
+// each annihilated entry to exactly zero and ends when a sweep would rotate nothing.
+// That is decided by testing every pair once after a sweep: a sweep that rotates
+// nothing leaves A unchanged, so it rotates nothing iff no pair passes the test now.
+__device__ __forceinline__ bool needs_rotation(zc g, double a, double b) {
+  const double g2 = zabs2(g);
+  return !(g2 <= 1e-30 * fabs(a * b) || g2 == 0.0);
+}
+
 __device__ void inner_jacobi(zc (*A)[JC + 1], zc (*Q)[JC + 1], double* rc_c, double* rc_s, zc* rc_w, int* s_round,
-                             int* s_sweep, int tid, int max_sweeps) {
+                             int* s_sweep, int* s_more, int tid, int max_sweeps) {
   for (int e = tid; e < JC * JC; e += 256) {
     const int r = e % JC, c = e / JC;
     Q[r][c] = zmk(r == c ? 1.0 : 0.0, 0.0);
@@ -187,8 +194,7 @@ __device__ void inner_jacobi(zc (*A)[JC + 1], zc (*Q)[JC + 1], double* rc_c, dou
         zc w = zmk(1.0, 0.0);
         const zc g = A[pp][qq];
         const double a = A[pp][pp].x, b = A[qq][qq].x;
-        const double g2 = zabs2(g);
-        if (!(g2 <= 1e-30 * fabs(a * b) || g2 == 0.0)) jrot(a, b, g, c, s, w);
+        if (needs_rotation(g, a, b)) jrot(a, b, g, c, s, w);
         rc_c[tid] = c; rc_s[tid] = s; rc_w[tid] = w;
         if (s != 0.0) {
           s_round[rd & 1] = 1;
@@ -245,6 +251,16 @@ __device__ void inner_jacobi(zc (*A)[JC + 1], zc (*Q)[JC + 1], double* rc_c, dou
       __syncthreads();
     }
     if (!s_sweep[isw & 1]) break;
+    if (isw + 1 < max_sweeps) {  // would the next sweep rotate anything?
+      if (tid == 0) *s_more = 0;
+      __syncthreads();
+      for (int e = tid; e < JC * JC; e += 256) {
+        const int pp = e % JC, qq = e / JC;
+        if (pp < qq && needs_rotation(A[pp][qq], A[pp][pp].x, A[qq][qq].x)) *s_more = 1;
+      }
+      __syncthreads();
+      if (!*s_more) break;
+    }
   }
   __syncthreads();
 }
@@ -273,7 +289,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict
   __shared__ double nred[4][2][16];
   __shared__ double rc_c[JC / 2], rc_s[JC / 2];
   __shared__ zc rc_w[JC / 2];
-  __shared__ int s_flag, s_round[2], s_sweep[2];
+  __shared__ int s_flag, s_round[2], s_sweep[2], s_more;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const zc* Wi = t.W + (size_t)t.R * (bi * JB);
@@ -343,7 +359,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict
   }
 
   // ---- eigenvectors of A -> Q, to the pair's slot; queue the pair for rotation
-  inner_jacobi(A, Q, rc_c, rc_s, rc_w, s_round, s_sweep, tid, inner_sweeps);
+  inner_jacobi(A, Q, rc_c, rc_s, rc_w, s_round, s_sweep, &s_more, tid, inner_sweeps);
   zc* qg = t.Qg + (size_t)p * JC * JC;
   for (int e = tid; e < JC * JC; e += 256) qg[e] = Q[e / JC][e % JC];  // qg[k JC + c] = Q[k][c]
   if (tid == 0) {

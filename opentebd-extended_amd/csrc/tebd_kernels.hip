@@ -157,6 +157,7 @@ struct MfmaThetaCfg {
   static constexpr int LDT = TR + 1;
   static constexpr int AB = 2 * KC * LDT;
   static constexpr int SM = AB > TR * LDT ? AB : TR * LDT;
+  static_assert((KC * TR) % 256 == 0, "theta chunks are staged 256 elements at a time");
 };
 
 template <int D>
@@ -190,20 +191,38 @@ __global__ __launch_bounds__(256) void k_theta_gate_mfma(const BondTask* __restr
   }
 
   const int chil = t.chil, chic = t.chic, chir = t.chir;
-  for (int g0 = 0; g0 < chic; g0 += KC) {
-    for (int e = tid; e < KC * TR; e += 256) {
-      const int r = e % TR, g = e / TR;
-      const int k = r / BA, a = a0 + (r % BA), gg = g0 + g;
-      zc v = zmk(0.0, 0.0);
-      if (a < chil && gg < chic) v = zscale(t.gl[a + (size_t)chil * (k + (size_t)D * gg)], t.lc[gg]);
-      As[g * LDT + r] = v;
+  // chunk g0 is loaded into registers while the previous one is multiplied from LDS:
+  // element e = tid + 256 u of the A chunk (r = e % TR, g = e / TR) and of the B chunk
+  // (g = e % KC, c = e / KC)
+  constexpr int NL = KC * TR / 256;
+  zc stage_a[NL], stage_b[NL];
+  auto load_chunk = [&](int g0) {
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+      const int e = tid + 256 * u;
+      {
+        const int r = e % TR, g = e / TR;
+        const int k = r / BA, a = a0 + (r % BA), gg = g0 + g;
+        stage_a[u] = (a < chil && gg < chic) ? zscale(t.gl[a + (size_t)chil * (k + (size_t)D * gg)], t.lc[gg])
+                                        : zmk(0.0, 0.0);
+      }
+      {
+        const int g = e % KC, c = e / KC;
+        const int l = c / BA, b = b0 + (c % BA), gg = g0 + g;
+        stage_b[u] = (b < chir && gg < chic) ? t.gr[gg + (size_t)chic * (l + (size_t)D * b)] : zmk(0.0, 0.0);
+      }
     }
-    for (int e = tid; e < KC * TR; e += 256) {
-      const int g = e % KC, c = e / KC;
-      const int l = c / BA, b = b0 + (c % BA), gg = g0 + g;
-      Bs[g * LDT + c] = (b < chir && gg < chic) ? t.gr[gg + (size_t)chic * (l + (size_t)D * b)] : zmk(0.0, 0.0);
+  };
+  load_chunk(0);
+  for (int g0 = 0; g0 < chic; g0 += KC) {
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+      const int e = tid + 256 * u;
+      As[(e / TR) * LDT + e % TR] = stage_a[u];
+      Bs[(e % KC) * LDT + e / KC] = stage_b[u];
     }
     __syncthreads();
+    if (g0 + KC < chic) load_chunk(g0 + KC);
 #pragma unroll
     for (int ks = 0; ks < KC / 4; ++ks) {
       const int kk = ks * 4 + lk;

@@ -14,7 +14,8 @@
 // With X V = U_x S:  W = (Q V) S (P U_x)^dagger, so the Rb-side singular vectors
 // are Q V and the Cb-side ones P U_x.
 //
-//   k_qrp_begin        column norms, ||W||_F, identity pivot, rank := running
+//   k_qrp_colnorms     column norms (one wave per column)
+//   k_qrp_begin        ||W||_F, identity pivot, rank := running
 //   k_qrp_panel_reg    panel j0 for Rb <= 1024: the QB largest remaining columns are
 //                      gathered into registers (512 threads, one or two rows each),
 //                      factorised there (one 32-value reduce-scatter per reflector
@@ -55,19 +56,24 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return red[0] + red[1] + red[2] + red[3];
 }
 
+// Squared column norms of the bond matrix: one wave per column, 4 columns per workgroup.
+__global__ __launch_bounds__(256) void k_qrp_colnorms(const BondTask* __restrict__ tasks) {
+  const BondTask t = tasks[blockIdx.y];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int k = 4 * blockIdx.x + wave;
+  if (k >= t.Cb) return;
+  const zc* col = t.Wq + (size_t)t.Rb * k;
+  double s = 0.0;
+  for (int i = lane; i < t.Rb; i += 64) s += zabs2(col[i]);
+  s = wave_sum(s);
+  if (lane == 0) t.qnrm[k] = s;
+}
+
+// After k_qrp_colnorms: identity pivot, ||W||_F^2 (fixed-order sum), stop threshold.
 __global__ __launch_bounds__(256) void k_qrp_begin(const BondTask* __restrict__ tasks, double tol_rel) {
   const BondTask t = tasks[blockIdx.x];
   __shared__ double red[4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int k = wave; k < t.Cb; k += 4) {
-    const zc* col = t.Wq + (size_t)t.Rb * k;
-    double s = 0.0;
-    for (int i = lane; i < t.Rb; i += 64) s += zabs2(col[i]);
-    s = wave_sum(s);
-    if (lane == 0) t.qnrm[k] = s;
-  }
   for (int k = threadIdx.x; k < t.Cb; k += 256) t.qpiv[k] = k;
-  __syncthreads();
   double part = 0.0;
   for (int k = threadIdx.x; k < t.Cb; k += 256) part += t.qnrm[k];
   const double tot = block_sum(part, red);
@@ -745,7 +751,10 @@ __global__ __launch_bounds__(256) void k_qrp_vectors_init(const BondTask* __rest
 
 }  // namespace
 
-hipError_t launch_qrp_begin(const BondTask* d_tasks, int ntasks, double tol_rel, hipStream_t st) {
+hipError_t launch_qrp_begin(const BondTask* d_tasks, int ntasks, int max_cb, double tol_rel, hipStream_t st) {
+  hipLaunchKernelGGL(k_qrp_colnorms, dim3((max_cb + 3) / 4, ntasks), dim3(256), 0, st, d_tasks);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_qrp_begin, dim3(ntasks), dim3(256), 0, st, d_tasks, tol_rel);
   return hipGetLastError();
 }

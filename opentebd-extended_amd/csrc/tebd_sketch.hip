@@ -113,27 +113,46 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__
       re[i][j] = d4{0.0, 0.0, 0.0, 0.0};
       im[i][j] = d4{0.0, 0.0, 0.0, 0.0};
     }
-  for (int k0 = 0; k0 < t.K; k0 += MK) {
-    for (int e = tid; e < MK * TM; e += 256) {
-      int kk, mm;
+  // chunk k0 of op(A) and B is loaded into registers while the previous one is
+  // multiplied from LDS: element e = tid + 256 u of the chunk
+  constexpr int NA = MK * TM / 256, NB = MK * TN / 256;
+  zc pa[NA], pb[NB];
+  auto load_chunk = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int e = tid + 256 * u;
       zc v = zmk(0.0, 0.0);
       if (!CONJ_A) {  // A is M x K: contiguous along m
-        mm = e % TM; kk = e / TM;
-        const int m = m0 + mm, k = k0 + kk;
+        const int m = m0 + e % TM, k = k0 + e / TM;
         if (m < t.M && k < t.K) v = t.A[m + (size_t)t.lda * k];
       } else {        // A is K x M: contiguous along k
-        kk = e % MK; mm = e / MK;
-        const int m = m0 + mm, k = k0 + kk;
+        const int m = m0 + e / MK, k = k0 + e % MK;
         if (m < t.M && k < t.K) v = zconj(t.A[k + (size_t)t.lda * m]);
       }
-      As[kk][mm] = v;
+      pa[u] = v;
     }
-    for (int e = tid; e < MK * TN; e += 256) {
-      const int kk = e % MK, nn = e / MK;
-      const int k = k0 + kk, n = n0 + nn;
-      Bs[kk][nn] = (k < t.K && n < t.N) ? t.B[k + (size_t)t.ldb * n] : zmk(0.0, 0.0);
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int e = tid + 256 * u;
+      const int k = k0 + e % MK, n = n0 + e / MK;
+      pb[u] = (k < t.K && n < t.N) ? t.B[k + (size_t)t.ldb * n] : zmk(0.0, 0.0);
+    }
+  };
+  load_chunk(0);
+  for (int k0 = 0; k0 < t.K; k0 += MK) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int e = tid + 256 * u;
+      if (!CONJ_A) As[e / TM][e % TM] = pa[u];
+      else As[e % MK][e / MK] = pa[u];
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int e = tid + 256 * u;
+      Bs[e % MK][e / MK] = pb[u];
     }
     __syncthreads();
+    if (k0 + MK < t.K) load_chunk(k0 + MK);
 #pragma unroll
     for (int ks = 0; ks < MK / 4; ++ks) {
       const int kk = 4 * ks + lk;

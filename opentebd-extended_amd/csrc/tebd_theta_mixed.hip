@@ -87,33 +87,50 @@ __global__ __launch_bounds__(256) void k_theta_gate_mixed(const BondTask* __rest
   }
 
   const int chil = t.chil, chic = t.chic, chir = t.chir;
-  for (int g0 = 0; g0 < chic; g0 += KC) {
-    // A(r, g) = Gl(a, k, g) lc(g), r = k BA + a_local (contiguous alpha within one k)
-    for (int e = tid; e < KC * TR; e += 256) {
-      const int r = e % TR, g = e / TR;
-      const int k = r / BA, a = a0 + (r % BA), gg = g0 + g;
-      cf v = {0.f, 0.f};
-      if (a < chil && gg < chic) {
-        const zc x = t.gl[a + (size_t)chil * (k + (size_t)D * gg)];
-        const double l = t.lc[gg];
-        v.x = (float)(x.x * l);
-        v.y = (float)(x.y * l);
+  // chunk g0 is loaded (and rounded) into registers while the previous one is
+  // multiplied from LDS: element e = tid + 256 u of each chunk
+  constexpr int NL = KC * TR / 256;
+  static_assert((KC * TR) % 256 == 0, "theta chunks are staged 256 elements at a time");
+  cf stage_a[NL], stage_b[NL];
+  auto load_chunk = [&](int g0) {
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+      const int e = tid + 256 * u;
+      {  // A(r, g) = Gl(a, k, g) lc(g), r = k BA + a_local (contiguous alpha within one k)
+        const int r = e % TR, g = e / TR;
+        const int k = r / BA, a = a0 + (r % BA), gg = g0 + g;
+        cf v = {0.f, 0.f};
+        if (a < chil && gg < chic) {
+          const zc x = t.gl[a + (size_t)chil * (k + (size_t)D * gg)];
+          const double l = t.lc[gg];
+          v.x = (float)(x.x * l);
+          v.y = (float)(x.y * l);
+        }
+        stage_a[u] = v;
       }
-      As[g][r] = v;
+      {  // B(g, c) = Gr(g, l, b), c = l BA + b_local (contiguous gamma)
+        const int g = e % KC, c = e / KC;
+        const int l = c / BA, b = b0 + (c % BA), gg = g0 + g;
+        cf v = {0.f, 0.f};
+        if (b < chir && gg < chic) {
+          const zc x = t.gr[gg + (size_t)chic * (l + (size_t)D * b)];
+          v.x = (float)x.x;
+          v.y = (float)x.y;
+        }
+        stage_b[u] = v;
+      }
     }
-    // B(g, c) = Gr(g, l, b), c = l BA + b_local (contiguous gamma)
-    for (int e = tid; e < KC * TR; e += 256) {
-      const int g = e % KC, c = e / KC;
-      const int l = c / BA, b = b0 + (c % BA), gg = g0 + g;
-      cf v = {0.f, 0.f};
-      if (b < chir && gg < chic) {
-        const zc x = t.gr[gg + (size_t)chic * (l + (size_t)D * b)];
-        v.x = (float)x.x;
-        v.y = (float)x.y;
-      }
-      Bs[g][c] = v;
+  };
+  load_chunk(0);
+  for (int g0 = 0; g0 < chic; g0 += KC) {
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+      const int e = tid + 256 * u;
+      As[e / TR][e % TR] = stage_a[u];
+      Bs[e % KC][e / KC] = stage_b[u];
     }
     __syncthreads();
+    if (g0 + KC < chic) load_chunk(g0 + KC);
 #pragma unroll
     for (int ks = 0; ks < KC / 4; ++ks) {
       const int kk = ks * 4 + lk;
