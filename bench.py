@@ -21,8 +21,12 @@ The mixed-precision form of the same path (theta and gate on the FP32 matrix
 cores, `--mixed-mode`) runs the same 100 steps; its steps/s and its observable
 errors against the exact batched path are reported under `mixed`.
 
+The timed run splits every half-sweep over two lanes (two HIP streams driven by two
+host threads, `--lanes`); the results are bit-identical to one lane.
+
 Prints one JSON line.  `roofline` is the single kernel with the most device time in
-the timed steps: its algorithmic work per launch (HBM bytes for the streaming
+the same timed steps run with one lane (`--roofline-lanes`, so that no launch shares
+the GPU with the other lane's): its algorithmic work per launch (HBM bytes for the streaming
 kernels -- Jacobi rounds and rotations, pivoted-QR panels and trailing updates --,
 FP64 flops for the GEMM-shaped ones) / its average launch duration, both measured
 with HIP events around each launch on the context's stream inside the library.
@@ -112,10 +116,11 @@ def device_observables(ctx, H):
     return dict(E=ctx.energy(H), n=ctx.one_site_expval(ntot), sz=ctx.one_site_expval(sz), S=ctx.chain_entropy())
 
 
-def run_mode(mode, steps, warmup, log, state=None, keep_state=False, rank_model=None):
+def run_mode(mode, steps, warmup, log, state=None, keep_state=False, rank_model=None, lanes=2, tag="timed"):
     from tebd_amd import models as M
 
     ctx = make_context(mode, rank_model)
+    ctx.set_lanes(lanes)
     if state is not None:
         ctx.upload(*state)
     trunc_warm = 0.0
@@ -132,7 +137,7 @@ def run_mode(mode, steps, warmup, log, state=None, keep_state=False, rank_model=
     st0 = ctx.stats()
     rx = _Roctx()
     trunc = 0.0
-    rx.push(f"bench_timed_{mode}")
+    rx.push(f"bench_{tag}_{mode}")
     t0 = time.perf_counter()
     for k in range(steps):
         trunc += ctx.trotter_step()
@@ -201,6 +206,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--compare-steps", type=int, default=3)
     ap.add_argument("--mode", default="batched,learned")
+    ap.add_argument("--lanes", type=int, default=2, help="streams / host threads per half-sweep of the timed run")
+    ap.add_argument("--roofline-lanes", type=int, default=1,
+                    help="lanes of the separate run whose per-kernel events give the roofline (0 = use the timed run)")
     ap.add_argument("--reference-mode", default="batched", help="exact path for the observable check ('' = skip)")
     ap.add_argument("--mixed-mode", default="batched,learned,mixed",
                     help="mixed-precision path timed and checked against the exact path ('' = skip)")
@@ -223,7 +231,7 @@ def main():
         return
 
     model = np.load(args.rank_model) if args.rank_model else None
-    res = run_mode(args.mode, args.steps, args.warmup, log, rank_model=model)
+    res = run_mode(args.mode, args.steps, args.warmup, log, rank_model=model, lanes=args.lanes)
     sps = args.steps / res["elapsed"]
     ref = None
     if args.reference_mode:
@@ -231,7 +239,13 @@ def main():
     mixed = None
     if args.mixed_mode:
         mixed = run_mode(args.mixed_mode, args.steps, args.warmup, log, rank_model=model)
-    prof = res["prof"]
+    # per-kernel event timings from a run whose launches do not share the GPU with a
+    # second lane, so each launch's duration is that kernel's own
+    roof_run = res
+    if args.roofline_lanes and args.roofline_lanes != args.lanes:
+        roof_run = run_mode(args.mode, args.steps, args.warmup, log, rank_model=model, lanes=args.roofline_lanes,
+                            tag="roofline")
+    prof = roof_run["prof"]
     dom = max(SINGLE_KERNEL, key=lambda n: prof[n][1])
     kernel, bound = SINGLE_KERNEL[dom]
     n_launch, dom_ms, dom_flops, dom_bytes = prof[dom]
@@ -249,6 +263,7 @@ def main():
                  "launches": int(n_launch), "bytes_per_launch": dom_bytes / max(n_launch, 1),
                  "flops_per_launch": dom_flops / max(n_launch, 1), "fp64_tflops": tflops,
                  "fp64_frac": tflops / FP64_PEAK_TFLOPS, "hbm_gbs": gbs,
+                 "lanes": args.roofline_lanes or args.lanes,
                  "device_ms": {n: prof[n][1] for n in prof}})
     st = res["stats"]
     half_steps = 3 * args.steps
@@ -303,7 +318,7 @@ def main():
         "data": "synthetic (Neel product state, Fermi-Hubbard gates)",
         "config": {"workload": "Fermi-Hubbard quench L=128 d=4 t=1 U=4 dt=0.01 chi_max=256, Neel start; "
                                f"timed steps {args.warmup + 1}-{args.warmup + args.steps}",
-                   "mode": args.mode, "chi_max_reached": int(res["dims"].max()),
+                   "mode": args.mode, "lanes": args.lanes, "chi_max_reached": int(res["dims"].max()),
                    "mean_bond_dim": float(res["dims"].mean()), "warmup_s": res["warm_s"]},
         "roofline": roof,
         "learned_rank": {"rank_model": os.path.basename(args.rank_model) if args.rank_model else None,

@@ -155,6 +155,14 @@ int tebd_get_stats(tebd_ctx* ctx, double* stats, int n);
 int tebd_set_profiling(tebd_ctx* ctx, int on);
 int tebd_get_profile(tebd_ctx* ctx, double* out, int n);
 
+/* Host concurrency of a batched context (no counterpart in the reference): with
+ * n = 2 lanes (the default) the bonds of each half-sweep are split in two halves that
+ * run on two HIP streams driven by two host threads, so one half's latency-bound
+ * phases (QR panels, inner Jacobi solves) overlap the other half's work; n = 1 runs a
+ * half-sweep as one batch on the context's stream.  Every bond's result is the same
+ * either way (a bond's arithmetic does not depend on the bonds that share its batch). */
+int tebd_set_lanes(tebd_ctx* ctx, int n);
+
 /* Autotuner of launch shapes (no counterpart in the reference).  The batched
  * complex GEMMs of the learned-rank path and of CanonicalForm have several tile
  * variants; with autotuning on (the default) the first launch of every shape class

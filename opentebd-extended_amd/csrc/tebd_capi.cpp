@@ -13,6 +13,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -136,6 +137,12 @@ struct tebd_ctx {
   std::map<uint64_t, int> tuned;
   hipEvent_t tev0 = nullptr, tev1 = nullptr;
   double tune_trials = 0;
+  // two lanes split the bonds of a batched half-sweep between two streams driven by two
+  // host threads (lanes[] are shallow copies whose per-task workspace starts at their
+  // first slot); a lane's own lanes[] are null
+  bool is_lane = false;
+  int nlanes = 2;
+  tebd_ctx* lanes[2] = {nullptr, nullptr};
   struct Pending { hipEvent_t a, b; int kind; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
@@ -955,6 +962,149 @@ int run_bonds(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs,
   return run_bonds_full(c, links, nb, gate_ptrs, trunc_sum);
 }
 
+void free_lane(tebd_ctx* l);
+
+// Lane `lane` of context c: workspace slots [slot0, slot0 + nslots), its own stream,
+// Jacobi queue counters and events; everything else is the context's.
+tebd_ctx* make_lane(tebd_ctx* c, int lane, int slot0, int nslots) {
+  tebd_ctx* l = new tebd_ctx(*c);
+  l->is_lane = true;
+  l->lanes[0] = l->lanes[1] = nullptr;
+  l->event_pool.clear();
+  l->pending.clear();
+  l->staging.clear();
+  l->staging.shrink_to_fit();
+  l->tev0 = l->tev1 = l->ev0 = l->ev1 = nullptr;
+  l->st = nullptr;
+  l->wl_count = nullptr;
+  if (hipStreamCreateWithFlags(&l->st, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&l->wl_count, sizeof(int) * 2) != hipSuccess) {
+    free_lane(l);
+    return nullptr;
+  }
+  const size_t s0 = (size_t)slot0;
+  l->ntask_max = nslots;
+  l->W = c->W + s0 * c->Rmax * c->Cpmax;
+  l->V = c->V + s0 * c->Cpmax * c->Cpmax;
+  l->s = c->s + s0 * c->Cpmax;
+  l->perm = c->perm + s0 * c->Cpmax;
+  l->X = c->X + s0 * c->Cpmax * c->Cpmax;
+  l->Yr = c->Yr + s0 * c->Rmax * c->chi_max;
+  l->Zc = c->Zc + s0 * c->Cpmax * c->chi_max;
+  l->qnrm = c->qnrm + s0 * c->Cpmax;
+  l->qpiv = c->qpiv + s0 * c->Cpmax;
+  l->tau = c->tau + s0 * c->Cpmax;
+  l->qstate = c->qstate + 2 * s0;
+  l->h_qstate = c->h_qstate + 2 * s0;
+  l->Tq = c->Tq + s0 * tq_stride(c);
+  l->counters = c->counters + 4 * s0;
+  l->h_counters = c->h_counters + 4 * s0;
+  l->blkver = c->blkver + s0 * c->nblk_max;
+  l->pairseen = c->pairseen + s0 * c->nblk_max * c->nblk_max;
+  l->Qg = c->Qg + s0 * (c->nblk_max / 2) * JC * JC;
+  l->wl_cap = nslots * (c->nblk_max / 2);
+  l->wl_items = c->wl_items + 2 * s0 * (c->nblk_max / 2);
+  l->results = c->results + RESULT_STRIDE * s0;
+  l->results2 = c->results2 + RESULT_STRIDE * s0;
+  l->h_results = c->h_results + RESULT_STRIDE * s0;
+  l->d_feat = c->d_feat + TEBD_MLP_INPUTS * s0;
+  l->h_feat = c->h_feat + TEBD_MLP_INPUTS * s0;
+  l->d_pred = c->d_pred + s0;
+  l->h_pred = c->h_pred + s0;
+  l->d_tasks = c->d_tasks + kStages * s0;
+  l->h_tasks = c->h_tasks + kStages * s0;
+  if (c->Zb) {
+    l->Zb = c->Zb + s0 * c->Rmax * c->Cpmax;
+    l->Bb = c->Bb + s0 * c->Rmax * c->Cpmax;
+    l->lam_scratch = c->lam_scratch + s0 * c->chi_max;
+    l->d_gemm = c->d_gemm + 5 * s0;
+    l->h_gemm = c->h_gemm + 5 * s0;
+    l->d_kprime = c->d_kprime + s0;
+    l->h_kprime = c->h_kprime + s0;
+  }
+  (void)lane;
+  return l;
+}
+
+// The settings and host mirrors a lane reads, from its context, before a run.
+void lane_pull(tebd_ctx* l, const tebd_ctx* c) {
+  l->chi = c->chi;
+  l->last_rank = c->last_rank;
+  l->tuned = c->tuned;
+  l->mode = c->mode;
+  l->cutoff = c->cutoff;
+  l->have_model = c->have_model;
+  l->rank_log = c->rank_log;
+  l->tol_missed = c->tol_missed;
+  l->tol_trunc = c->tol_trunc;
+  l->accept_threshold = c->accept_threshold;
+  l->oversample = c->oversample;
+  l->step_index = c->step_index;
+  l->profiling = c->profiling;
+  l->autotune = c->autotune;
+  l->have_gates = c->have_gates;
+  l->have_state = c->have_state;
+  for (double& v : l->stats) v = 0.0;
+  for (double& v : l->prof) v = 0.0;
+  l->tune_trials = 0.0;
+  l->log_X.clear();
+  l->log_y.clear();
+  l->err.clear();
+}
+
+// What a lane's run changed, back into its context: the bonds it updated, counters,
+// timings, autotuner choices and the rank-model record.
+void lane_push(tebd_ctx* c, const tebd_ctx* l, const int* links, int nb) {
+  for (int i = 0; i < nb; ++i) {
+    c->chi[links[i] + 1] = l->chi[links[i] + 1];
+    c->last_rank[links[i] + 1] = l->last_rank[links[i] + 1];
+  }
+  for (int k = 0; k < kNumStats; ++k) c->stats[k] += l->stats[k];
+  for (int k = 0; k < kNumProf; ++k) c->prof[k] += l->prof[k];
+  for (const auto& kv : l->tuned) c->tuned.insert(kv);
+  c->tune_trials += l->tune_trials;
+  c->log_X.insert(c->log_X.end(), l->log_X.begin(), l->log_X.end());
+  c->log_y.insert(c->log_y.end(), l->log_y.begin(), l->log_y.end());
+}
+
+// The bonds of one half-sweep: with two lanes, the first half on lane 0 (this thread)
+// and the second on lane 1 (a second thread), each on its own stream, so that one
+// lane's latency-bound phases (QR panels, inner Jacobi solves) overlap the other's
+// work.  Every bond's result is independent of which bonds share its batch, so the
+// split changes the timing, never the numbers.
+int run_bonds_split(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
+  if (c->nlanes < 2 || nb < 8 || c->ntask_max < 8) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
+  if (!c->lanes[0]) {
+    const int half = c->ntask_max / 2;
+    c->lanes[0] = make_lane(c, 0, 0, half);
+    c->lanes[1] = c->lanes[0] ? make_lane(c, 1, half, c->ntask_max - half) : nullptr;
+    if (!c->lanes[1]) {
+      free_lane(c->lanes[0]);
+      c->lanes[0] = nullptr;
+      c->nlanes = 1;
+      return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
+    }
+  }
+  const int h = nb / 2;
+  tebd_ctx *l0 = c->lanes[0], *l1 = c->lanes[1];
+  if (h > l0->ntask_max || nb - h > l1->ntask_max) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
+  // the lanes start after everything already queued on the context's stream
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  lane_pull(l0, c);
+  lane_pull(l1, c);
+  double t0 = 0.0, t1 = 0.0;
+  int e1 = TEBD_OK;
+  std::thread worker([&] { e1 = run_bonds(l1, links + h, nb - h, gate_ptrs + h, &t1); });
+  const int e0 = run_bonds(l0, links, h, gate_ptrs, &t0);
+  worker.join();
+  if (e0) return fail(c, e0, l0->err);
+  if (e1) return fail(c, e1, l1->err);
+  lane_push(c, l0, links, h);
+  lane_push(c, l1, links + h, nb - h);
+  *trunc_sum = t0 + t1;
+  return TEBD_OK;
+}
+
 int ensure_state(tebd_ctx* c, bool need_gates) {
   if (!c) return TEBD_ERR_ARG;
   if (!c->have_state) return fail(c, TEBD_ERR_STATE, "no state uploaded");
@@ -1024,7 +1174,26 @@ int download_impl(tebd_ctx* c, double* const* gamma, double* const* lambda, int 
   return TEBD_OK;
 }
 
+void free_lane(tebd_ctx* l) {
+  if (!l) return;
+  if (l->st) (void)hipStreamSynchronize(l->st);
+  for (auto& p : l->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : l->event_pool) (void)hipEventDestroy(e);
+  if (l->tev0) (void)hipEventDestroy(l->tev0);
+  if (l->tev1) (void)hipEventDestroy(l->tev1);
+  (void)hipFree(l->wl_count);
+  if (l->st) (void)hipStreamDestroy(l->st);
+  delete l;
+}
+
 void free_all(tebd_ctx* c) {
+  for (tebd_ctx*& l : c->lanes) {
+    free_lane(l);
+    l = nullptr;
+  }
   if (c->tev0) (void)hipEventDestroy(c->tev0);
   if (c->tev1) (void)hipEventDestroy(c->tev1);
   (void)hipFree(c->gam);
@@ -1269,7 +1438,7 @@ int tebd_trotter_step(tebd_ctx* c, double* truncerr) {
     }
     if (c->mode & TEBD_MODE_BATCHED) {
       double t = 0.0;
-      e = run_bonds(c, links.data(), (int)links.size(), gp.data(), &t);
+      e = run_bonds_split(c, links.data(), (int)links.size(), gp.data(), &t);
       if (e) return e;
       total += t;
     } else {
@@ -1466,6 +1635,12 @@ int tebd_get_profile(tebd_ctx* c, double* out, int n) {
   HIPCHK(c, hipStreamSynchronize(c->st));
   drain_profile(c);
   for (int i = 0; i < n && i < kNumProf; ++i) out[i] = c->prof[i];
+  return TEBD_OK;
+}
+
+int tebd_set_lanes(tebd_ctx* c, int n) {
+  if (!c || n < 1 || n > 2) return fail(c, TEBD_ERR_ARG, "set_lanes: 1 or 2");
+  c->nlanes = n;
   return TEBD_OK;
 }
 

@@ -240,6 +240,10 @@ class Context:
     def sync(self):
         self._check(self._lib.tebd_sync(self._h))
 
+    def set_lanes(self, n=2):
+        """Split each batched half-sweep over n (1 or 2) streams driven by n host threads."""
+        self._check(self._lib.tebd_set_lanes(self._h, int(n)))
+
     def set_autotune(self, on=True):
         """Autotuning of launch shapes: True on, False off, -1 on and forget the choices."""
         self._check(self._lib.tebd_set_autotune(self._h, int(on)))

@@ -1,0 +1,36 @@
+"""Two lanes (tebd_set_lanes): a batched half-sweep split over two streams and two host
+threads must give bit-identical states to the same half-sweep as one batch, because a
+bond's arithmetic does not depend on which bonds share its batch."""
+import numpy as np
+import pytest
+
+from tebd_amd import Context, models as M
+
+pytestmark = pytest.mark.gpu
+
+
+def evolve(mode, lanes, steps=12):
+    L, d, chi = 32, 4, 64
+    ctx = Context(L, d, chi, mode=mode)
+    ctx.set_lanes(lanes)
+    ctx.upload(*M.product_state(M.neel(L, d, 1, 2), chi))
+    ctx.set_gates(M.construct_propagators(M.fermi_hubbard(L, 1.0, 4.0), 0.05))
+    trunc = sum(ctx.trotter_step() for _ in range(steps))
+    g, l = ctx.download(chi)
+    dims = ctx.bond_dims()
+    st = ctx.stats()
+    ctx.close()
+    return g, l, dims, trunc, st
+
+
+@pytest.mark.parametrize("mode", ["batched", "batched,learned"])
+def test_two_lanes_are_bit_identical_to_one(mode):
+    print(f"[gpu] lanes fh L=32 chi=64 {mode}", flush=True)
+    g1, l1, d1, t1, s1 = evolve(mode, 1)
+    g2, l2, d2, t2, s2 = evolve(mode, 2)
+    np.testing.assert_array_equal(d1, d2)
+    np.testing.assert_array_equal(l1, l2)
+    np.testing.assert_array_equal(g1, g2)
+    assert t1 == t2
+    assert s1["bond_updates"] == s2["bond_updates"]
+    assert d1.max() == 64
