@@ -155,12 +155,21 @@ int tebd_get_stats(tebd_ctx* ctx, double* stats, int n);
 int tebd_set_profiling(tebd_ctx* ctx, int on);
 int tebd_get_profile(tebd_ctx* ctx, double* out, int n);
 
+/* hipGraph replay of the one-sided Jacobi sweeps (no counterpart in the reference):
+ * with on = 1 (the default) each sweep's launches (sweep start, and the Gram /
+ * eigenproblem and rotation kernels of its 2P-1 rounds) are captured once per launch
+ * shape into an executable graph and replayed, instead of being launched one by one;
+ * the convergence test between sweeps stays on the host.  Per-launch profiling
+ * (tebd_set_profiling) launches eagerly.  Results are identical either way. */
+int tebd_set_graphs(tebd_ctx* ctx, int on);
+
 /* Host concurrency of a batched context (no counterpart in the reference): with
- * n = 2 lanes (the default) the bonds of each half-sweep are split in two halves that
- * run on two HIP streams driven by two host threads, so one half's latency-bound
- * phases (QR panels, inner Jacobi solves) overlap the other half's work; n = 1 runs a
- * half-sweep as one batch on the context's stream.  Every bond's result is the same
- * either way (a bond's arithmetic does not depend on the bonds that share its batch). */
+ * n lanes (1 to 4, default 2) the bonds of each half-sweep are split into n contiguous
+ * parts that run on n HIP streams, each driven by its own host thread, so that one
+ * part's latency-bound phases (QR panels, inner Jacobi solves) overlap the others'
+ * work; n = 1 runs a half-sweep as one batch on the context's stream.  Every bond's
+ * result is the same either way (a bond's arithmetic does not depend on the bonds
+ * that share its batch). */
 int tebd_set_lanes(tebd_ctx* ctx, int n);
 
 /* Autotuner of launch shapes (no counterpart in the reference).  The batched

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -18,6 +19,7 @@
 
 namespace {
 
+constexpr int kMaxLanes = 4;
 constexpr int kStages = 6;  // task arrays per batch: 0-4 the learned path's stages, 5 the QR subset
 constexpr int kNumStats = 13;
 constexpr int kNumProf = 28;
@@ -137,12 +139,17 @@ struct tebd_ctx {
   std::map<uint64_t, int> tuned;
   hipEvent_t tev0 = nullptr, tev1 = nullptr;
   double tune_trials = 0;
-  // two lanes split the bonds of a batched half-sweep between two streams driven by two
-  // host threads (lanes[] are shallow copies whose per-task workspace starts at their
-  // first slot); a lane's own lanes[] are null
+  // lanes split the bonds of a batched half-sweep between up to kMaxLanes streams, each
+  // driven by its own host thread (lanes[] are shallow copies whose per-task workspace
+  // starts at their first slot); a lane's own lanes[] are null
+  // hipGraphs of whole Jacobi sweeps (sweep start, every round and rotation), keyed by
+  // the launch arguments; used while per-launch profiling is off
+  bool graphs = true;
+  std::map<std::array<long long, 7>, hipGraphExec_t> sweep_graphs;
   bool is_lane = false;
   int nlanes = 2;
-  tebd_ctx* lanes[2] = {nullptr, nullptr};
+  int lanes_made = 0;
+  tebd_ctx* lanes[kMaxLanes] = {};
   struct Pending { hipEvent_t a, b; int kind; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
@@ -224,6 +231,34 @@ void attach_jacobi(tebd_ctx* c, BondTask& t, int slot) {
   t.Qg = c->Qg + (size_t)slot * (c->nblk_max / 2) * JC * JC;
 }
 
+void clear_graphs(tebd_ctx* c) {
+  for (auto& kv : c->sweep_graphs) (void)hipGraphExecDestroy(kv.second);
+  c->sweep_graphs.clear();
+}
+
+// One Jacobi sweep (sweep start, then every round's Gram/eigenproblem kernel and its
+// rotation kernel) captured from the context's stream as an executable graph; null if
+// the capture fails.  The captured launches are the eager ones with the same arguments.
+hipGraphExec_t capture_sweep(tebd_ctx* c, BondTask* d_t, int nb, int Pmax, int cls, int inner, int rows_w,
+                             int rows_v) {
+  if (hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+  hipError_t e = launch_jacobi_sweep_begin(d_t, nb, cls, c->wl_count, c->st);
+  for (int r = 0; r < 2 * Pmax - 1 && e == hipSuccess; ++r) {
+    e = launch_jacobi_round(d_t, nb, Pmax, cls, r, kTolRel, kTolAbs, rows_w, rows_v, c->wl_count, c->wl_items,
+                            c->wl_cap, inner, c->st);
+    if (e == hipSuccess)
+      e = launch_jacobi_rotate(d_t, r, rows_w, rows_v, c->wl_count, c->wl_items, c->wl_cap, c->st);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(c->st, &g);
+  hipGraphExec_t ge = nullptr;
+  if (e == hipSuccess && ee == hipSuccess && g && hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess)
+    ge = nullptr;
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipGetLastError();  // a failed capture leaves no sticky error behind
+  return ge;
+}
+
 // Jacobi sweeps over the tasks of one stage (task i uses slot i) until a sweep
 // rotates nothing.
 int jacobi_sweeps(tebd_ctx* c, BondTask* d_t, const BondTask* h_t, int nb, int Pmax) {
@@ -237,15 +272,35 @@ int jacobi_sweeps(tebd_ctx* c, BondTask* d_t, const BondTask* h_t, int nb, int P
   bool converged = false;
   while (sweeps < kMaxSweeps && !converged) {
     const int inner = sweeps < kEarlySweeps ? kEarlyInnerSweeps : 0;  // 0: to convergence
-    HIPCHK(c, launch_jacobi_sweep_begin(d_t, nb, sweeps, c->wl_count, c->st));
-    for (int r = 0; r < 2 * Pmax - 1; ++r) {
-      {
-        ProfScope ps(c, 1);
-        HIPCHK(c, launch_jacobi_round(d_t, nb, Pmax, sweeps, r, kTolRel, kTolAbs, rows_w, rows_v, c->wl_count,
-                                      c->wl_items, c->wl_cap, inner, c->st));
+    // the kernels read the sweep index only as (sweep > 0) and its parity, so 0, 1 and 2
+    // stand for every sweep of their class and one graph serves the whole class
+    const int cls = sweeps == 0 ? 0 : ((sweeps & 1) ? 1 : 2);
+    bool done = false;
+    if (c->graphs && !c->profiling) {
+      const std::array<long long, 7> key = {(long long)(uintptr_t)d_t, nb, Pmax, rows_w, rows_v, cls, inner};
+      auto it = c->sweep_graphs.find(key);
+      if (it == c->sweep_graphs.end()) {
+        if (c->sweep_graphs.size() >= 256) clear_graphs(c);
+        hipGraphExec_t ge = capture_sweep(c, d_t, nb, Pmax, cls, inner, rows_w, rows_v);
+        if (ge) it = c->sweep_graphs.emplace(key, ge).first;
+        else c->graphs = false;  // capture unavailable: launch eagerly from now on
       }
-      ProfScope ps(c, 5);
-      HIPCHK(c, launch_jacobi_rotate(d_t, r, rows_w, rows_v, c->wl_count, c->wl_items, c->wl_cap, c->st));
+      if (it != c->sweep_graphs.end()) {
+        HIPCHK(c, hipGraphLaunch(it->second, c->st));
+        done = true;
+      }
+    }
+    if (!done) {
+      HIPCHK(c, launch_jacobi_sweep_begin(d_t, nb, cls, c->wl_count, c->st));
+      for (int r = 0; r < 2 * Pmax - 1; ++r) {
+        {
+          ProfScope ps(c, 1);
+          HIPCHK(c, launch_jacobi_round(d_t, nb, Pmax, cls, r, kTolRel, kTolAbs, rows_w, rows_v, c->wl_count,
+                                        c->wl_items, c->wl_cap, inner, c->st));
+        }
+        ProfScope ps(c, 5);
+        HIPCHK(c, launch_jacobi_rotate(d_t, r, rows_w, rows_v, c->wl_count, c->wl_items, c->wl_cap, c->st));
+      }
     }
     HIPCHK(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(int) * 4 * nb, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -969,12 +1024,14 @@ void free_lane(tebd_ctx* l);
 tebd_ctx* make_lane(tebd_ctx* c, int lane, int slot0, int nslots) {
   tebd_ctx* l = new tebd_ctx(*c);
   l->is_lane = true;
-  l->lanes[0] = l->lanes[1] = nullptr;
+  for (tebd_ctx*& x : l->lanes) x = nullptr;
+  l->lanes_made = 0;
   l->event_pool.clear();
   l->pending.clear();
   l->staging.clear();
   l->staging.shrink_to_fit();
   l->tev0 = l->tev1 = l->ev0 = l->ev1 = nullptr;
+  l->sweep_graphs.clear();  // graphs are captured per stream
   l->st = nullptr;
   l->wl_count = nullptr;
   if (hipStreamCreateWithFlags(&l->st, hipStreamNonBlocking) != hipSuccess ||
@@ -1042,6 +1099,7 @@ void lane_pull(tebd_ctx* l, const tebd_ctx* c) {
   l->step_index = c->step_index;
   l->profiling = c->profiling;
   l->autotune = c->autotune;
+  l->graphs = c->graphs;
   l->have_gates = c->have_gates;
   l->have_state = c->have_state;
   for (double& v : l->stats) v = 0.0;
@@ -1067,41 +1125,51 @@ void lane_push(tebd_ctx* c, const tebd_ctx* l, const int* links, int nb) {
   c->log_y.insert(c->log_y.end(), l->log_y.begin(), l->log_y.end());
 }
 
-// The bonds of one half-sweep: with two lanes, the first half on lane 0 (this thread)
-// and the second on lane 1 (a second thread), each on its own stream, so that one
-// lane's latency-bound phases (QR panels, inner Jacobi solves) overlap the other's
+// The bonds of one half-sweep: with n lanes, contiguous parts of the batch run on n
+// streams, part 0 from this thread and the others from their own threads, so that one
+// part's latency-bound phases (QR panels, inner Jacobi solves) overlap the others'
 // work.  Every bond's result is independent of which bonds share its batch, so the
 // split changes the timing, never the numbers.
 int run_bonds_split(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
-  if (c->nlanes < 2 || nb < 8 || c->ntask_max < 8) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
-  if (!c->lanes[0]) {
-    const int half = c->ntask_max / 2;
-    c->lanes[0] = make_lane(c, 0, 0, half);
-    c->lanes[1] = c->lanes[0] ? make_lane(c, 1, half, c->ntask_max - half) : nullptr;
-    if (!c->lanes[1]) {
-      free_lane(c->lanes[0]);
-      c->lanes[0] = nullptr;
-      c->nlanes = 1;
-      return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
+  const int n = std::min(c->nlanes, nb / 4);
+  if (n < 2 || c->ntask_max < 2 * kMaxLanes) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
+  if (c->lanes_made != n) {  // lanes of ntask_max / n slots each
+    for (tebd_ctx*& l : c->lanes) {
+      free_lane(l);
+      l = nullptr;
     }
+    c->lanes_made = 0;
+    const int per = c->ntask_max / n;
+    for (int k = 0; k < n; ++k) {
+      c->lanes[k] = make_lane(c, k, k * per, k + 1 < n ? per : c->ntask_max - k * per);
+      if (!c->lanes[k]) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
+    }
+    c->lanes_made = n;
   }
-  const int h = nb / 2;
-  tebd_ctx *l0 = c->lanes[0], *l1 = c->lanes[1];
-  if (h > l0->ntask_max || nb - h > l1->ntask_max) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
+  int first[kMaxLanes + 1];
+  for (int k = 0; k <= n; ++k) first[k] = (int)((long long)nb * k / n);
+  for (int k = 0; k < n; ++k)
+    if (first[k + 1] - first[k] > c->lanes[k]->ntask_max) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
   // the lanes start after everything already queued on the context's stream
   HIPCHK(c, hipStreamSynchronize(c->st));
-  lane_pull(l0, c);
-  lane_pull(l1, c);
-  double t0 = 0.0, t1 = 0.0;
-  int e1 = TEBD_OK;
-  std::thread worker([&] { e1 = run_bonds(l1, links + h, nb - h, gate_ptrs + h, &t1); });
-  const int e0 = run_bonds(l0, links, h, gate_ptrs, &t0);
-  worker.join();
-  if (e0) return fail(c, e0, l0->err);
-  if (e1) return fail(c, e1, l1->err);
-  lane_push(c, l0, links, h);
-  lane_push(c, l1, links + h, nb - h);
-  *trunc_sum = t0 + t1;
+  for (int k = 0; k < n; ++k) lane_pull(c->lanes[k], c);
+  double tr[kMaxLanes] = {};
+  int err[kMaxLanes] = {};
+  std::vector<std::thread> workers;
+  for (int k = 1; k < n; ++k)
+    workers.emplace_back([&, k] {
+      err[k] = run_bonds(c->lanes[k], links + first[k], first[k + 1] - first[k], gate_ptrs + first[k], &tr[k]);
+    });
+  err[0] = run_bonds(c->lanes[0], links, first[1], gate_ptrs, &tr[0]);
+  for (auto& w : workers) w.join();
+  for (int k = 0; k < n; ++k)
+    if (err[k]) return fail(c, err[k], c->lanes[k]->err);
+  double tsum = 0.0;
+  for (int k = 0; k < n; ++k) {
+    lane_push(c, c->lanes[k], links + first[k], first[k + 1] - first[k]);
+    tsum += tr[k];
+  }
+  *trunc_sum = tsum;
   return TEBD_OK;
 }
 
@@ -1184,6 +1252,7 @@ void free_lane(tebd_ctx* l) {
   for (auto e : l->event_pool) (void)hipEventDestroy(e);
   if (l->tev0) (void)hipEventDestroy(l->tev0);
   if (l->tev1) (void)hipEventDestroy(l->tev1);
+  clear_graphs(l);
   (void)hipFree(l->wl_count);
   if (l->st) (void)hipStreamDestroy(l->st);
   delete l;
@@ -1194,6 +1263,8 @@ void free_all(tebd_ctx* c) {
     free_lane(l);
     l = nullptr;
   }
+  c->lanes_made = 0;
+  clear_graphs(c);
   if (c->tev0) (void)hipEventDestroy(c->tev0);
   if (c->tev1) (void)hipEventDestroy(c->tev1);
   (void)hipFree(c->gam);
@@ -1638,8 +1709,14 @@ int tebd_get_profile(tebd_ctx* c, double* out, int n) {
   return TEBD_OK;
 }
 
+int tebd_set_graphs(tebd_ctx* c, int on) {
+  if (!c) return TEBD_ERR_ARG;
+  c->graphs = on != 0;
+  return TEBD_OK;
+}
+
 int tebd_set_lanes(tebd_ctx* c, int n) {
-  if (!c || n < 1 || n > 2) return fail(c, TEBD_ERR_ARG, "set_lanes: 1 or 2");
+  if (!c || n < 1 || n > kMaxLanes) return fail(c, TEBD_ERR_ARG, "set_lanes: 1 to 4 lanes");
   c->nlanes = n;
   return TEBD_OK;
 }

@@ -240,8 +240,12 @@ class Context:
     def sync(self):
         self._check(self._lib.tebd_sync(self._h))
 
+    def set_graphs(self, on=True):
+        """Replay each Jacobi sweep as a captured hipGraph (while per-launch profiling is off)."""
+        self._check(self._lib.tebd_set_graphs(self._h, int(bool(on))))
+
     def set_lanes(self, n=2):
-        """Split each batched half-sweep over n (1 or 2) streams driven by n host threads."""
+        """Split each batched half-sweep over n (1 to 4) streams driven by n host threads."""
         self._check(self._lib.tebd_set_lanes(self._h, int(n)))
 
     def set_autotune(self, on=True):

@@ -68,7 +68,7 @@ def test_python_contexts_expose_the_library_surface():
     for name in ("upload", "download", "set_gates", "trotter_step", "two_site_op", "canonical_form_all",
                  "canonical_form", "bond_dims", "stats", "set_profiling", "profile", "sync", "energy",
                  "one_site_expval", "chain_entropy", "expval_two_site", "set_rank_model", "rank_log",
-                 "set_autotune", "autotune_info", "set_cutoff", "lambdas", "set_lanes"):
+                 "set_autotune", "autotune_info", "set_cutoff", "lambdas", "set_lanes", "set_graphs"):
         assert callable(getattr(Context, name, None)), name
     for name in ("upload_chains", "download_chains", "energy_chains", "one_site_expval_chains",
                  "chain_entropy_chains", "expval_two_site_chains"):

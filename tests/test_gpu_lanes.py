@@ -1,4 +1,4 @@
-"""Two lanes (tebd_set_lanes): a batched half-sweep split over two streams and two host
+"""Lanes (tebd_set_lanes): a batched half-sweep split over several streams and host
 threads must give bit-identical states to the same half-sweep as one batch, because a
 bond's arithmetic does not depend on which bonds share its batch."""
 import numpy as np
@@ -23,14 +23,34 @@ def evolve(mode, lanes, steps=12):
     return g, l, dims, trunc, st
 
 
-@pytest.mark.parametrize("mode", ["batched", "batched,learned"])
-def test_two_lanes_are_bit_identical_to_one(mode):
-    print(f"[gpu] lanes fh L=32 chi=64 {mode}", flush=True)
+@pytest.mark.parametrize("mode,lanes", [("batched", 2), ("batched,learned", 2), ("batched,learned", 4)])
+def test_lanes_are_bit_identical_to_one(mode, lanes):
+    print(f"[gpu] lanes fh L=32 chi=64 {mode} lanes={lanes}", flush=True)
     g1, l1, d1, t1, s1 = evolve(mode, 1)
-    g2, l2, d2, t2, s2 = evolve(mode, 2)
+    g2, l2, d2, t2, s2 = evolve(mode, lanes)
     np.testing.assert_array_equal(d1, d2)
     np.testing.assert_array_equal(l1, l2)
     np.testing.assert_array_equal(g1, g2)
     assert t1 == t2
     assert s1["bond_updates"] == s2["bond_updates"]
     assert d1.max() == 64
+
+
+def test_graph_replayed_sweeps_are_bit_identical_to_eager_launches():
+    """Every Jacobi sweep replays a captured hipGraph (one per launch shape, many replays
+    per graph across 12 steps, eager host work in between) and must leave exactly the
+    state the eager launches leave."""
+    print("[gpu] graphs vs eager fh L=32 chi=64", flush=True)
+    out = {}
+    for graphs in (False, True):
+        L, d, chi = 32, 4, 64
+        ctx = Context(L, d, chi, mode="batched,learned")
+        ctx.set_graphs(graphs)
+        ctx.upload(*M.product_state(M.neel(L, d, 1, 2), chi))
+        ctx.set_gates(M.construct_propagators(M.fermi_hubbard(L, 1.0, 4.0), 0.05))
+        for _ in range(12):
+            ctx.trotter_step()
+        out[graphs] = ctx.download(chi)
+        ctx.close()
+    np.testing.assert_array_equal(out[False][1], out[True][1])
+    np.testing.assert_array_equal(out[False][0], out[True][0])
