@@ -147,6 +147,7 @@ struct tebd_ctx {
   bool graphs = true;
   std::map<std::array<long long, 7>, hipGraphExec_t> sweep_graphs;
   bool is_lane = false;
+  bool owns_gemm = false;  // a lane that allocated its own GEMM descriptors
   int nlanes = 2;
   int lanes_made = 0;
   tebd_ctx* lanes[kMaxLanes] = {};
@@ -1008,6 +1009,7 @@ int ensure_cf_workspace(tebd_ctx* c) {
   if (!c->h_gemm) {
     HIPCHK(c, hipMalloc(&c->d_gemm, sizeof(GemmTask) * c->ntask_max * 5));
     HIPCHK(c, hipHostMalloc(&c->h_gemm, sizeof(GemmTask) * c->ntask_max * 5));
+    c->owns_gemm = c->is_lane;
   }
   return TEBD_OK;
 }
@@ -1032,6 +1034,11 @@ tebd_ctx* make_lane(tebd_ctx* c, int lane, int slot0, int nslots) {
   l->staging.shrink_to_fit();
   l->tev0 = l->tev1 = l->ev0 = l->ev1 = nullptr;
   l->sweep_graphs.clear();  // graphs are captured per stream
+  l->cf = nullptr;           // CanonicalForm workspace: a lane allocates its own
+  l->cf_s = nullptr;
+  l->cf_lam = nullptr;
+  l->cf_perm = nullptr;
+  l->owns_gemm = false;
   l->st = nullptr;
   l->wl_count = nullptr;
   if (hipStreamCreateWithFlags(&l->st, hipStreamNonBlocking) != hipSuccess ||
@@ -1074,11 +1081,12 @@ tebd_ctx* make_lane(tebd_ctx* c, int lane, int slot0, int nslots) {
     l->Zb = c->Zb + s0 * c->Rmax * c->Cpmax;
     l->Bb = c->Bb + s0 * c->Rmax * c->Cpmax;
     l->lam_scratch = c->lam_scratch + s0 * c->chi_max;
-    l->d_gemm = c->d_gemm + 5 * s0;
-    l->h_gemm = c->h_gemm + 5 * s0;
     l->d_kprime = c->d_kprime + s0;
     l->h_kprime = c->h_kprime + s0;
   }
+  // GEMM descriptors: the lane's slice of the context's, or (none yet) its own later
+  l->d_gemm = c->d_gemm ? c->d_gemm + 5 * s0 : nullptr;
+  l->h_gemm = c->h_gemm ? c->h_gemm + 5 * s0 : nullptr;
   (void)lane;
   return l;
 }
@@ -1125,6 +1133,24 @@ void lane_push(tebd_ctx* c, const tebd_ctx* l, const int* links, int nb) {
   c->log_y.insert(c->log_y.end(), l->log_y.begin(), l->log_y.end());
 }
 
+// n lanes of ntask_max / n workspace slots each (made again when n changes); false if
+// they cannot be made.
+bool ensure_lanes(tebd_ctx* c, int n) {
+  if (c->lanes_made == n) return true;
+  for (tebd_ctx*& l : c->lanes) {
+    free_lane(l);
+    l = nullptr;
+  }
+  c->lanes_made = 0;
+  const int per = c->ntask_max / n;
+  for (int k = 0; k < n; ++k) {
+    c->lanes[k] = make_lane(c, k, k * per, k + 1 < n ? per : c->ntask_max - k * per);
+    if (!c->lanes[k]) return false;
+  }
+  c->lanes_made = n;
+  return true;
+}
+
 // The bonds of one half-sweep: with n lanes, contiguous parts of the batch run on n
 // streams, part 0 from this thread and the others from their own threads, so that one
 // part's latency-bound phases (QR panels, inner Jacobi solves) overlap the others'
@@ -1133,19 +1159,7 @@ void lane_push(tebd_ctx* c, const tebd_ctx* l, const int* links, int nb) {
 int run_bonds_split(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
   const int n = std::min(c->nlanes, nb / 4);
   if (n < 2 || c->ntask_max < 2 * kMaxLanes) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
-  if (c->lanes_made != n) {  // lanes of ntask_max / n slots each
-    for (tebd_ctx*& l : c->lanes) {
-      free_lane(l);
-      l = nullptr;
-    }
-    c->lanes_made = 0;
-    const int per = c->ntask_max / n;
-    for (int k = 0; k < n; ++k) {
-      c->lanes[k] = make_lane(c, k, k * per, k + 1 < n ? per : c->ntask_max - k * per);
-      if (!c->lanes[k]) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
-    }
-    c->lanes_made = n;
-  }
+  if (!ensure_lanes(c, n)) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
   int first[kMaxLanes + 1];
   for (int k = 0; k <= n; ++k) first[k] = (int)((long long)nb * k / n);
   for (int k = 0; k < n; ++k)
@@ -1253,6 +1267,14 @@ void free_lane(tebd_ctx* l) {
   if (l->tev0) (void)hipEventDestroy(l->tev0);
   if (l->tev1) (void)hipEventDestroy(l->tev1);
   clear_graphs(l);
+  (void)hipFree(l->cf);
+  (void)hipFree(l->cf_s);
+  (void)hipFree(l->cf_perm);
+  (void)hipFree(l->cf_lam);
+  if (l->owns_gemm) {
+    (void)hipFree(l->d_gemm);
+    (void)hipHostFree(l->h_gemm);
+  }
   (void)hipFree(l->wl_count);
   if (l->st) (void)hipStreamDestroy(l->st);
   delete l;
@@ -1546,16 +1568,48 @@ int tebd_two_site_op(tebd_ctx* c, int link, const double* op2, double* truncerr)
   return TEBD_OK;
 }
 
+// CanonicalFormAll of chain r of the context (its bonds left to right, then right to
+// left, as propagation_module.f90:617-634 does for one chain).
+int canonical_form_chain(tebd_ctx* c, int r) {
+  int e = ensure_cf_workspace(c);
+  if (e) return e;
+  const int b0 = r * c->chain_len, b1 = b0 + c->chain_len - 1;  // bonds b0 .. b1 - 1
+  for (int k = b0; k < b1; ++k)
+    if ((e = canonical_form(c, k))) return e;
+  for (int k = b1 - 1; k >= b0; --k)
+    if ((e = canonical_form(c, k))) return e;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  drain_profile(c);
+  return TEBD_OK;
+}
+
 int tebd_canonical_form_all(tebd_ctx* c) {
   int e = ensure_state(c, false);
   if (e) return e;
-  if ((e = ensure_cf_workspace(c))) return e;
-  for (int k = 0; k < c->L - 1; ++k)
-    if (!is_cut(c, k) && (e = canonical_form(c, k))) return e;
-  for (int k = c->L - 2; k >= 0; --k)
-    if (!is_cut(c, k) && (e = canonical_form(c, k))) return e;
-  HIPCHK(c, hipStreamSynchronize(c->st));
-  drain_profile(c);
+  // the chains of an ensemble are independent: with lanes, chain r goes to lane r mod n,
+  // each lane on its own stream and host thread; each chain sees the same sequence of
+  // bond operations either way
+  const int n = std::min(c->nlanes, c->nchain);
+  if (n >= 2 && c->ntask_max >= n && ensure_lanes(c, n)) {
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    int err[kMaxLanes] = {};
+    auto work = [&](int k) {
+      tebd_ctx* l = c->lanes[k];
+      lane_pull(l, c);
+      for (int r = k; r < c->nchain && !err[k]; r += n) err[k] = canonical_form_chain(l, r);
+    };
+    std::vector<std::thread> workers;
+    for (int k = 1; k < n; ++k) workers.emplace_back(work, k);
+    work(0);
+    for (auto& w : workers) w.join();
+    for (int k = 0; k < n; ++k) {
+      if (err[k]) return fail(c, err[k], c->lanes[k]->err);
+      lane_push(c, c->lanes[k], nullptr, 0);
+    }
+    return TEBD_OK;
+  }
+  for (int r = 0; r < c->nchain; ++r)
+    if ((e = canonical_form_chain(c, r))) return e;
   return TEBD_OK;
 }
 

@@ -54,3 +54,26 @@ def test_graph_replayed_sweeps_are_bit_identical_to_eager_launches():
         ctx.close()
     np.testing.assert_array_equal(out[False][1], out[True][1])
     np.testing.assert_array_equal(out[False][0], out[True][0])
+
+
+def test_autotuned_gemm_tiles_do_not_change_results():
+    """The learned-rank path's GEMMs with the autotuner on (tile variants timed on the real
+    operands, fastest kept per shape class) and off (64 x 64 tiles): identical states."""
+    print("[gpu] autotune on/off fh L=32 chi=64", flush=True)
+    out, info = {}, None
+    for tune in (False, True):
+        L, d, chi = 32, 4, 64
+        ctx = Context(L, d, chi, mode="batched,learned")
+        ctx.set_autotune(tune)
+        ctx.upload(*M.product_state(M.neel(L, d, 1, 2), chi))
+        ctx.set_gates(M.construct_propagators(M.fermi_hubbard(L, 1.0, 4.0), 0.05))
+        for _ in range(10):
+            ctx.trotter_step()
+        out[tune] = ctx.download(chi)
+        if tune:
+            info = ctx.autotune_info()
+        ctx.close()
+    np.testing.assert_array_equal(out[False][1], out[True][1])
+    np.testing.assert_array_equal(out[False][0], out[True][0])
+    assert info["shape_classes"] > 0 and info["timing_launches"] >= 2 * info["shape_classes"]
+    assert sum(info["chosen"].values()) == info["shape_classes"]
