@@ -104,7 +104,9 @@ int tebd_trotter_step(tebd_ctx* ctx, double* truncerr);
 int tebd_two_site_op(tebd_ctx* ctx, int link, const double* op2, double* truncerr);
 
 /* Re-orthogonalises every bipartition (left-to-right, then right-to-left
- * sweep).  Replaces CanonicalFormAll (propagation_module.f90:617-634). */
+ * sweep).  Replaces CanonicalFormAll (propagation_module.f90:617-634).  In an
+ * ensemble the same bond of every chain is processed as one batch (batched Jacobi
+ * SVDs and GEMMs), and the chains are spread over the lanes (tebd_set_lanes). */
 int tebd_canonical_form_all(tebd_ctx* ctx);
 
 /* Re-orthogonalises one bipartition, bond `link` (1-based).  Replaces
@@ -128,13 +130,14 @@ int tebd_set_cutoff(tebd_ctx* ctx, double cutoff);
 /* Current bond dimensions chi[l], l = 0..L (chi[0] = chi[L] = 1). */
 int tebd_get_bond_dims(tebd_ctx* ctx, int* chi);
 
-/* Counters since creation (n <= 13): [0] bond updates, [1] Jacobi sweeps,
+/* Counters since creation (n <= 14): [0] bond updates, [1] Jacobi sweeps,
  * [2] sketches of the learned-rank path redone by the full SVD, [3] device
  * milliseconds in trotter steps, [4] sketch columns requested, [5] sketches
  * accepted, [6] sum of the sketches' numerical ranks, [7] rank-model predictions,
  * [8] pair Gram matrices formed, [9] pairs rotated, [10] pair visits of the sweeps
  * run, [11] sketches skipped on the rank model's prediction, [12] sketches
- * rejected for their discarded weight (the rest of [2] for missed weight). */
+ * rejected for their discarded weight (the rest of [2] for missed weight),
+ * [13] half-sweeps split over lanes (tebd_set_lanes). */
 int tebd_get_stats(tebd_ctx* ctx, double* stats, int n);
 
 /* Per-kernel timing with HIP events recorded on the context's stream (no
@@ -167,7 +170,10 @@ int tebd_set_graphs(tebd_ctx* ctx, int on);
  * n lanes (1 to 4, default 2) the bonds of each half-sweep are split into n contiguous
  * parts that run on n HIP streams, each driven by its own host thread, so that one
  * part's latency-bound phases (QR panels, inner Jacobi solves) overlap the others'
- * work; n = 1 runs a half-sweep as one batch on the context's stream.  Every bond's
+ * work; n = 1 runs a half-sweep as one batch on the context's stream.  A half-sweep
+ * is split only when it has at least 4 bonds per lane and a bond matrix with a side
+ * of at least 512 (smaller ones run faster as one batch).  The chains of an ensemble
+ * are also distributed over the lanes in tebd_canonical_form_all.  Every bond's
  * result is the same either way (a bond's arithmetic does not depend on the bonds
  * that share its batch). */
 int tebd_set_lanes(tebd_ctx* ctx, int n);

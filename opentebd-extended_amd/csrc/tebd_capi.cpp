@@ -20,8 +20,9 @@
 namespace {
 
 constexpr int kMaxLanes = 4;
+constexpr int kLaneMinSide = 512;  // lanes split a batch only if a bond matrix has a side >= this
 constexpr int kStages = 6;  // task arrays per batch: 0-4 the learned path's stages, 5 the QR subset
-constexpr int kNumStats = 13;
+constexpr int kNumStats = 14;
 constexpr int kNumProf = 28;
 constexpr double kQrTol = 1e-17;  // QR stops when every remaining column norm is <= kQrTol ||theta||_F
 constexpr int kQrPanel = 16;      // panel width of the blocked pivoted QR (tebd_qrp.hip)
@@ -115,6 +116,7 @@ struct tebd_ctx {
   double* cf_s = nullptr;
   int* cf_perm = nullptr;
   double* cf_lam = nullptr;
+  int cf_sets = 0;               // bonds the CanonicalForm workspace holds at once
   std::vector<int> last_rank;    // kept rank of each bond's last update (rank model input)
   double* h_results = nullptr;   // pinned
   int* h_counters = nullptr;     // pinned, 4 per task slot
@@ -124,7 +126,7 @@ struct tebd_ctx {
   // [4] sketch columns requested, [5] sketches accepted, [6] sketch ranks (sum of r_y), [7] rank-model predictions,
   // [8] pair Gram matrices formed, [9] pairs rotated, [10] pair visits of the sweeps run,
   // [11] sketches skipped on the rank model's prediction, [12] sketches rejected for discarded weight
-  // (the others of [2] for missed weight)
+  // (the others of [2] for missed weight), [13] half-sweeps split over lanes
   double stats[kNumStats] = {};
   // optional per-kernel timing with HIP events on the context's stream:
   // prof[k*4 + 0] launches, [1] milliseconds, [2] algorithmic flops, [3] algorithmic HBM bytes,
@@ -925,87 +927,167 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
 // SVD of W (R x C, ld R, R >= C) by one-sided Jacobi.  On return W holds W J
 // (columns U s, unsorted), V (Cpad x Cpad) holds J, s[0..C) the singular values
 // in descending order and perm[b] the column of W / V that carries s[b].
-int device_svd(tebd_ctx* c, zc* W, int R, int C, zc* V, double* s, int* perm, double* lam_scratch) {
-  BondTask& t = c->h_tasks[0];
-  std::memset(&t, 0, sizeof(t));
-  t.W = W; t.R = R; t.C = C; t.Cpad = round_up(C, JC); t.P = t.Cpad / JC;
-  t.V = V; t.NV = t.Cpad; t.s = s; t.perm = perm;
-  attach_jacobi(c, t, 0);
-  t.result = c->results; t.chi_max = C; t.lc_out = lam_scratch;
-  HIPCHK(c, hipMemcpyAsync(c->d_tasks, c->h_tasks, sizeof(BondTask), hipMemcpyHostToDevice, c->st));
-  HIPCHK(c, launch_prepare_svd(c->d_tasks, 1, c->st));
-  HIPCHK(c, launch_frobenius(c->d_tasks, 1, t.Cpad, c->st, 0));
-  int e = jacobi_sweeps(c, c->d_tasks, c->h_tasks, 1, t.P);
+// SVDs of nb matrices at once by one-sided Jacobi (task i in slot i): W_i (R_i x C_i,
+// ld R_i, R_i >= C_i).  On return W_i holds W_i J (columns U s, unsorted), V_i
+// (Cpad x Cpad) holds J, s_i[0..C_i) the singular values in descending order and
+// perm_i[b] the column of W_i / V_i that carries s_i[b].
+struct SvdJob {
+  zc* W;
+  int R, C;
+  zc* V;
+  double* s;
+  int* perm;
+  double* lam_scratch;
+};
+
+int device_svd_batch(tebd_ctx* c, const SvdJob* jobs, int nb) {
+  int Pmax = 1, Cpmax = 1;
+  for (int i = 0; i < nb; ++i) {
+    BondTask& t = c->h_tasks[i];
+    std::memset(&t, 0, sizeof(t));
+    t.W = jobs[i].W; t.R = jobs[i].R; t.C = jobs[i].C; t.Cpad = round_up(t.C, JC); t.P = t.Cpad / JC;
+    t.V = jobs[i].V; t.NV = t.Cpad; t.s = jobs[i].s; t.perm = jobs[i].perm;
+    attach_jacobi(c, t, i);
+    t.result = c->results + (size_t)RESULT_STRIDE * i;
+    t.chi_max = t.C;
+    t.lc_out = jobs[i].lam_scratch;
+    Pmax = std::max(Pmax, t.P);
+    Cpmax = std::max(Cpmax, t.Cpad);
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_tasks, c->h_tasks, sizeof(BondTask) * nb, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, launch_prepare_svd(c->d_tasks, nb, c->st));
+  HIPCHK(c, launch_frobenius(c->d_tasks, nb, Cpmax, c->st, 0));
+  int e = jacobi_sweeps(c, c->d_tasks, c->h_tasks, nb, Pmax);
   if (e) return e;
-  HIPCHK(c, launch_col_norms(c->d_tasks, 1, t.Cpad, c->st));
-  HIPCHK(c, launch_truncate(c->d_tasks, 1, 0.0, c->st));
+  HIPCHK(c, launch_col_norms(c->d_tasks, nb, Cpmax, c->st));
+  HIPCHK(c, launch_truncate(c->d_tasks, nb, 0.0, c->st));
   return TEBD_OK;
 }
 
-int gemm1(tebd_ctx* c, const zc* A, const zc* B, zc* C, int M, int N, int K, int lda, int ldb, int ldc, bool conj) {
-  GemmTask* g = c->h_gemm;
-  *g = {A, B, C, M, N, K, lda, ldb, ldc, 1.0, 0};
-  HIPCHK(c, hipMemcpyAsync(c->d_gemm, g, sizeof(GemmTask), hipMemcpyHostToDevice, c->st));
-  HIPCHK(c, tuned_zgemm(c, c->d_gemm, 1, M, N, conj));
-  // the pinned descriptor is rewritten by the next call
+// nb GEMMs C_i = op(A_i) B_i as one batched launch; descriptor region `region` (0..4) of
+// the pinned descriptors, so the GEMMs of one CanonicalForm batch need no host wait.
+int gemm_batch(tebd_ctx* c, const GemmTask* g, int nb, int region, bool conj) {
+  GemmTask* h = c->h_gemm + (size_t)region * c->ntask_max;
+  GemmTask* d = c->d_gemm + (size_t)region * c->ntask_max;
+  int max_m = 1, max_n = 1;
+  for (int i = 0; i < nb; ++i) {
+    h[i] = g[i];
+    max_m = std::max(max_m, g[i].M);
+    max_n = std::max(max_n, g[i].N);
+  }
+  HIPCHK(c, hipMemcpyAsync(d, h, sizeof(GemmTask) * nb, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, tuned_zgemm(c, d, nb, max_m, max_n, conj));
+  return TEBD_OK;
+}
+
+// CanonicalForm (local_operations_module.f90:496-620) on nb bonds at once, bonds[i]
+// 0-based (sites l, l+1); the bonds must not share a site (e.g. the same bond of
+// different chains of an ensemble).  Bond i works in CanonicalForm workspace set i.
+// The SVDs of the weighted Gram matrices M1 = B1^H B1 and M2 = C2 C2^H are taken as
+// one-sided Jacobi SVDs of B1 and C2^H: S = s^2 and the same U, without squaring the
+// condition number.  Bond dimensions are kept, as in the reference.  Each bond's
+// arithmetic is the same whatever bonds share its batch.
+int canonical_form_batch(tebd_ctx* c, const int* bonds, int nb) {
+  const int d = c->d;
+  const size_t slab = (size_t)std::max(c->Rmax, c->Cpmax) * c->Cpmax;
+  const double thr = 1e-8;  // S > 1e-8 (local_operations_module.f90:536,571)
+  struct Bond {
+    int chi0, chi1, chi2, m1, n2, cp1;
+    zc *G1, *G2, *P[8];
+    const double *lam0, *lam2;
+    double *lam1, *s1, *s2, *s3, *lsc;
+    int *p1, *p2, *p3;
+  };
+  std::vector<Bond> B(nb);
+  for (int i = 0; i < nb; ++i) {
+    Bond& b = B[i];
+    const int l = bonds[i];
+    b.chi0 = c->chi[l]; b.chi1 = c->chi[l + 1]; b.chi2 = c->chi[l + 2];
+    b.m1 = d * b.chi0; b.n2 = d * b.chi2;
+    if (b.m1 < b.chi1 || b.n2 < b.chi1)
+      return fail(c, TEBD_ERR_SHAPE, "canonical_form: bond dimension exceeds d x neighbour");
+    b.cp1 = round_up(b.chi1, JC);
+    for (int j = 0; j < 8; ++j) b.P[j] = c->cf + slab * (8 * (size_t)i + j);
+    b.s1 = c->cf_s + (3 * (size_t)i + 0) * c->Cpmax;
+    b.s2 = c->cf_s + (3 * (size_t)i + 1) * c->Cpmax;
+    b.s3 = c->cf_s + (3 * (size_t)i + 2) * c->Cpmax;
+    b.p1 = c->cf_perm + (3 * (size_t)i + 0) * c->Cpmax;
+    b.p2 = c->cf_perm + (3 * (size_t)i + 1) * c->Cpmax;
+    b.p3 = c->cf_perm + (3 * (size_t)i + 2) * c->Cpmax;
+    b.lsc = c->cf_lam + (size_t)i * c->Cpmax;
+    b.G1 = c->gam + (size_t)l * c->gslab();
+    b.G2 = c->gam + (size_t)(l + 1) * c->gslab();
+    b.lam0 = c->lam + (size_t)l * c->chi_max;
+    b.lam1 = c->lam + (size_t)(l + 1) * c->chi_max;
+    b.lam2 = c->lam + (size_t)(l + 2) * c->chi_max;
+  }
+  std::vector<SvdJob> J(nb);
+  std::vector<GemmTask> G(nb);
+  int e;
+  // M1 = Gamma1^H lambda0^2 Gamma1  ->  U1, S1
+  for (const Bond& b : B)
+    HIPCHK(c, launch_scaled_copy(b.G1, b.m1, b.P[0], b.m1, b.m1, b.chi1, b.lam0, b.chi0, nullptr, 1, 0, c->st));
+  for (int i = 0; i < nb; ++i) J[i] = {B[i].P[0], B[i].m1, B[i].chi1, B[i].P[1], B[i].s1, B[i].p1, B[i].lsc};
+  if ((e = device_svd_batch(c, J.data(), nb))) return e;
+  for (const Bond& b : B) {
+    HIPCHK(c, launch_perm_cols(b.P[1], b.cp1, b.P[2], b.chi1, b.chi1, b.chi1, b.p1, b.s1, 0, thr, c->st));  // U1
+    HIPCHK(c, launch_perm_cols(b.P[1], b.cp1, b.P[3], b.chi1, b.chi1, b.chi1, b.p1, b.s1, 2, thr, c->st));  // U1 S1^-1/2
+  }
+  for (int i = 0; i < nb; ++i)  // Gamma1Temp = Gamma1 U1 S1^-1/2
+    G[i] = {B[i].G1, B[i].P[3], B[i].P[4], B[i].m1, B[i].chi1, B[i].chi1, B[i].m1, B[i].chi1, B[i].m1, 1.0, 0};
+  if ((e = gemm_batch(c, G.data(), nb, 0, false))) return e;
+  // M2 = Gamma2 lambda2^2 Gamma2^H  ->  U2, S2
+  for (const Bond& b : B)
+    HIPCHK(c, launch_scaled_copy(b.G2, b.chi1, b.P[0], b.n2, b.chi1, b.n2, nullptr, 1, b.lam2, d, 1, c->st));
+  for (int i = 0; i < nb; ++i) J[i] = {B[i].P[0], B[i].n2, B[i].chi1, B[i].P[1], B[i].s2, B[i].p2, B[i].lsc};
+  if ((e = device_svd_batch(c, J.data(), nb))) return e;
+  for (const Bond& b : B) {
+    HIPCHK(c, launch_perm_cols(b.P[1], b.cp1, b.P[5], b.chi1, b.chi1, b.chi1, b.p2, b.s2, 0, thr, c->st));  // U2
+    HIPCHK(c, launch_perm_cols(b.P[1], b.cp1, b.P[6], b.chi1, b.chi1, b.chi1, b.p2, b.s2, 2, thr, c->st));  // U2 S2^-1/2
+  }
+  for (int i = 0; i < nb; ++i)  // Gamma2Temp = (U2 S2^-1/2)^H Gamma2
+    G[i] = {B[i].P[6], B[i].G2, B[i].P[7], B[i].chi1, B[i].n2, B[i].chi1, B[i].chi1, B[i].chi1, B[i].chi1, 1.0, 0};
+  if ((e = gemm_batch(c, G.data(), nb, 1, true))) return e;
+  // X = S1^1/2 U1^H, Y = lambda1 U2 S2^1/2, M = X Y -> U, S, V^T
+  for (const Bond& b : B) HIPCHK(c, launch_cf_xy(b.P[2], b.s1, b.P[5], b.s2, b.lam1, b.P[3], b.P[6], b.chi1, c->st));
+  for (int i = 0; i < nb; ++i)
+    G[i] = {B[i].P[3], B[i].P[6], B[i].P[0], B[i].chi1, B[i].chi1, B[i].chi1, B[i].chi1, B[i].chi1, B[i].chi1, 1.0, 0};
+  if ((e = gemm_batch(c, G.data(), nb, 2, false))) return e;
+  for (int i = 0; i < nb; ++i) J[i] = {B[i].P[0], B[i].chi1, B[i].chi1, B[i].P[1], B[i].s3, B[i].p3, B[i].lsc};
+  if ((e = device_svd_batch(c, J.data(), nb))) return e;
+  for (const Bond& b : B) {
+    HIPCHK(c, launch_perm_cols(b.P[0], b.chi1, b.P[2], b.chi1, b.chi1, b.chi1, b.p3, b.s3, 1, 0.0, c->st));  // U
+    HIPCHK(c, launch_perm_cols(b.P[1], b.cp1, b.P[5], b.chi1, b.chi1, b.chi1, b.p3, b.s3, 0, 0.0, c->st));   // V
+    HIPCHK(c, launch_cf_lambda(b.s3, b.chi1, b.lam1, c->chi_max, c->st));
+  }
+  for (int i = 0; i < nb; ++i)  // Gamma1 = Gamma1Temp U
+    G[i] = {B[i].P[4], B[i].P[2], B[i].G1, B[i].m1, B[i].chi1, B[i].chi1, B[i].m1, B[i].chi1, B[i].m1, 1.0, 0};
+  if ((e = gemm_batch(c, G.data(), nb, 3, false))) return e;
+  for (int i = 0; i < nb; ++i)  // Gamma2 = V^T Gamma2Temp
+    G[i] = {B[i].P[5], B[i].P[7], B[i].G2, B[i].chi1, B[i].n2, B[i].chi1, B[i].chi1, B[i].chi1, B[i].chi1, 1.0, 0};
+  if ((e = gemm_batch(c, G.data(), nb, 4, true))) return e;
+  // the pinned GEMM descriptors are rewritten by the next batch
   HIPCHK(c, hipStreamSynchronize(c->st));
   return TEBD_OK;
 }
 
-// CanonicalForm on bond `l` (0-based; sites l, l+1), local_operations_module.f90:496-620.
-// The SVDs of the weighted Gram matrices M1 = B1^H B1 and M2 = C2 C2^H are taken
-// as one-sided Jacobi SVDs of B1 and C2^H: S = s^2 and the same U, without
-// squaring the condition number.  Bond dimensions are kept, as in the reference.
-int canonical_form(tebd_ctx* c, int l) {
-  const int d = c->d;
-  const int chi0 = c->chi[l], chi1 = c->chi[l + 1], chi2 = c->chi[l + 2];
-  const int m1 = d * chi0, n2 = d * chi2;
-  if (m1 < chi1 || n2 < chi1) return fail(c, TEBD_ERR_SHAPE, "canonical_form: bond dimension exceeds d x neighbour");
-  // every slab holds an R x Cpad matrix or a Cpad x Cpad Jacobi V
+// CanonicalForm workspace for `sets` bonds at once (8 matrices and 3 (s, perm) pairs
+// per bond), kept and grown as needed.
+int ensure_cf_workspace(tebd_ctx* c, int sets) {
+  if (c->cf && c->cf_sets >= sets) return TEBD_OK;
+  if (sets > c->ntask_max) return fail(c, TEBD_ERR_SHAPE, "canonical_form: more bonds than workspace slots");
+  (void)hipFree(c->cf);
+  (void)hipFree(c->cf_s);
+  (void)hipFree(c->cf_perm);
+  (void)hipFree(c->cf_lam);
+  c->cf = nullptr; c->cf_s = nullptr; c->cf_perm = nullptr; c->cf_lam = nullptr;
+  c->cf_sets = 0;
   const size_t slab = (size_t)std::max(c->Rmax, c->Cpmax) * c->Cpmax;
-  zc* P[8];
-  for (int i = 0; i < 8; ++i) P[i] = c->cf + slab * i;
-  double *s1 = c->cf_s, *s2 = c->cf_s + c->Cpmax, *s3 = c->cf_s + 2 * c->Cpmax;
-  int *p1 = c->cf_perm, *p2 = c->cf_perm + c->Cpmax, *p3 = c->cf_perm + 2 * c->Cpmax;
-  const int cp1 = round_up(chi1, JC);
-  zc* G1 = c->gam + (size_t)l * c->gslab();
-  zc* G2 = c->gam + (size_t)(l + 1) * c->gslab();
-  const double* lam0 = c->lam + (size_t)l * c->chi_max;
-  double* lam1 = c->lam + (size_t)(l + 1) * c->chi_max;
-  const double* lam2 = c->lam + (size_t)(l + 2) * c->chi_max;
-  const double thr = 1e-8;  // S > 1e-8 (local_operations_module.f90:536,571)
-  int e;
-  // M1 = Gamma1^H lambda0^2 Gamma1  ->  U1, S1
-  HIPCHK(c, launch_scaled_copy(G1, m1, P[0], m1, m1, chi1, lam0, chi0, nullptr, 1, 0, c->st));
-  if ((e = device_svd(c, P[0], m1, chi1, P[1], s1, p1, c->cf_lam))) return e;
-  HIPCHK(c, launch_perm_cols(P[1], cp1, P[2], chi1, chi1, chi1, p1, s1, 0, thr, c->st));  // U1
-  HIPCHK(c, launch_perm_cols(P[1], cp1, P[3], chi1, chi1, chi1, p1, s1, 2, thr, c->st));  // U1 S1^-1/2
-  if ((e = gemm1(c, G1, P[3], P[4], m1, chi1, chi1, m1, chi1, m1, false))) return e;      // Gamma1Temp
-  // M2 = Gamma2 lambda2^2 Gamma2^H  ->  U2, S2
-  HIPCHK(c, launch_scaled_copy(G2, chi1, P[0], n2, chi1, n2, nullptr, 1, lam2, d, 1, c->st));
-  if ((e = device_svd(c, P[0], n2, chi1, P[1], s2, p2, c->cf_lam))) return e;
-  HIPCHK(c, launch_perm_cols(P[1], cp1, P[5], chi1, chi1, chi1, p2, s2, 0, thr, c->st));  // U2
-  HIPCHK(c, launch_perm_cols(P[1], cp1, P[6], chi1, chi1, chi1, p2, s2, 2, thr, c->st));  // U2 S2^-1/2
-  if ((e = gemm1(c, P[6], G2, P[7], chi1, n2, chi1, chi1, chi1, chi1, true))) return e;    // Gamma2Temp
-  // X = S1^1/2 U1^H, Y = lambda1 U2 S2^1/2, M = X Y -> U, S, V^T
-  HIPCHK(c, launch_cf_xy(P[2], s1, P[5], s2, lam1, P[3], P[6], chi1, c->st));
-  if ((e = gemm1(c, P[3], P[6], P[0], chi1, chi1, chi1, chi1, chi1, chi1, false))) return e;
-  if ((e = device_svd(c, P[0], chi1, chi1, P[1], s3, p3, c->cf_lam))) return e;
-  HIPCHK(c, launch_perm_cols(P[0], chi1, P[2], chi1, chi1, chi1, p3, s3, 1, 0.0, c->st));  // U
-  HIPCHK(c, launch_perm_cols(P[1], cp1, P[5], chi1, chi1, chi1, p3, s3, 0, 0.0, c->st));   // V (V^T = V^H)
-  HIPCHK(c, launch_cf_lambda(s3, chi1, lam1, c->chi_max, c->st));
-  if ((e = gemm1(c, P[4], P[2], G1, m1, chi1, chi1, m1, chi1, m1, false))) return e;      // Gamma1 = Temp U
-  if ((e = gemm1(c, P[5], P[7], G2, chi1, n2, chi1, chi1, chi1, chi1, true))) return e;     // Gamma2 = V^T Temp
-  return TEBD_OK;
-}
-
-int ensure_cf_workspace(tebd_ctx* c) {
-  if (c->cf) return TEBD_OK;
-  const size_t slab = (size_t)std::max(c->Rmax, c->Cpmax) * c->Cpmax;
-  HIPCHK(c, hipMalloc(&c->cf, sizeof(zc) * slab * 8));
-  HIPCHK(c, hipMalloc(&c->cf_s, sizeof(double) * 3 * c->Cpmax));
-  HIPCHK(c, hipMalloc(&c->cf_perm, sizeof(int) * 3 * c->Cpmax));
-  HIPCHK(c, hipMalloc(&c->cf_lam, sizeof(double) * c->Cpmax));
+  HIPCHK(c, hipMalloc(&c->cf, sizeof(zc) * slab * 8 * sets));
+  HIPCHK(c, hipMalloc(&c->cf_s, sizeof(double) * 3 * c->Cpmax * sets));
+  HIPCHK(c, hipMalloc(&c->cf_perm, sizeof(int) * 3 * c->Cpmax * sets));
+  HIPCHK(c, hipMalloc(&c->cf_lam, sizeof(double) * c->Cpmax * sets));
+  c->cf_sets = sets;
   if (!c->h_gemm) {
     HIPCHK(c, hipMalloc(&c->d_gemm, sizeof(GemmTask) * c->ntask_max * 5));
     HIPCHK(c, hipHostMalloc(&c->h_gemm, sizeof(GemmTask) * c->ntask_max * 5));
@@ -1038,6 +1120,7 @@ tebd_ctx* make_lane(tebd_ctx* c, int lane, int slot0, int nslots) {
   l->cf_s = nullptr;
   l->cf_lam = nullptr;
   l->cf_perm = nullptr;
+  l->cf_sets = 0;
   l->owns_gemm = false;
   l->st = nullptr;
   l->wl_count = nullptr;
@@ -1157,8 +1240,12 @@ bool ensure_lanes(tebd_ctx* c, int n) {
 // work.  Every bond's result is independent of which bonds share its batch, so the
 // split changes the timing, never the numbers.
 int run_bonds_split(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
+  // small bond matrices leave nothing to overlap: the extra launches and threads only
+  // cost (measured: 64-site chains at chi = 64, d = 2 run slower with two lanes)
+  int side = 0;
+  for (int i = 0; i < nb; ++i) side = std::max(side, c->d * std::max(c->chi[links[i]], c->chi[links[i] + 2]));
   const int n = std::min(c->nlanes, nb / 4);
-  if (n < 2 || c->ntask_max < 2 * kMaxLanes) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
+  if (n < 2 || c->ntask_max < 2 * kMaxLanes || side < kLaneMinSide) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
   if (!ensure_lanes(c, n)) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
   int first[kMaxLanes + 1];
   for (int k = 0; k <= n; ++k) first[k] = (int)((long long)nb * k / n);
@@ -1183,6 +1270,7 @@ int run_bonds_split(tebd_ctx* c, const int* links, int nb, const zc* const* gate
     lane_push(c, c->lanes[k], links + first[k], first[k + 1] - first[k]);
     tsum += tr[k];
   }
+  c->stats[13] += 1;
   *trunc_sum = tsum;
   return TEBD_OK;
 }
@@ -1568,16 +1656,26 @@ int tebd_two_site_op(tebd_ctx* c, int link, const double* op2, double* truncerr)
   return TEBD_OK;
 }
 
-// CanonicalFormAll of chain r of the context (its bonds left to right, then right to
-// left, as propagation_module.f90:617-634 does for one chain).
-int canonical_form_chain(tebd_ctx* c, int r) {
-  int e = ensure_cf_workspace(c);
+// CanonicalFormAll of chains r0, r0 + dr, ... of the context together: per batch the
+// same bond of every one of them, left to right, then right to left, as
+// propagation_module.f90:617-634 does for one chain.
+int canonical_form_chains(tebd_ctx* c, int r0, int dr) {
+  std::vector<int> rs;
+  for (int r = r0; r < c->nchain; r += dr) rs.push_back(r);
+  const int n = (int)rs.size();
+  if (n == 0) return TEBD_OK;
+  int e = ensure_cf_workspace(c, n);
   if (e) return e;
-  const int b0 = r * c->chain_len, b1 = b0 + c->chain_len - 1;  // bonds b0 .. b1 - 1
-  for (int k = b0; k < b1; ++k)
-    if ((e = canonical_form(c, k))) return e;
-  for (int k = b1 - 1; k >= b0; --k)
-    if ((e = canonical_form(c, k))) return e;
+  std::vector<int> bonds(n);
+  const int Lc = c->chain_len;
+  for (int k = 0; k < Lc - 1; ++k) {
+    for (int j = 0; j < n; ++j) bonds[j] = rs[j] * Lc + k;
+    if ((e = canonical_form_batch(c, bonds.data(), n))) return e;
+  }
+  for (int k = Lc - 2; k >= 0; --k) {
+    for (int j = 0; j < n; ++j) bonds[j] = rs[j] * Lc + k;
+    if ((e = canonical_form_batch(c, bonds.data(), n))) return e;
+  }
   HIPCHK(c, hipStreamSynchronize(c->st));
   drain_profile(c);
   return TEBD_OK;
@@ -1589,14 +1687,17 @@ int tebd_canonical_form_all(tebd_ctx* c) {
   // the chains of an ensemble are independent: with lanes, chain r goes to lane r mod n,
   // each lane on its own stream and host thread; each chain sees the same sequence of
   // bond operations either way
-  const int n = std::min(c->nlanes, c->nchain);
-  if (n >= 2 && c->ntask_max >= n && ensure_lanes(c, n)) {
+  // (a lane needs one workspace slot here, so lanes already made for the Trotter step
+  // are used as they are)
+  int n = std::min(c->nlanes, c->nchain);
+  if (c->lanes_made >= 2 && n >= 2) n = std::min(c->lanes_made, c->nchain);
+  if (n >= 2 && c->ntask_max >= n && (c->lanes_made >= n || ensure_lanes(c, n))) {
     HIPCHK(c, hipStreamSynchronize(c->st));
     int err[kMaxLanes] = {};
     auto work = [&](int k) {
       tebd_ctx* l = c->lanes[k];
       lane_pull(l, c);
-      for (int r = k; r < c->nchain && !err[k]; r += n) err[k] = canonical_form_chain(l, r);
+      err[k] = canonical_form_chains(l, k, n);
     };
     std::vector<std::thread> workers;
     for (int k = 1; k < n; ++k) workers.emplace_back(work, k);
@@ -1608,9 +1709,7 @@ int tebd_canonical_form_all(tebd_ctx* c) {
     }
     return TEBD_OK;
   }
-  for (int r = 0; r < c->nchain; ++r)
-    if ((e = canonical_form_chain(c, r))) return e;
-  return TEBD_OK;
+  return canonical_form_chains(c, 0, 1);
 }
 
 int tebd_canonical_form(tebd_ctx* c, int link) {
@@ -1618,8 +1717,9 @@ int tebd_canonical_form(tebd_ctx* c, int link) {
   if (e) return e;
   if (link < 1 || link > c->L - 1) return fail(c, TEBD_ERR_ARG, "canonical_form: bad link");
   if (is_cut(c, link - 1)) return fail(c, TEBD_ERR_ARG, "canonical_form: bond joins two chains of the ensemble");
-  if ((e = ensure_cf_workspace(c))) return e;
-  if ((e = canonical_form(c, link - 1))) return e;
+  if ((e = ensure_cf_workspace(c, 1))) return e;
+  const int l0 = link - 1;
+  if ((e = canonical_form_batch(c, &l0, 1))) return e;
   HIPCHK(c, hipStreamSynchronize(c->st));
   drain_profile(c);
   return TEBD_OK;
