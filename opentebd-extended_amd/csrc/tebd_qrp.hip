@@ -711,6 +711,123 @@ __global__ __launch_bounds__(256) void k_qrp_reflect(const BondTask* __restrict_
   if (tid < nc) t.qnrm[kb + tid] = nsum[0][tid] + nsum[1][tid] + nsum[2][tid] + nsum[3][tid];
 }
 
+// Two stored panels applied at once (MODE 1: to the Rb-side vectors Yr, MODE 2: to the
+// explicit factor Qs): Y <- Q_{p-1} Q_p Y = Y - V (T (V^H Y)) with the 32-reflector
+// compact-WY block V = [V_{p-1} V_p] (columns j0 .. j0 + 31 of the factor, j0 the first
+// column of panel p - 1) and T = [[T_{p-1}, -T_{p-1} (V_{p-1}^H V_p) T_p], [0, T_p]].
+// Each target column block is streamed once per two panels instead of once per panel;
+// the arithmetic is that of k_qrp_reflect<MODE> on the aggregated block.  arg = p.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_qrp_reflect_pair(const BondTask* __restrict__ tasks, int arg) {
+  static_assert(MODE == 1 || MODE == 2, "pairs of panels only for the vector passes");
+  const BondTask t = tasks[blockIdx.y];
+  if (MODE == 1 && !t.qr) return;
+  const int r = t.C;
+  const int j0 = QB * (arg - 1);  // first column of panel p - 1
+  if (j0 >= r) return;
+  const int nb1 = (r - j0) < QB ? (r - j0) : QB;                    // reflectors of panel p - 1
+  const int nb2 = (r - j0 - QB) <= 0 ? 0 : ((r - j0 - QB) < QB ? (r - j0 - QB) : QB);  // of panel p
+  const int ncols = MODE == 2 ? r : (int)t.result[0];
+  const int kb = 32 * blockIdx.x;
+  if (kb >= ncols || (MODE == 2 && kb + 31 < j0)) return;
+  const int nc = (ncols - kb) < 32 ? (ncols - kb) : 32;
+  const size_t ld = t.Rb;
+  zc* A2 = (MODE == 2 ? t.Qs : t.Yr) + ld * kb + j0;
+  const int K = t.Rb - j0;
+  const zc* P = t.Wq + ld * j0 + j0;  // P[i + ld q] = A(j0 + i, j0 + q)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  auto valid = [&](int q) { return q < QB ? q < nb1 : (q - QB) < nb2; };
+  auto vload = [&](int i, int q) -> zc {
+    if (!valid(q) || i < q || i >= K) return zmk(0.0, 0.0);
+    if (i == q) return zmk(1.0, 0.0);
+    return P[i + ld * q];
+  };
+  auto v2load = [&](int i, int q) -> zc { return vload(i, QB + q); };
+  auto aload = [&](int i, int c) -> zc {
+    if (c >= nc || i >= K) return zmk(0.0, 0.0);
+    return A2[i + ld * c];
+  };
+  __shared__ double red[3][2][256];
+  __shared__ zc Y[2 * QB][33];
+  __shared__ zc G[QB][QB + 1];
+  __shared__ zc Wt[QB][QB + 1];
+  __shared__ zc Ts[2 * QB][2 * QB + 1];
+  const zc* T1 = t.Tq + (size_t)(arg - 1) * QB * QB;
+  const zc* T2 = t.Tq + (size_t)arg * QB * QB;
+  for (int e = tid; e < QB * QB; e += 256) {
+    const int a = e % QB, b = e / QB;
+    Ts[a][b] = T1[e];
+    Ts[QB + a][QB + b] = nb2 > 0 ? T2[e] : zmk(0.0, 0.0);
+    Ts[QB + a][b] = zmk(0.0, 0.0);
+  }
+  // ---- Y = V^H A2 (32 x 32) and G = V_{p-1}^H V_p (16 x 16), rows split over the waves
+  {
+    d4 yre[2][2], yim[2][2];
+    gram_partial<2, 2>(vload, aload, K, wave, 4, lane, yre, yim);
+    gram_reduce<2, 2>(yre, yim, red, &Y[0][0], 33, wave, lane);
+    d4 gre[1][1], gim[1][1];
+    gram_partial<1, 1>(vload, v2load, K, wave, 4, lane, gre, gim);
+    gram_reduce<1, 1>(gre, gim, red, &G[0][0], QB + 1, wave, lane);
+  }
+  // ---- T12 = -T_{p-1} (G T_p)
+  {
+    const int a = tid % QB, b = tid / QB;
+    zc acc = zmk(0.0, 0.0);
+    for (int m = 0; m < QB; ++m) zfma(acc, G[a][m], Ts[QB + m][QB + b]);
+    Wt[a][b] = acc;
+    __syncthreads();
+    acc = zmk(0.0, 0.0);
+    for (int k = a; k < QB; ++k) zfma(acc, Ts[a][k], Wt[k][b]);
+    Ts[a][QB + b] = zmk(-acc.x, -acc.y);
+    __syncthreads();
+  }
+  // ---- Z = T Y (T upper triangular), held as MFMA A operands Z[4ks + lk][16ct + lr]
+  double zre[2][8], zim[2][8];
+#pragma unroll
+  for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      const int pr = 4 * ks + lk, qc = 16 * ct + lr;
+      zc acc = zmk(0.0, 0.0);
+      for (int kk = pr; kk < 2 * QB; ++kk) zfma(acc, Ts[pr][kk], Y[kk][qc]);
+      zre[ct][ks] = acc.x;
+      zim[ct][ks] = acc.y;
+    }
+  // ---- A2 -= V Z, as (V Z)^T = Z^T V^T: A operand Z^T, B operand V^T
+  for (int r0 = 16 * wave; r0 < K; r0 += 64) {
+    const int i = r0 + lr;
+    zc x[8];
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) x[ks] = vload(i, 4 * ks + lk);
+    d4 ore[2], oim[2];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+      ore[ct] = d4{0.0, 0.0, 0.0, 0.0};
+      oim[ct] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < 8; ++ks) {
+        ore[ct] = mfma4(zre[ct][ks], x[ks].x, ore[ct]);
+        ore[ct] = mfma4(-zim[ct][ks], x[ks].y, ore[ct]);
+        oim[ct] = mfma4(zre[ct][ks], x[ks].y, oim[ct]);
+        oim[ct] = mfma4(zim[ct][ks], x[ks].x, oim[ct]);
+      }
+    }
+    if (i < K) {
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int c = 16 * ct + lk + 4 * g;
+          if (c < nc) {
+            zc* ap = A2 + i + ld * c;
+            *ap = zsub(*ap, zmk(ore[ct][g], oim[ct][g]));
+          }
+        }
+    }
+  }
+}
+
 // X (Cb x Cpad, ld Cb) = R(0:r, :)^dagger; columns r..Cpad-1 zero.  R is the upper
 // trapezoid of Wq.
 __global__ void k_qrp_build_x(const BondTask* __restrict__ tasks) {
@@ -778,22 +895,35 @@ hipError_t launch_qrp_build_x(const BondTask* d_tasks, int ntasks, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_qrp_vectors(const BondTask* d_tasks, int ntasks, int max_k, int max_r, hipStream_t st) {
-  hipLaunchKernelGGL(k_qrp_vectors_init<false>, dim3(max_k, ntasks), dim3(256), 0, st, d_tasks);
-  hipError_t e = hipGetLastError();
-  for (int p = (max_r + QB - 1) / QB - 1; p >= 0 && e == hipSuccess; --p) {
-    hipLaunchKernelGGL(k_qrp_reflect<1>, dim3((max_k + 31) / 32, ntasks), dim3(256), 0, st, d_tasks, p);
+// The stored panels applied last to first: a single panel first when their number is
+// odd, then pairs of panels as 32-reflector blocks.
+template <int MODE>
+hipError_t apply_panels(const BondTask* d_tasks, int ntasks, int ncols, int max_r, hipStream_t st) {
+  const dim3 grid((ncols + 31) / 32, ntasks);
+  int p = (max_r + QB - 1) / QB - 1;
+  hipError_t e = hipSuccess;
+  if (p >= 0 && (p + 1) % 2 == 1) {
+    hipLaunchKernelGGL(k_qrp_reflect<MODE>, grid, dim3(256), 0, st, d_tasks, p);
+    e = hipGetLastError();
+    --p;
+  }
+  for (; p >= 1 && e == hipSuccess; p -= 2) {
+    hipLaunchKernelGGL(k_qrp_reflect_pair<MODE>, grid, dim3(256), 0, st, d_tasks, p);
     e = hipGetLastError();
   }
   return e;
 }
 
+hipError_t launch_qrp_vectors(const BondTask* d_tasks, int ntasks, int max_k, int max_r, hipStream_t st) {
+  hipLaunchKernelGGL(k_qrp_vectors_init<false>, dim3(max_k, ntasks), dim3(256), 0, st, d_tasks);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return apply_panels<1>(d_tasks, ntasks, max_k, max_r, st);
+}
+
 hipError_t launch_qrp_form_q(const BondTask* d_tasks, int ntasks, int max_r, hipStream_t st) {
   hipLaunchKernelGGL(k_qrp_vectors_init<true>, dim3(max_r, ntasks), dim3(256), 0, st, d_tasks);
   hipError_t e = hipGetLastError();
-  for (int p = (max_r + QB - 1) / QB - 1; p >= 0 && e == hipSuccess; --p) {
-    hipLaunchKernelGGL(k_qrp_reflect<2>, dim3((max_r + 31) / 32, ntasks), dim3(256), 0, st, d_tasks, p);
-    e = hipGetLastError();
-  }
-  return e;
+  if (e != hipSuccess) return e;
+  return apply_panels<2>(d_tasks, ntasks, max_r, max_r, st);
 }
