@@ -151,11 +151,12 @@ def run_mode(mode, steps, warmup, log, state=None, keep_state=False, rank_model=
     prof = ctx.profile()
     st1 = ctx.stats()
     dims = ctx.bond_dims()
+    tuning = ctx.autotune_info()
     obs = device_observables(ctx, M.fermi_hubbard(L, T_HOP, U_INT)) if state is None else None
     ctx.close()
     delta = {k: st1[k] - st0[k] for k in st1}
     return dict(elapsed=elapsed, prof=prof, dims=dims, trunc=trunc + trunc_warm, stats=delta, warm_s=warm_s,
-                snap=snap, obs=obs)
+                snap=snap, obs=obs, tuning=tuning)
 
 
 def copy_calibration(mib):
@@ -320,7 +321,8 @@ def main():
         "data": "synthetic (Neel product state, Fermi-Hubbard gates)",
         "config": {"workload": "Fermi-Hubbard quench L=128 d=4 t=1 U=4 dt=0.01 chi_max=256, Neel start; "
                                f"timed steps {args.warmup + 1}-{args.warmup + args.steps}",
-                   "mode": args.mode, "lanes": args.lanes, "chi_max_reached": int(res["dims"].max()),
+                   "mode": args.mode, "lanes": args.lanes, "gemm_autotune": res["tuning"],
+                   "chi_max_reached": int(res["dims"].max()),
                    "mean_bond_dim": float(res["dims"].mean()), "warmup_s": res["warm_s"]},
         "roofline": roof,
         "learned_rank": {"rank_model": os.path.basename(args.rank_model) if args.rank_model else None,

@@ -30,7 +30,13 @@ hipError_t launch_mlp_features(const BondTask* d_tasks, int ntasks, int L, long 
 hipError_t launch_mlp_predict(const float* d_params, const float* d_X, int n, float* d_out, hipStream_t st);
 hipError_t launch_write_gammas(const BondTask* d_tasks, int ntasks, hipStream_t st);
 // tile variants of the batched complex GEMM (chosen per shape by the autotuner)
-enum ZgemmVariant { ZGEMM_VALU_64x64 = 0, ZGEMM_MFMA_64x64 = 1, ZGEMM_MFMA_128x64 = 2, ZGEMM_VARIANTS = 3 };
+enum ZgemmVariant {
+  ZGEMM_VALU_64x64 = 0,
+  ZGEMM_MFMA_64x64 = 1,
+  ZGEMM_MFMA_128x64 = 2,
+  ZGEMM_MFMA_64x96 = 3,
+  ZGEMM_VARIANTS = 4
+};
 hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_n, bool conj_a, hipStream_t st,
                         int variant);
 hipError_t launch_build_sketch(const BondTask* d_tasks, const int* d_kprime, int ntasks, unsigned long long seed,

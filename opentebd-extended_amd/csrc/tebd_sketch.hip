@@ -80,9 +80,10 @@ __global__ __launch_bounds__(256) void k_zgemm(const GemmTask* __restrict__ task
 }
 
 // ---------------------------------------------------------------------------
-// batched complex GEMM on the FP64 matrix cores: TM x 64 tiles, 4 waves as 2 x 2,
-// each wave (TM/2) x 32 = (TM/32) x 2 blocks of 16 x 16, complex products as four
-// real v_mfma_f64_16x16x4f64 per k-step, operands staged through LDS in chunks of 16.
+// batched complex GEMM on the FP64 matrix cores: TM x TN tiles, 4 waves as 2 x 2,
+// each wave (TM/2) x (TN/2) = (TM/32) x (TN/32) blocks of 16 x 16, complex products as
+// four real v_mfma_f64_16x16x4f64 per k-step, operands staged through LDS in chunks of
+// 16.  Every tile shape takes the same operations in the same order per element.
 // ---------------------------------------------------------------------------
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -94,9 +95,9 @@ __device__ __forceinline__ d4 mfma4(double a, double b, d4 c) {
 
 constexpr int MK = 16;  // k chunk staged in LDS
 
-template <bool CONJ_A, int TM>
+template <bool CONJ_A, int TM, int TN>
 __global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__ tasks) {
-  constexpr int TN = 64, RB = TM / 32;
+  constexpr int RB = TM / 32, CB = TN / 32;
   const GemmTask t = tasks[blockIdx.z];
   const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
   if (m0 >= t.M || n0 >= t.N) return;
@@ -105,11 +106,11 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int wr = wave >> 1, wc = wave & 1;
-  d4 re[RB][2], im[RB][2];
+  d4 re[RB][CB], im[RB][CB];
 #pragma unroll
   for (int i = 0; i < RB; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < CB; ++j) {
       re[i][j] = d4{0.0, 0.0, 0.0, 0.0};
       im[i][j] = d4{0.0, 0.0, 0.0, 0.0};
     }
@@ -156,15 +157,15 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__
 #pragma unroll
     for (int ks = 0; ks < MK / 4; ++ks) {
       const int kk = 4 * ks + lk;
-      zc a[RB], b[2];
+      zc a[RB], b[CB];
 #pragma unroll
       for (int i = 0; i < RB; ++i) a[i] = As[kk][wr * (TM / 2) + 16 * i + lr];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) b[j] = Bs[kk][wc * 32 + 16 * j + lr];
+      for (int j = 0; j < CB; ++j) b[j] = Bs[kk][wc * (TN / 2) + 16 * j + lr];
 #pragma unroll
       for (int i = 0; i < RB; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < CB; ++j) {
           re[i][j] = mfma4(a[i].x, b[j].x, re[i][j]);
           re[i][j] = mfma4(-a[i].y, b[j].y, re[i][j]);
           im[i][j] = mfma4(a[i].x, b[j].y, im[i][j]);
@@ -176,10 +177,10 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__
 #pragma unroll
   for (int i = 0; i < RB; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < CB; ++j)
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
-        const int m = m0 + wr * (TM / 2) + 16 * i + lk + 4 * g, n = n0 + wc * 32 + 16 * j + lr;
+        const int m = m0 + wr * (TM / 2) + 16 * i + lk + 4 * g, n = n0 + wc * (TN / 2) + 16 * j + lr;
         if (m < t.M && n < t.N) {
           zc v = zmk(re[i][j][g] * t.alpha, im[i][j][g] * t.alpha);
           if (t.accumulate) v = zadd(v, t.C[m + (size_t)t.ldc * n]);
@@ -238,9 +239,11 @@ hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_
     case ZGEMM_VALU_64x64:
       return conj_a ? go(k_zgemm<true>, GT, GT) : go(k_zgemm<false>, GT, GT);
     case ZGEMM_MFMA_64x64:
-      return conj_a ? go(k_zgemm_mfma<true, 64>, 64, 64) : go(k_zgemm_mfma<false, 64>, 64, 64);
+      return conj_a ? go(k_zgemm_mfma<true, 64, 64>, 64, 64) : go(k_zgemm_mfma<false, 64, 64>, 64, 64);
     case ZGEMM_MFMA_128x64:
-      return conj_a ? go(k_zgemm_mfma<true, 128>, 128, 64) : go(k_zgemm_mfma<false, 128>, 128, 64);
+      return conj_a ? go(k_zgemm_mfma<true, 128, 64>, 128, 64) : go(k_zgemm_mfma<false, 128, 64>, 128, 64);
+    case ZGEMM_MFMA_64x96:
+      return conj_a ? go(k_zgemm_mfma<true, 64, 96>, 64, 96) : go(k_zgemm_mfma<false, 64, 96>, 64, 96);
     default:
       return hipErrorInvalidValue;
   }

@@ -253,10 +253,11 @@ class Context:
         self._check(self._lib.tebd_set_autotune(self._h, int(on)))
 
     def autotune_info(self):
-        out = np.zeros(5)
-        self._check(self._lib.tebd_autotune_info(self._h, _p(out), 5))
+        out = np.zeros(6)
+        self._check(self._lib.tebd_autotune_info(self._h, _p(out), 6))
         return {"shape_classes": int(out[0]), "timing_launches": int(out[1]),
-                "chosen": {"valu_64x64": int(out[2]), "mfma_64x64": int(out[3]), "mfma_128x64": int(out[4])}}
+                "chosen": {"valu_64x64": int(out[2]), "mfma_64x64": int(out[3]), "mfma_128x64": int(out[4]),
+                           "mfma_64x96": int(out[5])}}
 
 
 class Ensemble(Context):
