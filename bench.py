@@ -117,11 +117,12 @@ def device_observables(ctx, H):
 
 
 def run_mode(mode, steps, warmup, log, state=None, keep_state=False, rank_model=None, lanes=2, tag="timed",
-             profile=False):
+             profile=False, graphs=True):
     from tebd_amd import models as M
 
     ctx = make_context(mode, rank_model)
     ctx.set_lanes(lanes)
+    ctx.set_graphs(graphs)
     if state is not None:
         ctx.upload(*state)
     trunc_warm = 0.0
@@ -211,6 +212,8 @@ def main():
     ap.add_argument("--compare-steps", type=int, default=3)
     ap.add_argument("--mode", default="batched,learned")
     ap.add_argument("--lanes", type=int, default=2, help="streams / host threads per half-sweep of the timed run")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="hipGraph replay of the Jacobi sweeps in the timed runs (0 = eager launches)")
     ap.add_argument("--roofline-lanes", type=int, default=1,
                     help="lanes of the separate run whose per-kernel events give the roofline (0 = use the timed run)")
     ap.add_argument("--reference-mode", default="batched", help="exact path for the observable check ('' = skip)")
@@ -236,14 +239,15 @@ def main():
 
     model = np.load(args.rank_model) if args.rank_model else None
     res = run_mode(args.mode, args.steps, args.warmup, log, rank_model=model, lanes=args.lanes,
-                   profile=not args.roofline_lanes)
+                   profile=not args.roofline_lanes, graphs=bool(args.graphs))
     sps = args.steps / res["elapsed"]
     ref = None
     if args.reference_mode:
-        ref = run_mode(args.reference_mode, args.steps, args.warmup, log, keep_state=bool(args.compare))
+        ref = run_mode(args.reference_mode, args.steps, args.warmup, log, keep_state=bool(args.compare),
+                       graphs=bool(args.graphs))
     mixed = None
     if args.mixed_mode:
-        mixed = run_mode(args.mixed_mode, args.steps, args.warmup, log, rank_model=model)
+        mixed = run_mode(args.mixed_mode, args.steps, args.warmup, log, rank_model=model, graphs=bool(args.graphs))
     # per-kernel event timings from a run whose launches do not share the GPU with a
     # second lane, so each launch's duration is that kernel's own
     roof_run = res
@@ -292,7 +296,7 @@ def main():
             extra["mixed"]["obs_err"] = max(merr.values())
             extra["mixed"]["obs_err_by_observable"] = merr
     if args.compare and ref is not None:
-        cmp_res = run_mode(args.compare, args.compare_steps, 0, log, state=ref["snap"])
+        cmp_res = run_mode(args.compare, args.compare_steps, 0, log, state=ref["snap"], graphs=bool(args.graphs))
         extra[f"{args.compare}_steps_per_s"] = args.compare_steps / cmp_res["elapsed"]
         extra[f"{args.compare}_window"] = f"steps {args.warmup + 1}-{args.warmup + args.compare_steps}"
         extra[f"speedup_vs_{args.compare}"] = sps / extra[f"{args.compare}_steps_per_s"]
@@ -321,7 +325,7 @@ def main():
         "data": "synthetic (Neel product state, Fermi-Hubbard gates)",
         "config": {"workload": "Fermi-Hubbard quench L=128 d=4 t=1 U=4 dt=0.01 chi_max=256, Neel start; "
                                f"timed steps {args.warmup + 1}-{args.warmup + args.steps}",
-                   "mode": args.mode, "lanes": args.lanes, "gemm_autotune": res["tuning"],
+                   "mode": args.mode, "lanes": args.lanes, "graphs": bool(args.graphs), "gemm_autotune": res["tuning"],
                    "chi_max_reached": int(res["dims"].max()),
                    "mean_bond_dim": float(res["dims"].mean()), "warmup_s": res["warm_s"]},
         "roofline": roof,
