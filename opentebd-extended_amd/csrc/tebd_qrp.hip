@@ -203,35 +203,66 @@ __device__ __forceinline__ bool householder(zc alpha, double xn2, zc& tau, doubl
   return true;
 }
 
-// Column moves of a panel: sel[0..nbp) (distinct columns >= j0) go to j0..j0+nbp-1 in
+// Column moves of a panel, by one wave (lane q < QB handles panel slot j0 + q and the
+// q-th selected column): sel[0..nbp) (distinct columns >= j0) go to j0..j0+nbp-1 in
 // that order; the columns of j0..j0+nbp-1 not selected go to the selected slots beyond
-// the panel (both lists ascending), so every column lands in exactly one place.  One
-// thread; updates qpiv and qnrm and returns the moves in mv_src/mv_dst.
+// the panel (both lists ascending), so every column lands in exactly one place.
+// Updates qpiv and qnrm, writes the moves to mv_src/mv_dst and returns their number
+// (the same in every lane).  Every global read is issued before any global write, and
+// the reads are independent of each other.
 __device__ int panel_moves(const BondTask& t, int j0, int nbp, const int* sel, int* mv_src, int* mv_dst,
-                           int* piv_tmp, double* nrm_tmp) {
-  int nmv = 0, nvac = 0;
-  for (int d = j0; d < j0 + nbp; ++d) {
-    bool in = false;
-    for (int q = 0; q < nbp; ++q) in = in || sel[q] == d;
-    if (!in) mv_src[nmv++] = d;
+                           int lane) {
+  int sr[QB];
+#pragma unroll
+  for (int q = 0; q < QB; ++q) sr[q] = q < nbp ? sel[q] : -1;
+  const int d = j0 + lane;                       // this lane's panel slot
+  const int mine = lane < nbp ? sel[lane] : -1;  // this lane's selected column
+  bool in = false;
+  int rank_dst = 0;
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    in = in || sr[q] == d;
+    if (sr[q] >= j0 + nbp && sr[q] < mine) ++rank_dst;
   }
-  for (int q = 0; q < nbp; ++q)
-    if (sel[q] >= j0 + nbp) {
-      int m = nvac++;  // insertion into the ascending list
-      for (; m > 0 && mv_dst[m - 1] > sel[q]; --m) mv_dst[m] = mv_dst[m - 1];
-      mv_dst[m] = sel[q];
-    }
-  for (int q = 0; q < nbp; ++q) piv_tmp[q] = t.qpiv[sel[q]];
-  for (int m = 0; m < nmv; ++m) {
-    piv_tmp[QB + m] = t.qpiv[mv_src[m]];
-    nrm_tmp[m] = t.qnrm[mv_src[m]];
+  const bool is_src = lane < nbp && !in;           // an unselected panel column: it moves out
+  const bool is_dst = lane < nbp && mine >= j0 + nbp;  // a selected column beyond the panel: it is vacated
+  const unsigned long long bsrc = __ballot(is_src);
+  const int nmv = __popcll(bsrc);
+  const int rank_src = __popcll(bsrc & ((1ull << lane) - 1ull));
+  const int piv_sel = lane < nbp ? t.qpiv[mine] : 0;
+  const int piv_src = is_src ? t.qpiv[d] : 0;
+  const double nrm_src = is_src ? t.qnrm[d] : 0.0;
+  // the moved-out column of rank m fills the vacated slot of rank m
+  int from = 0;
+  if (is_dst) {
+    unsigned long long b = bsrc;
+    for (int m = 0; m < rank_dst; ++m) b &= b - 1ull;
+    from = __builtin_ctzll(b);
   }
-  for (int q = 0; q < nbp; ++q) t.qpiv[j0 + q] = piv_tmp[q];
-  for (int m = 0; m < nmv; ++m) {
-    t.qpiv[mv_dst[m]] = piv_tmp[QB + m];
-    t.qnrm[mv_dst[m]] = nrm_tmp[m];
+  const int piv_mv = __shfl(piv_src, from, 64);
+  const double nrm_mv = __shfl(nrm_src, from, 64);
+  if (lane < nbp) t.qpiv[d] = piv_sel;
+  if (is_dst) {
+    t.qpiv[mine] = piv_mv;
+    t.qnrm[mine] = nrm_mv;
+    mv_dst[rank_dst] = mine;
   }
+  if (is_src) mv_src[rank_src] = d;
   return nmv;
+}
+
+// Rows [i] of the displaced columns moved to the vacated slots, four loads in flight
+// (sources and destinations are disjoint column sets).
+__device__ __forceinline__ void move_rows(zc* Wq, size_t ld, int i, int nmv, const int* mv_src, const int* mv_dst) {
+  for (int m0 = 0; m0 < nmv; m0 += 4) {
+    zc v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (m0 + u < nmv) v[u] = Wq[i + ld * mv_src[m0 + u]];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (m0 + u < nmv) Wq[i + ld * mv_dst[m0 + u]] = v[u];
+  }
 }
 
 // Panel j0 with the panel in registers (Rb <= RPT * PT; one workgroup of PT threads per
@@ -259,8 +290,7 @@ __global__ __launch_bounds__(PT) void k_qrp_panel_reg(const BondTask* __restrict
   const int ncand = Cb - j0;
   const size_t ld = Rb;
   __shared__ double snrm[kMaxCols];
-  __shared__ int sel[QB], mv_src[QB], mv_dst[QB], piv_tmp[2 * QB];
-  __shared__ double nrm_tmp[QB];
+  __shared__ int sel[QB], mv_src[QB], mv_dst[QB];
   __shared__ int snmv, sstop;
   __shared__ double red[PW][32];
   __shared__ double wsum[32];
@@ -305,7 +335,10 @@ __global__ __launch_bounds__(PT) void k_qrp_panel_reg(const BondTask* __restrict
     if (tid == 0) t.qstate[0] = sstop == 1 ? (double)j0 : 0.0;
     return;
   }
-  if (tid == 0) snmv = panel_moves(t, j0, nbp, sel, mv_src, mv_dst, piv_tmp, nrm_tmp);
+  if (wave == 0) {
+    const int n = panel_moves(t, j0, nbp, sel, mv_src, mv_dst, lane);
+    if (lane == 0) snmv = n;
+  }
   __syncthreads();
 
   // 2. gather the panel; each thread moves its own rows of the displaced columns
@@ -316,8 +349,7 @@ __global__ __launch_bounds__(PT) void k_qrp_panel_reg(const BondTask* __restrict
     const int i = tid + PT * s;
 #pragma unroll
     for (int q = 0; q < QB; ++q) a[s][q] = (i < Rb && q < nbp) ? t.Wq[i + ld * sel[q]] : zmk(0.0, 0.0);
-    if (i < Rb)
-      for (int m = 0; m < nmv; ++m) t.Wq[i + ld * mv_dst[m]] = t.Wq[i + ld * mv_src[m]];
+    if (i < Rb) move_rows(t.Wq, ld, i, nmv, mv_src, mv_dst);
   }
 
   // 3. Householder QR of the panel.  The columns rotate through a[]: at step q the
@@ -471,8 +503,7 @@ __global__ __launch_bounds__(256) void k_qrp_panel(const BondTask* __restrict__ 
   const int ncand = t.Cb - j0;
   const size_t ld = t.Rb;
   __shared__ double snrm[kMaxCols];
-  __shared__ int sel[QB], mv_src[QB], mv_dst[QB], piv_tmp[2 * QB];
-  __shared__ double nrm_tmp[QB];
+  __shared__ int sel[QB], mv_src[QB], mv_dst[QB];
   __shared__ int snmv, sstop;
   __shared__ double red[4];
   __shared__ zc G[QB][QB + 1];
@@ -512,7 +543,10 @@ __global__ __launch_bounds__(256) void k_qrp_panel(const BondTask* __restrict__ 
     if (tid == 0) t.qstate[0] = sstop == 1 ? (double)j0 : 0.0;
     return;
   }
-  if (tid == 0) snmv = panel_moves(t, j0, nbp, sel, mv_src, mv_dst, piv_tmp, nrm_tmp);
+  if (wave == 0) {
+    const int n = panel_moves(t, j0, nbp, sel, mv_src, mv_dst, lane);
+    if (lane == 0) snmv = n;
+  }
   __syncthreads();
   // row by row: gather the selected columns, move the displaced ones to the vacated
   // slots, then write the selected ones into the panel
@@ -520,7 +554,7 @@ __global__ __launch_bounds__(256) void k_qrp_panel(const BondTask* __restrict__ 
     zc v[QB];
 #pragma unroll
     for (int q = 0; q < QB; ++q) v[q] = q < nbp ? t.Wq[i + ld * sel[q]] : zmk(0.0, 0.0);
-    for (int m = 0; m < snmv; ++m) t.Wq[i + ld * mv_dst[m]] = t.Wq[i + ld * mv_src[m]];
+    move_rows(t.Wq, ld, i, snmv, mv_src, mv_dst);
 #pragma unroll
     for (int q = 0; q < QB; ++q)
       if (q < nbp) t.Wq[i + ld * (j0 + q)] = v[q];
