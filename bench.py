@@ -21,8 +21,8 @@ The mixed-precision form of the same path (theta and gate on the FP32 matrix
 cores, `--mixed-mode`) runs the same 100 steps; its steps/s and its observable
 errors against the exact batched path are reported under `mixed`.
 
-The timed run splits every half-sweep over two lanes (two HIP streams driven by two
-host threads, `--lanes`); the results are bit-identical to one lane.
+The timed run splits every half-sweep over three lanes (three HIP streams driven by
+three host threads, `--lanes`); the results are bit-identical to one lane.
 
 Prints one JSON line.  `roofline` is the single kernel with the most device time in
 the same timed steps run with one lane (`--roofline-lanes`, so that no launch shares
@@ -116,7 +116,7 @@ def device_observables(ctx, H):
     return dict(E=ctx.energy(H), n=ctx.one_site_expval(ntot), sz=ctx.one_site_expval(sz), S=ctx.chain_entropy())
 
 
-def run_mode(mode, steps, warmup, log, state=None, keep_state=False, rank_model=None, lanes=2, tag="timed",
+def run_mode(mode, steps, warmup, log, state=None, keep_state=False, rank_model=None, lanes=3, tag="timed",
              profile=False, graphs=True):
     from tebd_amd import models as M
 
@@ -211,7 +211,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--compare-steps", type=int, default=3)
     ap.add_argument("--mode", default="batched,learned")
-    ap.add_argument("--lanes", type=int, default=2, help="streams / host threads per half-sweep of the timed run")
+    ap.add_argument("--lanes", type=int, default=3, help="streams / host threads per half-sweep of the timed run")
     ap.add_argument("--graphs", type=int, default=1,
                     help="hipGraph replay of the Jacobi sweeps in the timed runs (0 = eager launches)")
     ap.add_argument("--roofline-lanes", type=int, default=1,

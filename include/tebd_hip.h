@@ -167,10 +167,13 @@ int tebd_get_profile(tebd_ctx* ctx, double* out, int n);
 int tebd_set_graphs(tebd_ctx* ctx, int on);
 
 /* Host concurrency of a batched context (no counterpart in the reference): with
- * n lanes (1 to 4, default 2) the bonds of each half-sweep are split into n contiguous
- * parts that run on n HIP streams, each driven by its own host thread, so that one
- * part's latency-bound phases (QR panels, inner Jacobi solves) overlap the others'
- * work; n = 1 runs a half-sweep as one batch on the context's stream.  A half-sweep
+ * n lanes (1 to 4, default 3) the bonds of each half-sweep are split into n contiguous
+ * parts that run on n HIP streams (the context's own and n - 1 more), each driven by
+ * its own host thread, so that one part's latency-bound phases (QR panels, inner
+ * Jacobi solves) overlap the others' work; n = 1 runs a half-sweep as one batch on the
+ * context's stream.  HIP runs a process's streams on a few hardware queues (four
+ * unless GPU_MAX_HW_QUEUES says otherwise), and busy streams that share one run in
+ * turn: more lanes than free queues make the step slower, not faster.  A half-sweep
  * is split only when it has at least 4 bonds per lane and a bond matrix with a side
  * of at least 512 (smaller ones run faster as one batch).  The chains of an ensemble
  * are also distributed over the lanes in tebd_canonical_form_all.  Every bond's
@@ -188,7 +191,7 @@ int tebd_set_lanes(tebd_ctx* ctx, int n);
  * classes take the 64 x 64 MFMA tiles), on = 1 on, on = -1 on and forgets what was
  * tuned.  tebd_autotune_info: out[0] shape classes tuned, out[1] timing launches,
  * out[2 + v] classes that chose variant v (0 VALU 64x64, 1 MFMA 64x64, 2 MFMA
- * 128x64, 3 MFMA 64x96); n <= 6. */
+ * 128x64, 3 MFMA 64x96, 4 MFMA 64x64 with k chunks of 32); n <= 7. */
 int tebd_set_autotune(tebd_ctx* ctx, int on);
 int tebd_autotune_info(tebd_ctx* ctx, double* out, int n);
 

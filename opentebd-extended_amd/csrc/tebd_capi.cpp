@@ -150,7 +150,8 @@ struct tebd_ctx {
   std::map<std::array<long long, 7>, hipGraphExec_t> sweep_graphs;
   bool is_lane = false;
   bool owns_gemm = false;  // a lane that allocated its own GEMM descriptors
-  int nlanes = 2;
+  bool owns_stream = true;  // false for lane 0, which runs on its context's stream
+  int nlanes = 3;
   int lanes_made = 0;
   tebd_ctx* lanes[kMaxLanes] = {};
   struct Pending { hipEvent_t a, b; int kind; };
@@ -1103,8 +1104,12 @@ int run_bonds(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs,
 
 void free_lane(tebd_ctx* l);
 
-// Lane `lane` of context c: workspace slots [slot0, slot0 + nslots), its own stream,
-// Jacobi queue counters and events; everything else is the context's.
+// Lane `lane` of context c: workspace slots [slot0, slot0 + nslots), its own Jacobi
+// queue counters and events, and a stream: lane 0 uses the context's own stream, which
+// is idle while the lanes run, the others one each.  HIP maps streams onto a few
+// hardware queues per process (four by default), and two busy streams that share a
+// queue run one after the other, so the lanes take as few streams as they can.
+// Everything else is the context's.
 tebd_ctx* make_lane(tebd_ctx* c, int lane, int slot0, int nslots) {
   tebd_ctx* l = new tebd_ctx(*c);
   l->is_lane = true;
@@ -1122,9 +1127,10 @@ tebd_ctx* make_lane(tebd_ctx* c, int lane, int slot0, int nslots) {
   l->cf_perm = nullptr;
   l->cf_sets = 0;
   l->owns_gemm = false;
-  l->st = nullptr;
+  l->owns_stream = lane != 0;
+  l->st = lane == 0 ? c->st : nullptr;
   l->wl_count = nullptr;
-  if (hipStreamCreateWithFlags(&l->st, hipStreamNonBlocking) != hipSuccess ||
+  if ((lane != 0 && hipStreamCreateWithFlags(&l->st, hipStreamNonBlocking) != hipSuccess) ||
       hipMalloc(&l->wl_count, sizeof(int) * 2) != hipSuccess) {
     free_lane(l);
     return nullptr;
@@ -1170,7 +1176,6 @@ tebd_ctx* make_lane(tebd_ctx* c, int lane, int slot0, int nslots) {
   // GEMM descriptors: the lane's slice of the context's, or (none yet) its own later
   l->d_gemm = c->d_gemm ? c->d_gemm + 5 * s0 : nullptr;
   l->h_gemm = c->h_gemm ? c->h_gemm + 5 * s0 : nullptr;
-  (void)lane;
   return l;
 }
 
@@ -1364,7 +1369,7 @@ void free_lane(tebd_ctx* l) {
     (void)hipHostFree(l->h_gemm);
   }
   (void)hipFree(l->wl_count);
-  if (l->st) (void)hipStreamDestroy(l->st);
+  if (l->st && l->owns_stream) (void)hipStreamDestroy(l->st);
   delete l;
 }
 

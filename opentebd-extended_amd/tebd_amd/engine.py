@@ -244,7 +244,7 @@ class Context:
         """Replay each Jacobi sweep as a captured hipGraph (while per-launch profiling is off)."""
         self._check(self._lib.tebd_set_graphs(self._h, int(bool(on))))
 
-    def set_lanes(self, n=2):
+    def set_lanes(self, n=3):
         """Split each batched half-sweep over n (1 to 4) streams driven by n host threads."""
         self._check(self._lib.tebd_set_lanes(self._h, int(n)))
 
@@ -253,11 +253,11 @@ class Context:
         self._check(self._lib.tebd_set_autotune(self._h, int(on)))
 
     def autotune_info(self):
-        out = np.zeros(6)
-        self._check(self._lib.tebd_autotune_info(self._h, _p(out), 6))
+        out = np.zeros(7)
+        self._check(self._lib.tebd_autotune_info(self._h, _p(out), 7))
         return {"shape_classes": int(out[0]), "timing_launches": int(out[1]),
                 "chosen": {"valu_64x64": int(out[2]), "mfma_64x64": int(out[3]), "mfma_128x64": int(out[4]),
-                           "mfma_64x96": int(out[5])}}
+                           "mfma_64x96": int(out[5]), "mfma_64x64_k32": int(out[6])}}
 
 
 class Ensemble(Context):
