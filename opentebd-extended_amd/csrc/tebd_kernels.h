@@ -35,7 +35,8 @@ enum ZgemmVariant {
   ZGEMM_MFMA_64x64 = 1,
   ZGEMM_MFMA_128x64 = 2,
   ZGEMM_MFMA_64x96 = 3,
-  ZGEMM_VARIANTS = 4
+  ZGEMM_MFMA_64x64_K32 = 4,  // 64 x 64 tiles, k chunks of 32
+  ZGEMM_VARIANTS = 5
 };
 hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_n, bool conj_a, hipStream_t st,
                         int variant);

@@ -83,7 +83,9 @@ __global__ __launch_bounds__(256) void k_zgemm(const GemmTask* __restrict__ task
 // batched complex GEMM on the FP64 matrix cores: TM x TN tiles, 4 waves as 2 x 2,
 // each wave (TM/2) x (TN/2) = (TM/32) x (TN/32) blocks of 16 x 16, complex products as
 // four real v_mfma_f64_16x16x4f64 per k-step, operands staged through LDS in chunks of
-// 16.  Every tile shape takes the same operations in the same order per element.
+// KCH = 16 (or 32: half the barriers per product and twice the time for the next
+// chunk's loads to arrive, for twice the LDS).  Every tile shape and chunk length takes
+// the same operations in the same order per element.
 // ---------------------------------------------------------------------------
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -95,14 +97,14 @@ __device__ __forceinline__ d4 mfma4(double a, double b, d4 c) {
 
 constexpr int MK = 16;  // k chunk staged in LDS
 
-template <bool CONJ_A, int TM, int TN>
+template <bool CONJ_A, int TM, int TN, int KCH>
 __global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__ tasks) {
   constexpr int RB = TM / 32, CB = TN / 32;
   const GemmTask t = tasks[blockIdx.z];
   const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
   if (m0 >= t.M || n0 >= t.N) return;
-  __shared__ zc As[MK][TM + 1];
-  __shared__ zc Bs[MK][TN + 1];
+  __shared__ zc As[KCH][TM + 1];
+  __shared__ zc Bs[KCH][TN + 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int wr = wave >> 1, wc = wave & 1;
@@ -116,7 +118,7 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__
     }
   // chunk k0 of op(A) and B is loaded into registers while the previous one is
   // multiplied from LDS: element e = tid + 256 u of the chunk
-  constexpr int NA = MK * TM / 256, NB = MK * TN / 256;
+  constexpr int NA = KCH * TM / 256, NB = KCH * TN / 256;
   zc pa[NA], pb[NB];
   auto load_chunk = [&](int k0) {
 #pragma unroll
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__
         const int m = m0 + e % TM, k = k0 + e / TM;
         if (m < t.M && k < t.K) v = t.A[m + (size_t)t.lda * k];
       } else {        // A is K x M: contiguous along k
-        const int m = m0 + e / MK, k = k0 + e % MK;
+        const int m = m0 + e / KCH, k = k0 + e % KCH;
         if (m < t.M && k < t.K) v = zconj(t.A[k + (size_t)t.lda * m]);
       }
       pa[u] = v;
@@ -135,27 +137,27 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
       const int e = tid + 256 * u;
-      const int k = k0 + e % MK, n = n0 + e / MK;
+      const int k = k0 + e % KCH, n = n0 + e / KCH;
       pb[u] = (k < t.K && n < t.N) ? t.B[k + (size_t)t.ldb * n] : zmk(0.0, 0.0);
     }
   };
   load_chunk(0);
-  for (int k0 = 0; k0 < t.K; k0 += MK) {
+  for (int k0 = 0; k0 < t.K; k0 += KCH) {
 #pragma unroll
     for (int u = 0; u < NA; ++u) {
       const int e = tid + 256 * u;
       if (!CONJ_A) As[e / TM][e % TM] = pa[u];
-      else As[e % MK][e / MK] = pa[u];
+      else As[e % KCH][e / KCH] = pa[u];
     }
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
       const int e = tid + 256 * u;
-      Bs[e % MK][e / MK] = pb[u];
+      Bs[e % KCH][e / KCH] = pb[u];
     }
     __syncthreads();
-    if (k0 + MK < t.K) load_chunk(k0 + MK);
-#pragma unroll
-    for (int ks = 0; ks < MK / 4; ++ks) {
+    if (k0 + KCH < t.K) load_chunk(k0 + KCH);
+#pragma unroll 4
+    for (int ks = 0; ks < KCH / 4; ++ks) {
       const int kk = 4 * ks + lk;
       zc a[RB], b[CB];
 #pragma unroll
@@ -239,11 +241,13 @@ hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_
     case ZGEMM_VALU_64x64:
       return conj_a ? go(k_zgemm<true>, GT, GT) : go(k_zgemm<false>, GT, GT);
     case ZGEMM_MFMA_64x64:
-      return conj_a ? go(k_zgemm_mfma<true, 64, 64>, 64, 64) : go(k_zgemm_mfma<false, 64, 64>, 64, 64);
+      return conj_a ? go(k_zgemm_mfma<true, 64, 64, MK>, 64, 64) : go(k_zgemm_mfma<false, 64, 64, MK>, 64, 64);
     case ZGEMM_MFMA_128x64:
-      return conj_a ? go(k_zgemm_mfma<true, 128, 64>, 128, 64) : go(k_zgemm_mfma<false, 128, 64>, 128, 64);
+      return conj_a ? go(k_zgemm_mfma<true, 128, 64, MK>, 128, 64) : go(k_zgemm_mfma<false, 128, 64, MK>, 128, 64);
     case ZGEMM_MFMA_64x96:
-      return conj_a ? go(k_zgemm_mfma<true, 64, 96>, 64, 96) : go(k_zgemm_mfma<false, 64, 96>, 64, 96);
+      return conj_a ? go(k_zgemm_mfma<true, 64, 96, MK>, 64, 96) : go(k_zgemm_mfma<false, 64, 96, MK>, 64, 96);
+    case ZGEMM_MFMA_64x64_K32:
+      return conj_a ? go(k_zgemm_mfma<true, 64, 64, 2 * MK>, 64, 64) : go(k_zgemm_mfma<false, 64, 64, 2 * MK>, 64, 64);
     default:
       return hipErrorInvalidValue;
   }
