@@ -62,23 +62,22 @@ __device__ __forceinline__ double rcp_nr(double x) {
 }
 
 // Hermitian 2x2 Jacobi rotation J = [[c, s w], [-s conj(w), c]], w = g/|g|, that
-// zeroes g in [[a, g], [conj(g), b]] via J^dagger H J (Rutishauser's formulas:
-// zeta = (b - a) / 2|g|, t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), c = 1/sqrt(1+t^2)).
+// zeroes g in [[a, g], [conj(g), b]] via J^dagger H J: with zeta = (b - a) / 2|g|,
+// t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)) (Rutishauser), written as
+// t = sign(b - a) 2|g| / (|b - a| + sqrt((b - a)^2 + 4|g|^2)) so that 1/|g| and the
+// square root are independent (two refined estimates in parallel instead of in
+// sequence on the rotation's critical path) and nothing overflows for tiny |g|;
+// c = 1/sqrt(1 + t^2), s = t c.
 __device__ __forceinline__ void jrot(double a, double b, zc g, double& c, double& s, zc& w) {
   const double g2 = zabs2(g);
   if (g2 == 0.0) { c = 1.0; s = 0.0; w = zmk(1.0, 0.0); return; }
+  const double dd = b - a;
+  const double e2 = fma(dd, dd, 4.0 * g2);
   const double inv = rsqrt_nr(g2);
+  const double re = rsqrt_nr(e2);
   w = zmk(g.x * inv, g.y * inv);
-  const double zeta = 0.5 * (b - a) * inv;
-  const double az = fabs(zeta);
-  double tt;
-  if (az > 1e150) {
-    tt = 0.5 / az;
-  } else {
-    const double q = fma(az, az, 1.0);
-    tt = rcp_nr(az + q * rsqrt_nr(q));
-  }
-  if (zeta < 0.0) tt = -tt;
+  double tt = (2.0 * g2 * inv) * rcp_nr(fma(e2, re, fabs(dd)));
+  if (dd < 0.0) tt = -tt;
   c = rsqrt_nr(fma(tt, tt, 1.0));
   s = tt * c;
 }
