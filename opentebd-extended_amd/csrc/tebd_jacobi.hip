@@ -156,9 +156,19 @@ __device__ __forceinline__ void reduce_block(zc (*A)[JC + 1], double (*red)[2][2
 
 constexpr int kInnerSweeps = 30;
 
+// Index pair t (t < JC/2) of inner round rd (rd < JC - 1), lower index first: rr_pair
+// on JC indices, with its reduction mod JC - 1 done by one compare (the argument is
+// below 2 (JC - 1)) instead of a general modulo, which costs several multiplies per
+// use inside the inner rounds.
 __device__ __forceinline__ void inner_pair(int rd, int t, int& p, int& q) {
-  rr_pair(JC, rd, t, p, q);
-  if (p > q) { const int x = p; p = q; q = x; }
+  auto pos = [rd](int k) {
+    int x = k - 1 + rd;
+    x = x >= JC - 1 ? x - (JC - 1) : x;
+    return k == 0 ? 0 : 1 + x;
+  };
+  const int a = pos(t), b = pos(JC - 1 - t);
+  p = a < b ? a : b;
+  q = a < b ? b : a;
 }
 
 // Cyclic Jacobi on A (JC x JC Hermitian, LDS); Q (LDS) accumulates the rotations.
