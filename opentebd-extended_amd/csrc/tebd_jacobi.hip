@@ -40,6 +40,13 @@ __device__ __forceinline__ d4 mfma4(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// k_jacobi_round runs 8 waves: the first four form the Gram matrix; in the inner
+// solve waves 0-3 update one 2x2 block of A per thread and waves 4-7 the rotation
+// accumulator Q, so that no wave issues both (a round costs about the longest wave's
+// instruction stream).
+constexpr int kRoundThreads = 512;
+constexpr int kGramWaves = 4;
+
 // Round-robin ("circle") schedule over nb = 2P blocks: block 0 stays, the others
 // rotate; over nb-1 rounds every pair of blocks meets exactly once.
 __device__ __forceinline__ void rr_pair(int nb, int round, int p, int& bi, int& bj) {
@@ -118,12 +125,14 @@ __device__ __forceinline__ void gram_block(const zc* __restrict__ X, const zc* _
   }
 }
 
-// Sums the four waves' partial 16x16 blocks in a fixed order (bitwise reproducible)
-// and stores rows [ro, ro+16) x cols [co, co+16) of A.  With `diag` the block is a
-// diagonal block of the Hermitian A: only the upper triangle is taken and mirrored.
+// Sums the four Gram waves' partial 16x16 blocks in a fixed order (bitwise
+// reproducible) and stores rows [ro, ro+16) x cols [co, co+16) of A.  With `diag` the
+// block is a diagonal block of the Hermitian A: only the upper triangle is taken and
+// mirrored.  Called by every thread of the workgroup (waves beyond the fourth only
+// take part in the barriers).
 __device__ __forceinline__ void reduce_block(zc (*A)[JC + 1], double (*red)[2][256], d4 gre, d4 gim, int wave,
                                              int lane, int ro, int co, bool diag) {
-  if (wave > 0) {
+  if (wave > 0 && wave < kGramWaves) {
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
       red[wave - 1][0][g * 64 + lane] = gre[g];
@@ -173,10 +182,10 @@ __device__ __forceinline__ void inner_pair(int rd, int t, int& p, int& q) {
 
 // Cyclic Jacobi on A (JC x JC Hermitian, LDS); Q (LDS) accumulates the rotations.
 // A round rotates 16 disjoint index pairs (round-robin order): threads 0..15
-// compute the rotations, then every thread updates one 2x2 block [p_r, q_r] x
-// [p_k, q_k] of A (J^H A J acts on each such block on its own, so the update is in
-// place) and two (row, pair) groups of Q; blocks whose two pairs both stay put are
-// skipped.  A Gram entry is accurate relative to sqrt(A_pp A_qq), so the solve uses
+// compute the rotations, then threads 0..255 update one 2x2 block [p_r, q_r] x
+// [p_k, q_k] of A each (J^H A J acts on each such block on its own, so the update is
+// in place) and threads 256..511 two (row, pair) groups of Q each; blocks whose two
+// pairs both stay put are skipped.  A Gram entry is accurate relative to sqrt(A_pp A_qq), so the solve uses
 // a purely relative test (no rotation when |A_pq| <= 1e-15 sqrt(A_pp A_qq)), sets
 // each annihilated entry to exactly zero and ends when a sweep would rotate nothing.
 // That is decided by testing every pair once after a sweep: a sweep that rotates
@@ -188,7 +197,7 @@ __device__ __forceinline__ bool needs_rotation(zc g, double a, double b) {
 
 __device__ void inner_jacobi(zc (*A)[JC + 1], zc (*Q)[JC + 1], double* rc_c, double* rc_s, zc* rc_w, int* s_round,
                              int* s_sweep, int* s_more, int tid, int max_sweeps) {
-  for (int e = tid; e < JC * JC; e += 256) {
+  for (int e = tid; e < JC * JC; e += kRoundThreads) {
     const int r = e % JC, c = e / JC;
     Q[r][c] = zmk(r == c ? 1.0 : 0.0, 0.0);
   }
@@ -212,7 +221,7 @@ __device__ void inner_jacobi(zc (*A)[JC + 1], zc (*Q)[JC + 1], double* rc_c, dou
       }
       __syncthreads();
       if (!s_round[rd & 1]) continue;
-      {
+      if (tid < 256) {
         const int tr = tid >> 4, tk = tid & 15;
         const double sr = rc_s[tr], sk = rc_s[tk];
         if (sr != 0.0 || sk != 0.0) {
@@ -243,9 +252,10 @@ __device__ void inner_jacobi(zc (*A)[JC + 1], zc (*Q)[JC + 1], double* rc_c, dou
           A[pr][pk] = a00; A[pr][qk] = a01; A[qr][pk] = a10; A[qr][qk] = a11;
         }
       }
+      if (tid >= 256) {
 #pragma unroll
       for (int h = 0; h < 2; ++h) {  // Q_:p <- c Q_:p - s conj(w) Q_:q ; Q_:q <- s w Q_:p + c Q_:q
-        const int grp = tid + 256 * h, k = grp >> 4, tp = grp & 15;
+        const int grp = tid - 256 + 256 * h, k = grp >> 4, tp = grp & 15;
         const double sn = rc_s[tp];
         if (sn == 0.0) continue;
         int pq, qq;
@@ -257,13 +267,14 @@ __device__ void inner_jacobi(zc (*A)[JC + 1], zc (*Q)[JC + 1], double* rc_c, dou
         Q[k][pq] = zsub(zscale(xp, c), zmul(swc, xq));
         Q[k][qq] = zadd(zmul(sw, xp), zscale(xq, c));
       }
+      }
       __syncthreads();
     }
     if (!s_sweep[isw & 1]) break;
     if (isw + 1 < max_sweeps) {  // would the next sweep rotate anything?
       if (tid == 0) *s_more = 0;
       __syncthreads();
-      for (int e = tid; e < JC * JC; e += 256) {
+      for (int e = tid; e < JC * JC; e += kRoundThreads) {
         const int pp = e % JC, qq = e / JC;
         if (pp < qq && needs_rotation(A[pp][qq], A[pp][pp].x, A[qq][qq].x)) *s_more = 1;
       }
@@ -275,7 +286,7 @@ __device__ void inner_jacobi(zc (*A)[JC + 1], zc (*Q)[JC + 1], double* rc_c, dou
 }
 
 template <bool FULL>
-__global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict__ tasks, int sweep, int round,
+__global__ __launch_bounds__(kRoundThreads) void k_jacobi_round(const BondTask* __restrict__ tasks, int sweep, int round,
                                                       double tol_rel, double tol_abs, int* __restrict__ wl_count,
                                                       int2* __restrict__ wl_items, int wl_cap, int inner_sweeps) {
   const BondTask t = tasks[blockIdx.y];
@@ -304,28 +315,38 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict
   const zc* Wi = t.W + (size_t)t.R * (bi * JB);
   const zc* Wj = t.W + (size_t)t.R * (bj * JB);
 
-  // ---- Gram: A = [Wi Wj]^H [Wi Wj]
+  // ---- Gram: A = [Wi Wj]^H [Wi Wj], by the first kGramWaves waves
   {
     d4 gre, gim;
     double nx, ny;
-    gram_block(Wi, Wj, t.R, wave, lane, gre, gim, nx, ny);
+    auto gram = [&](const zc* X, const zc* Y) {
+      if (wave < kGramWaves) {
+        gram_block(X, Y, t.R, wave, lane, gre, gim, nx, ny);
+      } else {
+        gre = d4{0.0, 0.0, 0.0, 0.0};
+        gim = d4{0.0, 0.0, 0.0, 0.0};
+        nx = 0.0;
+        ny = 0.0;
+      }
+    };
+    gram(Wi, Wj);
     reduce_block(A, red, gre, gim, wave, lane, 0, JB, false);
     if (FULL) {
-      gram_block(Wi, Wi, t.R, wave, lane, gre, gim, nx, ny);
+      gram(Wi, Wi);
       reduce_block(A, red, gre, gim, wave, lane, 0, 0, true);
-      gram_block(Wj, Wj, t.R, wave, lane, gre, gim, nx, ny);
+      gram(Wj, Wj);
       reduce_block(A, red, gre, gim, wave, lane, JB, JB, true);
     } else {
       nx += __shfl_xor(nx, 16, 64);
       nx += __shfl_xor(nx, 32, 64);
       ny += __shfl_xor(ny, 16, 64);
       ny += __shfl_xor(ny, 32, 64);
-      if (lane < 16) {
+      if (lane < 16 && wave < kGramWaves) {
         nred[wave][0][lane] = nx;
         nred[wave][1][lane] = ny;
       }
       // within-block entries: zero (the blocks' columns are orthogonal)
-      for (int e = tid; e < 2 * JB * JB; e += 256) {
+      for (int e = tid; e < 2 * JB * JB; e += kRoundThreads) {
         const int blk = e / (JB * JB), r = (e / JB) % JB, c = e % JB;
         if (r != c) A[blk * JB + r][blk * JB + c] = zmk(0.0, 0.0);
       }
@@ -333,7 +354,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict
       if (tid < JC) {
         const int side = tid / JB, col = tid % JB;
         double v = 0.0;
-        for (int w = 0; w < 4; ++w) v += nred[w][side][col];
+        for (int w = 0; w < kGramWaves; ++w) v += nred[w][side][col];
         A[tid][tid] = zmk(v, 0.0);
       }
       __syncthreads();
@@ -346,7 +367,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict
   {
     const double fnorm = sqrt(t.result[2]);  // largest column norm
     int need = 0;
-    for (int e = tid; e < JC * JC; e += 256) {
+    for (int e = tid; e < JC * JC; e += kRoundThreads) {
       const int pp = e % JC, qq = e / JC;
       if (pp >= qq) continue;
       const double g2 = zabs2(A[pp][qq]);
@@ -370,7 +391,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const BondTask* __restrict
   // ---- eigenvectors of A -> Q, to the pair's slot; queue the pair for rotation
   inner_jacobi(A, Q, rc_c, rc_s, rc_w, s_round, s_sweep, &s_more, tid, inner_sweeps);
   zc* qg = t.Qg + (size_t)p * JC * JC;
-  for (int e = tid; e < JC * JC; e += 256) qg[e] = Q[e / JC][e % JC];  // qg[k JC + c] = Q[k][c]
+  for (int e = tid; e < JC * JC; e += kRoundThreads) qg[e] = Q[e / JC][e % JC];  // qg[k JC + c] = Q[k][c]
   if (tid == 0) {
     atomicAdd(&cnt[sweep & 1], 1);
     atomicAdd(&cnt[2], 1);
@@ -484,10 +505,10 @@ hipError_t launch_jacobi_round(const BondTask* d_tasks, int ntasks, int max_pair
                                int2* wl_items, int wl_cap, int inner_sweeps, hipStream_t st) {
   if (inner_sweeps < 1 || inner_sweeps > kInnerSweeps) inner_sweeps = kInnerSweeps;
   if (round == 0)
-    hipLaunchKernelGGL(k_jacobi_round<true>, dim3(max_pairs, ntasks), dim3(256), 0, st, d_tasks, sweep, round,
+    hipLaunchKernelGGL(k_jacobi_round<true>, dim3(max_pairs, ntasks), dim3(kRoundThreads), 0, st, d_tasks, sweep, round,
                        tol_rel, tol_abs, wl_count, wl_items, wl_cap, inner_sweeps);
   else
-    hipLaunchKernelGGL(k_jacobi_round<false>, dim3(max_pairs, ntasks), dim3(256), 0, st, d_tasks, sweep, round,
+    hipLaunchKernelGGL(k_jacobi_round<false>, dim3(max_pairs, ntasks), dim3(kRoundThreads), 0, st, d_tasks, sweep, round,
                        tol_rel, tol_abs, wl_count, wl_items, wl_cap, inner_sweeps);
   return hipGetLastError();
 }
