@@ -155,24 +155,28 @@ struct MfmaThetaCfg {
   static constexpr int KP = (D2 + 3) / 4 * 4;
   static constexpr int NP = BA * BA;
   static constexpr int LDT = TR + 1;
-  static constexpr int AB = 2 * KC * LDT;
+  // middle-bond chunk staged in LDS: 32 (twice the staging of the VALU kernel) halves
+  // the barriers per MFMA and gives the next chunk's loads twice the time; the staging
+  // shares its LDS with the pre-gate tile, which is as large for d = 4 and 8
+  static constexpr int KM = 32;
+  static constexpr int AB = 2 * KM * LDT;
   static constexpr int SM = AB > TR * LDT ? AB : TR * LDT;
-  static_assert((KC * TR) % 256 == 0, "theta chunks are staged 256 elements at a time");
+  static_assert((KM * TR) % 256 == 0, "theta chunks are staged 256 elements at a time");
 };
 
 template <int D>
 __global__ __launch_bounds__(256) void k_theta_gate_mfma(const BondTask* __restrict__ tasks) {
   using C = MfmaThetaCfg<D>;
   constexpr int BA = C::BA, TR = C::TR, NB1 = C::NB1, NBW = C::NBW, D2 = C::D2;
-  constexpr int MP = C::MP, KP = C::KP, NP = C::NP, LDT = C::LDT;
+  constexpr int MP = C::MP, KP = C::KP, NP = C::NP, LDT = C::LDT, KM = C::KM;
   const BondTask t = tasks[blockIdx.z];
   const int a0 = blockIdx.x * BA, b0 = blockIdx.y * BA;
   if (a0 >= t.chil || b0 >= t.chir) return;
 
   __shared__ zc sm[C::SM];     // As, Bs during stage 1; the pre-gate tile T after it
   __shared__ zc sU[MP * KP];   // gate, row-major [ij][kl], zero padded
-  zc* As = sm;                 // [KC][LDT]
-  zc* Bs = sm + KC * LDT;      // [KC][LDT]
+  zc* As = sm;                 // [KM][LDT]
+  zc* Bs = sm + KM * LDT;      // [KM][LDT]
   zc* T = sm;                  // [TR][LDT]
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -193,8 +197,8 @@ __global__ __launch_bounds__(256) void k_theta_gate_mfma(const BondTask* __restr
   const int chil = t.chil, chic = t.chic, chir = t.chir;
   // chunk g0 is loaded into registers while the previous one is multiplied from LDS:
   // element e = tid + 256 u of the A chunk (r = e % TR, g = e / TR) and of the B chunk
-  // (g = e % KC, c = e / KC)
-  constexpr int NL = KC * TR / 256;
+  // (g = e % KM, c = e / KM)
+  constexpr int NL = KM * TR / 256;
   zc stage_a[NL], stage_b[NL];
   auto load_chunk = [&](int g0) {
 #pragma unroll
@@ -207,24 +211,24 @@ __global__ __launch_bounds__(256) void k_theta_gate_mfma(const BondTask* __restr
                                         : zmk(0.0, 0.0);
       }
       {
-        const int g = e % KC, c = e / KC;
+        const int g = e % KM, c = e / KM;
         const int l = c / BA, b = b0 + (c % BA), gg = g0 + g;
         stage_b[u] = (b < chir && gg < chic) ? t.gr[gg + (size_t)chic * (l + (size_t)D * b)] : zmk(0.0, 0.0);
       }
     }
   };
   load_chunk(0);
-  for (int g0 = 0; g0 < chic; g0 += KC) {
+  for (int g0 = 0; g0 < chic; g0 += KM) {
 #pragma unroll
     for (int u = 0; u < NL; ++u) {
       const int e = tid + 256 * u;
       As[(e / TR) * LDT + e % TR] = stage_a[u];
-      Bs[(e % KC) * LDT + e / KC] = stage_b[u];
+      Bs[(e % KM) * LDT + e / KM] = stage_b[u];
     }
     __syncthreads();
-    if (g0 + KC < chic) load_chunk(g0 + KC);
+    if (g0 + KM < chic) load_chunk(g0 + KM);
 #pragma unroll
-    for (int ks = 0; ks < KC / 4; ++ks) {
+    for (int ks = 0; ks < KM / 4; ++ks) {
       const int kk = ks * 4 + lk;
 #pragma unroll
       for (int s = 0; s < NBW; ++s) {
