@@ -40,12 +40,12 @@ __device__ __forceinline__ d4 mfma4(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// k_jacobi_round runs 8 waves: the first four form the Gram matrix; in the inner
-// solve waves 0-3 update one 2x2 block of A per thread and waves 4-7 the rotation
+// k_jacobi_round runs 8 waves: all of them form the Gram matrix; in the inner solve
+// waves 0-3 update one 2x2 block of A per thread and waves 4-7 the rotation
 // accumulator Q, so that no wave issues both (a round costs about the longest wave's
 // instruction stream).
 constexpr int kRoundThreads = 512;
-constexpr int kGramWaves = 4;
+constexpr int kGramWaves = kRoundThreads / 64;
 
 // Round-robin ("circle") schedule over nb = 2P blocks: block 0 stays, the others
 // rotate; over nb-1 rounds every pair of blocks meets exactly once.
@@ -90,10 +90,10 @@ __device__ __forceinline__ void jrot(double a, double b, zc g, double& c, double
 }
 
 // Gram block D = X^H Y (16 x 16) of two 16-column blocks over rows [0, R), as
-// per-wave partial sums: wave w takes rows 8 (w + 4 it) .. +7 in two MFMA k-steps
-// of 4 rows, with the loads of four such row groups issued before their MFMAs (the
-// pass is bound by memory latency).  Also accumulates the lane's partial squared
-// norms of X and Y.
+// per-wave partial sums of kGramWaves waves: wave w takes rows 8 (w + kGramWaves m)
+// .. +7, m = 0, 1, ..., each in two MFMA k-steps of 4 rows, with the loads of four
+// consecutive m issued before their MFMAs (the pass is bound by memory latency).  Also
+// accumulates the lane's partial squared norms of X and Y.
 __device__ __forceinline__ void gram_block(const zc* __restrict__ X, const zc* __restrict__ Y, int R, int wave,
                                            int lane, d4& gre, d4& gim, double& nx, double& ny) {
   constexpr int U = 4;
@@ -104,11 +104,12 @@ __device__ __forceinline__ void gram_block(const zc* __restrict__ X, const zc* _
   ny = 0.0;
   const zc* xc = X + (size_t)R * lr;
   const zc* yc = Y + (size_t)R * lr;
-  for (int r0 = 8 * wave; r0 < R; r0 += 32 * U) {
+  constexpr int PER = 8 * kGramWaves;  // rows per period: 8 of them for each wave
+  for (int r0 = 8 * wave; r0 < R; r0 += PER * U) {
     zc a[2 * U], b[2 * U];
 #pragma unroll
     for (int u = 0; u < 2 * U; ++u) {
-      const int r = r0 + 32 * (u >> 1) + 4 * (u & 1) + lk;
+      const int r = r0 + PER * (u >> 1) + 4 * (u & 1) + lk;
       a[u] = r < R ? xc[r] : zmk(0.0, 0.0);
       b[u] = r < R ? yc[r] : zmk(0.0, 0.0);
     }
@@ -125,11 +126,10 @@ __device__ __forceinline__ void gram_block(const zc* __restrict__ X, const zc* _
   }
 }
 
-// Sums the four Gram waves' partial 16x16 blocks in a fixed order (bitwise
-// reproducible) and stores rows [ro, ro+16) x cols [co, co+16) of A.  With `diag` the
-// block is a diagonal block of the Hermitian A: only the upper triangle is taken and
-// mirrored.  Called by every thread of the workgroup (waves beyond the fourth only
-// take part in the barriers).
+// Sums the Gram waves' partial 16x16 blocks in a fixed order (bitwise reproducible)
+// and stores rows [ro, ro+16) x cols [co, co+16) of A.  With `diag` the block is a
+// diagonal block of the Hermitian A: only the upper triangle is taken and mirrored.
+// Called by every thread of the workgroup.
 __device__ __forceinline__ void reduce_block(zc (*A)[JC + 1], double (*red)[2][256], d4 gre, d4 gim, int wave,
                                              int lane, int ro, int co, bool diag) {
   if (wave > 0 && wave < kGramWaves) {
@@ -144,7 +144,7 @@ __device__ __forceinline__ void reduce_block(zc (*A)[JC + 1], double (*red)[2][2
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
       double xr = gre[g], xi = gim[g];
-      for (int w = 0; w < 3; ++w) {
+      for (int w = 0; w < kGramWaves - 1; ++w) {
         xr += red[w][0][g * 64 + lane];
         xi += red[w][1][g * 64 + lane];
       }
@@ -305,8 +305,8 @@ __global__ __launch_bounds__(kRoundThreads) void k_jacobi_round(const BondTask* 
 
   __shared__ zc A[JC][JC + 1];
   __shared__ zc Q[JC][JC + 1];
-  __shared__ double red[3][2][256];
-  __shared__ double nred[4][2][16];
+  __shared__ double red[kGramWaves - 1][2][256];
+  __shared__ double nred[kGramWaves][2][16];
   __shared__ double rc_c[JC / 2], rc_s[JC / 2];
   __shared__ zc rc_w[JC / 2];
   __shared__ int s_flag, s_round[2], s_sweep[2], s_more;
@@ -315,20 +315,11 @@ __global__ __launch_bounds__(kRoundThreads) void k_jacobi_round(const BondTask* 
   const zc* Wi = t.W + (size_t)t.R * (bi * JB);
   const zc* Wj = t.W + (size_t)t.R * (bj * JB);
 
-  // ---- Gram: A = [Wi Wj]^H [Wi Wj], by the first kGramWaves waves
+  // ---- Gram: A = [Wi Wj]^H [Wi Wj]
   {
     d4 gre, gim;
     double nx, ny;
-    auto gram = [&](const zc* X, const zc* Y) {
-      if (wave < kGramWaves) {
-        gram_block(X, Y, t.R, wave, lane, gre, gim, nx, ny);
-      } else {
-        gre = d4{0.0, 0.0, 0.0, 0.0};
-        gim = d4{0.0, 0.0, 0.0, 0.0};
-        nx = 0.0;
-        ny = 0.0;
-      }
-    };
+    auto gram = [&](const zc* X, const zc* Y) { gram_block(X, Y, t.R, wave, lane, gre, gim, nx, ny); };
     gram(Wi, Wj);
     reduce_block(A, red, gre, gim, wave, lane, 0, JB, false);
     if (FULL) {
