@@ -191,7 +191,8 @@ int tebd_set_lanes(tebd_ctx* ctx, int n);
  * classes take the 64 x 64 MFMA tiles), on = 1 on, on = -1 on and forgets what was
  * tuned.  tebd_autotune_info: out[0] shape classes tuned, out[1] timing launches,
  * out[2 + v] classes that chose variant v (0 VALU 64x64, 1 MFMA 64x64, 2 MFMA
- * 128x64, 3 MFMA 64x96, 4 MFMA 64x64 with k chunks of 32); n <= 7. */
+ * 128x64, 3 MFMA 64x96, 4 MFMA 64x64 with k chunks of 32, 5 MFMA 128x96 on eight
+ * waves); n <= 8. */
 int tebd_set_autotune(tebd_ctx* ctx, int on);
 int tebd_autotune_info(tebd_ctx* ctx, double* out, int n);
 

@@ -538,12 +538,13 @@ uint64_t gemm_key(int max_m, int max_n, int ntasks, bool conj) {
 }
 
 // Batched complex GEMM with the tile variant the autotuner chose for this shape class.
-// The candidates (MFMA 64x64, 128x64 and 64x96 tiles with k chunks of 16, and 64x64 tiles
-// with k chunks of 32) take the same floating-point operations
+// The candidates (MFMA 64x64, 128x64 and 64x96 tiles with k chunks of 16, 64x64 tiles
+// with k chunks of 32, and 128x96 tiles on eight waves) take the same floating-point operations
 // in the same order for every element, so the choice changes the speed, never the
 // result; C = op(A) B is overwritten, so the timing runs leave the same C behind.
 hipError_t tuned_zgemm(tebd_ctx* c, const GemmTask* d_t, int nt, int max_m, int max_n, bool conj) {
-  static const int cand[] = {ZGEMM_MFMA_64x64, ZGEMM_MFMA_128x64, ZGEMM_MFMA_64x96, ZGEMM_MFMA_64x64_K32};
+  static const int cand[] = {ZGEMM_MFMA_64x64, ZGEMM_MFMA_128x64, ZGEMM_MFMA_64x96, ZGEMM_MFMA_64x64_K32,
+                             ZGEMM_MFMA_128x96_W8};
   const uint64_t key = gemm_key(max_m, max_n, nt, conj);
   auto it = c->tuned.find(key);
   if (it != c->tuned.end()) return launch_zgemm(d_t, nt, max_m, max_n, conj, c->st, it->second);

@@ -36,7 +36,8 @@ enum ZgemmVariant {
   ZGEMM_MFMA_128x64 = 2,
   ZGEMM_MFMA_64x96 = 3,
   ZGEMM_MFMA_64x64_K32 = 4,  // 64 x 64 tiles, k chunks of 32
-  ZGEMM_VARIANTS = 5
+  ZGEMM_MFMA_128x96_W8 = 5,  // 128 x 96 tiles on 8 waves
+  ZGEMM_VARIANTS = 6
 };
 hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_n, bool conj_a, hipStream_t st,
                         int variant);

@@ -80,9 +80,10 @@ __global__ __launch_bounds__(256) void k_zgemm(const GemmTask* __restrict__ task
 }
 
 // ---------------------------------------------------------------------------
-// batched complex GEMM on the FP64 matrix cores: TM x TN tiles, 4 waves as 2 x 2,
-// each wave (TM/2) x (TN/2) = (TM/32) x (TN/32) blocks of 16 x 16, complex products as
-// four real v_mfma_f64_16x16x4f64 per k-step, operands staged through LDS in chunks of
+// batched complex GEMM on the FP64 matrix cores: TM x TN tiles, 2 NWR waves as NWR x 2
+// (NWR = 2, or 4 for 128 x 96 tiles), each wave (TM/NWR) x (TN/2) in blocks of 16 x 16,
+// complex products as four real v_mfma_f64_16x16x4f64 per k-step, operands staged
+// through LDS in chunks of
 // KCH = 16 (or 32: half the barriers per product and twice the time for the next
 // chunk's loads to arrive, for twice the LDS).  Every tile shape and chunk length takes
 // the same operations in the same order per element.
@@ -97,9 +98,12 @@ __device__ __forceinline__ d4 mfma4(double a, double b, d4 c) {
 
 constexpr int MK = 16;  // k chunk staged in LDS
 
-template <bool CONJ_A, int TM, int TN, int KCH>
-__global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__ tasks) {
-  constexpr int RB = TM / 32, CB = TN / 32;
+template <bool CONJ_A, int TM, int TN, int KCH, int NWR>
+__global__ __launch_bounds__(128 * NWR) void k_zgemm_mfma(const GemmTask* __restrict__ tasks) {
+  constexpr int NT = 128 * NWR;                  // NWR x 2 waves
+  constexpr int RB = TM / (16 * NWR), CB = TN / 32;
+  static_assert(TM % (16 * NWR) == 0 && TN % 32 == 0 && (KCH * TM) % NT == 0 && (KCH * TN) % NT == 0,
+                "tile shape and thread count must match");
   const GemmTask t = tasks[blockIdx.z];
   const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
   if (m0 >= t.M || n0 >= t.N) return;
@@ -107,7 +111,7 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__
   __shared__ zc Bs[KCH][TN + 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
-  const int wr = wave >> 1, wc = wave & 1;
+  const int wr = wave >> 1, wc = wave & 1;  // wave tile (TM / NWR) x (TN / 2)
   d4 re[RB][CB], im[RB][CB];
 #pragma unroll
   for (int i = 0; i < RB; ++i)
@@ -117,13 +121,13 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__
       im[i][j] = d4{0.0, 0.0, 0.0, 0.0};
     }
   // chunk k0 of op(A) and B is loaded into registers while the previous one is
-  // multiplied from LDS: element e = tid + 256 u of the chunk
-  constexpr int NA = KCH * TM / 256, NB = KCH * TN / 256;
+  // multiplied from LDS: element e = tid + NT u of the chunk
+  constexpr int NA = KCH * TM / NT, NB = KCH * TN / NT;
   zc pa[NA], pb[NB];
   auto load_chunk = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < NA; ++u) {
-      const int e = tid + 256 * u;
+      const int e = tid + NT * u;
       zc v = zmk(0.0, 0.0);
       if (!CONJ_A) {  // A is M x K: contiguous along m
         const int m = m0 + e % TM, k = k0 + e / TM;
@@ -136,7 +140,7 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__
     }
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
-      const int e = tid + 256 * u;
+      const int e = tid + NT * u;
       const int k = k0 + e % KCH, n = n0 + e / KCH;
       pb[u] = (k < t.K && n < t.N) ? t.B[k + (size_t)t.ldb * n] : zmk(0.0, 0.0);
     }
@@ -145,13 +149,13 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__
   for (int k0 = 0; k0 < t.K; k0 += KCH) {
 #pragma unroll
     for (int u = 0; u < NA; ++u) {
-      const int e = tid + 256 * u;
+      const int e = tid + NT * u;
       if (!CONJ_A) As[e / TM][e % TM] = pa[u];
       else As[e % KCH][e / KCH] = pa[u];
     }
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
-      const int e = tid + 256 * u;
+      const int e = tid + NT * u;
       Bs[e % KCH][e / KCH] = pb[u];
     }
     __syncthreads();
@@ -161,7 +165,7 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__
       const int kk = 4 * ks + lk;
       zc a[RB], b[CB];
 #pragma unroll
-      for (int i = 0; i < RB; ++i) a[i] = As[kk][wr * (TM / 2) + 16 * i + lr];
+      for (int i = 0; i < RB; ++i) a[i] = As[kk][wr * (TM / NWR) + 16 * i + lr];
 #pragma unroll
       for (int j = 0; j < CB; ++j) b[j] = Bs[kk][wc * (TN / 2) + 16 * j + lr];
 #pragma unroll
@@ -182,7 +186,7 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(const GemmTask* __restrict__
     for (int j = 0; j < CB; ++j)
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
-        const int m = m0 + wr * (TM / 2) + 16 * i + lk + 4 * g, n = n0 + wc * (TN / 2) + 16 * j + lr;
+        const int m = m0 + wr * (TM / NWR) + 16 * i + lk + 4 * g, n = n0 + wc * (TN / 2) + 16 * j + lr;
         if (m < t.M && n < t.N) {
           zc v = zmk(re[i][j][g] * t.alpha, im[i][j][g] * t.alpha);
           if (t.accumulate) v = zadd(v, t.C[m + (size_t)t.ldc * n]);
@@ -232,22 +236,27 @@ __global__ void k_build_sketch(const BondTask* __restrict__ tasks, const int* __
 
 hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_n, bool conj_a, hipStream_t st,
                         int variant) {
-  auto go = [&](auto kern, int tm, int tn) {
+  auto go = [&](auto kern, int tm, int tn, int threads = 256) {
     dim3 grid((max_m + tm - 1) / tm, (max_n + tn - 1) / tn, ntasks);
-    hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, d_tasks);
+    hipLaunchKernelGGL(kern, grid, dim3(threads), 0, st, d_tasks);
     return hipGetLastError();
   };
   switch (variant) {
     case ZGEMM_VALU_64x64:
       return conj_a ? go(k_zgemm<true>, GT, GT) : go(k_zgemm<false>, GT, GT);
     case ZGEMM_MFMA_64x64:
-      return conj_a ? go(k_zgemm_mfma<true, 64, 64, MK>, 64, 64) : go(k_zgemm_mfma<false, 64, 64, MK>, 64, 64);
+      return conj_a ? go(k_zgemm_mfma<true, 64, 64, MK, 2>, 64, 64) : go(k_zgemm_mfma<false, 64, 64, MK, 2>, 64, 64);
     case ZGEMM_MFMA_128x64:
-      return conj_a ? go(k_zgemm_mfma<true, 128, 64, MK>, 128, 64) : go(k_zgemm_mfma<false, 128, 64, MK>, 128, 64);
+      return conj_a ? go(k_zgemm_mfma<true, 128, 64, MK, 2>, 128, 64)
+                    : go(k_zgemm_mfma<false, 128, 64, MK, 2>, 128, 64);
     case ZGEMM_MFMA_64x96:
-      return conj_a ? go(k_zgemm_mfma<true, 64, 96, MK>, 64, 96) : go(k_zgemm_mfma<false, 64, 96, MK>, 64, 96);
+      return conj_a ? go(k_zgemm_mfma<true, 64, 96, MK, 2>, 64, 96) : go(k_zgemm_mfma<false, 64, 96, MK, 2>, 64, 96);
     case ZGEMM_MFMA_64x64_K32:
-      return conj_a ? go(k_zgemm_mfma<true, 64, 64, 2 * MK>, 64, 64) : go(k_zgemm_mfma<false, 64, 64, 2 * MK>, 64, 64);
+      return conj_a ? go(k_zgemm_mfma<true, 64, 64, 2 * MK, 2>, 64, 64)
+                    : go(k_zgemm_mfma<false, 64, 64, 2 * MK, 2>, 64, 64);
+    case ZGEMM_MFMA_128x96_W8:
+      return conj_a ? go(k_zgemm_mfma<true, 128, 96, MK, 4>, 128, 96, 512)
+                    : go(k_zgemm_mfma<false, 128, 96, MK, 4>, 128, 96, 512);
     default:
       return hipErrorInvalidValue;
   }

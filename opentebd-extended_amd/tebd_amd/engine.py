@@ -253,11 +253,12 @@ class Context:
         self._check(self._lib.tebd_set_autotune(self._h, int(on)))
 
     def autotune_info(self):
-        out = np.zeros(7)
-        self._check(self._lib.tebd_autotune_info(self._h, _p(out), 7))
+        out = np.zeros(8)
+        self._check(self._lib.tebd_autotune_info(self._h, _p(out), 8))
         return {"shape_classes": int(out[0]), "timing_launches": int(out[1]),
                 "chosen": {"valu_64x64": int(out[2]), "mfma_64x64": int(out[3]), "mfma_128x64": int(out[4]),
-                           "mfma_64x96": int(out[5]), "mfma_64x64_k32": int(out[6])}}
+                           "mfma_64x96": int(out[5]), "mfma_64x64_k32": int(out[6]),
+                           "mfma_128x96_w8": int(out[7])}}
 
 
 class Ensemble(Context):
