@@ -127,8 +127,8 @@ __device__ __forceinline__ void gram_partial(LL lload, RL rload, int K, int wave
   }
 }
 
-// Fixed-order sum of the 4 waves' partials into S (row stride lds; bitwise reproducible).
-template <int NP, int NQ>
+// Fixed-order sum of the NW waves' partials into S (row stride lds; bitwise reproducible).
+template <int NP, int NQ, int NW = 4>
 __device__ __forceinline__ void gram_reduce(const d4 (&dre)[NP][NQ], const d4 (&dim)[NP][NQ],
                                             double (*red)[2][256], zc* S, int lds, int wave, int lane) {
   const int lr = lane & 15, lk = lane >> 4;
@@ -148,7 +148,7 @@ __device__ __forceinline__ void gram_reduce(const d4 (&dre)[NP][NQ], const d4 (&
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
           double xr = dre[pb][qb][g], xi = dim[pb][qb][g];
-          for (int w = 0; w < 3; ++w) {
+          for (int w = 0; w < NW - 1; ++w) {
             xr += red[w][0][g * 64 + lane];
             xi += red[w][1][g * 64 + lane];
           }
@@ -619,8 +619,13 @@ __global__ __launch_bounds__(256) void k_qrp_panel(const BondTask* __restrict__ 
 //   MODE 2  the same on the explicit factor Qs (r columns, identity start: column
 //           blocks entirely left of the panel are still unit vectors and are skipped).
 // V has rows j0.. with an implicit unit diagonal; only rows j0.. of the target change.
+// Eight waves: a launch has few workgroups (one per 32 target columns of each bond),
+// so each workgroup's stream over the rows is what the launch waits for.
+constexpr int kReflWaves = 8;
+constexpr int kReflThreads = 64 * kReflWaves;
+
 template <int MODE>
-__global__ __launch_bounds__(256) void k_qrp_reflect(const BondTask* __restrict__ tasks, int arg) {
+__global__ __launch_bounds__(kReflThreads) void k_qrp_reflect(const BondTask* __restrict__ tasks, int arg) {
   const BondTask t = tasks[blockIdx.y];
   int j0, nbp, kb, nc;
   const size_t ld = t.Rb;
@@ -662,16 +667,16 @@ __global__ __launch_bounds__(256) void k_qrp_reflect(const BondTask* __restrict_
     if (c >= nc || i >= K) return zmk(0.0, 0.0);
     return A2[i + ld * c];
   };
-  __shared__ double red[3][2][256];
+  __shared__ double red[kReflWaves - 1][2][256];
   __shared__ zc Y[QB][33];
   __shared__ zc Ts[QB][QB + 1];
-  __shared__ double nsum[4][32];
-  for (int e = tid; e < QB * QB; e += 256) Ts[e % QB][e / QB] = Tp[e];
+  __shared__ double nsum[kReflWaves][32];
+  for (int e = tid; e < QB * QB; e += kReflThreads) Ts[e % QB][e / QB] = Tp[e];
   // ---- Y = V^H A2 (16 x 32), rows split over the waves
   {
     d4 yre[1][2], yim[1][2];
-    gram_partial<1, 2>(vload, aload, K, wave, 4, lane, yre, yim);
-    gram_reduce<1, 2>(yre, yim, red, &Y[0][0], 33, wave, lane);
+    gram_partial<1, 2>(vload, aload, K, wave, kReflWaves, lane, yre, yim);
+    gram_reduce<1, 2, kReflWaves>(yre, yim, red, &Y[0][0], 33, wave, lane);
   }
   // ---- Z = T^H Y (MODE 0) or T Y, held as MFMA A operands Z[4ks + lk][16ct + lr]
   double zre[2][4], zim[2][4];
@@ -695,7 +700,7 @@ __global__ __launch_bounds__(256) void k_qrp_reflect(const BondTask* __restrict_
   for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
     for (int g = 0; g < 4; ++g) ns[ct][g] = 0.0;
-  for (int r0 = 16 * wave; r0 < K; r0 += 64) {
+  for (int r0 = 16 * wave; r0 < K; r0 += 16 * kReflWaves) {
     const int i = r0 + lr;
     zc x[4];
 #pragma unroll
@@ -742,7 +747,11 @@ __global__ __launch_bounds__(256) void k_qrp_reflect(const BondTask* __restrict_
       if (lr == 0) nsum[wave][16 * ct + lk + 4 * g] = v;
     }
   __syncthreads();
-  if (tid < nc) t.qnrm[kb + tid] = nsum[0][tid] + nsum[1][tid] + nsum[2][tid] + nsum[3][tid];
+  if (tid < nc) {
+    double v = 0.0;
+    for (int w = 0; w < kReflWaves; ++w) v += nsum[w][tid];
+    t.qnrm[kb + tid] = v;
+  }
 }
 
 // Two stored panels applied at once (MODE 1: to the Rb-side vectors Yr, MODE 2: to the
@@ -752,7 +761,7 @@ __global__ __launch_bounds__(256) void k_qrp_reflect(const BondTask* __restrict_
 // Each target column block is streamed once per two panels instead of once per panel;
 // the arithmetic is that of k_qrp_reflect<MODE> on the aggregated block.  arg = p.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_qrp_reflect_pair(const BondTask* __restrict__ tasks, int arg) {
+__global__ __launch_bounds__(kReflThreads) void k_qrp_reflect_pair(const BondTask* __restrict__ tasks, int arg) {
   static_assert(MODE == 1 || MODE == 2, "pairs of panels only for the vector passes");
   const BondTask t = tasks[blockIdx.y];
   if (MODE == 1 && !t.qr) return;
@@ -782,14 +791,14 @@ __global__ __launch_bounds__(256) void k_qrp_reflect_pair(const BondTask* __rest
     if (c >= nc || i >= K) return zmk(0.0, 0.0);
     return A2[i + ld * c];
   };
-  __shared__ double red[3][2][256];
+  __shared__ double red[kReflWaves - 1][2][256];
   __shared__ zc Y[2 * QB][33];
   __shared__ zc G[QB][QB + 1];
   __shared__ zc Wt[QB][QB + 1];
   __shared__ zc Ts[2 * QB][2 * QB + 1];
   const zc* T1 = t.Tq + (size_t)(arg - 1) * QB * QB;
   const zc* T2 = t.Tq + (size_t)arg * QB * QB;
-  for (int e = tid; e < QB * QB; e += 256) {
+  for (int e = tid; e < QB * QB; e += kReflThreads) {
     const int a = e % QB, b = e / QB;
     Ts[a][b] = T1[e];
     Ts[QB + a][QB + b] = nb2 > 0 ? T2[e] : zmk(0.0, 0.0);
@@ -798,22 +807,26 @@ __global__ __launch_bounds__(256) void k_qrp_reflect_pair(const BondTask* __rest
   // ---- Y = V^H A2 (32 x 32) and G = V_{p-1}^H V_p (16 x 16), rows split over the waves
   {
     d4 yre[2][2], yim[2][2];
-    gram_partial<2, 2>(vload, aload, K, wave, 4, lane, yre, yim);
-    gram_reduce<2, 2>(yre, yim, red, &Y[0][0], 33, wave, lane);
+    gram_partial<2, 2>(vload, aload, K, wave, kReflWaves, lane, yre, yim);
+    gram_reduce<2, 2, kReflWaves>(yre, yim, red, &Y[0][0], 33, wave, lane);
     d4 gre[1][1], gim[1][1];
-    gram_partial<1, 1>(vload, v2load, K, wave, 4, lane, gre, gim);
-    gram_reduce<1, 1>(gre, gim, red, &G[0][0], QB + 1, wave, lane);
+    gram_partial<1, 1>(vload, v2load, K, wave, kReflWaves, lane, gre, gim);
+    gram_reduce<1, 1, kReflWaves>(gre, gim, red, &G[0][0], QB + 1, wave, lane);
   }
-  // ---- T12 = -T_{p-1} (G T_p)
+  // ---- T12 = -T_{p-1} (G T_p), one entry per thread of the first 256
   {
     const int a = tid % QB, b = tid / QB;
-    zc acc = zmk(0.0, 0.0);
-    for (int m = 0; m < QB; ++m) zfma(acc, G[a][m], Ts[QB + m][QB + b]);
-    Wt[a][b] = acc;
+    if (tid < QB * QB) {
+      zc acc = zmk(0.0, 0.0);
+      for (int m = 0; m < QB; ++m) zfma(acc, G[a][m], Ts[QB + m][QB + b]);
+      Wt[a][b] = acc;
+    }
     __syncthreads();
-    acc = zmk(0.0, 0.0);
-    for (int k = a; k < QB; ++k) zfma(acc, Ts[a][k], Wt[k][b]);
-    Ts[a][QB + b] = zmk(-acc.x, -acc.y);
+    if (tid < QB * QB) {
+      zc acc = zmk(0.0, 0.0);
+      for (int k = a; k < QB; ++k) zfma(acc, Ts[a][k], Wt[k][b]);
+      Ts[a][QB + b] = zmk(-acc.x, -acc.y);
+    }
     __syncthreads();
   }
   // ---- Z = T Y (T upper triangular), held as MFMA A operands Z[4ks + lk][16ct + lr]
@@ -829,7 +842,7 @@ __global__ __launch_bounds__(256) void k_qrp_reflect_pair(const BondTask* __rest
       zim[ct][ks] = acc.y;
     }
   // ---- A2 -= V Z, as (V Z)^T = Z^T V^T: A operand Z^T, B operand V^T
-  for (int r0 = 16 * wave; r0 < K; r0 += 64) {
+  for (int r0 = 16 * wave; r0 < K; r0 += 16 * kReflWaves) {
     const int i = r0 + lr;
     zc x[8];
 #pragma unroll
@@ -920,7 +933,8 @@ hipError_t launch_qrp_panel(const BondTask* d_tasks, int ntasks, int max_rb, int
 
 hipError_t launch_qrp_trailing(const BondTask* d_tasks, int ntasks, int max_cb, int j0, hipStream_t st) {
   if (j0 + QB >= max_cb) return hipSuccess;
-  hipLaunchKernelGGL(k_qrp_reflect<0>, dim3((max_cb - j0 - 1) / 32 + 1, ntasks), dim3(256), 0, st, d_tasks, j0);
+  hipLaunchKernelGGL(k_qrp_reflect<0>, dim3((max_cb - j0 - 1) / 32 + 1, ntasks), dim3(kReflThreads), 0, st, d_tasks,
+                     j0);
   return hipGetLastError();
 }
 
@@ -937,12 +951,12 @@ hipError_t apply_panels(const BondTask* d_tasks, int ntasks, int ncols, int max_
   int p = (max_r + QB - 1) / QB - 1;
   hipError_t e = hipSuccess;
   if (p >= 0 && (p + 1) % 2 == 1) {
-    hipLaunchKernelGGL(k_qrp_reflect<MODE>, grid, dim3(256), 0, st, d_tasks, p);
+    hipLaunchKernelGGL(k_qrp_reflect<MODE>, grid, dim3(kReflThreads), 0, st, d_tasks, p);
     e = hipGetLastError();
     --p;
   }
   for (; p >= 1 && e == hipSuccess; p -= 2) {
-    hipLaunchKernelGGL(k_qrp_reflect_pair<MODE>, grid, dim3(256), 0, st, d_tasks, p);
+    hipLaunchKernelGGL(k_qrp_reflect_pair<MODE>, grid, dim3(kReflThreads), 0, st, d_tasks, p);
     e = hipGetLastError();
   }
   return e;
