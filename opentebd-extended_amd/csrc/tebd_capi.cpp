@@ -35,6 +35,8 @@ constexpr double kTolAbs = 1e-13;   // rounding floor: |A_pq| <= kTolAbs ||theta
 // kEarlyInnerSweeps inner sweeps (the pair is revisited anyway); later ones to convergence.
 constexpr int kEarlySweeps = 4;
 constexpr int kEarlyInnerSweeps = 2;
+constexpr size_t kMaxSweepGraphs = 512;  // executable Jacobi sweep graphs kept per context or lane
+constexpr int kGraphRowQuantum = 64;     // = the rotation kernel's slab height (tebd_jacobi.hip)
 
 int round_up(int x, int m) { return ((x + m - 1) / m) * m; }
 
@@ -272,6 +274,11 @@ int jacobi_sweeps(tebd_ctx* c, BondTask* d_t, const BondTask* h_t, int nb, int P
     rows_w = std::max(rows_w, h_t[i].R);
     if (h_t[i].V) rows_v = std::max(rows_v, h_t[i].NV);
   }
+  // The rotation kernel reads the rows only as a count of 64-row slabs and every slab
+  // checks its task's own row count, so rounding up to whole slabs launches the same
+  // work and lets one captured graph serve every row count of a slab class.
+  rows_w = round_up(rows_w, kGraphRowQuantum);
+  rows_v = round_up(rows_v, kGraphRowQuantum);
   int sweeps = 0;
   bool converged = false;
   while (sweeps < kMaxSweeps && !converged) {
@@ -283,8 +290,10 @@ int jacobi_sweeps(tebd_ctx* c, BondTask* d_t, const BondTask* h_t, int nb, int P
     if (c->graphs && !c->profiling) {
       const std::array<long long, 7> key = {(long long)(uintptr_t)d_t, nb, Pmax, rows_w, rows_v, cls, inner};
       auto it = c->sweep_graphs.find(key);
-      if (it == c->sweep_graphs.end()) {
-        if (c->sweep_graphs.size() >= 256) clear_graphs(c);
+      // A full cache is not emptied while the context runs: executable graphs are
+      // destroyed only when the context (or its lane) is freed, after its stream has
+      // drained.  Sweeps of shapes beyond the cap launch eagerly.
+      if (it == c->sweep_graphs.end() && c->sweep_graphs.size() < kMaxSweepGraphs) {
         hipGraphExec_t ge = capture_sweep(c, d_t, nb, Pmax, cls, inner, rows_w, rows_v);
         if (ge) it = c->sweep_graphs.emplace(key, ge).first;
         else c->graphs = false;  // capture unavailable: launch eagerly from now on
