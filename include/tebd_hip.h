@@ -196,6 +196,14 @@ int tebd_set_lanes(tebd_ctx* ctx, int n);
 int tebd_set_autotune(tebd_ctx* ctx, int on);
 int tebd_autotune_info(tebd_ctx* ctx, double* out, int n);
 
+/* Device memory of a context (no counterpart in the reference, whose workspace is the
+ * module-global SVD arrays of SVDInit, local_operations_module.f90:84-122): out[0] the
+ * per-bond workspace slots, i.e. the bonds one batched launch sequence processes (every
+ * bond of a half-sweep, up to a budget of 48 GiB of slots; longer half-sweeps run in
+ * batches of this many bonds), out[1] bytes per slot, out[2] bytes of the Vidal state
+ * and gates, out[3] out[2] + all slots; n <= 4. */
+int tebd_memory_info(tebd_ctx* ctx, double* out, int n);
+
 /* Blocks until the context's stream is idle. */
 int tebd_sync(tebd_ctx* ctx);
 

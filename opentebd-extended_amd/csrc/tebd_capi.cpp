@@ -35,6 +35,7 @@ constexpr double kTolAbs = 1e-13;   // rounding floor: |A_pq| <= kTolAbs ||theta
 // kEarlyInnerSweeps inner sweeps (the pair is revisited anyway); later ones to convergence.
 constexpr int kEarlySweeps = 4;
 constexpr int kEarlyInnerSweeps = 2;
+constexpr double kSlotBudget = 48.0 * (1ull << 30);  // bytes of per-bond workspace slots per context
 constexpr size_t kMaxSweepGraphs = 512;  // executable Jacobi sweep graphs kept per context or lane
 constexpr int kGraphRowQuantum = 64;     // = the rotation kernel's slab height (tebd_jacobi.hip)
 
@@ -1449,6 +1450,18 @@ void free_all(tebd_ctx* c) {
   if (c->st) (void)hipStreamDestroy(c->st);
 }
 
+// Device bytes of one per-bond workspace slot (every per-slot array tebd_create
+// allocates), for bond matrices of up to Rmax x Cpmax.
+double slot_bytes(const tebd_ctx* c) {
+  const double R = c->Rmax, Cp = c->Cpmax, chi = c->chi_max, nblk = c->nblk_max, z = sizeof(zc);
+  double b = z * (R * Cp + 2.0 * Cp * Cp + R * chi + Cp * chi + Cp) + 8.0 * Cp * 2 + 4.0 * Cp * 2 + 16.0 +
+             z * (double)tq_stride(c) + 16.0 + 4.0 * nblk + 8.0 * nblk * nblk + z * (nblk / 2) * JC * JC +
+             16.0 * (nblk / 2) + 2.0 * 8 * RESULT_STRIDE + 4.0 * TEBD_MLP_INPUTS + 4.0 +
+             2.0 * kStages * sizeof(BondTask);
+  if (c->mode & TEBD_MODE_LEARNED_RANK) b += 2.0 * z * R * Cp + 8.0 * chi + 2.0 * 5 * sizeof(GemmTask) + 8.0;
+  return b;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1469,7 +1482,12 @@ int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, u
   c->chi.assign(L + 1, 1);
   c->Rmax = d * chi_max;
   c->Cpmax = round_up(d * chi_max, JC);
-  c->ntask_max = (mode & TEBD_MODE_BATCHED) ? (L / 2) : 1;
+  c->nblk_max = c->Cpmax / JB;
+  // Batched mode gives every bond of a half-sweep a workspace slot, up to kSlotBudget
+  // bytes of slots; longer half-sweeps (ensembles, very large bond matrices) run in
+  // batches of that many bonds.
+  const int want = (mode & TEBD_MODE_BATCHED) ? std::max(1, L / 2) : 1;
+  c->ntask_max = (int)std::max(1.0, std::min((double)want, std::floor(kSlotBudget / slot_bytes(c))));
   auto bad = [&](hipError_t e) {
     if (e != hipSuccess) {
       free_all(c);
@@ -1488,7 +1506,6 @@ int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, u
   if (bad(hipMalloc(&c->V, sizeof(zc) * (size_t)c->Cpmax * c->Cpmax * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->s, sizeof(double) * (size_t)c->Cpmax * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->perm, sizeof(int) * (size_t)c->Cpmax * nt))) return TEBD_ERR_HIP;
-  c->nblk_max = c->Cpmax / JB;
   if (bad(hipMalloc(&c->X, sizeof(zc) * (size_t)c->Cpmax * c->Cpmax * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->Yr, sizeof(zc) * (size_t)c->Rmax * chi_max * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->Zc, sizeof(zc) * (size_t)c->Cpmax * chi_max * nt))) return TEBD_ERR_HIP;
@@ -1634,10 +1651,14 @@ int tebd_trotter_step(tebd_ctx* c, double* truncerr) {
       gp.push_back(c->gates + (size_t)b * c->d4());
     }
     if (c->mode & TEBD_MODE_BATCHED) {
-      double t = 0.0;
-      e = run_bonds_split(c, links.data(), (int)links.size(), gp.data(), &t);
-      if (e) return e;
-      total += t;
+      // batches of at most ntask_max bonds (one workspace slot each)
+      for (size_t b0 = 0; b0 < links.size(); b0 += c->ntask_max) {
+        const int nb = (int)std::min(links.size() - b0, (size_t)c->ntask_max);
+        double t = 0.0;
+        e = run_bonds_split(c, links.data() + b0, nb, gp.data() + b0, &t);
+        if (e) return e;
+        total += t;
+      }
     } else {
       for (size_t i = 0; i < links.size(); ++i) {
         double t = 0.0;
@@ -1676,21 +1697,26 @@ int tebd_two_site_op(tebd_ctx* c, int link, const double* op2, double* truncerr)
 // same bond of every one of them, left to right, then right to left, as
 // propagation_module.f90:617-634 does for one chain.
 int canonical_form_chains(tebd_ctx* c, int r0, int dr) {
-  std::vector<int> rs;
-  for (int r = r0; r < c->nchain; r += dr) rs.push_back(r);
-  const int n = (int)rs.size();
-  if (n == 0) return TEBD_OK;
-  int e = ensure_cf_workspace(c, n);
-  if (e) return e;
-  std::vector<int> bonds(n);
+  std::vector<int> all;
+  for (int r = r0; r < c->nchain; r += dr) all.push_back(r);
+  if (all.empty()) return TEBD_OK;
+  // chains in groups of at most ntask_max (CanonicalForm workspace sets); each group runs
+  // its chains' full left-to-right and right-to-left sweeps
+  int e = TEBD_OK;
   const int Lc = c->chain_len;
-  for (int k = 0; k < Lc - 1; ++k) {
-    for (int j = 0; j < n; ++j) bonds[j] = rs[j] * Lc + k;
-    if ((e = canonical_form_batch(c, bonds.data(), n))) return e;
-  }
-  for (int k = Lc - 2; k >= 0; --k) {
-    for (int j = 0; j < n; ++j) bonds[j] = rs[j] * Lc + k;
-    if ((e = canonical_form_batch(c, bonds.data(), n))) return e;
+  for (size_t g0 = 0; g0 < all.size(); g0 += c->ntask_max) {
+    const std::vector<int> rs(all.begin() + g0, all.begin() + std::min(all.size(), g0 + c->ntask_max));
+    const int n = (int)rs.size();
+    if ((e = ensure_cf_workspace(c, n))) return e;
+    std::vector<int> bonds(n);
+    for (int k = 0; k < Lc - 1; ++k) {
+      for (int j = 0; j < n; ++j) bonds[j] = rs[j] * Lc + k;
+      if ((e = canonical_form_batch(c, bonds.data(), n))) return e;
+    }
+    for (int k = Lc - 2; k >= 0; --k) {
+      for (int j = 0; j < n; ++j) bonds[j] = rs[j] * Lc + k;
+      if ((e = canonical_form_batch(c, bonds.data(), n))) return e;
+    }
   }
   HIPCHK(c, hipStreamSynchronize(c->st));
   drain_profile(c);
@@ -1905,6 +1931,15 @@ int tebd_autotune_info(tebd_ctx* c, double* out, int n) {
   v[1] = c->tune_trials;
   for (const auto& kv : c->tuned) v[2 + kv.second] += 1;
   for (int i = 0; i < n && i < (int)v.size(); ++i) out[i] = v[i];
+  return TEBD_OK;
+}
+
+int tebd_memory_info(tebd_ctx* c, double* out, int n) {
+  if (!c || !out || n < 0) return TEBD_ERR_ARG;
+  const double state = sizeof(zc) * (double)c->gslab() * c->L + 8.0 * (c->L + 1) * c->chi_max +
+                       sizeof(zc) * (double)c->d4() * c->L;
+  const double v[4] = {(double)c->ntask_max, slot_bytes(c), state, state + slot_bytes(c) * c->ntask_max};
+  for (int i = 0; i < n && i < 4; ++i) out[i] = v[i];
   return TEBD_OK;
 }
 

@@ -240,6 +240,12 @@ class Context:
     def sync(self):
         self._check(self._lib.tebd_sync(self._h))
 
+    def memory_info(self):
+        """{slots, slot_bytes, state_bytes, total_bytes} of the context's device memory."""
+        out = np.zeros(4)
+        self._check(self._lib.tebd_memory_info(self._h, _p(out), 4))
+        return {"slots": int(out[0]), "slot_bytes": out[1], "state_bytes": out[2], "total_bytes": out[3]}
+
     def set_graphs(self, on=True):
         """Replay each Jacobi sweep as a captured hipGraph (while per-launch profiling is off)."""
         self._check(self._lib.tebd_set_graphs(self._h, int(bool(on))))

@@ -25,9 +25,10 @@ def lib():
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
-        # liboracle links MKL's single dynamic library; sequential LAPACK per
-        # call, as the reference links mkl_sequential.
-        os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
+        # liboracle links MKL's single dynamic library with the GNU OpenMP layer; every
+        # LAPACK call runs on one thread (as the reference links mkl_sequential) unless
+        # checker_speed() asks for more.
+        os.environ.setdefault("MKL_THREADING_LAYER", "GNU")
         L = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
         P = ctypes.c_void_p
         I = ctypes.c_int
@@ -40,8 +41,15 @@ def lib():
         L.orc_two_site_rho_all.argtypes = [I, I, I, P, P, P]
         L.orc_one_site_expval.argtypes = [I, I, I, P, P, P, P]
         L.orc_chain_entropy.argtypes = [I, I, I, P, P]
+        L.orc_set_checker_speed.argtypes = [I]
         _LIB = L
     return _LIB
+
+
+def checker_speed(threads):
+    """threads > 0: theta by ZGEMM and ZGESVD with its optimal workspace on that many
+    threads (a checker for large bonds); 0: the reference-faithful mode (default)."""
+    lib().orc_set_checker_speed(int(threads))
 
 
 def _p(a):

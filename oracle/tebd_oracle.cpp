@@ -20,6 +20,11 @@
 //                                               (observables_module.f90:95-117,134-162,
 //                                                704-832,1432-1466,1583-1602).
 //
+// A faster CHECKER mode (orc_set_checker_speed) keeps the same algorithm but forms
+// theta with one ZGEMM and lets ZGESVD take its optimal (blocked) workspace, with
+// LAPACK threads; results agree with the reference-faithful mode to rounding.  The
+// timed CPU baseline never uses it.
+//
 // Storage (identical to the reference's per-site arrays, packed one after another):
 //   Gamma[s] : chi x d x chi complex(8), column-major, element (a,i,b) at a + chi*(i + d*b)
 //   lambda[l]: chi real(8), l = 0..L (reference Lambdas(1..L+1))
@@ -36,8 +41,16 @@ extern "C" void zgesvd_(const char* jobu, const char* jobvt, const int* m, const
                         cplx* a, const int* lda, double* s, cplx* u, const int* ldu,
                         cplx* vt, const int* ldvt, cplx* work, const int* lwork,
                         double* rwork, int* info);
+extern "C" void zgemm_(const char* ta, const char* tb, const int* m, const int* n, const int* k,
+                       const cplx* alpha, const cplx* a, const int* lda, const cplx* b, const int* ldb,
+                       const cplx* beta, cplx* c, const int* ldc);
+extern "C" int MKL_Set_Num_Threads_Local(int nt);  // MKL service call, by value
 
 namespace {
+
+// checker speed (orc_set_checker_speed): 0 = reference-faithful, else fast with this many
+// LAPACK threads per call
+int g_fast_threads = 0;
 
 struct View {
   int L, d, chi;
@@ -52,9 +65,17 @@ struct View {
 // Full SVD exactly as the reference calls it: jobu=jobvt='A', lwork = 5*n.
 int ref_zgesvd(int n, cplx* A, double* S, cplx* U, cplx* VT) {
   int lwork = 5 * n, info = 0;
+  const char job = 'A';
+  if (g_fast_threads > 0) {  // checker mode: LAPACK's own optimal workspace
+    cplx q;
+    int query = -1;
+    std::vector<double> rw(5 * n);
+    zgesvd_(&job, &job, &n, &n, A, &n, S, U, &n, VT, &n, &q, &query, rw.data(), &info);
+    lwork = std::max(lwork, (int)q.real());
+  }
   std::vector<cplx> work(lwork);
   std::vector<double> rwork(5 * n);
-  const char job = 'A';
+  MKL_Set_Num_Threads_Local(g_fast_threads > 0 ? g_fast_threads : 1);
   zgesvd_(&job, &job, &n, &n, A, &n, S, U, &n, VT, &n, work.data(), &lwork, rwork.data(), &info);
   return info;
 }
@@ -70,16 +91,32 @@ int two_site_op(const View& v, int link, const cplx* Uop, double* truncerr) {
   auto th = [&](int a, int i, int j, int b) -> cplx& {
     return theta[a + (size_t)chi * (i + (size_t)d * (j + (size_t)d * b))];
   };
-  std::vector<cplx> rowtmp(chi);
-  for (int b = 0; b < chi; ++b)
-    for (int j = 0; j < d; ++j)
+  if (g_fast_threads > 0) {
+    // checker mode: (lambda_l Gamma_l lambda_c) viewed as a (chi d) x chi matrix times
+    // Gamma_r viewed as chi x (d chi) is theta(a + chi i, j + d b) before lambda_r
+    std::vector<cplx> A(v.gsz());
+    for (int gm = 0; gm < chi; ++gm)
       for (int i = 0; i < d; ++i)
-        for (int a = 0; a < chi; ++a) {
-          cplx acc = 0.0;
-          for (int gm = 0; gm < chi; ++gm)
-            acc += v.g(link, a, i, gm) * l1[gm] * v.g(link + 1, gm, j, b);
-          th(a, i, j, b) = l0[a] * acc * l2[b];
-        }
+        for (int a = 0; a < chi; ++a) A[a + (size_t)chi * (i + (size_t)d * gm)] = l0[a] * v.g(link, a, i, gm) * l1[gm];
+    const int m = chi * d;
+    const cplx one = 1.0, zero = 0.0;
+    MKL_Set_Num_Threads_Local(g_fast_threads);
+    zgemm_("N", "N", &m, &m, &chi, &one, A.data(), &m, v.gam(link + 1), &chi, &zero, theta.data(), &m);
+    for (int b = 0; b < chi; ++b)
+      for (int j = 0; j < d; ++j)
+        for (int i = 0; i < d; ++i)
+          for (int a = 0; a < chi; ++a) th(a, i, j, b) *= l2[b];
+  } else {
+    for (int b = 0; b < chi; ++b)
+      for (int j = 0; j < d; ++j)
+        for (int i = 0; i < d; ++i)
+          for (int a = 0; a < chi; ++a) {
+            cplx acc = 0.0;
+            for (int gm = 0; gm < chi; ++gm)
+              acc += v.g(link, a, i, gm) * l1[gm] * v.g(link + 1, gm, j, b);
+            th(a, i, j, b) = l0[a] * acc * l2[b];
+          }
+  }
   // ThetaOperation_c: Theta'(a,i,j,b) = sum_{k,l} U((i-1)d+j,(k-1)d+l) Theta(a,k,l,b)
   const int d2 = d * d;
   std::vector<cplx> tmp(d2);
@@ -238,6 +275,11 @@ void two_site_rho(const View& v, int s, cplx* rho) {
 }  // namespace
 
 extern "C" {
+
+// Checker speed: threads = 0 restores the reference-faithful mode (the default, and
+// the only mode the timed CPU baseline uses); threads > 0 forms theta by ZGEMM and
+// gives ZGESVD its optimal workspace, with that many LAPACK threads per call.
+void orc_set_checker_speed(int threads) { g_fast_threads = threads > 0 ? threads : 0; }
 
 // One two-site update on bond `link` (1-based, reference convention).
 int orc_two_site_op(int L, int d, int chi, double* G, double* lam, int link, const double* U,

@@ -32,6 +32,16 @@ def test_mixed_trajectory_error_is_bounded(name, mode):
     assert dims.max() <= int(z["chi"])
 
 
+@pytest.mark.parametrize("mode", ["mixed", "batched,learned,mixed"])
+def test_mixed_imaginary_time_error_is_bounded(mode):
+    """Imaginary time with CanonicalFormAll every step against the reference."""
+    z = load_golden("itp_l10_chi8")
+    print(f"[gpu] mixed itp_l10_chi8 mode={mode}", flush=True)
+    worst, dims = run_device(z, mode, check_every=5)
+    print(f"[gpu] itp {mode} errors {worst}", flush=True)
+    assert max(worst.values()) < MIXED_TOL, worst
+
+
 def test_mixed_two_site_op_matches_fp64_to_fp32_rounding():
     """One update with a random non-unitary operator on a random state: the mixed
     theta differs from the FP64 one only by FP32 rounding, so lambda and the

@@ -53,7 +53,7 @@ def test_trajectory_matches_reference(name, mode):
     assert dims.max() <= int(z["chi"]) and dims[0] == 1 and dims[-1] == 1
 
 
-@pytest.mark.parametrize("mode", ["fp64", "batched"])
+@pytest.mark.parametrize("mode", ["fp64", "batched", "learned", "batched,learned"])
 def test_imaginary_time_with_canonical_form_matches_reference(mode):
     """TrotterStep with exp(-dtau H) gates + CanonicalFormAll every step
     (propagation_module.f90:741-743) against the reference's trajectory."""
