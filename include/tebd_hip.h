@@ -121,6 +121,30 @@ int tebd_canonical_form(tebd_ctx* ctx, int link);
  * rho = theta theta^dagger is formed on the device from the resident state. */
 int tebd_expval_two_site(tebd_ctx* ctx, const double* ops, double* out);
 
+/* The 2nd-order OBC Trotter gates of one chain from its nb = L-1 bond Hamiltonians H
+ * ((d*d) x (d*d) complex, column-major, back to back): U(b) = exp(-i t H(b)) with
+ * t = dt/2 on odd bonds (1-based) and dt on even ones, dt = dt_re + i dt_im (imaginary
+ * time: dt = -i dtau).  Host-only (no device call).  Replaces ConstructPropagators /
+ * NEWConstructPropagators and matrix_exponential_c (TEBDtools_module.f90:97-99,
+ * 176-222,1079-1101), which diagonalise H with ZHEEV; here by complex Jacobi rotations. */
+int tebd_construct_propagators(const double* H, int nb, int d, double dt_re, double dt_im, double* U);
+
+/* Imaginary-time ground-state search on the resident state.  Replaces ImagTimeProp and
+ * ImagTimePropSpin (propagation_module.f90:636-876): gates exp(-dtau H) built from the
+ * nh = L-1 bond Hamiltonians H (nchain * (L-1), chain after chain, in an ensemble);
+ * chi stages chi_in and then the context's chi_max (one stage when they are equal),
+ * each starting with every bond cut to the stage's chi (entries past chi dropped, as
+ * CopyGamLam does) and truncating at that chi; in each stage, for j = 1..max_steps:
+ * every steps_for_judge-th j the largest relative change of lambda_l(1) over all bonds
+ * since the last judgement is compared with conv1 (stages below chi_max) or conv2 (the
+ * chi_max stage, or the only stage) and the stage ends when it is smaller; otherwise one
+ * TrotterStep and CanonicalFormAll.  steps[0..3] = {j at which stage 1 ended (max_steps
+ * if it did not converge), 1 if it converged, the same for stage 2}.  The gates of the
+ * context are replaced by the imaginary-time ones.  The reference's printed totals
+ * (TotalNumber, TotalEnergy, TotalOneSite at each judgement) are not formed. */
+int tebd_imag_time_prop(tebd_ctx* ctx, const double* H, int nh, double dtau, int chi_in, int max_steps,
+                        int steps_for_judge, double conv1, double conv2, int* steps);
+
 /* Singular-value cutoff on the normalised weight s^2/||theta||^2.  Default
  * 1e-30, the reference's rule (keep |s| > 1e-15 after normalisation,
  * FormLambda1, local_operations_module.f90:360-375).  Larger cutoffs shrink

@@ -4,6 +4,8 @@
 #include "tebd_device.h"
 #include "tebd_kernels.h"
 
+#include <algorithm>
+
 namespace {
 
 // Out[r + ldo c] = rs(r) cs(c) In[r + ldi c]            (conj_t = 0)
@@ -67,6 +69,28 @@ __global__ void k_cf_lambda(const double* __restrict__ s, int n, double* __restr
   for (int a = threadIdx.x; a < chi_max; a += 256) lam[a] = a < n ? s[a] * inv : 0.0;
 }
 
+// Gamma(a, i, b) of a site slab cut to a < ncl, b < ncr (CopyGamLam,
+// TEBDtools_module.f90:523-553): src holds the chi_l x d x chi_r slab, dst receives the
+// ncl x d x ncr one, both packed with their own leading dimensions.
+__global__ void k_shrink_slab(const zc* __restrict__ src, int cl, int d, zc* __restrict__ dst, int ncl, int ncr) {
+  const size_t n = (size_t)ncl * d * ncr;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int a = (int)(e % ncl);
+    const size_t rest = e / ncl;
+    const int i = (int)(rest % d);
+    const int b = (int)(rest / d);
+    dst[e] = src[a + (size_t)cl * (i + (size_t)d * b)];
+  }
+}
+
+// lambda[l][a] = 0 for chi <= a < chi_max, every bond l < nbonds
+__global__ void k_zero_lambda_tail(double* __restrict__ lam, int nbonds, int chi, int chi_max) {
+  const int w = chi_max - chi;
+  const size_t n = (size_t)nbonds * w;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    lam[(e / w) * chi_max + chi + (e % w)] = 0.0;
+}
+
 }  // namespace
 
 hipError_t launch_scaled_copy(const zc* in, int ldi, zc* out, int ldo, int rows, int cols, const double* rowscale,
@@ -90,5 +114,20 @@ hipError_t launch_cf_xy(const zc* U1, const double* s1, const zc* U2, const doub
 
 hipError_t launch_cf_lambda(const double* s, int n, double* lam, int chi_max, hipStream_t st) {
   hipLaunchKernelGGL(k_cf_lambda, dim3(1), dim3(256), 0, st, s, n, lam, chi_max);
+  return hipGetLastError();
+}
+
+hipError_t launch_shrink_slab(const zc* src, int cl, int d, zc* dst, int ncl, int ncr, hipStream_t st) {
+  const size_t n = (size_t)ncl * d * ncr;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_shrink_slab, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, st, src, cl, d, dst, ncl, ncr);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_lambda_tail(double* lam, int nbonds, int chi, int chi_max, hipStream_t st) {
+  if (chi >= chi_max) return hipSuccess;
+  const size_t n = (size_t)nbonds * (chi_max - chi);
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_zero_lambda_tail, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, st, lam, nbonds, chi, chi_max);
   return hipGetLastError();
 }

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -50,6 +51,9 @@ struct tebd_ctx {
   int chain_len = 0, nchain = 1;
   unsigned mode = 0;
   double cutoff = 1e-30;
+  // chi stage of imaginary-time propagation (tebd_imag_time_prop): the truncation keeps at
+  // most this many values while it is below chi_max (0 = chi_max)
+  int stage_chi = 0;
   hipStream_t st = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // device state
@@ -162,6 +166,7 @@ struct tebd_ctx {
   std::vector<hipEvent_t> event_pool;
 
   size_t gslab() const { return (size_t)chi_max * d * chi_max; }
+  int cap() const { return stage_chi > 0 && stage_chi < chi_max ? stage_chi : chi_max; }
   size_t d4() const { return (size_t)d * d * d * d; }
 };
 
@@ -583,7 +588,7 @@ hipError_t tuned_zgemm(tebd_ctx* c, const GemmTask* d_t, int nt, int max_m, int 
 int sketch_width(tebd_ctx* c, int link) {
   const int chic = c->chi[link + 1];
   const int grow = std::max(4, chic / 8);
-  return std::min(c->chi_max, chic + grow) + c->oversample;
+  return std::min(c->cap(), chic + grow) + c->oversample;
 }
 
 // Process `nb` independent bonds (0-based links) as one batch of tasks.
@@ -621,7 +626,7 @@ void make_full_task(tebd_ctx* c, BondTask& t, int link, const zc* gate, int slot
   t.gl_out = const_cast<zc*>(t.gl);
   t.gr_out = const_cast<zc*>(t.gr);
   t.lc_out = const_cast<double*>(t.lc);
-  t.chi_max = c->chi_max;
+  t.chi_max = c->cap();
 }
 
 // Algorithmic work of the theta + gate kernel on the tasks (profiling only).
@@ -752,7 +757,7 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
       base.chil = chil; base.chic = chic; base.chir = chir; base.d = d;
       base.link = link;
       base.rng_key = link % c->chain_len;
-      base.chi_max = c->chi_max;
+      base.chi_max = c->cap();
       base.s = c->s + (size_t)q * c->Cpmax;
       base.perm = c->perm + (size_t)q * c->Cpmax;
       zc* Wt = c->W + (size_t)q * c->Rmax * c->Cpmax;   // theta, later the left vectors
@@ -1198,6 +1203,7 @@ void lane_pull(tebd_ctx* l, const tebd_ctx* c) {
   l->tuned = c->tuned;
   l->mode = c->mode;
   l->cutoff = c->cutoff;
+  l->stage_chi = c->stage_chi;
   l->have_model = c->have_model;
   l->rank_log = c->rank_log;
   l->tol_missed = c->tol_missed;
@@ -1448,6 +1454,120 @@ void free_all(tebd_ctx* c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->st) (void)hipStreamDestroy(c->st);
+}
+
+// ---- gates on the host: ConstructPropagators -----------------------------------------
+using cplx = std::complex<double>;
+
+// A = V diag(w) V^H for a Hermitian n x n matrix (column-major) by cyclic complex Jacobi
+// rotations; each rotation takes the phase off A_pq and then zeroes it with a real plane
+// rotation, so V stays unitary and w real.
+void herm_eig(int n, std::vector<cplx> A, std::vector<double>& w, std::vector<cplx>& V) {
+  V.assign((size_t)n * n, cplx(0.0));
+  for (int i = 0; i < n; ++i) V[i + (size_t)n * i] = 1.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int q = 0; q < n; ++q)
+      for (int p = 0; p < n; ++p) {
+        const double a2 = std::norm(A[p + (size_t)n * q]);
+        tot += a2;
+        if (p != q) off += a2;
+      }
+    if (off <= 1e-34 * tot) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const cplx apq = A[p + (size_t)n * q];
+        const double g = std::abs(apq);
+        if (g == 0.0) continue;
+        const cplx ph = std::conj(apq) / g;  // D = diag(1, ph) makes the (p, q) entry real
+        const double tau = (A[q + (size_t)n * q].real() - A[p + (size_t)n * p].real()) / (2.0 * g);
+        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+        const double cs = 1.0 / std::sqrt(1.0 + t * t), sn = t * cs;
+        // G = D J, J = [[cs, sn], [-sn, cs]] in the (p, q) plane
+        const cplx gpp = cs, gpq = sn, gqp = -sn * ph, gqq = cs * ph;
+        for (int r = 0; r < n; ++r) {  // A <- A G, V <- V G
+          const cplx ap = A[r + (size_t)n * p], aq = A[r + (size_t)n * q];
+          A[r + (size_t)n * p] = ap * gpp + aq * gqp;
+          A[r + (size_t)n * q] = ap * gpq + aq * gqq;
+          const cplx vp = V[r + (size_t)n * p], vq = V[r + (size_t)n * q];
+          V[r + (size_t)n * p] = vp * gpp + vq * gqp;
+          V[r + (size_t)n * q] = vp * gpq + vq * gqq;
+        }
+        for (int col = 0; col < n; ++col) {  // A <- G^H A
+          const cplx ap = A[p + (size_t)n * col], aq = A[q + (size_t)n * col];
+          A[p + (size_t)n * col] = std::conj(gpp) * ap + std::conj(gqp) * aq;
+          A[q + (size_t)n * col] = std::conj(gpq) * ap + std::conj(gqq) * aq;
+        }
+        A[p + (size_t)n * q] = A[q + (size_t)n * p] = 0.0;
+        A[p + (size_t)n * p] = A[p + (size_t)n * p].real();
+        A[q + (size_t)n * q] = A[q + (size_t)n * q].real();
+      }
+  }
+  w.resize(n);
+  for (int i = 0; i < n; ++i) w[i] = A[i + (size_t)n * i].real();
+}
+
+// The 2nd-order OBC gates of one chain (NEWConstructPropagators,
+// TEBDtools_module.f90:1079-1101, with matrix_exponential_c :176-222):
+// U(b) = exp(-i t H(b)), t = dt/2 on odd bonds (1-based) and dt on even ones; dt complex
+// (imaginary time passes dt = -i dtau).  H, U: nb matrices of n x n complex, column-major.
+void construct_propagators_host(const double* H, int nb, int n, cplx dt, double* U) {
+  std::vector<cplx> A((size_t)n * n), V;
+  std::vector<double> w;
+  for (int b = 0; b < nb; ++b) {
+    const cplx* h = reinterpret_cast<const cplx*>(H) + (size_t)b * n * n;
+    for (size_t e = 0; e < (size_t)n * n; ++e) A[e] = h[e];
+    herm_eig(n, A, w, V);
+    const cplx t = (b % 2 == 0) ? 0.5 * dt : dt;
+    std::vector<cplx> f(n);
+    for (int k = 0; k < n; ++k) f[k] = std::exp(cplx(0.0, -1.0) * t * w[k]);
+    cplx* u = reinterpret_cast<cplx*>(U) + (size_t)b * n * n;
+    for (int col = 0; col < n; ++col)
+      for (int r = 0; r < n; ++r) {
+        cplx acc = 0.0;
+        for (int k = 0; k < n; ++k) acc += V[r + (size_t)n * k] * f[k] * std::conj(V[col + (size_t)n * k]);
+        u[r + (size_t)n * col] = acc;
+      }
+  }
+}
+
+// ---- imaginary-time chi stages ----------------------------------------------------------
+// Cuts every bond to at most chi values, as the reference's copy of the state into chi-sized
+// arrays does at the start of a stage (CopyGamLam, TEBDtools_module.f90:523-553: entries past
+// chi dropped, lambda not renormalised), and caps later truncations at chi.
+int set_stage(tebd_ctx* c, int chi) {
+  const int L = c->L, d = c->d;
+  std::vector<int> nd(c->chi);
+  bool cut = false;
+  for (int l = 0; l <= L; ++l) {
+    nd[l] = std::min(c->chi[l], chi);
+    cut = cut || nd[l] != c->chi[l];
+  }
+  if (cut) {
+    zc* tmp = c->W;  // slot 0 holds Rmax x Cpmax >= chi_max d chi_max entries
+    for (int s = 0; s < L; ++s) {
+      if (nd[s] == c->chi[s] && nd[s + 1] == c->chi[s + 1]) continue;
+      zc* slab = c->gam + (size_t)s * c->gslab();
+      HIPCHK(c, hipMemcpyAsync(tmp, slab, sizeof(zc) * (size_t)c->chi[s] * d * c->chi[s + 1], hipMemcpyDeviceToDevice,
+                               c->st));
+      HIPCHK(c, launch_shrink_slab(tmp, c->chi[s], d, slab, nd[s], nd[s + 1], c->st));
+    }
+    HIPCHK(c, launch_zero_lambda_tail(c->lam, L + 1, chi, c->chi_max, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->chi = nd;
+    for (int l = 0; l <= L; ++l) c->last_rank[l] = std::min(c->last_rank[l], chi);
+  }
+  c->stage_chi = chi;
+  return TEBD_OK;
+}
+
+// lambda_l(1) of every bond l = 0..L (the quantity ImagTimeProp's convergence test follows).
+int largest_lambdas(tebd_ctx* c, std::vector<double>& out) {
+  out.resize(c->L + 1);
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy2D(out.data(), sizeof(double), c->lam, sizeof(double) * c->chi_max, sizeof(double), c->L + 1,
+                        hipMemcpyDeviceToHost));
+  return TEBD_OK;
 }
 
 // Device bytes of one per-bond workspace slot (every per-slot array tebd_create
@@ -1932,6 +2052,61 @@ int tebd_autotune_info(tebd_ctx* c, double* out, int n) {
   for (const auto& kv : c->tuned) v[2 + kv.second] += 1;
   for (int i = 0; i < n && i < (int)v.size(); ++i) out[i] = v[i];
   return TEBD_OK;
+}
+
+int tebd_construct_propagators(const double* H, int nb, int d, double dt_re, double dt_im, double* U) {
+  if (!H || !U || nb < 1 || d < 2 || d > 8) return TEBD_ERR_ARG;
+  construct_propagators_host(H, nb, d * d, cplx(dt_re, dt_im), U);
+  return TEBD_OK;
+}
+
+int tebd_imag_time_prop(tebd_ctx* c, const double* H, int nh, double dtau, int chi_in, int max_steps,
+                        int steps_for_judge, double conv1, double conv2, int* steps) {
+  int e = ensure_state(c, false);
+  if (e) return e;
+  const int per = c->chain_len - 1, n = c->d * c->d;
+  if (!H || nh != c->nchain * per || !(dtau > 0.0) || chi_in < 1 || chi_in > c->chi_max || max_steps < 0 ||
+      steps_for_judge < 1 || !(conv1 > 0.0) || !(conv2 > 0.0) || !steps)
+    return fail(c, TEBD_ERR_ARG, "imag_time_prop: bad arguments");
+  // Uitp = ConstructPropagators(H, -i dtau), chain by chain (propagation_module.f90:656-662)
+  std::vector<double> U((size_t)2 * n * n * nh);
+  for (int r = 0; r < c->nchain; ++r)
+    construct_propagators_host(H + (size_t)2 * n * n * per * r, per, n, cplx(0.0, -dtau),
+                               U.data() + (size_t)2 * n * n * per * r);
+  if ((e = tebd_set_gates(c, U.data(), nh))) return e;
+  // stages chi_in and chi_max (one when they are equal), the tighter criterion on the last
+  // (propagation_module.f90:663-690)
+  std::vector<int> stages = {c->chi_max};
+  if (chi_in != c->chi_max) stages = {chi_in, c->chi_max};
+  for (int k = 0; k < 4; ++k) steps[k] = 0;
+  std::vector<double> before, after;
+  for (size_t k = 0; k < stages.size(); ++k) {
+    const int chi = stages[k];
+    const double conv = (chi_in == c->chi_max || chi == c->chi_max) ? conv2 : conv1;
+    if ((e = set_stage(c, chi))) break;
+    if ((e = largest_lambdas(c, before))) break;
+    int j = 1;
+    bool converged = false;
+    for (; j <= max_steps; ++j) {
+      if (j % steps_for_judge == 0) {  // the largest relative change of any bond's lambda_1 (:704-738)
+        if ((e = largest_lambdas(c, after))) break;
+        double gap = 0.0;
+        for (size_t l = 0; l < after.size(); ++l) gap = std::max(gap, std::fabs((before[l] - after[l]) / before[l]));
+        if (gap < conv) {
+          converged = true;
+          break;
+        }
+        before = after;
+      }
+      double tr = 0.0;  // TrotterStep, then CanonicalFormAll (:741-743)
+      if ((e = tebd_trotter_step(c, &tr)) || (e = tebd_canonical_form_all(c))) break;
+    }
+    if (e) break;
+    steps[2 * k] = converged ? j : max_steps;
+    steps[2 * k + 1] = converged ? 1 : 0;
+  }
+  c->stage_chi = 0;
+  return e;
 }
 
 int tebd_memory_info(tebd_ctx* c, double* out, int n) {

@@ -52,3 +52,5 @@ hipError_t launch_perm_cols(const zc* src, int lds, zc* dst, int ldd, int rows, 
 hipError_t launch_cf_xy(const zc* U1, const double* s1, const zc* U2, const double* s2, const double* lam1, zc* X,
                         zc* Y, int n, hipStream_t st);
 hipError_t launch_cf_lambda(const double* s, int n, double* lam, int chi_max, hipStream_t st);
+hipError_t launch_shrink_slab(const zc* src, int cl, int d, zc* dst, int ncl, int ncr, hipStream_t st);
+hipError_t launch_zero_lambda_tail(double* lam, int nbonds, int chi, int chi_max, hipStream_t st);

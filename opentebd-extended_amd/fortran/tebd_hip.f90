@@ -68,6 +68,22 @@ INTERFACE
     REAL(c_double) :: truncerr
   END FUNCTION tebd_two_site_op
 
+  INTEGER(c_int) FUNCTION tebd_construct_propagators(H, nb, d, dt_re, dt_im, U) &
+      BIND(C, name="tebd_construct_propagators")
+    IMPORT :: c_ptr, c_int, c_double
+    TYPE(c_ptr), VALUE :: H, U
+    INTEGER(c_int), VALUE :: nb, d
+    REAL(c_double), VALUE :: dt_re, dt_im
+  END FUNCTION tebd_construct_propagators
+
+  INTEGER(c_int) FUNCTION tebd_imag_time_prop(ctx, H, nh, dtau, chi_in, max_steps, steps_for_judge, conv1, conv2, &
+                                              steps) BIND(C, name="tebd_imag_time_prop")
+    IMPORT :: c_ptr, c_int, c_double
+    TYPE(c_ptr), VALUE :: ctx, H, steps
+    INTEGER(c_int), VALUE :: nh, chi_in, max_steps, steps_for_judge
+    REAL(c_double), VALUE :: dtau, conv1, conv2
+  END FUNCTION tebd_imag_time_prop
+
   INTEGER(c_int) FUNCTION tebd_canonical_form_all(ctx) BIND(C, name="tebd_canonical_form_all")
     IMPORT :: c_ptr, c_int
     TYPE(c_ptr), VALUE :: ctx

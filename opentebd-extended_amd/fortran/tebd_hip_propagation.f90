@@ -7,9 +7,13 @@
 !   TrotterStep(Udt, Gammas, Lambdas, totalTruncerr)
 !                                             propagation_module.f90:42-114
 !   CanonicalFormAll(Gammas, Lambdas)         propagation_module.f90:617-634
+!   ImagTimeProp(H, Gammas, Lambdas, chiIn[, intDegFree]), ImagTimePropSpin(H, Gammas,
+!   Lambdas, chiIn)                           propagation_module.f90:636-876
+!   ConstructPropagators(H, U, dt)            TEBDtools_module.f90:97-99,1079-1101
 ! on the reference's own types (TYPE(tensor), TYPE(vector), TYPE(matrix),
 ! TEBDtools_module.f90:31-45) and globals (systemSize, localSize, trotterOrder,
-! BoundaryCond from system_parameters).  Build it after the reference's
+! BoundaryCond, and for imaginary time chiMax, dtITP, maxITPsteps, stepsForJudge,
+! convCriterion1/2 from system_parameters).  Build it after the reference's
 ! system_parameters and TEBDtools_module and link libtebdhip.so; a program that
 ! also needs the reference's own procedures of the same names renames one set
 ! on USE (e.g. USE tebd_hip_propagation, TrotterStepHIP => TrotterStep).
@@ -24,12 +28,14 @@
 ! Lambdas) before reading it on the host and tebd_hip_to_device after changing it.
 MODULE tebd_hip_propagation
 USE, INTRINSIC :: iso_c_binding
-USE system_parameters, ONLY: rKind, systemSize, localSize, trotterOrder, BoundaryCond
+USE system_parameters, ONLY: rKind, systemSize, localSize, trotterOrder, BoundaryCond, chiMax, dtITP, &
+                             maxITPsteps, stepsForJudge, convCriterion1, convCriterion2
 USE TEBDtools_module, ONLY: tensor, vector, matrix
 USE tebd_hip
 IMPLICIT NONE
 PRIVATE
-PUBLIC :: SVDInit, SVDFinish, TwoSiteOp, TrotterStep, CanonicalFormAll
+PUBLIC :: SVDInit, SVDFinish, TwoSiteOp, TrotterStep, CanonicalFormAll, ImagTimeProp, ImagTimePropSpin, &
+          ConstructPropagators
 PUBLIC :: tebd_hip_resident, tebd_hip_mode, tebd_hip_to_host, tebd_hip_to_device, tebd_hip_context
 
 TYPE(c_ptr), SAVE :: ctx = c_null_ptr
@@ -152,5 +158,71 @@ SUBROUTINE CanonicalFormAll(Gammas, Lambdas)
   CALL tebd_check(ctx, tebd_canonical_form_all(ctx), 'CanonicalFormAll')
   CALL after_call(Gammas, Lambdas)
 END SUBROUTINE CanonicalFormAll
+
+! Udt(i) = exp(-i t H(i)), t = dt/2 on odd bonds and dt on even ones (2nd-order OBC), as
+! the reference's ConstructPropagators; U must be allocated like H.
+SUBROUTINE ConstructPropagators(H, U, dt)
+  TYPE(matrix), POINTER :: H(:), U(:)
+  COMPLEX(KIND=rKind), INTENT(IN) :: dt
+  COMPLEX(c_double_complex), ALLOCATABLE, TARGET :: hb(:, :, :), ub(:, :, :)
+  INTEGER :: i, nb, d2
+  nb = SIZE(H)
+  d2 = SIZE(H(1)%m, 1)
+  ALLOCATE(hb(d2, d2, nb), ub(d2, d2, nb))
+  DO i = 1, nb
+    hb(:, :, i) = H(i)%m
+  END DO
+  CALL tebd_check(ctx, tebd_construct_propagators(c_loc(hb), INT(nb, c_int), INT(localSize, c_int), &
+                  REAL(dt, c_double), AIMAG(dt), c_loc(ub)), 'ConstructPropagators')
+  DO i = 1, nb
+    U(i)%m = ub(:, :, i)
+  END DO
+END SUBROUTINE ConstructPropagators
+
+! Imaginary-time ground-state search with the reference's chi stages (chiIn, then chiMax)
+! and its convergence test on every bond's largest lambda, run on the device.  On return
+! GammasOuter / LambdasOuter are reallocated at chiMax and hold the result, as the
+! reference leaves them.  intDegFree only selects what the reference prints at each
+! judgement and is not used.
+SUBROUTINE ImagTimeProp(H, GammasOuter, LambdasOuter, chiIn, intDegFree)
+  TYPE(matrix), POINTER :: H(:)
+  TYPE(tensor), POINTER :: GammasOuter(:)
+  TYPE(vector), POINTER :: LambdasOuter(:)
+  INTEGER, INTENT(INOUT) :: chiIn
+  INTEGER, INTENT(IN), OPTIONAL :: intDegFree
+  COMPLEX(c_double_complex), ALLOCATABLE, TARGET :: hb(:, :, :)
+  INTEGER(c_int), TARGET :: steps(4)
+  INTEGER :: i, nb, d2
+  IF (.NOT. c_associated(ctx) .OR. ctx_chi /= chiMax) CALL SVDInit(chiMax)
+  CALL tebd_hip_to_device(GammasOuter, LambdasOuter)
+  nb = SIZE(H)
+  d2 = SIZE(H(1)%m, 1)
+  ALLOCATE(hb(d2, d2, nb))
+  DO i = 1, nb
+    hb(:, :, i) = H(i)%m
+  END DO
+  CALL tebd_check(ctx, tebd_imag_time_prop(ctx, c_loc(hb), INT(nb, c_int), REAL(dtITP, c_double), &
+                  INT(chiIn, c_int), INT(maxITPsteps, c_int), INT(stepsForJudge, c_int), &
+                  REAL(convCriterion1, c_double), REAL(convCriterion2, c_double), c_loc(steps)), 'ImagTimeProp')
+  DO i = 1, SIZE(GammasOuter)
+    DEALLOCATE(GammasOuter(i)%t)
+    ALLOCATE(GammasOuter(i)%t(chiMax, localSize, chiMax))
+  END DO
+  DO i = 1, SIZE(LambdasOuter)
+    DEALLOCATE(LambdasOuter(i)%v)
+    ALLOCATE(LambdasOuter(i)%v(chiMax))
+  END DO
+  CALL tebd_hip_to_host(GammasOuter, LambdasOuter)
+  IF (PRESENT(intDegFree)) CONTINUE
+END SUBROUTINE ImagTimeProp
+
+! ImagTimePropSpin differs from ImagTimeProp only in what it prints at each judgement.
+SUBROUTINE ImagTimePropSpin(H, GammasOuter, LambdasOuter, chiIn)
+  TYPE(matrix), POINTER :: H(:)
+  TYPE(tensor), POINTER :: GammasOuter(:)
+  TYPE(vector), POINTER :: LambdasOuter(:)
+  INTEGER, INTENT(INOUT) :: chiIn
+  CALL ImagTimeProp(H, GammasOuter, LambdasOuter, chiIn)
+END SUBROUTINE ImagTimePropSpin
 
 END MODULE tebd_hip_propagation

@@ -351,27 +351,36 @@ class Ensemble(Context):
         return out
 
 
-def imag_time_prop(ctx, H, dtau, max_steps, steps_for_judge=10, conv_criterion=1e-6):
-    """Imaginary-time ground-state search on the device.
+def construct_propagators(H, dt):
+    """ConstructPropagators (TEBDtools_module.f90:1079-1101) by the library's host routine:
+    exp(-i t H(b)) with t = dt/2 on odd bonds (1-based) and dt on even ones; dt complex
+    (imaginary time: dt = -i dtau).  H: (L-1, d^2, d^2) -> U of the same shape."""
+    H = np.asarray(H, np.complex128)
+    nb, n = H.shape[0], H.shape[1]
+    d = int(round(np.sqrt(n)))
+    h = np.ascontiguousarray(np.transpose(H, (0, 2, 1)))
+    u = np.zeros_like(h)
+    dt = complex(dt)
+    rc = _lib.lib().tebd_construct_propagators(_p(h), nb, d, dt.real, dt.imag, _p(u))
+    if rc != 0:
+        raise TEBDError(rc, "tebd_construct_propagators failed")
+    return np.transpose(u, (0, 2, 1))
 
-    Follows ImagTimeProp's inner loop (propagation_module.f90:636-760): gates
-    exp(-dtau H) from ConstructPropagators with dt = -i dtau (:656-662); every
-    ``steps_for_judge`` steps the relative change of each bond's largest lambda
-    is compared with ``conv_criterion`` (:704-738); otherwise one TrotterStep
-    followed by CanonicalFormAll (:741-743).  The reference's chi stages
-    (chiMin -> chiMax) are separate contexts here.  Returns the steps taken.
-    """
-    from . import models
 
-    ctx.set_gates(models.construct_propagators(H, -1j * dtau))
-    before = ctx.lambdas()[:, 0].copy()
-    for j in range(1, max_steps + 1):
-        if j % steps_for_judge == 0:
-            after = ctx.lambdas()[:, 0]
-            gap = np.abs((before - after) / np.where(before != 0.0, before, 1.0))
-            if gap.max() < conv_criterion:
-                return j
-            before = after.copy()
-        ctx.trotter_step()
-        ctx.canonical_form_all()
-    return max_steps
+def imag_time_prop(ctx, H, dtau, chi_in=None, max_steps=2000, steps_for_judge=100, conv1=1e-5, conv2=1e-6):
+    """ImagTimeProp (propagation_module.f90:636-760) on the device: gates exp(-dtau H),
+    chi stages chi_in -> ctx.chi_max (each starting with every bond cut to that chi), and
+    in each stage TrotterStep + CanonicalFormAll until, at a multiple of steps_for_judge,
+    the largest relative change of any bond's largest lambda is below conv1 (stages below
+    chi_max) or conv2 (the chi_max stage).  The defaults are the reference's
+    system_parameters (maxITPsteps, stepsForJudge, convCriterion1/2).  H: (L-1, d^2, d^2),
+    or (nchain, L-1, d^2, d^2) for an Ensemble.  Returns [(steps, converged), ...] per stage."""
+    H = np.asarray(H, np.complex128)
+    n = H.shape[-1]
+    h = np.ascontiguousarray(np.transpose(H.reshape(-1, n, n), (0, 2, 1)))
+    steps = np.zeros(4, np.int32)
+    chi_in = ctx.chi_max if chi_in is None else int(chi_in)
+    ctx._check(ctx._lib.tebd_imag_time_prop(ctx._h, _p(h), h.shape[0], float(dtau), chi_in, int(max_steps),
+                                            int(steps_for_judge), float(conv1), float(conv2), _p(steps)))
+    nst = 1 if chi_in == ctx.chi_max else 2
+    return [(int(steps[2 * k]), bool(steps[2 * k + 1])) for k in range(nst)]

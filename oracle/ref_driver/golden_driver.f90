@@ -22,8 +22,10 @@
 !   dtre dtim nsteps canon    complex dt passed to ConstructPropagators,
 !                             canon=1 runs CanonicalFormAll after each step
 !   (re,im) coefficients of the product state, d per site, L sites
-!   mode                      0 = trajectory, 1 = single TwoSiteOp on a given state
+!   mode                      0 = trajectory, 1 = single TwoSiteOp on a given state,
+!                             2 = ImagTimeProp (propagation_module.f90:636-760) with chi = chiMax
 !   [mode 1: link, then Gamma(chi,d,chi) re im for each site, Lambda(chi) for L+1 links]
+!   [mode 2: chiIn dtITP maxITPsteps stepsForJudge convCriterion1 convCriterion2]
 PROGRAM golden_driver
 USE system_parameters
 USE TEBDtools_module
@@ -42,7 +44,7 @@ COMPLEX(KIND=rKind), ALLOCATABLE :: carray(:,:)
 REAL(KIND=rKind), ALLOCATABLE :: field(:), expl(:), expn(:), buf(:)
 REAL(KIND=rKind) :: c1, c2, c3, dtre, dtim, err, energy, totTrunc
 REAL(KIND=rKind), ALLOCATABLE :: sz(:,:), nop(:,:)
-INTEGER :: model, L, param, chi, nsteps, canon, i, j, k, step, nf, mode, link
+INTEGER :: model, L, param, chi, nsteps, canon, i, j, k, step, nf, mode, link, chiIn
 INTEGER :: a, b, dd
 CHARACTER(len=256) :: jobfile
 
@@ -184,6 +186,16 @@ IF (mode == 1) THEN
 END IF
 
 ALLOCATE(expl(L), expn(L))
+IF (mode == 2) THEN
+  ! imaginary-time ground state through the reference's own driver routine; it prints its
+  ! convergence judgements ("ITP step j ...", "Percent difference ...") itself
+  READ(11,*) chiIn, dtITP, maxITPsteps, stepsForJudge, convCriterion1, convCriterion2
+  CALL SVDFinish()
+  CALL ImagTimeProp(H, Gammas, Lambdas, chiIn)
+  CALL PrintObs(0, 0.0_rKind)
+  CALL PrintState()
+  STOP
+END IF
 totTrunc = 0.0_rKind
 CALL PrintObs(0, 0.0_rKind)
 DO step=1,nsteps

@@ -66,6 +66,40 @@ def run(job):
     return parse(out.stdout)
 
 
+def run_itp(job):
+    """A mode-2 job (ImagTimeProp): the reference's own convergence printout -- one
+    "ITP step j ..." and one "Percent difference t convergence Criterion c" line per
+    judgement -- split into stages where j restarts, plus the usual blocks."""
+    import re
+
+    out = subprocess.run([DRIVER, job], check=True, capture_output=True, text=True).stdout
+    num = re.compile(r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eEdD][-+]?\d+)?")
+    judged, rest = [], []
+    in_judgement = False  # list-directed output wraps the "ITP step j" line at varying places
+    for line in out.splitlines():
+        t = line.strip()
+        if t.startswith("ITP step j"):
+            judged.append([int(num.findall(t)[0])])
+            in_judgement = True
+        elif t.startswith("Percent difference"):
+            v = [float(x.replace("D", "E").replace("d", "e")) for x in num.findall(t)]
+            judged[-1] += v[:2]
+            in_judgement = False
+        elif in_judgement or t.startswith("number") or t.startswith("Sx"):
+            continue  # the wrapped tail of a judgement line, and the printed totals
+        else:
+            rest.append(line)
+    stages, cur = [], []
+    for j in judged:
+        if cur and j[0] <= cur[-1][0]:
+            stages.append(cur)
+            cur = []
+        cur.append(j)
+    if cur:
+        stages.append(cur)
+    return stages, parse("\n".join(rest))
+
+
 def collect(blocks, L, d, chi):
     d2 = d * d
     res = {"H": [], "U": [], "energy": [], "trunc": [], "sz": [], "n": [], "S": [],
@@ -199,5 +233,39 @@ def main():
     print("two_site_op_l4 trunc=", res["trunc"])
 
 
+def itp_stages():
+    """ImagTimeProp with chi stages chiIn=4 -> chiMax=8 (propagation_module.f90:636-760):
+    spin-1/2 Heisenberg L=10, h_i~U[-1,1] seed 5, equal-weight product start, dtITP=0.2,
+    stepsForJudge=10, convCriterion1=1e-4, convCriterion2=1e-5, maxITPsteps=600.  Stored: for every stage
+    the judgements (j, largest relative change of lambda_1, criterion), and the final
+    state's observables, lambdas and Gammas."""
+    L, chi, chi_in = 10, 8, 4
+    hf = np.random.default_rng(5).uniform(-1.0, 1.0, L)
+    c = np.full((L, 2), 1.0 / np.sqrt(2.0), np.complex128)
+    params = dict(chi_in=chi_in, dtau=0.2, max_steps=600, judge=10, conv1=1e-4, conv2=1e-5)
+    tmp = tempfile.mkdtemp()
+    path = os.path.join(tmp, "itp_stages.txt")
+    extra = f"{chi_in} {params['dtau']!r} {params['max_steps']} {params['judge']} {params['conv1']!r} {params['conv2']!r}\n"
+    write_job(path, 1, L, 1, chi, [1.0, 1.0, 1.0], hf, 0j, 0, 0, c, mode=2, extra=extra)
+    stages, blocks = run_itp(path)
+    res = collect(blocks, L, 2, chi)
+    steps = [s[-1][0] for s in stages]
+    conv = [bool(s[-1][1] < s[-1][2]) for s in stages]
+    # the last judgement of each stage and the one before must not sit at the criterion (the
+    # device's lambdas agree with the reference's to ~1e-13, far inside this margin)
+    for s in stages:
+        for j, t, cr in s[-2:]:
+            assert abs(t / cr - 1.0) > 1e-4, (j, t, cr)
+    meta = dict(model=1, L=L, param=1, chi=chi, d=2, consts=np.ones(3), field=hf, coeffs=c,
+                stage_steps=np.array(steps), stage_converged=np.array(conv),
+                judgements=np.array([[k, j, t, cr] for k, s in enumerate(stages) for j, t, cr in s]),
+                **{k: np.array(v) for k, v in params.items()})
+    np.savez_compressed(os.path.join(HERE, "itp_stages_l10.npz"), **meta, **res)
+    print("itp_stages_l10 stages", steps, conv, "energy", res["energy"][-1])
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "itp_stages":
+        itp_stages()
+    else:
+        main()

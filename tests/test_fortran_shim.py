@@ -1,37 +1,98 @@
-"""The Fortran iso_c_binding boundary.
+"""The Fortran iso_c_binding boundary (opentebd-extended_amd/fortran).
 
-CPU: the standalone binding module compiles with amdflang.
-GPU: tests/fortran/shim_vs_reference.f90, built against the reference's own
-Fortran modules (test infrastructure in oracle/_ref, made by build() where the
-reference is present), runs the reference's TrotterStep / CanonicalFormAll and
-the HIP-backed ones from tebd_hip_propagation on the same chains and compares
-the reference's own observables.
+CPU: the binding module compiles with amdflang; where the reference's modules have
+been compiled (oracle/_ref, this container only) the shim also compiles against the
+reference's own system_parameters and TEBDtools_module, i.e. it is a drop-in for
+host code built on them.
+GPU: tests/fortran/shim_golden (built by build() from the shim and
+tests/fortran/vidal_types.f90, no reference code) runs golden jobs through the
+reference's call surface -- SVDInit, TrotterStep, CanonicalFormAll, SVDFinish, and
+ImagTimeProp with its chi stages, on the reference's derived types -- and the final
+states are compared with the reference's own outputs in tests/golden: energy,
+<Sz_i>, <n_i>, entropy and the cumulative truncation error within 1e-9 after the last
+step.
 """
 import os
-import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
-from conftest import ROOT
+from conftest import ROOT, load_golden
+from helpers import golden_obs, max_obs_err, observables
+from tebd_amd import models as M
 
 FC = "/opt/rocm/bin/amdflang"
-EXE = os.path.join(ROOT, "oracle", "_ref", "shim_vs_reference")
+SHIM = os.path.join(ROOT, "opentebd-extended_amd", "fortran")
+EXE = os.path.join(ROOT, "tests", "fortran", "_build", "shim_golden")
+REF_MODS = os.path.join(ROOT, "oracle", "_ref", "obj")
+TOL = 1e-9
 
 
 @pytest.mark.skipif(not os.path.exists(FC), reason="amdflang not installed")
 def test_binding_module_compiles(tmp_path):
-    src = os.path.join(ROOT, "opentebd-extended_amd", "fortran", "tebd_hip.f90")
+    src = os.path.join(SHIM, "tebd_hip.f90")
     r = subprocess.run([FC, "-fsyntax-only", "-module-dir", str(tmp_path), src], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
 
 
+@pytest.mark.skipif(not (os.path.exists(FC) and os.path.exists(os.path.join(REF_MODS, "tebdtools_module.mod"))),
+                    reason="the reference's modules are compiled only where its sources are (make -C oracle ref)")
+def test_shim_compiles_against_the_reference_modules(tmp_path):
+    r = subprocess.run([FC, "-module-dir", str(tmp_path), "-I", REF_MODS, "-c", os.path.join(SHIM, "tebd_hip.f90"),
+                        "-o", str(tmp_path / "a.o")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([FC, "-module-dir", str(tmp_path), "-I", str(tmp_path), "-I", REF_MODS, "-c",
+                        os.path.join(SHIM, "tebd_hip_propagation.f90"), "-o", str(tmp_path / "b.o")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def write_job(path, z, resident, mode=0):
+    L, d, chi = int(z["L"]), int(z["d"]), int(z["chi"])
+    itp = "stage_steps" in z
+    gam, lam = M.product_state(z["coeffs"], chi)
+    with open(path, "wb") as f:
+        n = 0 if itp else int(z["nsteps"])
+        canon = 0 if itp else int(z["canon"])
+        np.array([L, d, chi, n, canon, resident, mode, int(itp)], np.int32).tofile(f)
+        if itp:
+            np.array([z["chi_in"], z["max_steps"], z["judge"]], np.int32).tofile(f)
+            np.array([z["dtau"], z["conv1"], z["conv2"]], np.float64).tofile(f)
+        gates = z["H"] if itp else z["U"]
+        np.ascontiguousarray(np.transpose(gates, (0, 2, 1))).astype(np.complex128).tofile(f)
+        np.ascontiguousarray(gam, np.complex128).tofile(f)
+        np.ascontiguousarray(lam, np.float64).tofile(f)
+
+
+def read_out(path, L, d, chi):
+    raw = np.fromfile(path, np.uint8)
+    total = raw[:8].view(np.float64)[0]
+    ng = L * chi * d * chi
+    gam = raw[8:8 + 16 * ng].view(np.complex128).reshape(L, chi, d, chi)
+    lam = raw[8 + 16 * ng:].view(np.float64).reshape(L + 1, chi)
+    return total, gam, lam
+
+
 @pytest.mark.gpu
-def test_fortran_shim_matches_reference_modules():
-    if not os.path.exists(EXE):
-        pytest.skip("shim test binary not built (needs the reference sources at build time)")
-    print("[gpu] fortran shim vs reference TrotterStep/CanonicalFormAll", flush=True)
-    r = subprocess.run([EXE], capture_output=True, text=True, timeout=600)
-    print(r.stdout, r.stderr)
+@pytest.mark.parametrize("name,resident,mode", [("xxz_l16_chi8", 0, 0), ("xxz_l16_chi8", 1, 0),
+                                                ("fh_l8_chi8", 0, 2), ("itp_l10_chi8", 0, 0),
+                                                ("itp_l10_chi8", 1, 2), ("itp_stages_l10", 0, 0),
+                                                ("itp_stages_l10", 0, 2)])
+def test_fortran_shim_reproduces_reference_outputs(tmp_path, name, resident, mode):
+    assert os.path.exists(EXE), "tests/fortran/_build/shim_golden is not built (build() makes it)"
+    z = load_golden(name)
+    itp = "stage_steps" in z
+    L, d, chi = int(z["L"]), int(z["d"]), int(z["chi"])
+    n = -1 if itp else int(z["nsteps"])
+    job, out = str(tmp_path / "job.bin"), str(tmp_path / "out.bin")
+    write_job(job, z, resident, mode)
+    print(f"[gpu] fortran shim {name} resident={resident} mode={mode}", flush=True)
+    r = subprocess.run([EXE, job, out], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("shim ") == 4
+    total, gam, lam = read_out(out, L, d, chi)
+    err = max_obs_err(observables(gam, lam, z["H"], z["szop"], z["nop"]), golden_obs(z, n))
+    if not itp:
+        err["trunc"] = abs(total - float(np.sum(z["trunc"][1:n + 1])))
+    print(f"[gpu] {name}: {err}", flush=True)
+    assert max(err.values()) < TOL, err

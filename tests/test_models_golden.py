@@ -59,3 +59,32 @@ def test_two_site_op_fixture_gates():
     assert np.max(np.abs(np.asarray(H) - z["H"])) <= 1e-14
     U = M.construct_propagators(H, complex(z["dt"]))
     assert np.max(np.abs(np.asarray(U) - z["U"])) <= 1e-13
+
+
+@pytest.mark.parametrize("name", JOBS)
+def test_library_propagators_match_the_reference(name):
+    """tebd_construct_propagators (host-only, no device call): the library's own
+    Hermitian exponential against the reference's U(i)."""
+    from tebd_amd import construct_propagators
+
+    z = load_golden(name)
+    U = construct_propagators(z["H"], complex(z["dt"]))
+    assert np.max(np.abs(U - z["U"])) <= 1e-13
+
+
+def test_library_propagators_on_degenerate_and_random_hamiltonians():
+    """Closed forms: H = 0 gives the identity; a diagonal H gives exp(-i t diag); a
+    random Hermitian H agrees with scipy's expm (odd bonds dt/2, even bonds dt)."""
+    from tebd_amd import construct_propagators
+
+    rng = np.random.default_rng(3)
+    n = 16
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    Hr = A + A.conj().T
+    D = np.diag(rng.normal(size=n)).astype(np.complex128)
+    Hs = np.stack([np.zeros((n, n), np.complex128), D, Hr, Hr])
+    for dt in (0.07, -0.05j, 0.1 - 0.02j):
+        U = construct_propagators(Hs, dt)
+        for b in range(4):
+            t = dt / 2 if b % 2 == 0 else dt
+            assert np.max(np.abs(U[b] - scipy.linalg.expm(-1j * t * Hs[b]))) <= 1e-12 * max(1.0, np.abs(U[b]).max())
