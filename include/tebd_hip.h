@@ -220,6 +220,29 @@ int tebd_set_lanes(tebd_ctx* ctx, int n);
 int tebd_set_autotune(tebd_ctx* ctx, int on);
 int tebd_autotune_info(tebd_ctx* ctx, double* out, int n);
 
+/* The two-site update of tebd_two_site_op on caller-provided device buffers, with no
+ * allocation of any kind.  No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Inputs, all device pointers, in the packed layout of a context:
+ * gamma_l chi_l x d x chi_c and gamma_r chi_c x d x chi_r complex (element a + rows*(i + d*b)),
+ * lambda_l/c/r (chi_l, chi_c, chi_r reals), op2 (d*d x d*d complex, column-major, row
+ * (a-1)d+b).  Outputs: gamma_l_out (chi_l x d x k) and gamma_r_out (k x d x chi_r, leading
+ * dimension k) with k <= chi_cap the kept rank, each with room for chi_cap columns or rows;
+ * lambda_c_out (chi_cap reals, zero past k); *chi_c_new = k and *truncerr on the host.  The
+ * outputs may alias the inputs.  Truncation is the reference's (at most chi_cap values,
+ * normalised |s| > 1e-15).  The workspace is one fixed device array of 6 MiB per GPU, held
+ * until the process ends: d*chi_l, d*chi_r, chi_c and chi_cap must each be at most 256
+ * (TEBD_ERR_SHAPE otherwise, before any device call); d in {2, 3, 4, 5, 6, 8}.  Every call on
+ * a GPU shares that GPU's array and would race with an overlapping one, so the library
+ * serialises the calls of a process: a call queues its work on `stream` (a hipStream_t;
+ * null = the default stream) and returns only when that work has finished, and concurrent
+ * callers wait for each other.  The array keeps the last call's contents; each call clears
+ * it first. */
+int tebd_two_site_op_noalloc(int d, int chi_l, int chi_c, int chi_r, int chi_cap, const double* gamma_l,
+                             const double* gamma_r, const double* lambda_l, const double* lambda_c,
+                             const double* lambda_r, const double* op2, double* gamma_l_out, double* gamma_r_out,
+                             double* lambda_c_out, int* chi_c_new, double* truncerr, void* stream);
+
 /* Device memory of a context (no counterpart in the reference, whose workspace is the
  * module-global SVD arrays of SVDInit, local_operations_module.f90:84-122): out[0] the
  * per-bond workspace slots, i.e. the bonds one batched launch sequence processes (every

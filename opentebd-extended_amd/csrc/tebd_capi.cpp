@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1570,6 +1571,71 @@ int largest_lambdas(tebd_ctx* c, std::vector<double>& out) {
   return TEBD_OK;
 }
 
+// ---- allocation-free two-site update (tebd_two_site_op_noalloc) -------------------
+// One bond's workspace lives in a file-scope device array instead of hipMalloc'd slots:
+// bond matrices of up to kNoAllocSide x kNoAllocSide (d chi_l, d chi_r <= 256) and at
+// most kNoAllocSide kept values.  Every device holds its own copy; its address is
+// looked up once per device with hipGetSymbolAddress, and only that address is passed
+// to kernels and copies.
+constexpr int kNoAllocSide = 256;
+constexpr size_t kNoAllocAlign = 256;
+constexpr size_t na_up(size_t b) { return (b + kNoAllocAlign - 1) / kNoAllocAlign * kNoAllocAlign; }
+constexpr size_t kNoAllocNblk = kNoAllocSide / JB;
+constexpr size_t kNoAllocTq = (kNoAllocSide / kQrPanel + 1) * kQrPanel * kQrPanel;  // = tq_stride at Cpmax 256
+constexpr size_t kNoAllocBytes =
+    5 * na_up(sizeof(zc) * kNoAllocSide * kNoAllocSide) +                      // W, V, X, Yr, Zc
+    2 * na_up(sizeof(double) * kNoAllocSide) + 2 * na_up(sizeof(int) * kNoAllocSide) +  // s, qnrm, perm, qpiv
+    na_up(sizeof(zc) * kNoAllocSide) + na_up(sizeof(double) * 2) +              // tau, qstate
+    na_up(sizeof(zc) * kNoAllocTq) + na_up(sizeof(int) * 4) +                   // Tq, counters
+    na_up(sizeof(unsigned) * kNoAllocNblk) + na_up(sizeof(unsigned long long) * kNoAllocNblk * kNoAllocNblk) +
+    na_up(sizeof(zc) * (kNoAllocNblk / 2) * JC * JC) + na_up(sizeof(int) * 2) +  // Qg, wl_count
+    na_up(sizeof(int2) * 2 * (kNoAllocNblk / 2)) + 2 * na_up(sizeof(double) * RESULT_STRIDE) +
+    na_up(sizeof(BondTask) * kStages);
+constexpr size_t kNoAllocScratch = 6u << 20;
+static_assert(kNoAllocBytes <= kNoAllocScratch, "no-alloc workspace exceeds its array");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;                        // one call at a time per process (shared host staging)
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};   // device address of `scratch` on each device
+
+__global__ void k_zero_words(unsigned long long* p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+// Points the workspace members of `c` (one slot) into the array at `base`.
+void noalloc_layout(tebd_ctx* c, unsigned char* base) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* q = base + off;
+    off += na_up(bytes);
+    return q;
+  };
+  const size_t n = kNoAllocSide, zsq = sizeof(zc) * n * n;
+  c->W = (zc*)take(zsq);
+  c->V = (zc*)take(zsq);
+  c->X = (zc*)take(zsq);
+  c->Yr = (zc*)take(zsq);
+  c->Zc = (zc*)take(zsq);
+  c->s = (double*)take(sizeof(double) * n);
+  c->qnrm = (double*)take(sizeof(double) * n);
+  c->perm = (int*)take(sizeof(int) * n);
+  c->qpiv = (int*)take(sizeof(int) * n);
+  c->tau = (zc*)take(sizeof(zc) * n);
+  c->qstate = (double*)take(sizeof(double) * 2);
+  c->Tq = (zc*)take(sizeof(zc) * kNoAllocTq);
+  c->counters = (int*)take(sizeof(int) * 4);
+  c->blkver = (unsigned*)take(sizeof(unsigned) * kNoAllocNblk);
+  c->pairseen = (unsigned long long*)take(sizeof(unsigned long long) * kNoAllocNblk * kNoAllocNblk);
+  c->Qg = (zc*)take(sizeof(zc) * (kNoAllocNblk / 2) * JC * JC);
+  c->wl_count = (int*)take(sizeof(int) * 2);
+  c->wl_items = (int2*)take(sizeof(int2) * 2 * (kNoAllocNblk / 2));
+  c->results = (double*)take(sizeof(double) * RESULT_STRIDE);
+  c->results2 = (double*)take(sizeof(double) * RESULT_STRIDE);
+  c->d_tasks = (BondTask*)take(sizeof(BondTask) * kStages);
+}
+
 // Device bytes of one per-bond workspace slot (every per-slot array tebd_create
 // allocates), for bond matrices of up to Rmax x Cpmax.
 double slot_bytes(const tebd_ctx* c) {
@@ -2107,6 +2173,94 @@ int tebd_imag_time_prop(tebd_ctx* c, const double* H, int nh, double dtau, int c
   }
   c->stage_chi = 0;
   return e;
+}
+
+int tebd_two_site_op_noalloc(int d, int chi_l, int chi_c, int chi_r, int chi_cap, const double* gamma_l,
+                             const double* gamma_r, const double* lambda_l, const double* lambda_c,
+                             const double* lambda_r, const double* op2, double* gamma_l_out, double* gamma_r_out,
+                             double* lambda_c_out, int* chi_c_new, double* truncerr, void* stream) {
+  if (!(d == 2 || d == 3 || d == 4 || d == 5 || d == 6 || d == 8)) return TEBD_ERR_ARG;
+  if (chi_l < 1 || chi_c < 1 || chi_r < 1 || chi_cap < 1) return TEBD_ERR_ARG;
+  if (d * chi_l > kNoAllocSide || d * chi_r > kNoAllocSide || chi_cap > kNoAllocSide || chi_c > kNoAllocSide)
+    return TEBD_ERR_SHAPE;  // beyond the fixed workspace: use a context (tebd_two_site_op)
+  if (!gamma_l || !gamma_r || !lambda_l || !lambda_c || !lambda_r || !op2 || !gamma_l_out || !gamma_r_out ||
+      !lambda_c_out || !chi_c_new || !truncerr)
+    return TEBD_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices) return TEBD_ERR_HIP;
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return TEBD_ERR_HIP;
+    g_noalloc_ptr[dev] = p;
+  }
+  // host staging of the one task (pageable: no pinned allocation either)
+  static BondTask h_tasks[kStages];
+  static double h_results[RESULT_STRIDE], h_qstate[2];
+  static int h_counters[4];
+  tebd_ctx c;
+  c.d = d;
+  c.chi_max = chi_cap;
+  c.Rmax = kNoAllocSide;
+  c.Cpmax = kNoAllocSide;
+  c.nblk_max = (int)kNoAllocNblk;
+  c.ntask_max = 1;
+  c.wl_cap = (int)(kNoAllocNblk / 2);
+  c.graphs = false;
+  c.st = (hipStream_t)stream;
+  c.h_tasks = h_tasks;
+  c.h_results = h_results;
+  c.h_qstate = h_qstate;
+  c.h_counters = h_counters;
+  noalloc_layout(&c, (unsigned char*)g_noalloc_ptr[dev]);
+  // the array keeps the previous call's contents: start from zero, as a fresh workspace
+  hipLaunchKernelGGL(k_zero_words, dim3(256), dim3(256), 0, c.st, (unsigned long long*)g_noalloc_ptr[dev],
+                     kNoAllocBytes / sizeof(unsigned long long));
+  HIPCHK(&c, hipGetLastError());
+  BondTask& t = h_tasks[0];
+  std::memset(&t, 0, sizeof(t));
+  t.chil = chi_l;
+  t.chic = chi_c;
+  t.chir = chi_r;
+  t.d = d;
+  const int m = d * chi_l, n = d * chi_r;
+  t.transposed = n > m ? 1 : 0;
+  t.R = std::max(m, n);
+  t.C = std::min(m, n);
+  t.Cpad = round_up(t.C, JC);
+  t.P = t.Cpad / JC;
+  t.gl = (const zc*)gamma_l;
+  t.gr = (const zc*)gamma_r;
+  t.ll = lambda_l;
+  t.lc = lambda_c;
+  t.lr = lambda_r;
+  t.gate = (const zc*)op2;
+  t.W = c.W;
+  t.V = c.V;
+  t.s = c.s;
+  t.perm = c.perm;
+  t.NV = t.Cpad;
+  attach_jacobi(&c, t, 0);
+  attach_qr(&c, t, 0);
+  t.result = c.results;
+  t.gl_out = (zc*)gamma_l_out;
+  t.gr_out = (zc*)gamma_r_out;
+  t.lc_out = lambda_c_out;
+  t.chi_max = chi_cap;
+  if (hipMemcpyAsync(c.d_tasks, h_tasks, sizeof(BondTask), hipMemcpyHostToDevice, c.st) != hipSuccess ||
+      theta_gate(&c, c.d_tasks, 1, chi_l, chi_r) != hipSuccess) {
+    (void)hipStreamSynchronize(c.st);
+    return TEBD_ERR_HIP;
+  }
+  const int link = 0;
+  int e = svd_truncate(&c, &link, 1, t.Cpad, t.P);
+  if (e) {
+    (void)hipStreamSynchronize(c.st);  // nothing of this call may still run when the next one starts
+    return e;
+  }
+  *chi_c_new = (int)h_results[0];
+  *truncerr = h_results[1];
+  return TEBD_OK;
 }
 
 int tebd_memory_info(tebd_ctx* c, double* out, int n) {
