@@ -253,17 +253,28 @@ def oracle_check(snap, modes, log, rank_model=None):
 
 
 def copy_calibration(mib):
-    """Device-to-device copies of a known size: the HBM byte counters (FETCH_SIZE,
-    WRITE_SIZE) of this kernel against the bytes it must move give their scale."""
-    import torch
+    """Device-to-device copies of a known size through the HIP runtime the library uses
+    (its blit kernel reads and writes mib MiB each): the HBM byte counters (FETCH_SIZE,
+    WRITE_SIZE) of these dispatches against the bytes they must move give their scale."""
+    import ctypes
 
-    n = mib * (1 << 20) // 4
-    x = torch.ones(n, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    n = mib * (1 << 20)
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(src), n) or hip.hipMalloc(ctypes.byref(dst), n):
+        raise BenchError("copy calibration: allocation failed")
+    hip.hipMemset(src, 1, n)
     for _ in range(5):
-        y.copy_(x)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 5}), flush=True)
+        if hip.hipMemcpy(dst, src, n, 3):  # device to device
+            raise BenchError("copy calibration: copy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes_read": n, "bytes_written": n, "launches": 5}), flush=True)
 
 
 def host_cpu():
@@ -469,17 +480,20 @@ def rocprof_summary(path, window_file=None, traffic_out=None, calib_dir=None, ca
             k["hbm_bytes_per_launch"] = k.get("FETCH_SIZE_bytes", 0.0) + k.get("WRITE_SIZE_bytes", 0.0)
         out.append("")
         if traffic_out:
-            old = {}
+            old, old_scale = {}, {}
             if os.path.exists(traffic_out):
                 with open(traffic_out) as fh:
-                    old = json.load(fh).get("kernels", {})
+                    prev = json.load(fh)
+                old, old_scale = prev.get("kernels", {}), prev.get("scale", {})
+            seen = {cn for (_, cn) in by}
+            old_scale.update({cn: v for cn, v in scale.items() if cn in seen})
             for name, k in kern.items():  # one counter per pass: merge with the other pass
                 o = old.get(name, {})
                 o.update(k)
                 o["hbm_bytes_per_launch"] = o.get("FETCH_SIZE_bytes", 0.0) + o.get("WRITE_SIZE_bytes", 0.0)
                 old[name] = o
             with open(traffic_out, "w") as fh:
-                json.dump({"workload": WORKLOAD_TAG, "scale": scale, "kernels": old}, fh, indent=1)
+                json.dump({"workload": WORKLOAD_TAG, "scale": old_scale, "kernels": old}, fh, indent=1)
     print("\n".join(out))
 
 
