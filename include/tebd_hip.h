@@ -47,10 +47,10 @@ enum tebd_mode {
 };
 
 /* Allocates the device state for a chain of L sites, local dimension d and
- * bond dimension cap chi_max.  order must be 2 and bc 'O' (the configurations'
- * schedule, TrotterStep -> TrotterStep2ndOrder, propagation_module.f90:42-114).
- * Replaces AllocateGamLam (TEBDtools_module.f90:486) + SVDInit
- * (local_operations_module.f90:84-122). */
+ * bond dimension cap chi_max, with the Trotter schedule of order 2 or 5 (Forest-Ruth) and
+ * open ('O') or periodic ('P', L >= 3, not in an ensemble) boundaries
+ * (trotterOrder / BoundaryCond, propagation_module.f90:42-614).  Replaces
+ * AllocateGamLam (TEBDtools_module.f90:486) + SVDInit (local_operations_module.f90:84-122). */
 int tebd_create(tebd_ctx** ctx, int L, int d, int chi_max, int order, char bc, unsigned mode);
 
 /* An ensemble: nchain independent chains of L sites each (e.g. disorder
@@ -72,9 +72,10 @@ int tebd_create_ensemble(tebd_ctx** ctx, int nchain, int L, int d, int chi_max, 
  * and DeallocateGamLam. */
 void tebd_destroy(tebd_ctx* ctx);
 
-/* Trotter gates of every bond, nb = L-1 matrices U(i) of (d*d) x (d*d)
- * complex, column-major, packed back to back (the Udt table built by
- * ConstructPropagators, TEBDtools_module.f90:1079-1101). */
+/* The Udt table built by ConstructPropagators (TEBDtools_module.f90:1079-1193) for the
+ * context's schedule: nb = its size (L-1 for 2nd order with open boundaries; see
+ * tebd_construct_propagators), matrices of (d*d) x (d*d) complex, column-major, packed
+ * back to back; an ensemble takes the tables of its chains one after another. */
 int tebd_set_gates(tebd_ctx* ctx, const double* U, int nb);
 
 /* Uploads a Vidal state held in the reference's padded layout: gamma[s] points
@@ -93,9 +94,13 @@ int tebd_download_packed(tebd_ctx* ctx, double* gamma, double* lambda, int chi_o
  * (propagation_module.f90:704-738) read. */
 int tebd_download_lambdas(tebd_ctx* ctx, double* lambda, int chi_out);
 
-/* One 2nd-order OBC Trotter step: odd bonds (gate dt/2), even bonds (dt),
- * odd bonds (dt/2); *truncerr = sum of the bond truncation errors.
- * Replaces TrotterStep / TrotterStep2ndOrder (propagation_module.f90:42-114). */
+/* One Trotter step of the context's schedule; *truncerr = sum of the bond truncation
+ * errors.  2nd order, open boundaries: odd bonds (gate dt/2), even bonds (dt), odd bonds
+ * (dt/2).  The other schedules follow TrotterStep2ndOrderPBC, TrotterStep5thOrder and
+ * TrotterStep5thOrderPBC: the periodic edge gate is applied on bond 1 after swapping site L
+ * next to site 1 (two-site updates with the SWAP gate on bonds L-1 .. 2, the reference's
+ * Swapping, local_operations_module.f90:665-695) and the swaps are undone afterwards; the
+ * swaps' truncation errors count.  Replaces TrotterStep (propagation_module.f90:42-614). */
 int tebd_trotter_step(tebd_ctx* ctx, double* truncerr);
 
 /* One two-site update with an arbitrary (d*d)x(d*d) operator op2 on bond
@@ -121,13 +126,21 @@ int tebd_canonical_form(tebd_ctx* ctx, int link);
  * rho = theta theta^dagger is formed on the device from the resident state. */
 int tebd_expval_two_site(tebd_ctx* ctx, const double* ops, double* out);
 
-/* The 2nd-order OBC Trotter gates of one chain from its nb = L-1 bond Hamiltonians H
- * ((d*d) x (d*d) complex, column-major, back to back): U(b) = exp(-i t H(b)) with
- * t = dt/2 on odd bonds (1-based) and dt on even ones, dt = dt_re + i dt_im (imaginary
- * time: dt = -i dtau).  Host-only (no device call).  Replaces ConstructPropagators /
- * NEWConstructPropagators and matrix_exponential_c (TEBDtools_module.f90:97-99,
- * 176-222,1079-1101), which diagonalise H with ZHEEV; here by complex Jacobi rotations. */
-int tebd_construct_propagators(const double* H, int nb, int d, double dt_re, double dt_im, double* U);
+/* The propagator table of one chain of L sites for the Trotter schedule (order, bc), from
+ * its bond Hamiltonians H ((d*d) x (d*d) complex, column-major, back to back: L-1 for open
+ * boundaries, L for periodic ones, H[L-1] coupling sites L and 1).  Entry k of the table
+ * U is exp(-i f_k dt H(bond_k)), dt = dt_re + i dt_im (imaginary time: dt = -i dtau), laid
+ * out as the reference's NEWConstructPropagators fills it (TEBDtools_module.f90:1079-1193):
+ * 2nd order, open: bond i gets f = 1/2 (odd i) or 1 (even i); 2nd order, periodic: also
+ * the edge with f = 1, and even bonds f = 1/2 when L is odd; 5th order (Forest-Ruth,
+ * theta = 1 / (2 - 2^(1/3)) with the reference's single-precision exponent): two entries
+ * per bond, and for periodic chains two (L even) or three (L odd, with four per even
+ * bond) for the edge.  Returns the table size (L-1, L, 2(L-1), 2L or 3L) or a negative
+ * status.  Host-only (no device call).  Replaces ConstructPropagators and
+ * matrix_exponential_c (TEBDtools_module.f90:97-99,176-222), which diagonalise H with
+ * ZHEEV; here by complex Jacobi rotations. */
+int tebd_construct_propagators(const double* H, int L, int d, int order, char bc, double dt_re, double dt_im,
+                               double* U);
 
 /* Imaginary-time ground-state search on the resident state.  Replaces ImagTimeProp and
  * ImagTimePropSpin (propagation_module.f90:636-876): gates exp(-dtau H) built from the

@@ -50,6 +50,11 @@ struct tebd_ctx {
   // an ensemble holds nchain independent chains of chain_len sites as one chain of
   // L = nchain * chain_len sites whose bonds between chains are never updated
   int chain_len = 0, nchain = 1;
+  // Trotter schedule: order 2 or 5 (Forest-Ruth), open ('O') or periodic ('P') boundaries;
+  // `table` gates per chain (the reference's propagator table for that schedule)
+  int order = 2;
+  char bc = 'O';
+  int table = 0;
   unsigned mode = 0;
   double cutoff = 1e-30;
   // chi stage of imaginary-time propagation (tebd_imag_time_prop): the truncation keeps at
@@ -60,7 +65,7 @@ struct tebd_ctx {
   // device state
   zc* gam = nullptr;      // L slabs of chi_max*d*chi_max
   double* lam = nullptr;  // (L+1) x chi_max
-  zc* gates = nullptr;    // (L-1) x d^4, plus one slot for tebd_two_site_op
+  zc* gates = nullptr;    // nchain x table gates of d^4, then tebd_two_site_op's operator, then SWAP
   bool have_gates = false, have_state = false;
   std::vector<int> chi;   // bond dimensions, host mirror
   // per-task workspace
@@ -1508,27 +1513,101 @@ void herm_eig(int n, std::vector<cplx> A, std::vector<double>& w, std::vector<cp
   for (int i = 0; i < n; ++i) w[i] = A[i + (size_t)n * i].real();
 }
 
-// The 2nd-order OBC gates of one chain (NEWConstructPropagators,
-// TEBDtools_module.f90:1079-1101, with matrix_exponential_c :176-222):
-// U(b) = exp(-i t H(b)), t = dt/2 on odd bonds (1-based) and dt on even ones; dt complex
-// (imaginary time passes dt = -i dtau).  H, U: nb matrices of n x n complex, column-major.
-void construct_propagators_host(const double* H, int nb, int n, cplx dt, double* U) {
+// ---- Trotter schedules ------------------------------------------------------------------
+// Size of the reference's propagator table (AllocateProp, TEBDtools_module.f90:937-998).
+int gate_table_size(int L, int order, char bc) {
+  const bool pbc = bc == 'P';
+  if (order == 2) return pbc ? L : L - 1;
+  return !pbc ? 2 * (L - 1) : (L % 2 == 0 ? 2 * L : 3 * L);
+}
+
+// 0-based table entry of bond i (1-based; i = L is the periodic edge, sites L and 1) in
+// slot s (1-based), as NEWConstructPropagators fills the table
+// (TEBDtools_module.f90:1079-1193).
+int gate_index(int L, int order, char bc, int i, int s) {
+  if (order == 2) return i - 1;
+  if (bc != 'P' || L % 2 == 0) return 2 * (i - 1) + s - 1;
+  return (i % 2 == 1) ? 3 * (i - 1) + s - 1 : 2 + 3 * (i - 2) + s - 1;  // odd L: i = L is odd
+}
+
+// Time factor of table entry (i, s) (dt times this is the gate's exponent), with the
+// Forest-Ruth theta as the reference evaluates it, 1 / (2 - 2**(1.0/3.0)) with a
+// single-precision exponent 1.0/3.0 (TEBDtools_module.f90:1135).
+double gate_factor(int L, int order, char bc, int i, int s) {
+  const bool edge = i == L, odd = i % 2 == 1;
+  if (order == 2) {
+    if (edge) return 1.0;
+    if (odd) return 0.5;
+    return (bc == 'P' && L % 2 == 1) ? 0.5 : 1.0;
+  }
+  const double th = 1.0 / (2.0 - std::pow(2.0, (double)(1.0f / 3.0f)));
+  if (bc != 'P' || L % 2 == 0) {
+    if (edge || !odd) return s == 1 ? th : 1.0 - 2.0 * th;
+    return s == 1 ? 0.5 * th : 0.5 * (1.0 - th);
+  }
+  if (edge) return s == 1 ? th * th : (s == 2 ? (1.0 - 2.0 * th) * th : (1.0 - 2.0 * th) * (1.0 - 2.0 * th));
+  if (odd) return s == 1 ? 0.5 * th : 0.5 * (1.0 - th);
+  const double f[4] = {0.5 * th * th, 0.5 * (1.0 - th) * th, 0.5 * (1.0 - 2.0 * th) * th,
+                       0.5 * (1.0 - th) * (1.0 - 2.0 * th)};
+  return f[s - 1];
+}
+
+// One stage of a TrotterStep: 'G' every odd (parity 1) or even (parity 2) bond with its
+// slot-s gate; 'E' the edge gate of slot s, applied on bond 1 after swapping site L next
+// to site 1 (swaps on bonds L-1 .. 2) and followed by the swaps back (bonds 2 .. L-1).
+struct Stage {
+  char kind;
+  int parity, slot;
+};
+
+// The stage sequence of TrotterStep for each order and boundary
+// (propagation_module.f90:42-614).
+std::vector<Stage> trotter_schedule(int L, int order, char bc) {
+  auto G = [](int p, int s) { return Stage{'G', p, s}; };
+  auto E = [](int s) { return Stage{'E', 0, s}; };
+  if (order == 2 && bc != 'P') return {G(1, 1), G(2, 1), G(1, 1)};          // :82-114
+  if (order == 2) {                                                         // :116-198
+    if (L % 2 == 1) return {G(1, 1), G(2, 1), E(1), G(2, 1), G(1, 1)};
+    return {G(1, 1), E(1), G(2, 1), G(1, 1)};
+  }
+  if (bc != 'P')                                                            // :200-245
+    return {G(1, 1), G(2, 1), G(1, 2), G(2, 2), G(1, 2), G(2, 1), G(1, 1)};
+  if (L % 2 == 0)                                                           // :541-612
+    return {G(1, 1), E(1), G(2, 1), G(1, 2), E(2), G(2, 2), G(1, 2), E(1), G(2, 1), G(1, 1)};
+  return {G(1, 1), G(2, 1), E(1), G(2, 2), E(2), G(2, 2), E(1), G(2, 1), G(1, 2),  // :260-538
+          G(2, 3), E(2), G(2, 4), E(3), G(2, 4), E(2), G(2, 3), G(1, 2), G(2, 1), E(1),
+          G(2, 2), E(2), G(2, 2), E(1), G(2, 1), G(1, 1)};
+}
+
+// The propagator table of one chain of L sites (NEWConstructPropagators,
+// TEBDtools_module.f90:1079-1193, with matrix_exponential_c :176-222): entry (i, s) is
+// exp(-i f dt H(i)) with the factor f of gate_factor; dt complex (imaginary time passes
+// dt = -i dtau).  H: L-1 (open) or L (periodic, the last one the edge) matrices of n x n
+// complex, column-major; U: gate_table_size entries.
+void construct_propagators_host(const double* H, int L, int n, int order, char bc, cplx dt, double* U) {
+  const int nb = bc == 'P' ? L : L - 1;
+  const int slots_odd = order == 2 ? 1 : 2;
+  const int slots_even = order == 2 ? 1 : ((bc == 'P' && L % 2 == 1) ? 4 : 2);
+  const int slots_edge = order == 2 ? 1 : ((L % 2 == 1) ? 3 : 2);
   std::vector<cplx> A((size_t)n * n), V;
   std::vector<double> w;
-  for (int b = 0; b < nb; ++b) {
-    const cplx* h = reinterpret_cast<const cplx*>(H) + (size_t)b * n * n;
+  std::vector<cplx> f(n);
+  for (int i = 1; i <= nb; ++i) {
+    const cplx* h = reinterpret_cast<const cplx*>(H) + (size_t)(i - 1) * n * n;
     for (size_t e = 0; e < (size_t)n * n; ++e) A[e] = h[e];
     herm_eig(n, A, w, V);
-    const cplx t = (b % 2 == 0) ? 0.5 * dt : dt;
-    std::vector<cplx> f(n);
-    for (int k = 0; k < n; ++k) f[k] = std::exp(cplx(0.0, -1.0) * t * w[k]);
-    cplx* u = reinterpret_cast<cplx*>(U) + (size_t)b * n * n;
-    for (int col = 0; col < n; ++col)
-      for (int r = 0; r < n; ++r) {
-        cplx acc = 0.0;
-        for (int k = 0; k < n; ++k) acc += V[r + (size_t)n * k] * f[k] * std::conj(V[col + (size_t)n * k]);
-        u[r + (size_t)n * col] = acc;
-      }
+    const int ns = i == L ? slots_edge : (i % 2 == 1 ? slots_odd : slots_even);
+    for (int sl = 1; sl <= ns; ++sl) {
+      const cplx t = gate_factor(L, order, bc, i, sl) * dt;
+      for (int k = 0; k < n; ++k) f[k] = std::exp(cplx(0.0, -1.0) * t * w[k]);
+      cplx* u = reinterpret_cast<cplx*>(U) + (size_t)gate_index(L, order, bc, i, sl) * n * n;
+      for (int col = 0; col < n; ++col)
+        for (int r = 0; r < n; ++r) {
+          cplx acc = 0.0;
+          for (int k = 0; k < n; ++k) acc += V[r + (size_t)n * k] * f[k] * std::conj(V[col + (size_t)n * k]);
+          u[r + (size_t)n * col] = acc;
+        }
+    }
   }
 }
 
@@ -1652,18 +1731,27 @@ double slot_bytes(const tebd_ctx* c) {
 
 extern "C" {
 
-int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, unsigned mode) {
+// nchain chains of Lc sites (L = nchain Lc) with the schedule (order, bc).
+int create_impl(tebd_ctx** out, int nchain, int Lc, int d, int chi_max, int order, char bc, unsigned mode) {
   if (!out) return TEBD_ERR_ARG;
   *out = nullptr;
-  if (L < 2 || d < 2 || chi_max < 1) return TEBD_ERR_ARG;
-  if (order != 2 || (bc != 'O' && bc != 'o')) return TEBD_ERR_ARG;
+  if (bc == 'o') bc = 'O';
+  if (bc == 'p') bc = 'P';
+  if (nchain < 1 || Lc < 2 || d < 2 || chi_max < 1 || (long long)nchain * Lc > (1 << 24)) return TEBD_ERR_ARG;
+  if ((order != 2 && order != 5) || (bc != 'O' && bc != 'P')) return TEBD_ERR_ARG;
+  if (bc == 'P' && (Lc < 3 || nchain > 1)) return TEBD_ERR_ARG;  // periodic: one chain of at least 3 sites
   if (!(d == 2 || d == 3 || d == 4 || d == 5 || d == 6 || d == 8)) return TEBD_ERR_ARG;
   if (d * chi_max > 4096) return TEBD_ERR_ARG;
+  const int L = nchain * Lc;
   tebd_ctx* c = new tebd_ctx();
   c->L = L;
   c->d = d;
   c->chi_max = chi_max;
-  c->chain_len = L;
+  c->chain_len = Lc;
+  c->nchain = nchain;
+  c->order = order;
+  c->bc = bc;
+  c->table = gate_table_size(Lc, order, bc);
   c->mode = mode;
   c->chi.assign(L + 1, 1);
   c->Rmax = d * chi_max;
@@ -1687,7 +1775,16 @@ int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, u
   if (bad(hipEventCreate(&c->ev0)) || bad(hipEventCreate(&c->ev1))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->gam, sizeof(zc) * c->gslab() * L))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->lam, sizeof(double) * (size_t)(L + 1) * chi_max))) return TEBD_ERR_HIP;
-  if (bad(hipMalloc(&c->gates, sizeof(zc) * c->d4() * L))) return TEBD_ERR_HIP;
+  if (bad(hipMalloc(&c->gates, sizeof(zc) * c->d4() * ((size_t)nchain * c->table + 2)))) return TEBD_ERR_HIP;
+  {  // the SWAP gate of the periodic schedules' swaps: U(i d + j, k d + l) = delta(i,l) delta(j,k)
+    const int d2 = d * d;
+    std::vector<zc> sw((size_t)d2 * d2, zc{0.0, 0.0});
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; j < d; ++j) sw[(i * d + j) + (size_t)d2 * (j * d + i)] = zc{1.0, 0.0};
+    if (bad(hipMemcpy(c->gates + c->d4() * ((size_t)nchain * c->table + 1), sw.data(), sizeof(zc) * sw.size(),
+                      hipMemcpyHostToDevice)))
+      return TEBD_ERR_HIP;
+  }
   if (bad(hipMalloc(&c->W, sizeof(zc) * (size_t)c->Rmax * c->Cpmax * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->V, sizeof(zc) * (size_t)c->Cpmax * c->Cpmax * nt))) return TEBD_ERR_HIP;
   if (bad(hipMalloc(&c->s, sizeof(double) * (size_t)c->Cpmax * nt))) return TEBD_ERR_HIP;
@@ -1733,15 +1830,12 @@ int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, u
   return TEBD_OK;
 }
 
+int tebd_create(tebd_ctx** out, int L, int d, int chi_max, int order, char bc, unsigned mode) {
+  return create_impl(out, 1, L, d, chi_max, order, bc, mode);
+}
+
 int tebd_create_ensemble(tebd_ctx** out, int nchain, int L, int d, int chi_max, int order, char bc, unsigned mode) {
-  if (!out) return TEBD_ERR_ARG;
-  *out = nullptr;
-  if (nchain < 1 || L < 2 || (long long)nchain * L > (1 << 24)) return TEBD_ERR_ARG;
-  int e = tebd_create(out, nchain * L, d, chi_max, order, bc, mode);
-  if (e) return e;
-  (*out)->chain_len = L;
-  (*out)->nchain = nchain;
-  return TEBD_OK;
+  return create_impl(out, nchain, L, d, chi_max, order, bc, mode);
 }
 
 void tebd_destroy(tebd_ctx* c) {
@@ -1753,27 +1847,11 @@ void tebd_destroy(tebd_ctx* c) {
 
 int tebd_set_gates(tebd_ctx* c, const double* U, int nb) {
   if (!c) return TEBD_ERR_ARG;
-  const int per = c->chain_len - 1;
-  if (!U || nb != c->nchain * per) return fail(c, TEBD_ERR_ARG, "set_gates: need L-1 gates per chain");
-  if (c->nchain == 1) {
-    HIPCHK(c, hipMemcpy(c->gates, U, sizeof(zc) * c->d4() * nb, hipMemcpyHostToDevice));
-  } else {
-    // gate b of chain r goes to bond r * chain_len + b; the bonds between chains get the
-    // identity (never applied)
-    const size_t d4 = c->d4();
-    const int d2 = c->d * c->d;
-    std::vector<zc> all(d4 * (c->L - 1), zc{0.0, 0.0});
-    for (int link = 0; link < c->L - 1; ++link) {
-      zc* dst = all.data() + d4 * link;
-      if (is_cut(c, link)) {
-        for (int q = 0; q < d2; ++q) dst[q + (size_t)d2 * q] = zc{1.0, 0.0};
-      } else {
-        const int r = link / c->chain_len, b = link % c->chain_len;
-        std::memcpy(dst, U + 2 * d4 * ((size_t)r * per + b), sizeof(zc) * d4);
-      }
-    }
-    HIPCHK(c, hipMemcpy(c->gates, all.data(), sizeof(zc) * all.size(), hipMemcpyHostToDevice));
-  }
+  if (!U || nb != c->nchain * c->table)
+    return fail(c, TEBD_ERR_ARG, "set_gates: need the schedule's propagator table (L-1 gates per chain for "
+                                 "2nd order with open boundaries)");
+  // chain after chain, each its whole table
+  HIPCHK(c, hipMemcpy(c->gates, U, sizeof(zc) * c->d4() * nb, hipMemcpyHostToDevice));
   c->have_gates = true;
   return TEBD_OK;
 }
@@ -1820,21 +1898,37 @@ int tebd_download_lambdas(tebd_ctx* c, double* lambda, int chi_out) {
 int tebd_trotter_step(tebd_ctx* c, double* truncerr) {
   int e = ensure_state(c, true);
   if (e) return e;
-  const int L = c->L;
+  const int L = c->L, Lc = c->chain_len;
   double total = 0.0;
   HIPCHK(c, hipEventRecord(c->ev0, c->st));
-  const int starts[3] = {0, 1, 0};  // reference bonds 1,3,5.. / 2,4,.. / 1,3,5..
   std::vector<int> links;
   std::vector<const zc*> gp;
   c->step_index++;
-  for (int half = 0; half < 3; ++half) {
+  const zc* swap = c->gates + c->d4() * ((size_t)c->nchain * c->table + 1);
+  // bonds one after another (the swaps and edge gates of the periodic schedules, and
+  // every bond in the per-bond FP64 mode)
+  auto one = [&](int link, const zc* g) {
+    double t = 0.0;
+    int r = run_bonds(c, &link, 1, &g, &t);
+    total += t;
+    return r;
+  };
+  for (const Stage& stg : trotter_schedule(Lc, c->order, c->bc)) {
+    if (stg.kind == 'E') {  // site L next to site 1, the edge gate on bond 1, and back
+      for (int i = Lc - 1; i >= 2 && !e; --i) e = one(i - 1, swap);
+      if (!e) e = one(0, c->gates + c->d4() * (size_t)gate_index(Lc, c->order, c->bc, Lc, stg.slot));
+      for (int i = 2; i <= Lc - 1 && !e; ++i) e = one(i - 1, swap);
+      if (e) return e;
+      continue;
+    }
     links.clear();
     gp.clear();
     for (int b = 0; b < L - 1; ++b) {
-      // reference bond numbering restarts in every chain of an ensemble
-      if (is_cut(c, b) || (b % c->chain_len) % 2 != starts[half]) continue;
+      // the reference's bond numbering restarts in every chain of an ensemble
+      const int i = b % Lc + 1, r = b / Lc;
+      if (is_cut(c, b) || (i % 2 == 1 ? 1 : 2) != stg.parity) continue;
       links.push_back(b);
-      gp.push_back(c->gates + (size_t)b * c->d4());
+      gp.push_back(c->gates + c->d4() * ((size_t)r * c->table + gate_index(Lc, c->order, c->bc, i, stg.slot)));
     }
     if (c->mode & TEBD_MODE_BATCHED) {
       // batches of at most ntask_max bonds (one workspace slot each)
@@ -1846,12 +1940,8 @@ int tebd_trotter_step(tebd_ctx* c, double* truncerr) {
         total += t;
       }
     } else {
-      for (size_t i = 0; i < links.size(); ++i) {
-        double t = 0.0;
-        e = run_bonds(c, &links[i], 1, &gp[i], &t);
-        if (e) return e;
-        total += t;
-      }
+      for (size_t k = 0; k < links.size(); ++k)
+        if ((e = one(links[k], gp[k]))) return e;
     }
   }
   HIPCHK(c, hipEventRecord(c->ev1, c->st));
@@ -1868,7 +1958,7 @@ int tebd_two_site_op(tebd_ctx* c, int link, const double* op2, double* truncerr)
   if (e) return e;
   if (link < 1 || link > c->L - 1 || !op2) return fail(c, TEBD_ERR_ARG, "two_site_op: bad link or operator");
   if (is_cut(c, link - 1)) return fail(c, TEBD_ERR_ARG, "two_site_op: bond joins two chains of the ensemble");
-  zc* slot = c->gates + (size_t)(c->L - 1) * c->d4();
+  zc* slot = c->gates + c->d4() * (size_t)c->nchain * c->table;
   HIPCHK(c, hipMemcpyAsync(slot, op2, sizeof(zc) * c->d4(), hipMemcpyHostToDevice, c->st));
   const int l0 = link - 1;
   const zc* gp = slot;
@@ -2120,26 +2210,32 @@ int tebd_autotune_info(tebd_ctx* c, double* out, int n) {
   return TEBD_OK;
 }
 
-int tebd_construct_propagators(const double* H, int nb, int d, double dt_re, double dt_im, double* U) {
-  if (!H || !U || nb < 1 || d < 2 || d > 8) return TEBD_ERR_ARG;
-  construct_propagators_host(H, nb, d * d, cplx(dt_re, dt_im), U);
-  return TEBD_OK;
+int tebd_construct_propagators(const double* H, int L, int d, int order, char bc, double dt_re, double dt_im,
+                               double* U) {
+  if (bc == 'o') bc = 'O';
+  if (bc == 'p') bc = 'P';
+  if (!H || !U || L < 2 || d < 2 || d > 8 || (order != 2 && order != 5) || (bc != 'O' && bc != 'P') ||
+      (bc == 'P' && L < 3))
+    return TEBD_ERR_ARG;
+  construct_propagators_host(H, L, d * d, order, bc, cplx(dt_re, dt_im), U);
+  return gate_table_size(L, order, bc);
 }
 
 int tebd_imag_time_prop(tebd_ctx* c, const double* H, int nh, double dtau, int chi_in, int max_steps,
                         int steps_for_judge, double conv1, double conv2, int* steps) {
   int e = ensure_state(c, false);
   if (e) return e;
-  const int per = c->chain_len - 1, n = c->d * c->d;
+  const int Lc = c->chain_len, n = c->d * c->d;
+  const int per = c->bc == 'P' ? Lc : Lc - 1;  // bond Hamiltonians per chain
   if (!H || nh != c->nchain * per || !(dtau > 0.0) || chi_in < 1 || chi_in > c->chi_max || max_steps < 0 ||
       steps_for_judge < 1 || !(conv1 > 0.0) || !(conv2 > 0.0) || !steps)
     return fail(c, TEBD_ERR_ARG, "imag_time_prop: bad arguments");
   // Uitp = ConstructPropagators(H, -i dtau), chain by chain (propagation_module.f90:656-662)
-  std::vector<double> U((size_t)2 * n * n * nh);
+  std::vector<double> U((size_t)2 * n * n * c->table * c->nchain);
   for (int r = 0; r < c->nchain; ++r)
-    construct_propagators_host(H + (size_t)2 * n * n * per * r, per, n, cplx(0.0, -dtau),
-                               U.data() + (size_t)2 * n * n * per * r);
-  if ((e = tebd_set_gates(c, U.data(), nh))) return e;
+    construct_propagators_host(H + (size_t)2 * n * n * per * r, Lc, n, c->order, c->bc, cplx(0.0, -dtau),
+                               U.data() + (size_t)2 * n * n * c->table * r);
+  if ((e = tebd_set_gates(c, U.data(), c->nchain * c->table))) return e;
   // stages chi_in and chi_max (one when they are equal), the tighter criterion on the last
   // (propagation_module.f90:663-690)
   std::vector<int> stages = {c->chi_max};

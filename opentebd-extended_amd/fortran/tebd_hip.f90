@@ -68,11 +68,12 @@ INTERFACE
     REAL(c_double) :: truncerr
   END FUNCTION tebd_two_site_op
 
-  INTEGER(c_int) FUNCTION tebd_construct_propagators(H, nb, d, dt_re, dt_im, U) &
+  INTEGER(c_int) FUNCTION tebd_construct_propagators(H, L, d, order, bc, dt_re, dt_im, U) &
       BIND(C, name="tebd_construct_propagators")
-    IMPORT :: c_ptr, c_int, c_double
+    IMPORT :: c_ptr, c_int, c_double, c_char
     TYPE(c_ptr), VALUE :: H, U
-    INTEGER(c_int), VALUE :: nb, d
+    INTEGER(c_int), VALUE :: L, d, order
+    CHARACTER(kind=c_char), VALUE :: bc
     REAL(c_double), VALUE :: dt_re, dt_im
   END FUNCTION tebd_construct_propagators
 

@@ -139,7 +139,7 @@ SUBROUTINE TrotterStep(Udt, Gammas, Lambdas, totalTruncerr)
   REAL(c_double) :: t
   INTEGER :: i, nb, d2
   CALL before_call(Gammas, Lambdas)
-  nb = systemSize - 1
+  nb = SIZE(Udt)  ! the propagator table of the schedule (trotterOrder, BoundaryCond)
   d2 = localSize * localSize
   ALLOCATE(u(d2, d2, nb))
   DO i = 1, nb
@@ -159,22 +159,27 @@ SUBROUTINE CanonicalFormAll(Gammas, Lambdas)
   CALL after_call(Gammas, Lambdas)
 END SUBROUTINE CanonicalFormAll
 
-! Udt(i) = exp(-i t H(i)), t = dt/2 on odd bonds and dt on even ones (2nd-order OBC), as
-! the reference's ConstructPropagators; U must be allocated like H.
+! The propagator table of the reference's ConstructPropagators for trotterOrder and
+! BoundaryCond; U must be allocated as AllocateProp does (one entry per table slot).
 SUBROUTINE ConstructPropagators(H, U, dt)
   TYPE(matrix), POINTER :: H(:), U(:)
   COMPLEX(KIND=rKind), INTENT(IN) :: dt
   COMPLEX(c_double_complex), ALLOCATABLE, TARGET :: hb(:, :, :), ub(:, :, :)
-  INTEGER :: i, nb, d2
+  INTEGER :: i, nb, nu, d2
+  INTEGER(c_int) :: rc
+  CHARACTER(kind=c_char) :: bc
   nb = SIZE(H)
+  nu = SIZE(U)
   d2 = SIZE(H(1)%m, 1)
-  ALLOCATE(hb(d2, d2, nb), ub(d2, d2, nb))
+  bc = BoundaryCond
+  ALLOCATE(hb(d2, d2, nb), ub(d2, d2, nu))
   DO i = 1, nb
     hb(:, :, i) = H(i)%m
   END DO
-  CALL tebd_check(ctx, tebd_construct_propagators(c_loc(hb), INT(nb, c_int), INT(localSize, c_int), &
-                  REAL(dt, c_double), AIMAG(dt), c_loc(ub)), 'ConstructPropagators')
-  DO i = 1, nb
+  rc = tebd_construct_propagators(c_loc(hb), INT(systemSize, c_int), INT(localSize, c_int), &
+                                  INT(trotterOrder, c_int), bc, REAL(dt, c_double), AIMAG(dt), c_loc(ub))
+  IF (rc /= nu) STOP 'ConstructPropagators: propagator table size does not match trotterOrder / BoundaryCond'
+  DO i = 1, nu
     U(i)%m = ub(:, :, i)
   END DO
 END SUBROUTINE ConstructPropagators

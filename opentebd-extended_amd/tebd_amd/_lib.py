@@ -65,7 +65,7 @@ def lib():
         L.tebd_get_stats.argtypes = [P, P, I]
         L.tebd_sync.argtypes = [P]
         L.tebd_memory_info.argtypes = [P, P, I]
-        L.tebd_construct_propagators.argtypes = [P, I, I, D, D, P]
+        L.tebd_construct_propagators.argtypes = [P, I, I, I, ctypes.c_char, D, D, P]
         L.tebd_imag_time_prop.argtypes = [P, P, I, D, I, I, I, D, D, P]
         L.tebd_set_autotune.argtypes = [P, I]
         L.tebd_set_lanes.argtypes = [P, I]

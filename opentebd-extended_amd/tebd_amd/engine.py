@@ -100,7 +100,8 @@ class Context:
         self._check(self._lib.tebd_set_cutoff(self._h, float(cutoff)))
 
     def set_gates(self, U):
-        """U: (L-1, d^2, d^2) complex gates indexed [bond, row, col]."""
+        """U: the propagator table of the context's schedule, (n, d^2, d^2) complex indexed
+        [entry, row, col]; n = L-1 for 2nd order with open boundaries (one gate per bond)."""
         u = np.ascontiguousarray(np.transpose(np.asarray(U, np.complex128), (0, 2, 1)))
         self._check(self._lib.tebd_set_gates(self._h, _p(u), u.shape[0]))
 
@@ -288,9 +289,10 @@ class Ensemble(Context):
             self.set_cutoff(cutoff)
 
     def set_gates(self, U):
-        """U: (nchain, L-1, d^2, d^2) complex gates, chain by chain."""
+        """U: (nchain, n, d^2, d^2) complex, every chain's propagator table (n = L-1 for 2nd
+        order with open boundaries)."""
         U = np.asarray(U, np.complex128)
-        n = self.nchain * (self.chain_len - 1)
+        n = self.nchain * U.shape[1]
         u = np.ascontiguousarray(np.transpose(U.reshape(n, self.d ** 2, self.d ** 2), (0, 2, 1)))
         self._check(self._lib.tebd_set_gates(self._h, _p(u), n))
 
@@ -351,18 +353,22 @@ class Ensemble(Context):
         return out
 
 
-def construct_propagators(H, dt):
-    """ConstructPropagators (TEBDtools_module.f90:1079-1101) by the library's host routine:
-    exp(-i t H(b)) with t = dt/2 on odd bonds (1-based) and dt on even ones; dt complex
-    (imaginary time: dt = -i dtau).  H: (L-1, d^2, d^2) -> U of the same shape."""
+def construct_propagators(H, dt, order=2, bc="O"):
+    """ConstructPropagators (TEBDtools_module.f90:1079-1193) by the library's host routine:
+    the propagator table of the Trotter schedule (order 2 or 5, open 'O' or periodic 'P'
+    boundaries), entry k = exp(-i f_k dt H(bond_k)); dt complex (imaginary time:
+    dt = -i dtau).  H: (L-1, d^2, d^2), or (L, d^2, d^2) with periodic boundaries (the last
+    one the edge) -> U: (table size, d^2, d^2)."""
     H = np.asarray(H, np.complex128)
     nb, n = H.shape[0], H.shape[1]
     d = int(round(np.sqrt(n)))
+    L = nb if bc == "P" else nb + 1
+    size = {2: (L if bc == "P" else L - 1), 5: (2 * (L - 1) if bc != "P" else (2 * L if L % 2 == 0 else 3 * L))}
     h = np.ascontiguousarray(np.transpose(H, (0, 2, 1)))
-    u = np.zeros_like(h)
+    u = np.zeros((size[int(order)], n, n), np.complex128)
     dt = complex(dt)
-    rc = _lib.lib().tebd_construct_propagators(_p(h), nb, d, dt.real, dt.imag, _p(u))
-    if rc != 0:
+    rc = _lib.lib().tebd_construct_propagators(_p(h), L, d, int(order), bc.encode()[:1], dt.real, dt.imag, _p(u))
+    if rc != u.shape[0]:
         raise TEBDError(rc, "tebd_construct_propagators failed")
     return np.transpose(u, (0, 2, 1))
 

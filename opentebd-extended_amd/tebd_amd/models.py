@@ -82,10 +82,12 @@ def fermion_ops(max_filling=2):
 # ---------------------------------------------------------------------------
 
 
-def heisenberg(L, spin=0.5, Jx=1.0, Jy=1.0, Jz=1.0, field=None):
+def heisenberg(L, spin=0.5, Jx=1.0, Jy=1.0, Jz=1.0, field=None, bc="O"):
     """Jx SxSx + Jy SySy + Jz SzSz per bond (HamiltonianHeisenberg,
     heisenberg_module.f90:143-174) plus a site-dependent field h_i Sz_i split
-    over bonds the way the reference splits its uniform field (:158-167)."""
+    over bonds the way the reference splits its uniform field (:158-167).  With
+    periodic boundaries (bc='P') a last entry H(L) couples site L to site 1, with the
+    coupling only (:169-171)."""
     sx, sy, sz = spin_ops(spin)
     d = sx.shape[0]
     one = np.eye(d)
@@ -98,6 +100,8 @@ def heisenberg(L, spin=0.5, Jx=1.0, Jy=1.0, Jz=1.0, field=None):
             H[b] += h[b] * left + h[b + 1] * right
         H[0] += h[0] * left
         H[L - 2] += h[L - 1] * right
+    if bc == "P":
+        H = np.concatenate([H, base.astype(np.complex128)[None]], axis=0)
     return H
 
 
@@ -154,13 +158,58 @@ def matrix_exponential(H, t):
     return (V * np.exp(-1j * t * E)[None, :]) @ V.conj().T
 
 
-def construct_propagators(H, dt):
-    """2nd-order OBC Trotter gates: odd bonds (1-based) exp(-i dt/2 H),
-    even bonds exp(-i dt H) (NEWConstructPropagators, TEBDtools_module.f90:1091-1101)."""
+# Forest-Ruth theta as the reference evaluates it, 1 / (2 - 2**(1.0/3.0)) with a
+# single-precision exponent 1.0/3.0 (TEBDtools_module.f90:1135): 1.3512072077991604
+# instead of 1.3512071919596578.
+FR_THETA = 1.0 / (2.0 - 2.0 ** float(np.float32(1.0) / np.float32(3.0)))
+
+
+def gate_table(L, order=2, bc="O"):
+    """The propagator table of NEWConstructPropagators (TEBDtools_module.f90:1104-1193)
+    as a list of (bond, time factor) per entry, bond 1-based (L = the edge bond between
+    sites L and 1), so that entry k is exp(-i factor dt H(bond)).  Its length is
+    AllocateProp's (:937-998): L-1 (2nd order, open), L (2nd, periodic), 2(L-1) (5th,
+    open), 2L or 3L (5th, periodic, L even or odd)."""
+    th = FR_THETA
+    n = L - 1
+    if order == 2 and bc == "O":
+        return [(i, 0.5 if i % 2 == 1 else 1.0) for i in range(1, L)]
+    if order == 2 and bc == "P":
+        even = 1.0 if L % 2 == 0 else 0.5
+        return [(i, 0.5 if i % 2 == 1 else even) for i in range(1, L)] + [(L, 1.0)]
+    if order != 5:
+        raise ValueError("trotter order must be 2 or 5")
+    if bc == "O" or L % 2 == 0:
+        size = 2 * n if bc == "O" else 2 * L
+        tab = [None] * size
+        for i in range(1, L):
+            f = (0.5 * th, 0.5 * (1.0 - th)) if i % 2 == 1 else (th, 1.0 - 2.0 * th)
+            tab[2 * (i - 1)], tab[2 * (i - 1) + 1] = (i, f[0]), (i, f[1])
+        if bc == "P":
+            tab[2 * n], tab[2 * n + 1] = (L, th), (L, 1.0 - 2.0 * th)
+        return tab
+    tab = [None] * (3 * L)
+    for i in range(1, L, 2):
+        tab[3 * (i - 1)], tab[3 * (i - 1) + 1] = (i, 0.5 * th), (i, 0.5 * (1.0 - th))
+    for i in range(2, L, 2):
+        b = 2 + 3 * (i - 2)
+        tab[b:b + 4] = [(i, 0.5 * th * th), (i, 0.5 * (1.0 - th) * th), (i, 0.5 * (1.0 - 2.0 * th) * th),
+                        (i, 0.5 * (1.0 - th) * (1.0 - 2.0 * th))]
+    tab[3 * n:3 * n + 3] = [(L, th * th), (L, (1.0 - 2.0 * th) * th), (L, (1.0 - 2.0 * th) ** 2)]
+    return tab
+
+
+def construct_propagators(H, dt, order=2, bc="O"):
+    """Trotter gates of the reference's schedules (NEWConstructPropagators,
+    TEBDtools_module.f90:1079-1193): entry k of gate_table is exp(-i factor dt H(bond)).
+    2nd order, open boundaries: odd bonds (1-based) exp(-i dt/2 H), even bonds
+    exp(-i dt H).  H: L-1 bond Hamiltonians (open) or L (periodic, H[L-1] the edge)."""
     H = np.asarray(H)
-    U = np.empty_like(H, dtype=np.complex128)
-    for b in range(H.shape[0]):
-        U[b] = matrix_exponential(H[b], 0.5 * dt if b % 2 == 0 else dt)
+    L = H.shape[0] + 1 if bc == "O" else H.shape[0]
+    tab = gate_table(L, order, bc)
+    U = np.empty((len(tab),) + H.shape[1:], dtype=np.complex128)
+    for k, (b, f) in enumerate(tab):
+        U[k] = matrix_exponential(H[b - 1], f * dt)
     return U
 
 

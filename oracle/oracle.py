@@ -42,6 +42,7 @@ def lib():
         L.orc_one_site_expval.argtypes = [I, I, I, P, P, P, P]
         L.orc_chain_entropy.argtypes = [I, I, I, P, P]
         L.orc_set_checker_speed.argtypes = [I]
+        L.orc_trotter_step_sched.argtypes = [I, I, I, P, P, P, I, I, ctypes.c_char, P]
         _LIB = L
     return _LIB
 
@@ -101,6 +102,17 @@ class OracleState:
         info = lib().orc_trotter_step(*self._args(), _p(g), ctypes.byref(t), int(nthreads))
         if info != 0:
             raise RuntimeError(f"zgesvd info={info}")
+        return t.value
+
+    def trotter_step_sched(self, gates, order=2, bc="O"):
+        """TrotterStep of any of the reference's schedules: order 2 or 5, open ('O') or
+        periodic ('P') boundaries; gates = the matching propagator table."""
+        g = _gate_stack(gates, self.d)
+        t = ctypes.c_double()
+        info = lib().orc_trotter_step_sched(*self._args(), _p(g), g.shape[0], int(order), bc.encode()[:1],
+                                            ctypes.byref(t))
+        if info != 0:
+            raise RuntimeError(f"zgesvd info={info} (or a gate table of the wrong size)")
         return t.value
 
     def canonical_form_all(self):

@@ -23,7 +23,9 @@
 !                             canon=1 runs CanonicalFormAll after each step
 !   (re,im) coefficients of the product state, d per site, L sites
 !   mode                      0 = trajectory, 1 = single TwoSiteOp on a given state,
-!                             2 = ImagTimeProp (propagation_module.f90:636-760) with chi = chiMax
+!                             2 = ImagTimeProp (propagation_module.f90:636-760) with chi = chiMax,
+!                             3 = trajectory with the Trotter order and boundary of the next line
+!   [mode 3: trotterOrder BoundaryCond  (2 or 5, 'O' or 'P'); periodic adds H(L), the L-1 edge]
 !   [mode 1: link, then Gamma(chi,d,chi) re im for each site, Lambda(chi) for L+1 links]
 !   [mode 2: chiIn dtITP maxITPsteps stepsForJudge convCriterion1 convCriterion2]
 PROGRAM golden_driver
@@ -45,7 +47,8 @@ REAL(KIND=rKind), ALLOCATABLE :: field(:), expl(:), expn(:), buf(:)
 REAL(KIND=rKind) :: c1, c2, c3, dtre, dtim, err, energy, totTrunc
 REAL(KIND=rKind), ALLOCATABLE :: sz(:,:), nop(:,:)
 INTEGER :: model, L, param, chi, nsteps, canon, i, j, k, step, nf, mode, link, chiIn
-INTEGER :: a, b, dd
+INTEGER :: a, b, dd, nH
+CHARACTER :: bcc
 CHARACTER(len=256) :: jobfile
 
 CALL GET_COMMAND_ARGUMENT(1, jobfile)
@@ -91,8 +94,14 @@ DO i=1,L
   END DO
 END DO
 READ(11,*) mode
+nH = L-1
+IF (mode == 3) THEN
+  READ(11,*) trotterOrder, bcc
+  BoundaryCond = bcc
+  IF (BoundaryCond == 'P') nH = L
+END IF
 
-CALL AllocateOps(H, L-1, dd*dd)
+CALL AllocateOps(H, nH, dd*dd)
 IF (model == 1) THEN
   CALL HamiltonianHeisenberg(H, c1, c2, c3)
   DO i=1,L-1
@@ -123,7 +132,7 @@ ELSE
 END IF
 
 PRINT '(A,3I6)', 'DIMS', L, dd, chi
-DO i=1,L-1
+DO i=1,nH
   PRINT '(A,I6)', 'H', i
   DO a=1,dd*dd
     PRINT '(64ES26.17E3)', (REAL(H(i)%m(a,b)), AIMAG(H(i)%m(a,b)), b=1,dd*dd)
@@ -148,7 +157,7 @@ END IF
 
 CALL AllocateProp(U)
 CALL ConstructPropagators(H, U, CMPLX(dtre, dtim, KIND=rKind))
-DO i=1,L-1
+DO i=1,SIZE(U)
   PRINT '(A,I6)', 'U', i
   DO a=1,dd*dd
     PRINT '(64ES26.17E3)', (REAL(U(i)%m(a,b)), AIMAG(U(i)%m(a,b)), b=1,dd*dd)

@@ -14,6 +14,10 @@
 //                                               (local_operations_module.f90:347-379),
 //   * Gamma = U/lambda_l, V^T/lambda_r with 1e-15 guard (local_operations_module.f90:381-428),
 //   * 2nd-order OBC schedule odd/even/odd       (propagation_module.f90:82-114),
+//   * 2nd-order PBC, 5th-order (Forest-Ruth) OBC and PBC schedules, with the edge gate
+//     applied after swapping site L next to site 1 (propagation_module.f90:116-614;
+//     Swapping, local_operations_module.f90:635-695: theta with its two physical
+//     indices exchanged, SVD without the normalisation of ReshapeTheta),
 //   * CanonicalForm / CanonicalFormAll          (local_operations_module.f90:496-633,
 //                                                propagation_module.f90:617-634),
 //   * observables: TotalEnergy, OneSiteExpVal, ChainEntropy
@@ -81,7 +85,9 @@ int ref_zgesvd(int n, cplx* A, double* S, cplx* U, cplx* VT) {
 }
 
 // TwoSiteOp_c (local_operations_module.f90:463-494); link is 0-based bond (sites link, link+1).
-int two_site_op(const View& v, int link, const cplx* Uop, double* truncerr) {
+// With swap = true it is Swapping (local_operations_module.f90:665-695) instead: Uop is
+// ignored, theta's physical indices are exchanged and theta is not normalised.
+int two_site_op(const View& v, int link, const cplx* Uop, double* truncerr, bool swap = false) {
   const int chi = v.chi, d = v.d, n = chi * d;
   const double* l0 = v.lm(link);
   const double* l1 = v.lm(link + 1);
@@ -118,6 +124,7 @@ int two_site_op(const View& v, int link, const cplx* Uop, double* truncerr) {
           }
   }
   // ThetaOperation_c: Theta'(a,i,j,b) = sum_{k,l} U((i-1)d+j,(k-1)d+l) Theta(a,k,l,b)
+  // (SwapTheta for a swap: Theta'(a,i,j,b) = Theta(a,j,i,b))
   const int d2 = d * d;
   std::vector<cplx> tmp(d2);
   for (int b = 0; b < chi; ++b)
@@ -127,14 +134,18 @@ int two_site_op(const View& v, int link, const cplx* Uop, double* truncerr) {
       for (int i = 0; i < d; ++i)
         for (int j = 0; j < d; ++j) {
           cplx acc = 0.0;
-          for (int kl = 0; kl < d2; ++kl) acc += Uop[(i * d + j) + (size_t)d2 * kl] * tmp[kl];
+          if (swap)
+            acc = tmp[j * d + i];
+          else
+            for (int kl = 0; kl < d2; ++kl) acc += Uop[(i * d + j) + (size_t)d2 * kl] * tmp[kl];
           th(a, i, j, b) = acc;
         }
     }
-  // ReshapeTheta: normalise, ThetaRS((i-1)chi+a,(j-1)chi+b) = Theta/norm
+  // ReshapeTheta: normalise, ThetaRS((i-1)chi+a,(j-1)chi+b) = Theta/norm (a swap keeps
+  // theta as it is)
   double nrm2 = 0.0;
   for (auto& z : theta) nrm2 += std::norm(z);
-  const double nrm = std::sqrt(nrm2);
+  const double nrm = swap ? 1.0 : std::sqrt(nrm2);
   std::vector<cplx> RS((size_t)n * n);
   for (int b = 0; b < chi; ++b)
     for (int j = 0; j < d; ++j)
@@ -309,6 +320,67 @@ int orc_trotter_step(int L, int d, int chi, double* G, double* lam, const double
       inf[k] = two_site_op(v, bond - 1, Ug + gs * (bond - 1), &te[k]);
     }
     for (int k = 0; k < nb; ++k) { total += te[k]; info |= inf[k]; }
+  }
+  *total_truncerr = total;
+  return info;
+}
+
+// One TrotterStep of any of the reference's schedules (propagation_module.f90:42-614):
+// order 2 or 5, bc 'O' or 'P'; gates = the propagator table of NEWConstructPropagators
+// (TEBDtools_module.f90:1079-1193) for that order and boundary, ngates entries of
+// d^2 x d^2 complex, entry k at gates + k d^4 (1-based index k+1 in the reference).
+int orc_trotter_step_sched(int L, int d, int chi, double* G, double* lam, const double* gates, int ngates,
+                           int order, char bc, double* total_truncerr) {
+  View v{L, d, chi, (cplx*)G, lam};
+  const size_t gs = (size_t)d * d * d * d;
+  const cplx* Ug = (const cplx*)gates;
+  double total = 0.0, t = 0.0;
+  int info = 0;
+  const bool pbc = bc == 'P' || bc == 'p';
+  const int want = order == 2 ? (pbc ? L : L - 1) : (!pbc ? 2 * (L - 1) : (L % 2 == 0 ? 2 * L : 3 * L));
+  if ((order != 2 && order != 5) || ngates != want) return -1;
+  // gate of bond i (1-based) in the reference's 1-based table slot k
+  auto op = [&](int i, int k) {
+    info |= two_site_op(v, i - 1, Ug + gs * (k - 1), &t);
+    total += t;
+  };
+  // every odd (first = 1) or even (first = 2) bond, slot k(i)
+  auto sweep = [&](int first, auto slot) {
+    for (int i = first; i <= L - 1; i += 2) op(i, slot(i));
+  };
+  // site L next to site 1 by swaps L-1 .. 2, the edge gate on bond 1, swaps 2 .. L-1 back
+  auto edge = [&](int k) {
+    for (int i = L - 1; i >= 2; --i) { info |= two_site_op(v, i - 1, nullptr, &t, true); total += t; }
+    op(1, k);
+    for (int i = 2; i <= L - 1; ++i) { info |= two_site_op(v, i - 1, nullptr, &t, true); total += t; }
+  };
+  auto s1 = [](int i) { return i; };
+  if (order == 2 && !pbc) {  // :82-114
+    sweep(1, s1); sweep(2, s1); sweep(1, s1);
+  } else if (order == 2) {   // :116-198
+    sweep(1, s1);
+    if (L % 2 == 1) { sweep(2, s1); edge(L); sweep(2, s1); }
+    else { edge(L); sweep(2, s1); }
+    sweep(1, s1);
+  } else if (!pbc || L % 2 == 0) {  // :200-245 and the even-L branch of :247-614
+    auto a = [](int i) { return 2 * (i - 1) + 1; };
+    auto b = [](int i) { return 2 * (i - 1) + 2; };
+    const int e1 = 2 * (L - 1) + 1, e2 = 2 * (L - 1) + 2;
+    if (!pbc) {
+      sweep(1, a); sweep(2, a); sweep(1, b); sweep(2, b); sweep(1, b); sweep(2, a); sweep(1, a);
+    } else {
+      sweep(1, a); edge(e1); sweep(2, a); sweep(1, b); edge(e2); sweep(2, b); sweep(1, b); edge(e1); sweep(2, a);
+      sweep(1, a);
+    }
+  } else {  // odd-L branch of :247-614
+    auto o1 = [](int i) { return 3 * (i - 1) + 1; };
+    auto o2 = [](int i) { return 3 * (i - 1) + 2; };
+    auto ev = [](int n) { return [n](int i) { return 2 + 3 * (i - 2) + n; }; };
+    const int e = 3 * (L - 1);
+    sweep(1, o1); sweep(2, ev(1)); edge(e + 1); sweep(2, ev(2)); edge(e + 2); sweep(2, ev(2)); edge(e + 1);
+    sweep(2, ev(1)); sweep(1, o2); sweep(2, ev(3)); edge(e + 2); sweep(2, ev(4)); edge(e + 3); sweep(2, ev(4));
+    edge(e + 2); sweep(2, ev(3)); sweep(1, o2); sweep(2, ev(1)); edge(e + 1); sweep(2, ev(2)); edge(e + 2);
+    sweep(2, ev(2)); edge(e + 1); sweep(2, ev(1)); sweep(1, o1);
   }
   *total_truncerr = total;
   return info;

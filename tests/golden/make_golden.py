@@ -39,11 +39,15 @@ def write_job(path, model, L, param, chi, consts, field, dt, nsteps, canon, coef
             f.write(extra)
 
 
+# progress lines the reference's Swapping prints on every call (local_operations_module.f90:680-693)
+SWAP_CHATTER = ("chis", "theta", "SVD", "L1", "G1", "G2")
+
+
 def parse(text):
     blocks = []
     for line in text.splitlines():
         tok = line.split()
-        if not tok:
+        if not tok or tok[0] in SWAP_CHATTER:
             continue
         if tok[0][0].isalpha():
             head = tok[0]
@@ -264,8 +268,34 @@ def itp_stages():
     print("itp_stages_l10 stages", steps, conv, "energy", res["energy"][-1])
 
 
+def schedules():
+    """Trajectories under the reference's other Trotter schedules (mode 3): 5th-order
+    Forest-Ruth with open boundaries, and 2nd / 5th order with periodic boundaries (edge
+    gate after swapping site L next to site 1), for even and odd L.  Spin-1/2 XXZ with
+    h_i~U[-1/2,1/2] (seed 21) from a random product state (seed 22); chi=16 is full rank
+    at L=8, chi=8 truncates at L=9."""
+    tmp = tempfile.mkdtemp()
+    cases = {"sched_o5_obc_l10": (10, 16, 5, "O", 20), "sched_o2_pbc_l8": (8, 16, 2, "P", 15),
+             "sched_o2_pbc_l9": (9, 8, 2, "P", 15), "sched_o5_pbc_l8": (8, 16, 5, "P", 10),
+             "sched_o5_pbc_l9": (9, 8, 5, "P", 10)}
+    for name, (L, chi, order, bc, nsteps) in cases.items():
+        h = np.random.default_rng(21).uniform(-0.5, 0.5, L)
+        r = np.random.default_rng(22)
+        c = r.normal(size=(L, 2)) + 1j * r.normal(size=(L, 2))
+        c /= np.linalg.norm(c, axis=1, keepdims=True)
+        path = os.path.join(tmp, name + ".txt")
+        write_job(path, 1, L, 1, chi, [1.0, 1.0, 0.7], h, 0.05 + 0j, nsteps, 0, c, mode=3, extra=f"{order} {bc}\n")
+        res = collect(run(path), L, 2, chi)
+        meta = dict(model=1, L=L, param=1, chi=chi, d=2, consts=np.array([1.0, 1.0, 0.7]), dt=np.complex128(0.05),
+                    nsteps=nsteps, canon=0, coeffs=c, field=h, order=order, bc=bc)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **meta, **res)
+        print(name, "energy[-1]=", res["energy"][-1], "sum trunc=", np.sum(res["trunc"]), "gates", len(res["U"]))
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "itp_stages":
         itp_stages()
+    elif len(sys.argv) > 1 and sys.argv[1] == "schedules":
+        schedules()
     else:
         main()

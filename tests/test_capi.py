@@ -26,8 +26,9 @@ def test_library_exports_every_declared_symbol():
 @pytest.mark.parametrize("args", [
     (1, 2, 4, 2, b"O"),      # L < 2
     (8, 7, 4, 2, b"O"),      # unsupported local dimension
-    (8, 2, 4, 5, b"O"),      # 5th-order schedule not provided
-    (8, 2, 4, 2, b"P"),      # periodic boundaries not provided
+    (8, 2, 4, 3, b"O"),      # Trotter order other than 2 or 5
+    (8, 2, 4, 2, b"X"),      # boundary other than open or periodic
+    (2, 2, 4, 2, b"P"),      # periodic chain of fewer than 3 sites
     (8, 2, 0, 2, b"O"),      # chi_max < 1
     (8, 4, 2048, 2, b"O"),   # d * chi_max beyond the SVD work size
 ])
@@ -51,7 +52,7 @@ def test_null_context_calls_are_errors():
     (0, 8, 2, 4, 2, b"O"),   # no chain
     (3, 1, 2, 4, 2, b"O"),   # chains of one site
     (3, 8, 7, 4, 2, b"O"),   # unsupported local dimension
-    (3, 8, 2, 4, 2, b"P"),   # periodic boundaries not provided
+    (3, 8, 2, 4, 2, b"P"),   # periodic boundaries only for a single chain
 ])
 def test_create_ensemble_rejects_bad_arguments_without_touching_the_gpu(args):
     lib = _lib.lib()
