@@ -94,6 +94,25 @@ int tebd_download_packed(tebd_ctx* ctx, double* gamma, double* lambda, int chi_o
  * (propagation_module.f90:704-738) read. */
 int tebd_download_lambdas(tebd_ctx* ctx, double* lambda, int chi_out);
 
+/* Checkpoint files of a Vidal state in the reference's text formats (RecordLambdas and
+ * RecordGammas, ToolsCore_v_2.0/io_module.f90:191-260, read as readGammaLambda does,
+ * :284-373): the lambda file holds L+1 records of chi values, the gamma file for every site,
+ * alpha and beta a record of the d real parts of Gamma(alpha, i, beta), then the same of the
+ * imaginary parts.  kind 'S' formats values as Fortran's E30.15 (15 significant digits),
+ * kind 'B' as B80 (the 64 bits; a restart from it is bit for bit).  gamma / lambda are
+ * host arrays in the padded layout of tebd_upload_packed.  Host-only (no device call). */
+int tebd_write_state_files(int L, int d, int chi, const double* gamma, const double* lambda, const char* lambda_path,
+                           const char* gamma_path, char kind);
+int tebd_read_state_files(int L, int d, int chi, double* gamma, double* lambda, const char* lambda_path,
+                          const char* gamma_path, char kind);
+/* Checkpoint of the context's state to those files at its chi_max (the size the reference's
+ * arrays are recorded at), and restart from files recorded at chi (<= chi_max), e.g. by the
+ * reference itself.  Every failure -- files missing or short, chi too large -- returns an
+ * error (the reference's RestartParallel returns success after a mismatch,
+ * Extensions_Core_v_2.0/SitesParallel_Extension_v_1.0/SitesParallelSetup_Module.f90:722-733). */
+int tebd_checkpoint(tebd_ctx* ctx, const char* lambda_path, const char* gamma_path, char kind);
+int tebd_restart(tebd_ctx* ctx, const char* lambda_path, const char* gamma_path, char kind, int chi);
+
 /* One Trotter step of the context's schedule; *truncerr = sum of the bond truncation
  * errors.  2nd order, open boundaries: odd bonds (gate dt/2), even bonds (dt), odd bonds
  * (dt/2).  The other schedules follow TrotterStep2ndOrderPBC, TrotterStep5thOrder and
@@ -157,6 +176,20 @@ int tebd_construct_propagators(const double* H, int L, int d, int order, char bc
  * (TotalNumber, TotalEnergy, TotalOneSite at each judgement) are not formed. */
 int tebd_imag_time_prop(tebd_ctx* ctx, const double* H, int nh, double dtau, int chi_in, int max_steps,
                         int steps_for_judge, double conv1, double conv2, int* steps);
+
+/* Two-site reduced density matrices of every pair of sites l1 < l2 of each chain, and the
+ * one-site ones, formed on the device from the resident canonical state the way
+ * TwoSiteExpVal and EvaluateMeasures form them (observables_module.f90:836-883,2286-2526):
+ * for each right site l2 the environment ThetaKernal(l2) is carried leftwards one site at
+ * a time (ThetaNext) and each step's TwoSiteRho is one pair.  With phase (d reals, the
+ * diagonal of a one-site operator such as the fermi phase (-1)^n) every ket Gamma of the
+ * sites l1 .. l2-1 carries it, as the reference's phaseStat / fermicorr do; NULL for none.
+ * rho2 (host): per chain Lc(Lc-1)/2 matrices of d^2 x d^2 complex, column-major, pair
+ * (l1, l2) (0-based within the chain) at index l1 Lc - l1(l1+1)/2 + (l2 - l1 - 1); row
+ * i d + j, column l d + k for ket i, bra l on site l1 and ket j, bra k on site l2 (the
+ * reference's rho2((i-1)d+j,(l-1)d+k)).  rho1 (host, or NULL): L matrices of d x d, element
+ * i + d j (FormSingleSiteRho, :95-117).  Tr(O rho) gives any correlator or local value. */
+int tebd_two_point_rdms(tebd_ctx* ctx, const double* phase, double* rho2, double* rho1);
 
 /* Singular-value cutoff on the normalised weight s^2/||theta||^2.  Default
  * 1e-30, the reference's rule (keep |s| > 1e-15 after normalisation,

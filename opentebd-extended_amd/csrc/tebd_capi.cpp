@@ -96,6 +96,12 @@ struct tebd_ctx {
   double* results = nullptr;
   double* results2 = nullptr;    // learned-rank path: per task ||theta||^2 and column norms of theta
   zc* obs_ops = nullptr;         // (L-1) x d^4 operators of tebd_expval_two_site, allocated on first use
+  // all-pairs reduced density matrices (tebd_two_point_rdms), allocated on first use:
+  // environments Theta (two, chi^2 d^2), T (chi^2 d^3), three chi d chi factors, row scales,
+  // the phase, and the device copies of the pair (d^4) and site (d^2) matrices
+  zc* cr_theta[2] = {nullptr, nullptr};
+  zc *cr_T = nullptr, *cr_K = nullptr, *cr_B = nullptr, *cr_A = nullptr, *cr_rho2 = nullptr, *cr_rho1 = nullptr;
+  double *cr_s = nullptr, *cr_phase = nullptr;
   zc* ident = nullptr;           // d^4 identity gate
   // rank model (learned-rank path) and the training record of the full path
   float* mlp = nullptr;          // TEBD_MLP_PARAMS on the device
@@ -1149,6 +1155,8 @@ tebd_ctx* make_lane(tebd_ctx* c, int lane, int slot0, int nslots) {
   l->cf_lam = nullptr;
   l->cf_perm = nullptr;
   l->cf_sets = 0;
+  l->cr_theta[0] = l->cr_theta[1] = l->cr_T = l->cr_K = l->cr_B = l->cr_A = l->cr_rho2 = l->cr_rho1 = nullptr;
+  l->cr_s = l->cr_phase = nullptr;
   l->owns_gemm = false;
   l->owns_stream = lane != 0;
   l->st = lane == 0 ? c->st : nullptr;
@@ -1431,6 +1439,10 @@ void free_all(tebd_ctx* c) {
   (void)hipFree(c->results);
   (void)hipFree(c->results2);
   (void)hipFree(c->obs_ops);
+  for (zc* p : {c->cr_theta[0], c->cr_theta[1], c->cr_T, c->cr_K, c->cr_B, c->cr_A, c->cr_rho2, c->cr_rho1})
+    (void)hipFree(p);
+  (void)hipFree(c->cr_s);
+  (void)hipFree(c->cr_phase);
   (void)hipFree(c->mlp);
   (void)hipFree(c->d_feat);
   (void)hipFree(c->d_pred);
@@ -2356,6 +2368,114 @@ int tebd_two_site_op_noalloc(int d, int chi_l, int chi_c, int chi_r, int chi_cap
   }
   *chi_c_new = (int)h_results[0];
   *truncerr = h_results[1];
+  return TEBD_OK;
+}
+
+int tebd_checkpoint(tebd_ctx* c, const char* lambda_path, const char* gamma_path, char kind) {
+  int e = ensure_state(c, false);
+  if (e) return e;
+  const int chi = c->chi_max;  // the reference records its arrays at their allocated size
+  std::vector<double> g((size_t)2 * chi * c->d * chi * c->L), l((size_t)(c->L + 1) * chi);
+  std::vector<double*> gp(c->L), lp(c->L + 1);
+  for (int s = 0; s < c->L; ++s) gp[s] = g.data() + (size_t)2 * chi * c->d * chi * s;
+  for (int b = 0; b <= c->L; ++b) lp[b] = l.data() + (size_t)chi * b;
+  if ((e = download_impl(c, gp.data(), lp.data(), chi))) return e;
+  e = tebd_write_state_files(c->L, c->d, chi, g.data(), l.data(), lambda_path, gamma_path, kind);
+  return e ? fail(c, e, "checkpoint: cannot write the state files") : TEBD_OK;
+}
+
+int tebd_restart(tebd_ctx* c, const char* lambda_path, const char* gamma_path, char kind, int chi) {
+  if (!c) return TEBD_ERR_ARG;
+  if (chi < 1 || chi > c->chi_max) return fail(c, TEBD_ERR_SHAPE, "restart: file chi exceeds the context's chi_max");
+  std::vector<double> g((size_t)2 * chi * c->d * chi * c->L), l((size_t)(c->L + 1) * chi);
+  int e = tebd_read_state_files(c->L, c->d, chi, g.data(), l.data(), lambda_path, gamma_path, kind);
+  if (e == TEBD_ERR_ARG) return fail(c, e, "restart: cannot open the state files (or bad kind)");
+  if (e) return fail(c, e, "restart: the state files are shorter than L, d and chi require, or unreadable");
+  std::vector<const double*> gp(c->L), lp(c->L + 1);
+  for (int s = 0; s < c->L; ++s) gp[s] = g.data() + (size_t)2 * chi * c->d * chi * s;
+  for (int b = 0; b <= c->L; ++b) lp[b] = l.data() + (size_t)chi * b;
+  return upload_impl(c, gp.data(), lp.data(), chi);
+}
+
+int tebd_two_point_rdms(tebd_ctx* c, const double* phase, double* rho2, double* rho1) {
+  int e = ensure_state(c, false);
+  if (e) return e;
+  if (!rho2) return fail(c, TEBD_ERR_ARG, "two_point_rdms: null output");
+  const int d = c->d, Lc = c->chain_len, chi = c->chi_max, d2 = d * d;
+  const size_t np = (size_t)Lc * (Lc - 1) / 2, d4 = (size_t)d2 * d2;
+  if (!c->cr_T) {
+    const size_t th = (size_t)chi * chi * d2, fac = (size_t)chi * d * chi;
+    HIPCHK(c, hipMalloc(&c->cr_theta[0], sizeof(zc) * th));
+    HIPCHK(c, hipMalloc(&c->cr_theta[1], sizeof(zc) * th));
+    HIPCHK(c, hipMalloc(&c->cr_T, sizeof(zc) * th * d));
+    HIPCHK(c, hipMalloc(&c->cr_K, sizeof(zc) * fac));
+    HIPCHK(c, hipMalloc(&c->cr_B, sizeof(zc) * fac));
+    HIPCHK(c, hipMalloc(&c->cr_A, sizeof(zc) * fac));
+    HIPCHK(c, hipMalloc(&c->cr_s, sizeof(double) * chi * d));
+    HIPCHK(c, hipMalloc(&c->cr_phase, sizeof(double) * d));
+    HIPCHK(c, hipMalloc(&c->cr_rho2, sizeof(zc) * d4 * np * c->nchain));
+    HIPCHK(c, hipMalloc(&c->cr_rho1, sizeof(zc) * d2 * c->L));
+  }
+  if (!c->h_gemm) {
+    HIPCHK(c, hipMalloc(&c->d_gemm, sizeof(GemmTask) * c->ntask_max * 5));
+    HIPCHK(c, hipHostMalloc(&c->h_gemm, sizeof(GemmTask) * c->ntask_max * 5));
+  }
+  const double* ph = nullptr;
+  if (phase) {
+    HIPCHK(c, hipMemcpy(c->cr_phase, phase, sizeof(double) * d, hipMemcpyHostToDevice));
+    ph = c->cr_phase;
+  }
+  // one GEMM at a time; the pinned descriptors of a region are rewritten only after the
+  // stream has consumed them
+  int region = 0;
+  auto gemm = [&](const zc* A, const zc* B, zc* C, int M, int N, int K, int lda, int ldb, int ldc, int acc) {
+    if (region == 5) {
+      HIPCHK(c, hipStreamSynchronize(c->st));
+      region = 0;
+    }
+    GemmTask t = {A, B, C, M, N, K, lda, ldb, ldc, 1.0, acc};
+    return gemm_batch(c, &t, 1, region++, false);
+  };
+  auto slab = [&](int s) { return c->gam + (size_t)s * c->gslab(); };
+  auto lam = [&](int b) { return c->lam + (size_t)b * chi; };
+  for (int r = 0; r < c->nchain; ++r) {
+    const int base = r * Lc;
+    zc* rho2_chain = c->cr_rho2 + d4 * np * r;
+    for (int l2 = Lc - 1; l2 >= 0; --l2) {
+      const int s2 = base + l2, chl = c->chi[s2], chr = c->chi[s2 + 1];
+      // ThetaKernal of site l2 (observables_module.f90:704-730): ket factor (lambda Gamma
+      // lambda as a (chi_l d) x chi_r matrix) times its bra counterpart
+      HIPCHK(c, launch_scaled_copy(slab(s2), chl * d, c->cr_K, chl * d, chl * d, chr, lam(s2), chl, lam(s2 + 1), 1, 0,
+                                   c->st));
+      HIPCHK(c, launch_bra_kernal(slab(s2), lam(s2), lam(s2 + 1), chl, d, chr, c->cr_B, c->st));
+      int cur = 0;
+      if ((e = gemm(c->cr_K, c->cr_B, c->cr_theta[cur], chl * d, d * chl, chr, chl * d, chr, chl * d, 0))) return e;
+      HIPCHK(c, launch_rho1_trace(c->cr_theta[cur], chl, d, c->cr_rho1 + (size_t)d2 * s2, c->st));
+      for (int l1 = l2 - 1; l1 >= 0; --l1) {
+        const int s1 = base + l1, ca = c->chi[s1], cb = c->chi[s1 + 1];
+        // rows of the left site's ket Gamma scaled by lambda(a) phase(i)
+        HIPCHK(c, launch_scale_vec(lam(s1), ph, ca, d, c->cr_s, c->st));
+        // bra Gamma, conjugate-transposed with lambda(a): Gbar(eta, a + chi_a l)
+        HIPCHK(c, launch_scaled_copy(slab(s1), ca * d, c->cr_B, cb, ca * d, cb, lam(s1), ca, nullptr, 1, 1, c->st));
+        // T = Theta Gbar (ThetaNext's / TwoSiteRho's first contraction)
+        if ((e = gemm(c->cr_theta[cur], c->cr_B, c->cr_T, cb * d2, ca * d, cb, cb * d2, cb, cb * d2, 0))) return e;
+        const size_t pair = (size_t)l1 * Lc - (size_t)l1 * (l1 + 1) / 2 + (l2 - l1 - 1);
+        HIPCHK(c, launch_rho_reduce(slab(s1), c->cr_s, c->cr_T, ca, cb, d, rho2_chain + d4 * pair, c->st));
+        if (l1 == 0) break;  // no site further left: the environment is not needed
+        // ket Gamma scaled by lambda(a) phase(k), then Theta' = sum_k A_k T_k
+        HIPCHK(c, launch_scaled_copy(slab(s1), ca * d, c->cr_A, ca * d, ca * d, cb, c->cr_s, ca * d, nullptr, 1, 0,
+                                     c->st));
+        for (int k = 0; k < d; ++k)
+          if ((e = gemm(c->cr_A + (size_t)ca * k, c->cr_T + (size_t)cb * d2 * ca * k, c->cr_theta[1 - cur], ca,
+                        d2 * ca, cb, ca * d, cb, ca, k > 0)))
+            return e;
+        cur = 1 - cur;
+      }
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy(rho2, c->cr_rho2, sizeof(zc) * d4 * np * c->nchain, hipMemcpyDeviceToHost));
+  if (rho1) HIPCHK(c, hipMemcpy(rho1, c->cr_rho1, sizeof(zc) * d2 * c->L, hipMemcpyDeviceToHost));
   return TEBD_OK;
 }
 

@@ -52,5 +52,11 @@ hipError_t launch_perm_cols(const zc* src, int lds, zc* dst, int ldd, int rows, 
 hipError_t launch_cf_xy(const zc* U1, const double* s1, const zc* U2, const double* s2, const double* lam1, zc* X,
                         zc* Y, int n, hipStream_t st);
 hipError_t launch_cf_lambda(const double* s, int n, double* lam, int chi_max, hipStream_t st);
+hipError_t launch_scale_vec(const double* lambda, const double* phase, int chi, int d, double* s, hipStream_t st);
+hipError_t launch_bra_kernal(const zc* G, const double* lam1, const double* lam2, int chi_l, int d, int chi_r, zc* out,
+                             hipStream_t st);
+hipError_t launch_rho_reduce(const zc* G, const double* s, const zc* T, int chi_a, int chi_b, int d, zc* rho,
+                             hipStream_t st);
+hipError_t launch_rho1_trace(const zc* Theta, int chi, int d, zc* rho, hipStream_t st);
 hipError_t launch_shrink_slab(const zc* src, int cl, int d, zc* dst, int ncl, int ncr, hipStream_t st);
 hipError_t launch_zero_lambda_tail(double* lam, int nbonds, int chi, int chi_max, hipStream_t st);

@@ -1,6 +1,7 @@
 """MI355X (gfx950) two-site Trotter update for OpenSourceTEBD-style Vidal states."""
 from . import models, rank_model
 from ._lib import TEBDError, LIB_PATH
-from .engine import Context, Ensemble, construct_propagators, imag_time_prop, parse_mode
+from .engine import (Context, Ensemble, construct_propagators, imag_time_prop, parse_mode, read_state_files,
+                     write_state_files)
 
-__all__ = ["Context", "Ensemble", "TEBDError", "construct_propagators", "imag_time_prop", "models", "rank_model", "parse_mode", "LIB_PATH"]
+__all__ = ["Context", "Ensemble", "TEBDError", "construct_propagators", "imag_time_prop", "read_state_files", "write_state_files", "models", "rank_model", "parse_mode", "LIB_PATH"]

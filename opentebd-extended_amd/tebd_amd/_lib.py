@@ -65,6 +65,12 @@ def lib():
         L.tebd_get_stats.argtypes = [P, P, I]
         L.tebd_sync.argtypes = [P]
         L.tebd_memory_info.argtypes = [P, P, I]
+        L.tebd_two_point_rdms.argtypes = [P, P, P, P]
+        Cs = ctypes.c_char_p
+        L.tebd_write_state_files.argtypes = [I, I, I, P, P, Cs, Cs, ctypes.c_char]
+        L.tebd_read_state_files.argtypes = [I, I, I, P, P, Cs, Cs, ctypes.c_char]
+        L.tebd_checkpoint.argtypes = [P, Cs, Cs, ctypes.c_char]
+        L.tebd_restart.argtypes = [P, Cs, Cs, ctypes.c_char, I]
         L.tebd_construct_propagators.argtypes = [P, I, I, I, ctypes.c_char, D, D, P]
         L.tebd_imag_time_prop.argtypes = [P, P, I, D, I, I, I, D, D, P]
         L.tebd_set_autotune.argtypes = [P, I]
@@ -87,7 +93,8 @@ def lib():
         L.tebd_last_error.restype = ctypes.c_char_p
         for name in ("tebd_create", "tebd_create_ensemble", "tebd_set_gates", "tebd_upload", "tebd_upload_packed", "tebd_download",
                      "tebd_download_packed", "tebd_download_lambdas", "tebd_trotter_step", "tebd_two_site_op", "tebd_canonical_form_all", "tebd_canonical_form",
-                     "tebd_set_cutoff", "tebd_get_bond_dims", "tebd_get_stats", "tebd_sync", "tebd_memory_info", "tebd_construct_propagators", "tebd_imag_time_prop", "tebd_set_autotune", "tebd_set_lanes", "tebd_set_graphs",
+                     "tebd_set_cutoff", "tebd_get_bond_dims", "tebd_get_stats", "tebd_sync", "tebd_memory_info", "tebd_two_point_rdms", "tebd_write_state_files", "tebd_read_state_files", "tebd_checkpoint", "tebd_restart",
+                     "tebd_construct_propagators", "tebd_imag_time_prop", "tebd_set_autotune", "tebd_set_lanes", "tebd_set_graphs",
                      "tebd_autotune_info",
                      "tebd_set_profiling", "tebd_get_profile", "tebd_expval_two_site", "tebd_rank_model_set",
                      "tebd_set_learned_params", "tebd_rank_model_get", "tebd_rank_log", "tebd_rank_log_get", "tebd_mlp_gradient",

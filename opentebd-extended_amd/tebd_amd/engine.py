@@ -120,6 +120,17 @@ class Context:
         self._check(self._lib.tebd_download_packed(self._h, _p(gam), _p(lam), chi))
         return gam, lam
 
+    def checkpoint(self, lambda_path, gamma_path, kind="B"):
+        """The state to checkpoint files in the reference's formats (RecordLambdas /
+        RecordGammas, io_module.f90:191-260) at chi_max; kind 'B' restarts bit for bit."""
+        self._check(self._lib.tebd_checkpoint(self._h, str(lambda_path).encode(), str(gamma_path).encode(),
+                                              kind.encode()[:1]))
+
+    def restart(self, lambda_path, gamma_path, chi, kind="B"):
+        """The state from checkpoint files recorded at chi (readGammaLambda, io_module.f90:284-373)."""
+        self._check(self._lib.tebd_restart(self._h, str(lambda_path).encode(), str(gamma_path).encode(),
+                                           kind.encode()[:1], int(chi)))
+
     def lambdas(self, chi_out=None):
         chi = int(chi_out or self.chi_max)
         lam = np.zeros((self.L + 1, chi))
@@ -148,6 +159,33 @@ class Context:
         out = np.zeros(2 * (self.L - 1))
         self._check(self._lib.tebd_expval_two_site(self._h, _p(buf), _p(out)))
         return out[0::2] + 1j * out[1::2]
+
+    def two_point_rdms(self, phase=None):
+        """Two-site reduced density matrices of every pair l1 < l2 within each chain and the
+        one-site ones (TwoSiteRho / FormSingleSiteRho, observables_module.f90:95-117,704-832),
+        from the device.  phase: diagonal of the ket operator carried over sites l1..l2-1 (the
+        fermi phase), or None.  Returns (rho2, rho1): rho2[(chain,) l1, l2] is a
+        (d^2, d^2) matrix indexed [i d + j, l d + k] (ket i / bra l on l1, ket j / bra k on l2),
+        zero for l1 >= l2; rho1[site] is (d, d)."""
+        d = self.d
+        nchain = getattr(self, "nchain", 1)
+        Lc = self.L // nchain
+        npair = Lc * (Lc - 1) // 2
+        buf = np.zeros((nchain, npair, d * d, d * d), np.complex128)
+        r1 = np.zeros((self.L, d, d), np.complex128)
+        ph = None if phase is None else np.ascontiguousarray(np.real(np.diag(phase) if np.ndim(phase) == 2 else phase),
+                                                             np.float64)
+        self._check(self._lib.tebd_two_point_rdms(self._h, None if ph is None else _p(ph), _p(buf), _p(r1)))
+        rho2 = np.zeros((nchain, Lc, Lc, d * d, d * d), np.complex128)
+        p = 0
+        for l1 in range(Lc):
+            for l2 in range(l1 + 1, Lc):
+                rho2[:, l1, l2] = np.transpose(buf[:, p], (0, 2, 1))  # column-major -> [row, col]
+                p += 1
+        rho1 = np.transpose(r1, (0, 2, 1))
+        if nchain == 1:
+            rho2 = rho2[0]
+        return rho2, rho1
 
     def energy(self, H):
         """TotalEnergy: sum over bonds of Tr(H_b rho_b) (observables_module.f90:1432-1466)."""
@@ -351,6 +389,30 @@ class Ensemble(Context):
                 v = v[v != 0.0]
                 out[r, link - 2] = -np.sum(v * v * np.log(np.abs(v * v))) / np.log(self.d)
         return out
+
+
+def read_state_files(L, d, chi, lambda_path, gamma_path, kind="S"):
+    """A Vidal state (gam [site, b, i, a], lam) from checkpoint files in the reference's
+    formats, recorded at chi (readGammaLambda, io_module.f90:284-373).  Host-only."""
+    gam = np.zeros((L, chi, d, chi), np.complex128)
+    lam = np.zeros((L + 1, chi))
+    rc = _lib.lib().tebd_read_state_files(L, d, chi, _p(gam), _p(lam), str(lambda_path).encode(),
+                                          str(gamma_path).encode(), kind.encode()[:1])
+    if rc != 0:
+        raise TEBDError(rc, "cannot read the state files")
+    return gam, lam
+
+
+def write_state_files(gam, lam, lambda_path, gamma_path, kind="S"):
+    """Checkpoint files of a Vidal state in the reference's formats (RecordLambdas /
+    RecordGammas, io_module.f90:191-260).  Host-only."""
+    gam = np.ascontiguousarray(gam, np.complex128)
+    lam = np.ascontiguousarray(lam, np.float64)
+    L, chi, d, _ = gam.shape
+    rc = _lib.lib().tebd_write_state_files(L, d, chi, _p(gam), _p(lam), str(lambda_path).encode(),
+                                           str(gamma_path).encode(), kind.encode()[:1])
+    if rc != 0:
+        raise TEBDError(rc, "cannot write the state files")
 
 
 def construct_propagators(H, dt, order=2, bc="O"):

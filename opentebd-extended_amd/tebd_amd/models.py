@@ -213,6 +213,32 @@ def construct_propagators(H, dt, order=2, bc="O"):
     return U
 
 
+def read_propagator(real_path, imag_path):
+    """A gate from two list-directed text files of its real and imaginary parts, read as
+    Fortran reads a whole n x n array (column-major), the way the Extensible wrapper loads
+    its Propagator_*.nml files (Case_Studies/Extensible_Wrapper/TEBDSolve.f90:172-206)."""
+    re = np.array(open(real_path).read().replace(",", " ").split(), float)
+    im = np.array(open(imag_path).read().replace(",", " ").split(), float)
+    n = int(round(np.sqrt(re.size)))
+    if n * n != re.size or im.size != re.size:
+        raise ValueError(f"{real_path}, {imag_path}: not the two parts of one square matrix")
+    return re.reshape(n, n, order="F") + 1j * im.reshape(n, n, order="F")
+
+
+def file_gate_table(L, left, bulk_odd, bulk_even, right):
+    """The gate table TEBDSolve.f90:172-206 assembles from files: bond 1 `left`, odd bonds
+    3, 5, .. <= L-2 `bulk_odd`, even bonds 2, 4, .. <= L-2 `bulk_even`, bond L-1 `right`
+    (assigned last, so for odd L it replaces the even-bond gate there)."""
+    U = np.empty((L - 1,) + left.shape, np.complex128)
+    U[0] = left
+    for i in range(3, L - 1, 2):
+        U[i - 1] = bulk_odd
+    for i in range(2, L - 1, 2):
+        U[i - 1] = bulk_even
+    U[L - 2] = right
+    return U
+
+
 # ---------------------------------------------------------------------------
 # initial states (reference padded layout: gam [site, b, i, a], lam [bond, a])
 # ---------------------------------------------------------------------------

@@ -141,3 +141,34 @@ class OracleState:
         out = np.zeros(self.L - 1)
         lib().orc_chain_entropy(self.L, self.d, self.chi, _p(self.lam), _p(out))
         return out
+
+
+def two_point_rdms(gam, lam, phase=None):
+    """All-pairs two-site and one-site reduced density matrices, restating the
+    reference's ThetaKernal / ThetaNext / TwoSiteRho recursion (observables_module.f90:
+    704-832, as TwoSiteExpVal and EvaluateMeasures drive it, :836-883,2286-2526) with numpy
+    on the padded state (gam [site, b, i, a] = Gamma(a, i, b), lam [bond, a]).  phase: the
+    diagonal of the one-site operator applied to every ket Gamma of sites l1 .. l2-1, or
+    None.  Returns rho2 [l1, l2, i d + j, l d + k] for l1 < l2 and rho1 [s, i, j]."""
+    gam = np.asarray(gam)
+    L, chi, d, _ = gam.shape
+    G = np.transpose(gam, (0, 3, 2, 1))  # [site, a, i, b]
+    ph = np.ones(d) if phase is None else np.asarray(phase, float)
+    rho2 = np.zeros((L, L, d * d, d * d), np.complex128)
+    rho1 = np.zeros((L, d, d), np.complex128)
+    for l2 in range(L - 1, -1, -1):
+        l1v, l2v = lam[l2], lam[l2 + 1]
+        # ThetaKernal: Theta(a,i,j,b) = sum_e l1(a) G(a,i,e) l2(e)^2 conj(G(b,j,e)) l1(b)
+        th = np.einsum("a,aie,e,bje,b->aijb", l1v, G[l2], l2v ** 2, G[l2].conj(), l1v, optimize=True)
+        rho1[l2] = np.einsum("aija->ij", th)
+        for l1 in range(l2 - 1, -1, -1):
+            lv = lam[l1]
+            Gp = G[l1] * ph[None, :, None]
+            # ThetaTemp(b,j,k,l,a) = sum_e Theta(b,j,k,e) conj(G(a,l,e)) lam(a)
+            tt = np.einsum("bjke,ale,a->bjkla", th, G[l1].conj(), lv, optimize=True)
+            # rho2((i)d+j,(l)d+k) = sum_{a,b} lam(a) Gp(a,i,b) ThetaTemp(b,j,k,l,a)
+            r = np.einsum("a,aib,bjkla->ijlk", lv, Gp, tt, optimize=True)
+            rho2[l1, l2] = r.reshape(d * d, d * d)
+            # ThetaNext: Theta(a,i,j,b) = sum_{k,e} lam(a) Gp(a,k,e) ThetaTemp(e,i,j,k,b)
+            th = np.einsum("a,ake,eijkb->aijb", lv, Gp, tt, optimize=True)
+    return rho2, rho1

@@ -26,6 +26,14 @@
 !                             2 = ImagTimeProp (propagation_module.f90:636-760) with chi = chiMax,
 !                             3 = trajectory with the Trotter order and boundary of the next line
 !   [mode 3: trotterOrder BoundaryCond  (2 or 5, 'O' or 'P'); periodic adds H(L), the L-1 edge]
+!   mode 4 = trajectory, then the final state written with the reference's checkpoint
+!            writers (io_module RecordLambdas / RecordGammas) in both kinds, 'S' and 'B',
+!            to ckpt_lambdas_S.dat, ckpt_gammas_S.dat, ckpt_lambdas_B.dat, ckpt_gammas_B.dat
+!            in the working directory
+!   mode 5 = trajectory (trotterOrder BoundaryCond line as mode 3), then EvaluateMeasures
+!            (observables_module.f90:2286-2526) with local and averaged Sz (spins) or n,
+!            the correlator of the bond two-site operator Sz x Sz (spins) or n x n, for
+!            fermions the fermi-phase correlator c+_up x c_up, and entropies of level 2
 !   [mode 1: link, then Gamma(chi,d,chi) re im for each site, Lambda(chi) for L+1 links]
 !   [mode 2: chiIn dtITP maxITPsteps stepsForJudge convCriterion1 convCriterion2]
 PROGRAM golden_driver
@@ -38,6 +46,7 @@ USE fermi_hubbard_module
 USE local_operations_module
 USE observables_module
 USE propagation_module
+USE io_module, ONLY: RecordLambdas, RecordGammas, openUnit
 IMPLICIT NONE
 TYPE(tensor), POINTER :: Gammas(:)
 TYPE(vector), POINTER :: Lambdas(:)
@@ -47,7 +56,8 @@ REAL(KIND=rKind), ALLOCATABLE :: field(:), expl(:), expn(:), buf(:)
 REAL(KIND=rKind) :: c1, c2, c3, dtre, dtim, err, energy, totTrunc
 REAL(KIND=rKind), ALLOCATABLE :: sz(:,:), nop(:,:)
 INTEGER :: model, L, param, chi, nsteps, canon, i, j, k, step, nf, mode, link, chiIn
-INTEGER :: a, b, dd, nH
+INTEGER :: a, b, dd, nH, nfc
+TYPE(measure) :: Meas
 CHARACTER :: bcc
 CHARACTER(len=256) :: jobfile
 
@@ -95,7 +105,7 @@ DO i=1,L
 END DO
 READ(11,*) mode
 nH = L-1
-IF (mode == 3) THEN
+IF (mode == 3 .OR. mode == 5) THEN
   READ(11,*) trotterOrder, bcc
   BoundaryCond = bcc
   IF (BoundaryCond == 'P') nH = L
@@ -214,6 +224,56 @@ DO step=1,nsteps
   CALL PrintObs(step, err)
 END DO
 CALL PrintState()
+IF (mode == 5) THEN
+  nfc = 0
+  IF (model == 3) nfc = 1
+  CALL AllocateMeasures(Meas, 1, 1, 1, nfc, 2)
+  IF (model == 2) THEN
+    Meas%local(1)%Op = nop
+    Meas%avg(1)%Op = nop
+    Meas%corr(1)%Op = tensorProd(nop, nop)
+  ELSE
+    Meas%local(1)%Op = sz
+    Meas%avg(1)%Op = sz
+    Meas%corr(1)%Op = tensorProd(sz, sz)
+  END IF
+  IF (model == 3) Meas%fermicorr(1)%Op = tensorProd(TRANSPOSE(a_opS(1)%mr), a_opS(1)%mr)
+  CALL EvaluateMeasures(Meas, Gammas, Lambdas, H)
+  PRINT '(A,ES26.17E3)', 'MEN', Meas%en
+  PRINT '(A,ES26.17E3)', 'MQME', Meas%ent%qme
+  PRINT '(A,256ES26.17E3)', 'MLOCAL', (REAL(Meas%local(1)%value(i)), AIMAG(Meas%local(1)%value(i)), i=1,L)
+  PRINT '(A,2ES26.17E3)', 'MAVG', REAL(Meas%avg(1)%value), AIMAG(Meas%avg(1)%value)
+  PRINT '(A,256ES26.17E3)', 'MVN', (Meas%ent%vN(i), i=1,L)
+  PRINT '(A,256ES26.17E3)', 'MCHAIN', (Meas%ent%chain(i), i=1,L+1)
+  DO i=1,L
+    PRINT '(A,I6)', 'MTB', i
+    PRINT '(256ES26.17E3)', (Meas%ent%tbvN(i,j), j=1,L)
+  END DO
+  DO i=1,L
+    PRINT '(A,I6)', 'MCORR', i
+    PRINT '(512ES26.17E3)', (REAL(Meas%corr(1)%value(i,j)), AIMAG(Meas%corr(1)%value(i,j)), j=1,L)
+  END DO
+  IF (nfc > 0) THEN
+    DO i=1,L
+      PRINT '(A,I6)', 'MFCORR', i
+      PRINT '(512ES26.17E3)', (REAL(Meas%fermicorr(1)%value(i,j)), AIMAG(Meas%fermicorr(1)%value(i,j)), j=1,L)
+    END DO
+  END IF
+END IF
+IF (mode == 4) THEN
+  CALL openUnit('ckpt_lambdas_S.dat', 21)
+  CALL RecordLambdas(21, Lambdas, 'S')
+  CLOSE(21)
+  CALL openUnit('ckpt_gammas_S.dat', 22)
+  CALL RecordGammas(22, Gammas, 'S')
+  CLOSE(22)
+  CALL openUnit('ckpt_lambdas_B.dat', 23)
+  CALL RecordLambdas(23, Lambdas, 'B')
+  CLOSE(23)
+  CALL openUnit('ckpt_gammas_B.dat', 24)
+  CALL RecordGammas(24, Gammas, 'B')
+  CLOSE(24)
+END IF
 
 CONTAINS
 

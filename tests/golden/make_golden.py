@@ -55,7 +55,7 @@ def parse(text):
             rest = tok[1:]
             blocks.append((head, nums))
             # integer tag arguments (indices) for H/U/LAMBDA/GAMMA/AOP/DIMS
-            if head in ("H", "U", "LAMBDA", "GAMMA", "AOP", "DIMS"):
+            if head in ("H", "U", "LAMBDA", "GAMMA", "AOP", "DIMS", "MTB", "MCORR", "MFCORR"):
                 blocks[-1] = (head, nums, [int(t) for t in rest])
                 continue
             nums.extend(float(t) for t in rest)
@@ -292,8 +292,83 @@ def schedules():
         print(name, "energy[-1]=", res["energy"][-1], "sum trunc=", np.sum(res["trunc"]), "gates", len(res["U"]))
 
 
+def measures():
+    """EvaluateMeasures (observables_module.f90:2286-2526) after a trajectory (mode 5):
+    Fermi-Hubbard L=6 chi=16 (with the fermi-phase correlator c+_up x c_up), spin-1/2 XXZ
+    L=8 chi=8 (truncated), and a periodic XXZ chain L=7 (energy with the edge bond), each
+    from a random product state."""
+    tmp = tempfile.mkdtemp()
+    cases = {"meas_fh_l6": (3, 6, 2, 16, [1.0, 4.0, 0.0], None, 4, 0.05, 8, 2, "O"),
+             "meas_xxz_l8": (1, 8, 1, 8, [1.0, 1.0, 0.6], 2, 2, 0.05, 12, 2, "O"),
+             "meas_xxz_pbc_l7": (1, 7, 1, 8, [1.0, 1.0, 0.6], 2, 2, 0.05, 8, 2, "P")}
+    for name, (model, L, param, chi, consts, nf, d, dt, nsteps, order, bc) in cases.items():
+        r = np.random.default_rng(51)
+        c = r.normal(size=(L, d)) + 1j * r.normal(size=(L, d))
+        c /= np.linalg.norm(c, axis=1, keepdims=True)
+        field = np.random.default_rng(52).uniform(-0.5, 0.5, L) if model == 1 else None
+        path = os.path.join(tmp, name + ".txt")
+        write_job(path, model, L, param, chi, consts, field, dt + 0j, nsteps, 0, c, mode=5, extra=f"{order} {bc}\n")
+        blocks = run(path)
+        res = collect(blocks, L, d, chi)
+        meas = {}
+        for head, nums, args in blocks:
+            a = np.array(nums)
+            if head == "MEN":
+                meas["m_en"] = a[0]
+            elif head == "MQME":
+                meas["m_qme"] = a[0]
+            elif head == "MLOCAL":
+                meas["m_local"] = a[0::2] + 1j * a[1::2]
+            elif head == "MAVG":
+                meas["m_avg"] = a[0] + 1j * a[1]
+            elif head == "MVN":
+                meas["m_vn"] = a
+            elif head == "MCHAIN":
+                meas["m_chain"] = a
+            elif head in ("MTB", "MCORR", "MFCORR"):
+                key = {"MTB": "m_tbvn", "MCORR": "m_corr", "MFCORR": "m_fcorr"}[head]
+                row = a if head == "MTB" else a[0::2] + 1j * a[1::2]
+                meas.setdefault(key, []).append(row)
+        meas = {k: np.array(v) for k, v in meas.items()}
+        meta = dict(model=model, L=L, param=param, chi=chi, d=d, consts=np.array(consts), dt=np.complex128(dt),
+                    nsteps=nsteps, canon=0, coeffs=c, field=np.array([] if field is None else field), order=order,
+                    bc=bc)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **meta, **res, **meas)
+        print(name, {k: v.shape for k, v in meas.items()})
+
+
+def checkpoint():
+    """The reference's own checkpoint files (io_module RecordLambdas / RecordGammas, kinds
+    'S' and 'B', mode 4) of a spin-1/2 XXZ state, L=4, chi=4 (bond 2 truncated from 4 to
+    4 -- full rank -- and padded zeros at the edges), after 6 steps from a random product
+    state (seed 31); kept as the files themselves in tests/golden/ckpt_l4/ beside the state."""
+    import shutil
+
+    L, chi = 4, 4
+    r = np.random.default_rng(31)
+    c = r.normal(size=(L, 2)) + 1j * r.normal(size=(L, 2))
+    c /= np.linalg.norm(c, axis=1, keepdims=True)
+    h = np.random.default_rng(32).uniform(-0.5, 0.5, L)
+    tmp = tempfile.mkdtemp()
+    path = os.path.join(tmp, "ckpt.txt")
+    write_job(path, 1, L, 1, chi, [1.0, 1.0, 0.5], h, 0.05 + 0j, 6, 0, c, mode=4)
+    out = subprocess.run([DRIVER, path], check=True, capture_output=True, text=True, cwd=tmp).stdout
+    res = collect(parse(out), L, 2, chi)
+    dest = os.path.join(HERE, "ckpt_l4")
+    os.makedirs(dest, exist_ok=True)
+    for f in ("ckpt_lambdas_S.dat", "ckpt_gammas_S.dat", "ckpt_lambdas_B.dat", "ckpt_gammas_B.dat"):
+        shutil.copy(os.path.join(tmp, f), os.path.join(dest, f))
+    np.savez_compressed(os.path.join(dest, "state.npz"), L=L, d=2, chi=chi, gam=res["gam"][0] if res["gam"].ndim == 5
+                        else res["gam"], lam=res["lam"], H=res["H"], U=res["U"], coeffs=c)
+    print("checkpoint files written", os.listdir(dest))
+
+
 if __name__ == "__main__":
-    if len(sys.argv) > 1 and sys.argv[1] == "itp_stages":
+    if len(sys.argv) > 1 and sys.argv[1] == "checkpoint":
+        checkpoint()
+    elif len(sys.argv) > 1 and sys.argv[1] == "measures":
+        measures()
+    elif len(sys.argv) > 1 and sys.argv[1] == "itp_stages":
         itp_stages()
     elif len(sys.argv) > 1 and sys.argv[1] == "schedules":
         schedules()
