@@ -626,7 +626,7 @@ def main():
                    "mean_bond_dim": float(res["dims"].mean()), "warmup_s": res["warm_s"]},
         "roofline": roof,
         "learned_rank": {"rank_model": os.path.basename(args.rank_model) if args.rank_model else None,
-                         "sketches_skipped_by_model": st["sketches_skipped"],
+                         "bonds_sent_to_full_svd_by_model": st["model_full_svd"],
                          "sketch_accepted": st["sketch_accepted"], "full_svd_fallbacks": st["fallbacks"],
                          "mean_sketch_width": st["sketch_cols"] / max(st["sketch_accepted"] + st["fallbacks"], 1)},
         "jacobi": {"sweeps_per_half_step_per_lane": st["jacobi_sweeps"] / (half_steps * max(args.lanes, 1)),

@@ -205,8 +205,10 @@ int tebd_get_bond_dims(tebd_ctx* ctx, int* chi);
  * milliseconds in trotter steps, [4] sketch columns requested, [5] sketches
  * accepted, [6] sum of the sketches' numerical ranks, [7] rank-model predictions,
  * [8] pair Gram matrices formed, [9] pairs rotated, [10] pair visits of the sweeps
- * run, [11] sketches skipped on the rank model's prediction, [12] sketches
- * rejected for their discarded weight (the rest of [2] for missed weight),
+ * run, [11] bonds the rank model sent to the full SVD at once (predicted rank
+ * beyond the cap, or too wide to sketch; the fixed width would have sketched
+ * them), [12] sketches
+ * rejected for their discarded weight (the rest of [2] by the probe check),
  * [13] half-sweeps split over lanes (tebd_set_lanes). */
 int tebd_get_stats(tebd_ctx* ctx, double* stats, int n);
 
@@ -304,34 +306,52 @@ const char* tebd_last_error(const tebd_ctx* ctx);
 
 /* ---- rank model of the learned-rank path -------------------------------------
  * No counterpart in the reference, which always takes the full SVD of every bond
- * (local_operations_module.f90:322-345).  The learned-rank path sketches theta with
- * chi_c + max(4, chi_c / 8) + 8 columns (chi_c: the bond's dimension) and redoes the
- * bond by the full SVD when the sketch misses more than tol_missed of the weight.  A
- * 66 -> 128 -> 128 -> 1 FP32 perceptron, trained on the path's own record of accepted
- * (1) and rejected (0) sketches, predicts the acceptance from the last (smallest) 64
- * Schmidt values of the bond (log10 / 10), the bond index / L and log(1 + step) / 5;
- * bonds predicted below the threshold go to the full SVD at once, without the wasted
- * sketch.  Parameters: W1[128][66], b1[128], W2[128][128], b2[128], W3[128], b3
- * (row-major floats). */
+ * (local_operations_module.f90:322-345).  The learned-rank path takes the SVD of theta
+ * from a sketch theta Omega, Omega = [pre-gate right Schmidt vectors | Gaussian] with
+ * width l: without a model l = chi_c + max(4, chi_c / 8) + 8 (chi_c: the bond's
+ * dimension); with one, a 66 -> 128 -> 128 -> 1 FP32 perceptron predicts the rank k the
+ * update needs, as log2(k / chi_c): the rank it keeps, or 2 kmax (kmax = min(chi_max,
+ * d chi_l, d chi_r)) when no sketch up to the cap passes (the update discards more
+ * than tol_trunc, or its sketch kept kmax values and failed the probe check).
+ * Then l = k + max(4, rank_slack k) + 8, rounded up to 32; bonds with k above sqrt(2)
+ * kmax or 2 l above the smaller side of theta take the full SVD at once.  Inputs: the
+ * last (smallest) 63 Schmidt values of the bond before the update (log10 / 10, -3 in
+ * front when fewer), log2(kmax / chi_c) / 4, log2(chi_c) / 12 and log10(||G - 1||_F) / 4
+ * of the bond's gate G.  Parameters: W1[128][66], b1[128], W2[128][128], b2[128],
+ * W3[128], b3 (row-major floats). */
 #define TEBD_MLP_INPUTS 66
 #define TEBD_MLP_HIDDEN 128
 #define TEBD_MLP_PARAMS (TEBD_MLP_HIDDEN * TEBD_MLP_INPUTS + TEBD_MLP_HIDDEN + TEBD_MLP_HIDDEN * TEBD_MLP_HIDDEN + \
                          TEBD_MLP_HIDDEN + TEBD_MLP_HIDDEN + 1)
 
-/* Learned-rank path settings: a sketch is accepted when it misses at most tol_missed
- * of ||theta||^2 and discards at most tol_trunc (defaults 1e-20, 1e-10); with a rank
- * model, bonds whose predicted acceptance is below accept_threshold (default 0.2) skip it. */
-int tebd_set_learned_params(tebd_ctx* ctx, double tol_missed, double tol_trunc, double accept_threshold);
+/* Learned-rank path settings.  A sketch is kept when its discarded weight (missed
+ * weight included) is at most tol_trunc ||theta||^2 (default 1e-10) and it passes the
+ * probe check.  With Q the sketch's range basis, E = theta - Q Q^H theta and kept triplets
+ * (s_j, u_j, v_j), j < k, two sets of 16 Gaussian probes estimate eta = sum_j ||E v_j||^2
+ * (how far the kept vectors are from singular vectors of theta: to first order the
+ * state's squared error) and w = ||E||_F^2 (the weight outside the sketch); with 10x
+ * each estimate (a tenth of the true value is undershot with probability < 2e-11):
+ * eta <= tol_missed ||theta||^2 (default 1e-20), and w <= tol_missed ||theta||^2 -- or,
+ * when k = kmax = min(chi_max, d chi_l, d chi_r) (the cap binds, and the full SVD
+ * discards the weight beyond it too), the unscaled estimate of w at most s_k^2, so that
+ * no missed direction ten times heavier than the smallest kept one goes undetected.
+ * Otherwise the bond is redone by the full SVD.  rank_slack (default 1/16) widens the
+ * model's predicted rank. */
+int tebd_set_learned_params(tebd_ctx* ctx, double tol_missed, double tol_trunc, double rank_slack);
 /* Install (n = TEBD_MLP_PARAMS) or remove (params = NULL or n = 0) the context's rank
- * model; without one every bond whose sketch fits (2 x width <= its smaller side) is
- * sketched. */
+ * model; without one every sketch has the fixed width above. */
 int tebd_rank_model_set(tebd_ctx* ctx, const float* params, int n);
 int tebd_rank_model_get(tebd_ctx* ctx, float* params, int n);
-/* Record (features, accepted) of every sketch of the learned-rank path while on;
+/* Record (features, log2(k / chi_c before)) of every bond update of the learned-rank
+ * path while on, k the needed rank as above;
  * switching on clears the record.  tebd_rank_log_get copies up to max_n samples (X: max_n x 66,
  * y: max_n) and returns the number recorded (or a negative status). */
 int tebd_rank_log(tebd_ctx* ctx, int on);
 int tebd_rank_log_get(tebd_ctx* ctx, float* X, float* y, int max_n);
+/* The probe checks of the same record: per sketch whose discarded weight passed, 8 values
+ * (k, kmax, ||theta||^2, discarded fraction, smallest kept s^2, estimate of
+ * sum_j ||E v_j||^2, estimate of ||E||_F^2, accepted); returns the number of records. */
+int tebd_sketch_log_get(tebd_ctx* ctx, double* out, int max_n);
 /* Stand-alone model kernels on the current device (no context):
  * mean loss sum w (model(X_s) - y_s)^2 / n, w = w_under where the model is below y;
  * its gradient; Adam training for `epochs` full-batch steps (loss_history: epochs

@@ -24,6 +24,13 @@ namespace {
 constexpr int kMaxLanes = 4;
 constexpr int kLaneMinSide = 512;  // lanes split a batch only if a bond matrix has a side >= this
 constexpr int kStages = 6;  // task arrays per batch: 0-4 the learned path's stages, 5 the QR subset
+constexpr int kGemmRegions = 8;  // GEMM descriptor arrays per batch (learned path: 0-2 sketch, 3-7 probe check)
+// Safety factor of the sketch's probe check.  With p = kProbes complex Gaussian probes an
+// estimate ||M G||_F^2 / p of ||M||_F^2 is sum_i mu_i^2 Gamma_i(p, 1) / p over M's
+// singular values mu_i, least concentrated when one mu_i dominates; then it falls below
+// 1/kProbeSafety of the true value with probability
+// P(Gamma(16, 1) < 1.6) = P(Poisson(1.6) >= 16) < 2e-11.
+constexpr double kProbeSafety = 10.0;
 constexpr int kNumStats = 14;
 constexpr int kNumProf = 28;
 constexpr double kQrTol = 1e-17;  // QR stops when every remaining column norm is <= kQrTol ||theta||_F
@@ -112,23 +119,31 @@ struct tebd_ctx {
   float* h_pred = nullptr;
   bool rank_log = false;
   std::vector<float> log_X, log_y;
+  // with the record on, per probe-checked sketch: k, kmax, ||theta||^2, discarded fraction,
+  // smallest kept s^2, the two probe estimates, accepted
+  std::vector<double> log_check;
   BondTask* d_tasks = nullptr;   // kStages x ntask_max
   BondTask* h_tasks = nullptr;   // pinned, kStages x ntask_max
   // sketched (learned-rank) path workspace
   zc* Zb = nullptr;              // per task: range basis Q of the sketch (m x l)
   zc* Bb = nullptr;              // per task: theta Omega and its QR, then B^dagger
   double* lam_scratch = nullptr; // per task chi_max: lambda before it is committed
+  zc* Pb = nullptr;              // per task: probe workspace of the sketch check (probe_stride())
+  ProbeTask* d_probe = nullptr;  // ntask_max probe descriptors
+  ProbeTask* h_probe = nullptr;  // pinned
   GemmTask* d_gemm = nullptr;
   GemmTask* h_gemm = nullptr;    // pinned
   int* d_kprime = nullptr;
   int* h_kprime = nullptr;       // pinned
   unsigned long long sketch_seed = 0x5DEECE66Dull;
-  double tol_missed = 1e-20;     // max weight of theta outside the sketch
-  double tol_trunc = 1e-10;      // max discarded weight before the full SVD judges
-  int oversample = 8;             // heuristic sketch width: chi_c + max(4, chi_c / 8) + oversample
-  // with a model: sketch only bonds whose predicted acceptance >= this.  A skipped bond costs a
-  // full SVD (several sketches), so only confident rejections are skipped.
-  float accept_threshold = 0.2f;
+  // a sketch is accepted when the probe estimate of sum_j s_j^2 ||E v_j||^2 (E: theta outside
+  // the sketch, v_j: kept right vectors), times kProbeSafety, is at most tol_missed ||theta||^4,
+  // and the discarded weight at most tol_trunc ||theta||^2
+  double tol_missed = 1e-20;
+  double tol_trunc = 1e-10;
+  int oversample = 8;             // sketch columns beyond the expected rank
+  // with a model: width = predicted rank k + max(4, rank_slack k) + oversample
+  double rank_slack = 1.0 / 16.0;
   long long step_index = 0;
   // CanonicalForm workspace (allocated on first use): 8 matrices, 3 (s, perm) pairs
   zc* cf = nullptr;
@@ -137,6 +152,7 @@ struct tebd_ctx {
   double* cf_lam = nullptr;
   int cf_sets = 0;               // bonds the CanonicalForm workspace holds at once
   std::vector<int> last_rank;    // kept rank of each bond's last update (rank model input)
+  std::vector<double> last_trunc;  // discarded weight (fraction) of each bond's last update
   double* h_results = nullptr;   // pinned
   int* h_counters = nullptr;     // pinned, 4 per task slot
   std::vector<zc> staging;
@@ -144,8 +160,10 @@ struct tebd_ctx {
   // [0] bond updates, [1] Jacobi sweeps, [2] sketch fallbacks, [3] device ms of trotter steps,
   // [4] sketch columns requested, [5] sketches accepted, [6] sketch ranks (sum of r_y), [7] rank-model predictions,
   // [8] pair Gram matrices formed, [9] pairs rotated, [10] pair visits of the sweeps run,
-  // [11] sketches skipped on the rank model's prediction, [12] sketches rejected for discarded weight
-  // (the others of [2] for missed weight), [13] half-sweeps split over lanes
+  // [11] bonds the rank model sent to the full SVD at once (predicted rank beyond the cap or too wide
+  // to sketch), [12] sketches
+  // rejected for discarded weight (the others of [2] by the probe check), [13] half-sweeps split
+  // over lanes
   double stats[kNumStats] = {};
   // optional per-kernel timing with HIP events on the context's stream:
   // prof[k*4 + 0] launches, [1] milliseconds, [2] algorithmic flops, [3] algorithmic HBM bytes,
@@ -365,6 +383,8 @@ int jacobi_sweeps(tebd_ctx* c, BondTask* d_t, const BondTask* h_t, int nb, int P
 
 // Compact-WY factors kept per slot: one kQrPanel^2 block per panel, min(Rb, Cb) <= Cpmax.
 size_t tq_stride(const tebd_ctx* c) { return (size_t)(c->Cpmax / kQrPanel + 1) * kQrPanel * kQrPanel; }
+// Probe workspace of one slot (sketch check): Om and Z (n, m x 2p), T (r x p), C1, C2 (k x p).
+size_t probe_stride(const tebd_ctx* c) { return (size_t)(4 * c->Rmax + 3 * c->Cpmax) * kProbes; }
 
 // Gives task `slot` its pivoted-QR buffers; Wq is the bond matrix slot W.
 void attach_qr(tebd_ctx* c, BondTask& t, int slot) {
@@ -676,6 +696,7 @@ int run_bonds_full(tebd_ctx* c, const int* links, int nb, const zc* const* gate_
   for (int i = 0; i < nb; ++i) {
     const double* r = c->h_results + RESULT_STRIDE * i;
     c->chi[links[i] + 1] = (int)r[0];
+    c->last_trunc[links[i] + 1] = r[1];
     tsum += r[1];
   }
   c->stats[0] += nb;
@@ -689,39 +710,53 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
   const int d = c->d, nt = c->ntask_max;
   std::vector<int> sk, full;
   std::vector<int> ell;
+  std::vector<int> chic_before(nb);
+  for (int i = 0; i < nb; ++i) chic_before[i] = c->chi[links[i] + 1];
+  std::vector<char> capped_reject(nb, 0);  // sketch at the cap rejected by the probe check
   // rank model: features of every bond of the batch (for the prediction and / or the
-  // record), one launch; bonds predicted to be rejected skip the sketch
-  std::vector<char> skip(nb, 0);
+  // record), one launch; the predicted rank sizes each bond's sketch
+  const bool use_model = c->have_model;
   if (c->have_model || c->rank_log) {
     BondTask* F = c->h_tasks + 4 * nt;
     for (int i = 0; i < nb; ++i) {
       std::memset(&F[i], 0, sizeof(BondTask));
-      F[i].link = links[i] % c->chain_len;  // position within its chain
+      F[i].chil = c->chi[links[i]];
       F[i].chic = c->chi[links[i] + 1];
+      F[i].chir = c->chi[links[i] + 2];
+      F[i].d = d;
+      F[i].chi_max = c->cap();
       F[i].lc = c->lam + (size_t)(links[i] + 1) * c->chi_max;
+      F[i].gate = gate_ptrs[i];
     }
     BondTask* DF = c->d_tasks + 4 * nt;
     HIPCHK(c, hipMemcpyAsync(DF, F, sizeof(BondTask) * nb, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, launch_mlp_features(DF, nb, c->chain_len, c->step_index, c->d_feat, c->st));
-    if (c->have_model) HIPCHK(c, launch_mlp_predict(c->mlp, c->d_feat, nb, c->d_pred, c->st));
+    HIPCHK(c, launch_mlp_features(DF, nb, c->d_feat, c->st));
+    if (use_model) HIPCHK(c, launch_mlp_predict(c->mlp, c->d_feat, nb, c->d_pred, c->st));
     HIPCHK(c, hipMemcpyAsync(c->h_feat, c->d_feat, sizeof(float) * TEBD_MLP_INPUTS * nb, hipMemcpyDeviceToHost, c->st));
-    if (c->have_model)
-      HIPCHK(c, hipMemcpyAsync(c->h_pred, c->d_pred, sizeof(float) * nb, hipMemcpyDeviceToHost, c->st));
+    if (use_model) HIPCHK(c, hipMemcpyAsync(c->h_pred, c->d_pred, sizeof(float) * nb, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    if (c->have_model) {
-      for (int i = 0; i < nb; ++i) skip[i] = c->h_pred[i] < c->accept_threshold;
-      c->stats[7] += nb;
-    }
+    if (use_model) c->stats[7] += nb;
   }
   for (int i = 0; i < nb; ++i) {
     const int link = links[i];
-    const int m = d * c->chi[link], n = d * c->chi[link + 2];
-    const int l = round_up(std::min(std::min(m, n), sketch_width(c, link)), JC);
-    if (2 * l > std::min(m, n)) {
+    const int m = d * c->chi[link], n = d * c->chi[link + 2], side = std::min(m, n);
+    const int l_formula = round_up(std::min(side, sketch_width(c, link)), JC);
+    int l = l_formula;
+    bool beyond = false;
+    if (use_model) {
+      // predicted rank k = chi_c 2^y; beyond sqrt(2) kmax the model expects the update to
+      // discard more than tol_trunc even at the cap, which rejects any sketch
+      const int kmax = std::min(c->cap(), side);
+      const double y = std::min(std::max((double)c->h_pred[i], -8.0), 12.0);
+      const double kraw = c->chi[link + 1] * std::exp2(y);
+      beyond = kraw > std::sqrt(2.0) * kmax;
+      const double khat = std::min((double)kmax, kraw);
+      const int want = std::min(kmax, (int)std::ceil(khat + std::max(4.0, c->rank_slack * khat))) + c->oversample;
+      l = round_up(std::min(side, want), JC);
+    }
+    if (beyond || 2 * l > side) {
       full.push_back(i);
-    } else if (skip[i]) {
-      full.push_back(i);
-      c->stats[11] += 1;
+      if (use_model && 2 * l_formula <= side) c->stats[11] += 1;
     } else {
       sk.push_back(i);
       ell.push_back(l);
@@ -874,29 +909,106 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
       const int link = links[merged[j]];
       c->chi[link + 1] = (int)r[0];
       c->last_rank[link + 1] = (int)r[0];
+      c->last_trunc[link + 1] = r[1];
       tsum += r[1];
     }
     c->stats[0] += nf;
-    // left vectors of theta: Q U_B (m x k), U_B = the factor's Cb-side output (r x k)
-    int na = 0;
-    std::vector<int> acc;
+    // Every sketch whose discarded weight passes gets its left vectors Q U_B (U_B: the
+    // factor's Cb-side output, r x k) and the probe check of ProbeTask: with E = theta -
+    // Q Q^dagger theta, set 1 gives Z1 = theta V G1 - (Q U_B) S G1 = E V G1 (theta^dagger Q U_B
+    // = V S holds exactly, so theta v_j - s_j u_j = E v_j), set 2 gives Z2 = E G2.  theta is
+    // read by the probe product before Q U_B overwrites it.
+    std::vector<int> cand;
+    int max_kc = 1, max_kn = 1, max_r = 1;
     for (int q = 0; q < ns; ++q) {
       const double* r = c->h_results + RESULT_STRIDE * q;
-      const double total = r[2], captured = r[4];
-      const bool ok = used_qr && (total - captured) <= c->tol_missed * total && r[1] <= c->tol_trunc;
-      if (c->rank_log) {
-        const float* f = c->h_feat + (size_t)TEBD_MLP_INPUTS * sk[q];
-        c->log_X.insert(c->log_X.end(), f, f + TEBD_MLP_INPUTS);
-        c->log_y.push_back(ok ? 1.0f : 0.0f);
-      }
       c->stats[4] += ell[q];
       c->stats[6] += S3[q].C;
+      if (used_qr && r[1] <= c->tol_trunc) {
+        cand.push_back(q);
+        max_kc = std::max(max_kc, (int)r[0]);
+        max_kn = std::max(max_kn, (int)r[0] + S0[q].C);
+        max_r = std::max(max_r, S2[q].C);
+      } else {
+        c->stats[12] += 1;
+      }
+    }
+    const int nc = (int)cand.size();
+    GemmTask* G[kGemmRegions];
+    for (int g = 0; g < kGemmRegions; ++g) G[g] = HG + g * nt;
+    for (int j = 0; j < nc; ++j) {
+      const int q = cand[j];
+      const int k = (int)c->h_results[RESULT_STRIDE * q];
+      const BondTask& b0 = S0[q];
+      const int m = b0.R, n = b0.C, r = S2[q].C;
+      zc* Om = c->Pb + (size_t)q * probe_stride(c);  // n x 2p
+      zc* Z = Om + (size_t)c->Rmax * 2 * kProbes;      // m x 2p
+      zc* T = Z + (size_t)c->Rmax * 2 * kProbes;       // r x p
+      zc* C1 = T + (size_t)c->Cpmax * kProbes;         // k x p
+      zc* C2 = C1 + (size_t)c->Cpmax * kProbes;        // k x p
+      zc* Z2 = Z + (size_t)m * kProbes;
+      c->h_probe[j] = {S3[q].s, k, m, n, sk_links[q] % c->chain_len, C1, C2, Om, Z,
+                       c->results + (size_t)RESULT_STRIDE * q + 6};
+      G[3][j] = {S3[q].Yr, C1, Om, n, kProbes, k, n, k, n, 1.0, 0};                 // Om(:, :p) = V G1
+      G[4][j] = {b0.W, Om, Z, m, 2 * kProbes, n, m, n, m, 1.0, 0};                  // Z = theta Om
+      G[5][j] = {S2[q].Qs, Z2, T, r, kProbes, m, m, m, r, 1.0, 0};                  // T = Q^dagger Z2
+      G[6][j] = {S2[q].Qs, T, Z2, m, kProbes, r, m, r, m, -1.0, 1};                 // Z2 -= Q T
+      G[2][j] = {S2[q].Qs, S3[q].Zc, b0.W, m, k, S3[q].Cb, m, S3[q].Cb, m, 1.0, 0};  // theta <- Q U_B
+      G[7][j] = {b0.W, C2, Z, m, kProbes, k, m, k, m, -1.0, 1};                     // Z1 -= Q U_B S G1
+    }
+    if (nc > 0) {
+      HIPCHK(c, hipMemcpyAsync(c->d_probe, c->h_probe, sizeof(ProbeTask) * nc, hipMemcpyHostToDevice, c->st));
+      for (int g = 2; g < kGemmRegions; ++g)
+        HIPCHK(c, hipMemcpyAsync(DG + g * nt, G[g], sizeof(GemmTask) * nc, hipMemcpyHostToDevice, c->st));
+      {
+        ProfScope ps(c, 2);
+        HIPCHK(c, launch_probe_coeffs(c->d_probe, nc, max_kn, c->sketch_seed ^ (unsigned long long)c->step_index,
+                                      c->st));
+      }
+      struct Launch { int region, max_m, max_n; bool conj; };
+      const Launch order[6] = {{3, max_n, kProbes, false}, {4, max_m, 2 * kProbes, false}, {5, max_r, kProbes, true},
+                               {6, max_m, kProbes, false}, {2, max_m, max_kc, false},     {7, max_m, kProbes, false}};
+      for (const Launch& o : order) {
+        ProfScope ps(c, 4);
+        HIPCHK(c, tuned_zgemm(c, DG + o.region * nt, nc, o.max_m, o.max_n, o.conj));
+      }
+      for (int g = 2; g < kGemmRegions; ++g) account_gemm(c, G[g], nc);
+      {
+        ProfScope ps(c, 2);
+        HIPCHK(c, launch_probe_norm(c->d_probe, nc, c->st));
+      }
+      HIPCHK(c, hipMemcpyAsync(c->h_results, c->results, sizeof(double) * RESULT_STRIDE * ns, hipMemcpyDeviceToHost,
+                               c->st));
+      HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+    int na = 0;
+    for (int j = 0, q = 0; q < ns; ++q) {
+      const double* r = c->h_results + RESULT_STRIDE * q;
+      const bool is_cand = j < nc && cand[j] == q;
+      if (is_cand) ++j;
+      bool ok = false;
+      if (is_cand) {
+        // r[2] = ||theta||^2, r[5] = the smallest kept s^2, r[6] / r[7] the two probe sums
+        const double total = r[2], eta = kProbeSafety * r[6] / kProbes, missed = kProbeSafety * r[7] / kProbes;
+        const int k = (int)r[0];
+        const int kmax = std::min(c->cap(), std::min(S0[q].R, S0[q].C));
+        // The kept vectors must be singular vectors of theta to first order (eta), and the
+        // weight outside the sketch negligible -- or, when the cap binds (k = kmax), where
+        // the full SVD discards the weight beyond the cap too, its unscaled estimate no
+        // larger than the smallest kept value: a missed direction ten times heavier than
+        // that would pass with probability < 2e-11, and one within that factor changes the
+        // state by no more than the truncation at the cap already does.
+        const double floor_w = c->tol_missed * total;
+        ok = eta <= floor_w && (missed <= floor_w || (k == kmax && r[7] / kProbes <= r[5]));
+        if (!ok && k == kmax) capped_reject[sk[q]] = 1;
+        if (c->rank_log) {
+          const double rec[8] = {(double)k, (double)kmax, total, r[1], r[5], r[6] / kProbes, r[7] / kProbes, ok ? 1.0 : 0.0};
+          c->log_check.insert(c->log_check.end(), rec, rec + 8);
+        }
+      }
       if (ok) {
-        acc.push_back(q);
         const int k = (int)r[0];
         const BondTask& b0 = S0[q];
-        // U_B has Cb = r rows (the factor's C is B^dagger's own rank, which may be smaller)
-        HG[2 * nt + na] = {S2[q].Qs, S3[q].Zc, b0.W, b0.R, k, S3[q].Cb, b0.R, S3[q].Cb, b0.R, 1.0, 0};
         BondTask& w = S4[na];
         w = S1[q];
         w.qr = 1;
@@ -913,24 +1025,18 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
         w.lc_commit = const_cast<double*>(w.lc);
         c->chi[sk_links[q] + 1] = k;
         c->last_rank[sk_links[q] + 1] = k;
+        c->last_trunc[sk_links[q] + 1] = r[1];
         tsum += r[1];
         ++na;
       } else {
         fb_links.push_back(sk_links[q]);
         fb_gates.push_back(gate_ptrs[sk[q]]);
         c->stats[2] += 1;
-        if ((total - captured) <= c->tol_missed * total) c->stats[12] += 1;
       }
     }
     c->stats[5] += na;
     if (na > 0) {
-      HIPCHK(c, hipMemcpyAsync(DG + 2 * nt, HG + 2 * nt, sizeof(GemmTask) * na, hipMemcpyHostToDevice, c->st));
       HIPCHK(c, hipMemcpyAsync(D4, S4, sizeof(BondTask) * na, hipMemcpyHostToDevice, c->st));
-      {
-        ProfScope ps(c, 4);
-        HIPCHK(c, tuned_zgemm(c, DG + 2 * nt, na, max_m, max_k, false));
-      }
-      account_gemm(c, HG + 2 * nt, na);
       {
         ProfScope ps(c, 2);
         HIPCHK(c, launch_write_gammas(D4, na, c->st));
@@ -950,6 +1056,19 @@ int run_bonds_learned(tebd_ctx* c, const int* links, int nb, const zc* const* ga
     if (e) return e;
     tsum += t;
   }
+  // the record: (features, log2(k / chi_c before)) of every bond of the batch, with k the
+  // kept rank, or 2 kmax when no sketch the model can ask for would pass: the update
+  // discarded more than tol_trunc, or its sketch kept the cap and failed the probe check
+  if (c->rank_log)
+    for (int i = 0; i < nb; ++i) {
+      const int link = links[i];
+      const float* f = c->h_feat + (size_t)TEBD_MLP_INPUTS * i;
+      c->log_X.insert(c->log_X.end(), f, f + TEBD_MLP_INPUTS);
+      const int kmax = std::min(c->cap(), d * std::min(c->chi[link], c->chi[link + 2]));
+      const bool hard = c->last_trunc[link + 1] > c->tol_trunc || capped_reject[i];
+      const double k = hard ? 2.0 * kmax : (double)c->chi[link + 1];
+      c->log_y.push_back((float)std::log2(k / (double)chic_before[i]));
+    }
   *trunc_sum = tsum;
   return TEBD_OK;
 }
@@ -1119,8 +1238,8 @@ int ensure_cf_workspace(tebd_ctx* c, int sets) {
   HIPCHK(c, hipMalloc(&c->cf_lam, sizeof(double) * c->Cpmax * sets));
   c->cf_sets = sets;
   if (!c->h_gemm) {
-    HIPCHK(c, hipMalloc(&c->d_gemm, sizeof(GemmTask) * c->ntask_max * 5));
-    HIPCHK(c, hipHostMalloc(&c->h_gemm, sizeof(GemmTask) * c->ntask_max * 5));
+    HIPCHK(c, hipMalloc(&c->d_gemm, sizeof(GemmTask) * c->ntask_max * kGemmRegions));
+    HIPCHK(c, hipHostMalloc(&c->h_gemm, sizeof(GemmTask) * c->ntask_max * kGemmRegions));
     c->owns_gemm = c->is_lane;
   }
   return TEBD_OK;
@@ -1201,12 +1320,15 @@ tebd_ctx* make_lane(tebd_ctx* c, int lane, int slot0, int nslots) {
     l->Zb = c->Zb + s0 * c->Rmax * c->Cpmax;
     l->Bb = c->Bb + s0 * c->Rmax * c->Cpmax;
     l->lam_scratch = c->lam_scratch + s0 * c->chi_max;
+    l->Pb = c->Pb + s0 * probe_stride(c);
+    l->d_probe = c->d_probe + s0;
+    l->h_probe = c->h_probe + s0;
     l->d_kprime = c->d_kprime + s0;
     l->h_kprime = c->h_kprime + s0;
   }
   // GEMM descriptors: the lane's slice of the context's, or (none yet) its own later
-  l->d_gemm = c->d_gemm ? c->d_gemm + 5 * s0 : nullptr;
-  l->h_gemm = c->h_gemm ? c->h_gemm + 5 * s0 : nullptr;
+  l->d_gemm = c->d_gemm ? c->d_gemm + kGemmRegions * s0 : nullptr;
+  l->h_gemm = c->h_gemm ? c->h_gemm + kGemmRegions * s0 : nullptr;
   return l;
 }
 
@@ -1214,6 +1336,7 @@ tebd_ctx* make_lane(tebd_ctx* c, int lane, int slot0, int nslots) {
 void lane_pull(tebd_ctx* l, const tebd_ctx* c) {
   l->chi = c->chi;
   l->last_rank = c->last_rank;
+  l->last_trunc = c->last_trunc;
   l->tuned = c->tuned;
   l->mode = c->mode;
   l->cutoff = c->cutoff;
@@ -1222,7 +1345,7 @@ void lane_pull(tebd_ctx* l, const tebd_ctx* c) {
   l->rank_log = c->rank_log;
   l->tol_missed = c->tol_missed;
   l->tol_trunc = c->tol_trunc;
-  l->accept_threshold = c->accept_threshold;
+  l->rank_slack = c->rank_slack;
   l->oversample = c->oversample;
   l->step_index = c->step_index;
   l->profiling = c->profiling;
@@ -1235,6 +1358,7 @@ void lane_pull(tebd_ctx* l, const tebd_ctx* c) {
   l->tune_trials = 0.0;
   l->log_X.clear();
   l->log_y.clear();
+  l->log_check.clear();
   l->err.clear();
 }
 
@@ -1244,6 +1368,7 @@ void lane_push(tebd_ctx* c, const tebd_ctx* l, const int* links, int nb) {
   for (int i = 0; i < nb; ++i) {
     c->chi[links[i] + 1] = l->chi[links[i] + 1];
     c->last_rank[links[i] + 1] = l->last_rank[links[i] + 1];
+    c->last_trunc[links[i] + 1] = l->last_trunc[links[i] + 1];
   }
   for (int k = 0; k < kNumStats; ++k) c->stats[k] += l->stats[k];
   for (int k = 0; k < kNumProf; ++k) c->prof[k] += l->prof[k];
@@ -1251,6 +1376,7 @@ void lane_push(tebd_ctx* c, const tebd_ctx* l, const int* links, int nb) {
   c->tune_trials += l->tune_trials;
   c->log_X.insert(c->log_X.end(), l->log_X.begin(), l->log_X.end());
   c->log_y.insert(c->log_y.end(), l->log_y.begin(), l->log_y.end());
+  c->log_check.insert(c->log_check.end(), l->log_check.begin(), l->log_check.end());
 }
 
 // n lanes of ntask_max / n workspace slots each (made again when n changes); false if
@@ -1457,6 +1583,9 @@ void free_all(tebd_ctx* c) {
   (void)hipFree(c->Zb);
   (void)hipFree(c->Bb);
   (void)hipFree(c->lam_scratch);
+  (void)hipFree(c->Pb);
+  (void)hipFree(c->d_probe);
+  (void)hipHostFree(c->h_probe);
   (void)hipFree(c->d_gemm);
   (void)hipHostFree(c->h_gemm);
   (void)hipFree(c->d_kprime);
@@ -1735,7 +1864,9 @@ double slot_bytes(const tebd_ctx* c) {
              z * (double)tq_stride(c) + 16.0 + 4.0 * nblk + 8.0 * nblk * nblk + z * (nblk / 2) * JC * JC +
              16.0 * (nblk / 2) + 2.0 * 8 * RESULT_STRIDE + 4.0 * TEBD_MLP_INPUTS + 4.0 +
              2.0 * kStages * sizeof(BondTask);
-  if (c->mode & TEBD_MODE_LEARNED_RANK) b += 2.0 * z * R * Cp + 8.0 * chi + 2.0 * 5 * sizeof(GemmTask) + 8.0;
+  if (c->mode & TEBD_MODE_LEARNED_RANK)
+    b += 2.0 * z * R * Cp + 8.0 * chi + 2.0 * kGemmRegions * sizeof(GemmTask) + 8.0 + z * (double)probe_stride(c) +
+         2.0 * sizeof(ProbeTask);
   return b;
 }
 
@@ -1830,12 +1961,16 @@ int create_impl(tebd_ctx** out, int nchain, int Lc, int d, int chi_max, int orde
     if (bad(hipMalloc(&c->Zb, sizeof(zc) * (size_t)c->Rmax * c->Cpmax * nt))) return TEBD_ERR_HIP;
     if (bad(hipMalloc(&c->Bb, sizeof(zc) * (size_t)c->Rmax * c->Cpmax * nt))) return TEBD_ERR_HIP;
     if (bad(hipMalloc(&c->lam_scratch, sizeof(double) * (size_t)chi_max * nt))) return TEBD_ERR_HIP;
-    if (bad(hipMalloc(&c->d_gemm, sizeof(GemmTask) * nt * 5))) return TEBD_ERR_HIP;
-    if (bad(hipHostMalloc(&c->h_gemm, sizeof(GemmTask) * nt * 5))) return TEBD_ERR_HIP;
+    if (bad(hipMalloc(&c->Pb, sizeof(zc) * probe_stride(c) * nt))) return TEBD_ERR_HIP;
+    if (bad(hipMalloc(&c->d_probe, sizeof(ProbeTask) * nt))) return TEBD_ERR_HIP;
+    if (bad(hipHostMalloc(&c->h_probe, sizeof(ProbeTask) * nt))) return TEBD_ERR_HIP;
+    if (bad(hipMalloc(&c->d_gemm, sizeof(GemmTask) * nt * kGemmRegions))) return TEBD_ERR_HIP;
+    if (bad(hipHostMalloc(&c->h_gemm, sizeof(GemmTask) * nt * kGemmRegions))) return TEBD_ERR_HIP;
     if (bad(hipMalloc(&c->d_kprime, sizeof(int) * nt))) return TEBD_ERR_HIP;
     if (bad(hipHostMalloc(&c->h_kprime, sizeof(int) * nt))) return TEBD_ERR_HIP;
   }
   c->last_rank.assign(L + 1, 0);
+  c->last_trunc.assign(L + 1, 0.0);
   if (bad(hipHostMalloc(&c->h_results, sizeof(double) * RESULT_STRIDE * nt))) return TEBD_ERR_HIP;
   if (bad(hipHostMalloc(&c->h_counters, sizeof(int) * 4 * nt))) return TEBD_ERR_HIP;
   *out = c;
@@ -2105,14 +2240,14 @@ int tebd_expval_two_site(tebd_ctx* c, const double* ops, double* out) {
   return TEBD_OK;
 }
 
-int tebd_set_learned_params(tebd_ctx* c, double tol_missed, double tol_trunc, double accept_threshold) {
+int tebd_set_learned_params(tebd_ctx* c, double tol_missed, double tol_trunc, double rank_slack) {
   if (!c) return TEBD_ERR_ARG;
   if (!(tol_missed > 0.0 && tol_missed < 1.0) || !(tol_trunc > 0.0 && tol_trunc < 1.0) ||
-      !(accept_threshold >= 0.0 && accept_threshold <= 1.0))
-    return fail(c, TEBD_ERR_ARG, "learned params: tolerances in (0, 1), threshold in [0, 1]");
+      !(rank_slack >= 0.0 && rank_slack <= 4.0))
+    return fail(c, TEBD_ERR_ARG, "learned params: tolerances in (0, 1), rank slack in [0, 4]");
   c->tol_missed = tol_missed;
   c->tol_trunc = tol_trunc;
-  c->accept_threshold = (float)accept_threshold;
+  c->rank_slack = rank_slack;
   return TEBD_OK;
 }
 
@@ -2141,8 +2276,18 @@ int tebd_rank_log(tebd_ctx* c, int on) {
   if (c->rank_log) {
     c->log_X.clear();
     c->log_y.clear();
+    c->log_check.clear();
   }
   return TEBD_OK;
+}
+
+int tebd_sketch_log_get(tebd_ctx* c, double* out, int max_n) {
+  if (!c || max_n < 0) return TEBD_ERR_ARG;
+  const int n = (int)(c->log_check.size() / 8);
+  const int m = std::min(n, max_n);
+  if (m > 0 && !out) return fail(c, TEBD_ERR_ARG, "sketch_log_get: null output");
+  if (m > 0) std::memcpy(out, c->log_check.data(), sizeof(double) * 8 * m);
+  return n;
 }
 
 int tebd_rank_log_get(tebd_ctx* c, float* X, float* y, int max_n) {
@@ -2417,8 +2562,8 @@ int tebd_two_point_rdms(tebd_ctx* c, const double* phase, double* rho2, double* 
     HIPCHK(c, hipMalloc(&c->cr_rho1, sizeof(zc) * d2 * c->L));
   }
   if (!c->h_gemm) {
-    HIPCHK(c, hipMalloc(&c->d_gemm, sizeof(GemmTask) * c->ntask_max * 5));
-    HIPCHK(c, hipHostMalloc(&c->h_gemm, sizeof(GemmTask) * c->ntask_max * 5));
+    HIPCHK(c, hipMalloc(&c->d_gemm, sizeof(GemmTask) * c->ntask_max * kGemmRegions));
+    HIPCHK(c, hipHostMalloc(&c->h_gemm, sizeof(GemmTask) * c->ntask_max * kGemmRegions));
   }
   const double* ph = nullptr;
   if (phase) {

@@ -31,6 +31,25 @@ struct GemmTask {
 
 constexpr int RESULT_STRIDE = 8;
 
+// A-posteriori check of one sketched bond (learned-rank path), by two sets of p
+// complex Gaussian probes.  theta (m x n), E = theta - Q Q^dagger theta the part of theta
+// outside the sketch's range basis Q (m x r), kept triplets (s_j, u_j, v_j), j < k:
+//   set 1: Z1 = theta V G1 - U S G1 = E V G1        (G1: k x p), since theta v_j = s_j u_j + E v_j;
+//   set 2: Z2 = (I - Q Q^dagger) theta G2 = E G2   (G2: n x p);
+// ||Z1||_F^2 / p estimates sum_j ||E v_j||^2 (how far the kept vectors are from singular
+// vectors of theta), ||Z2||_F^2 / p estimates ||E||_F^2 (the weight the sketch missed).
+constexpr int kProbes = 16;
+struct ProbeTask {
+  const double* s;   // k kept singular values of theta, descending
+  int k, m, n;
+  int rng_key;       // the bond's position in its chain
+  zc* C1;            // k x p: G1
+  zc* C2;            // k x p: S G1
+  zc* Om;            // n x 2p: [V G1 | G2] (the coefficient kernel writes G2)
+  zc* Z;             // m x 2p: [Z1 | Z2]
+  double* out;       // [0] ||Z1||_F^2, [1] ||Z2||_F^2
+};
+
 __device__ __forceinline__ zc zmk(double x, double y) { zc r; r.x = x; r.y = y; return r; }
 __device__ __forceinline__ zc zadd(zc a, zc b) { return zmk(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ zc zsub(zc a, zc b) { return zmk(a.x - b.x, a.y - b.y); }

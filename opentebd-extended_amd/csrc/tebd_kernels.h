@@ -26,7 +26,7 @@ hipError_t launch_qrp_vectors(const BondTask* d_tasks, int ntasks, int max_k, in
 hipError_t launch_qrp_form_q(const BondTask* d_tasks, int ntasks, int max_r, hipStream_t st);
 hipError_t launch_zdotc(const BondTask* d_a, const BondTask* d_b, int ntasks, double* d_out, hipStream_t st);
 hipError_t launch_finalize(const BondTask* d_tasks, int ntasks, int max_cpad, double cutoff, hipStream_t st);
-hipError_t launch_mlp_features(const BondTask* d_tasks, int ntasks, int L, long long step, float* d_X, hipStream_t st);
+hipError_t launch_mlp_features(const BondTask* d_tasks, int ntasks, float* d_X, hipStream_t st);
 hipError_t launch_mlp_predict(const float* d_params, const float* d_X, int n, float* d_out, hipStream_t st);
 hipError_t launch_write_gammas(const BondTask* d_tasks, int ntasks, hipStream_t st);
 // tile variants of the batched complex GEMM (chosen per shape by the autotuner)
@@ -43,6 +43,9 @@ hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_
                         int variant);
 hipError_t launch_build_sketch(const BondTask* d_tasks, const int* d_kprime, int ntasks, unsigned long long seed,
                                hipStream_t st);
+hipError_t launch_probe_coeffs(const ProbeTask* d_tasks, int ntasks, int max_kn, unsigned long long seed,
+                               hipStream_t st);
+hipError_t launch_probe_norm(const ProbeTask* d_tasks, int ntasks, hipStream_t st);
 hipError_t launch_col_norms(const BondTask* d_tasks, int ntasks, int max_cpad, hipStream_t st);
 hipError_t launch_truncate(const BondTask* d_tasks, int ntasks, double cutoff, hipStream_t st);
 hipError_t launch_scaled_copy(const zc* in, int ldi, zc* out, int ldo, int rows, int cols, const double* rowscale,

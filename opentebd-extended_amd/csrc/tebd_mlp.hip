@@ -1,11 +1,11 @@
 // The rank model of the learned-rank path: a 66 -> 128 -> 128 -> 1 perceptron in FP32
 // (ReLU hidden layers) with hand-written inference, forward/backward and Adam kernels.
 //
-// Features of a bond (k_mlp_features): the last (smallest) 64 of its chi_c Schmidt values
-// before the update as log10(lambda)/10, smallest last (fewer than 64 values: -3 in
-// front), the bond index / L and log(1 + step)/5.  Target: whether the learned-rank
-// path's sketch of that bond was accepted (1) or had to be redone by the full SVD (0).
-// Loss: mean of w (out - y)^2 with w = w_under when the model is below the target.
+// It predicts the rank a bond update will keep, as log2(k_new / chi_c), from features
+// of the bond before the update (k_mlp_features); the learned-rank path sizes the
+// bond's sketch from the prediction.  Loss: mean of w (out - y)^2 with w = w_under when
+// the model is below the target (an under-predicted rank costs a rejected sketch, an
+// over-predicted one only wider GEMMs).
 //
 // Parameter layout (floats): W1[128][66], b1[128], W2[128][128], b2[128], W3[128], b3.
 #include "tebd_device.h"
@@ -139,18 +139,35 @@ __global__ __launch_bounds__(NH) void k_mlp_predict(const float* __restrict__ p,
   if (threadIdx.x == 0) out[s] = o;
 }
 
-// Features of every task's bond (the state before the update): one workgroup per task.
-__global__ void k_mlp_features(const BondTask* __restrict__ tasks, int L, long long step, float* __restrict__ X) {
+// Features of every task's bond (the state before the update): one workgroup of 64
+// threads per task.  x[0..62]: the last (smallest) 63 of its chi_c Schmidt values as
+// log10(lambda)/10, in order (-3 in front when fewer); x[63]: log2(kmax / chi_c) / 4, the
+// room to grow, with kmax = min(chi_max, d chi_l, d chi_r) the largest rank the update can
+// keep; x[64]: log2(chi_c) / 12; x[65]: log10(||G - 1||_F) / 4 of the bond's gate G, how
+// far the update moves theta (dt ||H_bond|| for a real-time gate).
+constexpr int kSchmidtFeatures = NI - 3;
+__global__ void k_mlp_features(const BondTask* __restrict__ tasks, float* __restrict__ X) {
   const BondTask t = tasks[blockIdx.x];
   float* x = X + (size_t)blockIdx.x * NI;
-  for (int q = threadIdx.x; q < NI - 2; q += blockDim.x) {
-    const int idx = t.chic - (NI - 2) + q;  // last NI-2 values, in order
+  for (int q = threadIdx.x; q < kSchmidtFeatures; q += blockDim.x) {
+    const int idx = t.chic - kSchmidtFeatures + q;  // last values, in order
     const double l = idx >= 0 ? t.lc[idx] : 0.0;
     x[q] = l > 1e-30 ? (float)(log10(l) / 10.0) : -3.0f;
   }
+  const int d2 = t.d * t.d;
+  double g2 = 0.0;
+  for (int e = threadIdx.x; e < d2 * d2; e += blockDim.x) {
+    zc g = t.gate[e];
+    if (e % d2 == e / d2) g.x -= 1.0;
+    g2 += g.x * g.x + g.y * g.y;
+  }
+  for (int off = 32; off > 0; off >>= 1) g2 += __shfl_xor(g2, off, 64);
   if (threadIdx.x == 0) {
-    x[NI - 2] = (float)t.link / (float)L;
-    x[NI - 1] = (float)(log1p((double)step) / 5.0);
+    int kmax = t.chi_max < t.d * t.chil ? t.chi_max : t.d * t.chil;
+    kmax = kmax < t.d * t.chir ? kmax : t.d * t.chir;
+    x[NI - 3] = (float)(log2((double)kmax / (double)t.chic) / 4.0);
+    x[NI - 2] = (float)(log2((double)t.chic) / 12.0);
+    x[NI - 1] = (float)(log10(fmax(sqrt(g2), 1e-16)) / 4.0);
   }
 }
 
@@ -193,8 +210,8 @@ struct MlpBuffers {
 
 }  // namespace
 
-hipError_t launch_mlp_features(const BondTask* d_tasks, int ntasks, int L, long long step, float* d_X, hipStream_t st) {
-  hipLaunchKernelGGL(k_mlp_features, dim3(ntasks), dim3(64), 0, st, d_tasks, L, step, d_X);
+hipError_t launch_mlp_features(const BondTask* d_tasks, int ntasks, float* d_X, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_features, dim3(ntasks), dim3(64), 0, st, d_tasks, d_X);
   return hipGetLastError();
 }
 

@@ -232,7 +232,73 @@ __global__ void k_build_sketch(const BondTask* __restrict__ tasks, const int* __
   }
 }
 
+// Standard complex Gaussian (E|g|^2 = 1) number e of a probe stream.
+__device__ __forceinline__ zc probe_gauss(unsigned long long seed, int key, unsigned long long e) {
+  const unsigned long long x = seed ^ 0xA5A5F00DCAFEull ^ ((unsigned long long)key << 48) ^ (e << 1);
+  const double u1 = u01(x), u2 = u01(x + 0x632BE59BD9B4E019ull);
+  const double r = sqrt(-log(u1));  // |g|^2 ~ Exp(1)
+  return zmk(r * cos(6.283185307179586 * u2), r * sin(6.283185307179586 * u2));
+}
+
+// Probes of the sketch check: C1 = G1, C2 = S G1 (k x p) and G2 into columns p..2p-1 of
+// Om (n x p), from the bond's own stream, so that the check of a bond does not depend on
+// its batch slot.
+__global__ void k_probe_coeffs(const ProbeTask* __restrict__ tasks, unsigned long long seed) {
+  const ProbeTask t = tasks[blockIdx.y];
+  const int n1 = t.k * kProbes, n2 = t.n * kProbes;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n1 + n2; e += gridDim.x * blockDim.x) {
+    const zc g = probe_gauss(seed, t.rng_key, (unsigned long long)e);
+    if (e < n1) {
+      const double sj = t.s[e % t.k];
+      t.C1[e] = g;
+      t.C2[e] = zmk(sj * g.x, sj * g.y);
+    } else {
+      t.Om[(size_t)t.n * kProbes + (e - n1)] = g;
+    }
+  }
+}
+
+// out[0] = ||Z1||_F^2, out[1] = ||Z2||_F^2 (Z = [Z1 | Z2], m x 2p), one workgroup per task,
+// fixed-order reductions.
+__global__ __launch_bounds__(256) void k_probe_norm(const ProbeTask* __restrict__ tasks) {
+  const ProbeTask t = tasks[blockIdx.x];
+  const size_t n = (size_t)t.m * kProbes;
+  __shared__ double red[2][256];
+  double a1 = 0.0, a2 = 0.0;
+  for (size_t e = threadIdx.x; e < n; e += 256) {
+    const zc z1 = t.Z[e], z2 = t.Z[n + e];
+    a1 += z1.x * z1.x + z1.y * z1.y;
+    a2 += z2.x * z2.x + z2.y * z2.y;
+  }
+  red[0][threadIdx.x] = a1;
+  red[1][threadIdx.x] = a2;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + st];
+      red[1][threadIdx.x] += red[1][threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    t.out[0] = red[0][0];
+    t.out[1] = red[1][0];
+  }
+}
+
 }  // namespace
+
+hipError_t launch_probe_coeffs(const ProbeTask* d_tasks, int ntasks, int max_kn, unsigned long long seed,
+                               hipStream_t st) {
+  const int blocks = (max_kn * kProbes + 255) / 256;
+  hipLaunchKernelGGL(k_probe_coeffs, dim3(blocks, ntasks), dim3(256), 0, st, d_tasks, seed);
+  return hipGetLastError();
+}
+
+hipError_t launch_probe_norm(const ProbeTask* d_tasks, int ntasks, hipStream_t st) {
+  hipLaunchKernelGGL(k_probe_norm, dim3(ntasks), dim3(256), 0, st, d_tasks);
+  return hipGetLastError();
+}
 
 hipError_t launch_zgemm(const GemmTask* d_tasks, int ntasks, int max_m, int max_n, bool conj_a, hipStream_t st,
                         int variant) {

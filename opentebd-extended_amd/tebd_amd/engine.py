@@ -203,13 +203,13 @@ class Context:
         return np.concatenate([vals, [last]])
 
     # -- rank model of the learned-rank path -----------------------------------
-    def set_learned_params(self, tol_missed=1e-20, tol_trunc=1e-10, accept_threshold=0.2):
-        """Acceptance tolerances of the sketched SVD; with a rank model, bonds predicted
-        below accept_threshold skip the sketch."""
-        self._check(self._lib.tebd_set_learned_params(self._h, tol_missed, tol_trunc, accept_threshold))
+    def set_learned_params(self, tol_missed=1e-20, tol_trunc=1e-10, rank_slack=1.0 / 16.0):
+        """Acceptance tolerances of the sketched SVD (probe check, discarded weight) and the
+        relative margin added to the rank model's predicted rank."""
+        self._check(self._lib.tebd_set_learned_params(self._h, tol_missed, tol_trunc, rank_slack))
 
     def set_rank_model(self, params):
-        """Decide with this model which bonds the learned-rank path sketches (None: all that fit)."""
+        """Size each bond's sketch from this model's predicted rank (None: the fixed width)."""
         if params is None:
             self._check(self._lib.tebd_rank_model_set(self._h, None, 0))
             return
@@ -217,7 +217,8 @@ class Context:
         self._check(self._lib.tebd_rank_model_set(self._h, _p(p), p.size))
 
     def rank_log(self, on=True):
-        """Record (features, accepted) of every sketch of the learned-rank path while on."""
+        """Record (features, log2(kept rank / chi_c before)) of every bond update of the
+        learned-rank path while on."""
         self._check(self._lib.tebd_rank_log(self._h, int(bool(on))))
 
     def rank_log_data(self):
@@ -231,6 +232,21 @@ class Context:
             if m < 0:
                 self._check(m)
         return X, y
+
+    def sketch_log_data(self):
+        """Per probe-checked sketch of the record: dict of arrays k, kmax, total (||theta||^2),
+        discarded (fraction), smallest (kept s^2), eta (estimate of sum_j ||E v_j||^2),
+        missed (estimate of ||E||_F^2), accepted."""
+        n = self._lib.tebd_sketch_log_get(self._h, None, 0)
+        if n < 0:
+            self._check(n)
+        out = np.zeros((n, 8))
+        if n:
+            m = self._lib.tebd_sketch_log_get(self._h, _p(out), n)
+            if m < 0:
+                self._check(m)
+        names = ("k", "kmax", "total", "discarded", "smallest", "eta", "missed", "accepted")
+        return {nm: out[:, i] for i, nm in enumerate(names)}
 
     # -- the hot path ---------------------------------------------------------
     def trotter_step(self):
@@ -263,7 +279,7 @@ class Context:
         return {"bond_updates": out[0], "jacobi_sweeps": out[1], "fallbacks": out[2], "device_ms": out[3],
                 "sketch_cols": out[4], "sketch_accepted": out[5], "sketch_rank": out[6], "model_predictions": out[7],
                 "pair_grams": out[8], "pair_rotations": out[9], "pair_visits": out[10],
-                "sketches_skipped": out[11], "rejected_discarded": out[12], "lane_splits": out[13]}
+                "model_full_svd": out[11], "rejected_discarded": out[12], "lane_splits": out[13]}
 
     def set_profiling(self, on=True):
         self._check(self._lib.tebd_set_profiling(self._h, int(bool(on))))

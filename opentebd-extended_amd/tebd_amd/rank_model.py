@@ -1,12 +1,18 @@
 """The learned-rank path's rank model: a 66 -> 128 -> 128 -> 1 FP32 perceptron whose
 training, gradient and inference run in the library's own GPU kernels (tebd_mlp.hip).
 
-The learned-rank path sketches each bond with chi_c + max(4, chi_c/8) + 8 columns and
-redoes it by the full SVD when the sketch misses more than 1e-20 of the weight.  The
-model predicts that outcome from the last (smallest) 64 of the bond's Schmidt values
-before the update (log10(lambda)/10, -3 in front when fewer), bond index / L and
-log(1 + step)/5; bonds predicted to be rejected go to the full SVD at once.  Training
-data: Context.rank_log records (features, accepted) of every sketch the path tries.
+The model predicts the rank k a bond update needs, as log2(k / chi_c): the rank it
+keeps, or 2 kmax (kmax = min(chi_max, d chi_l, d chi_r)) when no sketch up to the cap
+passes -- the update discards more than the sketch's tolerance 1e-10, or its sketch kept
+kmax values and failed the probe check of the kept vectors.  Inputs: the last
+(smallest) 63 of the bond's Schmidt values before the update (log10(lambda)/10, -3 in
+front when fewer), log2(kmax / chi_c)/4, log2(chi_c)/12 and log10(||G - 1||_F)/4 of the
+bond's gate G (how far the update moves theta).  The learned-rank path
+sketches the bond with k + max(4, k/16) + 8 columns (rounded up to 32), or takes the
+full SVD at once when k > sqrt(2) kmax or the width is more than half of theta's smaller
+side.  Training data: Context.rank_log records (features, log2 ratio) of every bond
+update.  The loss weights under-predictions by w_under: a sketch that is too narrow is
+rejected and redone, one that is too wide only costs wider GEMMs.
 """
 import ctypes
 
@@ -28,7 +34,7 @@ def init_params(seed=0):
     w1 = rng.normal(0.0, np.sqrt(2.0 / MLP_INPUTS), (MLP_HIDDEN, MLP_INPUTS))
     w2 = rng.normal(0.0, np.sqrt(2.0 / MLP_HIDDEN), (MLP_HIDDEN, MLP_HIDDEN))
     w3 = rng.normal(0.0, np.sqrt(1.0 / MLP_HIDDEN), MLP_HIDDEN)
-    p = np.concatenate([w1.ravel(), np.zeros(MLP_HIDDEN), w2.ravel(), np.zeros(MLP_HIDDEN), w3, [0.5]])
+    p = np.concatenate([w1.ravel(), np.zeros(MLP_HIDDEN), w2.ravel(), np.zeros(MLP_HIDDEN), w3, [0.0]])
     assert p.size == MLP_PARAMS
     return p.astype(np.float32)
 
@@ -71,3 +77,4 @@ def predict(params, X):
     out = np.zeros(X.shape[0], np.float32)
     _check(lib().tebd_mlp_predict(_P(p), _P(X), X.shape[0], _P(out)))
     return out
+
