@@ -151,6 +151,30 @@ struct tebd_ctx {
   int* cf_perm = nullptr;
   double* cf_lam = nullptr;
   int cf_sets = 0;               // bonds the CanonicalForm workspace holds at once
+  // number-conserving update (tebd_set_conserved): the number of every local state, the
+  // total, and on the host the labels of every Schmidt state -- number left / right of each
+  // link, (L+1) x chi_max, 10000 = unused (the reference's LabelLeft / LabelRight)
+  bool nc = false;
+  int nc_tot = 0, nc_maxf = 0;
+  std::vector<int> nc_conserv, nc_LL, nc_LR;
+  // NC workspace (allocated on first use): per bond of a batch theta and the blocks'
+  // singular vectors; index lists, block and kept-state descriptors; per slot the
+  // truncation's lambda; pinned staging of singular values and lambdas
+  zc* nc_theta = nullptr;
+  zc* nc_vec = nullptr;
+  int nc_bonds = 0;
+  int* d_nc_idx = nullptr;
+  size_t nc_idx_cap = 0;
+  NcBlock* d_nc_blk = nullptr;
+  size_t nc_blk_cap = 0;
+  NcKept* d_nc_kept = nullptr;
+  size_t nc_kept_cap = 0;
+  double* d_nc_part = nullptr;
+  size_t nc_part_cap = 0;
+  double* nc_lscratch = nullptr;
+  double* h_nc_s = nullptr;      // pinned, ntask_max x Cpmax
+  double* h_nc_lam = nullptr;    // pinned, ntask_max x chi_max
+  double* h_nc_part = nullptr;   // pinned, grows with d_nc_part
   std::vector<int> last_rank;    // kept rank of each bond's last update (rank model input)
   std::vector<double> last_trunc;  // discarded weight (fraction) of each bond's last update
   double* h_results = nullptr;   // pinned
@@ -1245,7 +1269,348 @@ int ensure_cf_workspace(tebd_ctx* c, int sets) {
   return TEBD_OK;
 }
 
+// ---- number-conserving update (TwoSiteOpNC_c, local_operations_module.f90:1845-1964) -----
+// The reference's 10.0**(-10): a default-kind (single precision) literal, promoted when
+// compared with double precision values (UpdateLabelLeft, FormLambdaNC, FormGamma1/2NC).
+const double kNcCut = (double)std::pow(10.0f, -10.0f);
+
+// Grows a device buffer to hold `need` elements (the stream is drained first, so nothing
+// in flight reads the old one).
+template <class T>
+hipError_t nc_grow(tebd_ctx* c, T** p, size_t* cap, size_t need) {
+  if (need <= *cap) return hipSuccess;
+  hipError_t e = hipStreamSynchronize(c->st);
+  if (e != hipSuccess) return e;
+  (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  const size_t n = std::max(need, *cap * 2);
+  e = hipMalloc((void**)p, sizeof(T) * n);
+  if (e == hipSuccess) *cap = n;
+  return e;
+}
+
+// Per-bond theta and singular-vector stores for batches of up to ntask_max bonds, the
+// per-slot truncation lambdas and the pinned staging.
+int ensure_nc_workspace(tebd_ctx* c) {
+  if (c->nc_theta) return TEBD_OK;
+  const size_t nt = c->ntask_max, sq = (size_t)c->Rmax * c->Rmax;
+  HIPCHK(c, hipMalloc(&c->nc_theta, sizeof(zc) * sq * nt));
+  HIPCHK(c, hipMalloc(&c->nc_vec, sizeof(zc) * 2 * sq * nt));
+  HIPCHK(c, hipMalloc(&c->nc_lscratch, sizeof(double) * c->Cpmax * nt));
+  HIPCHK(c, hipHostMalloc(&c->h_nc_s, sizeof(double) * c->Cpmax * nt));
+  HIPCHK(c, hipHostMalloc(&c->h_nc_lam, sizeof(double) * c->chi_max * nt));
+  c->nc_bonds = (int)nt;
+  return TEBD_OK;
+}
+
+// One bond's number blocks (minmaxNLR, SizeOfBlocks, IndexLeft, IndexRight,
+// local_operations_module.f90:1201-1443), rows as theta rows a + chi_l i and columns as
+// theta columns g + chi_r j, in the reference's traversal order.
+struct NcBondBlocks {
+  int minNL = 0;
+  std::vector<std::vector<int>> rows, cols;
+};
+
+int nc_blocks(tebd_ctx* c, int link, NcBondBlocks& B) {
+  const int chi = c->chi_max, d = c->d, tot = c->nc_tot, maxf = c->nc_maxf;
+  const int chil = c->chi[link], chir = c->chi[link + 2];
+  const int* LLl = c->nc_LL.data() + (size_t)link * chi;
+  const int* LRr = c->nc_LR.data() + (size_t)(link + 2) * chi;
+  int minNL = 1 << 30, maxNL = 0, minNR = 1 << 30, maxNR = 0;
+  for (int a = 0; a < chi; ++a) {
+    minNL = std::min(minNL, LLl[a]);
+    maxNL = std::max(maxNL, LLl[a] == 10000 ? 0 : LLl[a]);
+    minNR = std::min(minNR, LRr[a]);
+    maxNR = std::max(maxNR, LRr[a] == 10000 ? 0 : LRr[a]);
+  }
+  maxNL += maxf;
+  maxNR += maxf;
+  if (maxNL > tot - minNR) maxNL = tot - minNR; else minNR = tot - maxNL;
+  if (maxNR > tot - minNL) maxNR = tot - minNL; else minNL = tot - maxNR;
+  const int nblk = maxNL - minNL + 1;
+  if (nblk < 1) return fail(c, TEBD_ERR_STATE, "number-conserving update: no number block on bond " + std::to_string(link + 1));
+  B.minNL = minNL;
+  B.rows.assign(nblk, {});
+  B.cols.assign(nblk, {});
+  for (int a = 0; a < chi; ++a)
+    for (int i = 0; i < d; ++i) {
+      const int lab = LLl[a], nl = lab + c->nc_conserv[i];
+      if (lab < 1000 && minNL <= nl && nl <= maxNL) {
+        if (a >= chil) return fail(c, TEBD_ERR_STATE, "labels: a label past the bond dimension on link " + std::to_string(link + 1));
+        B.rows[nl - minNL].push_back(a + chil * i);
+      }
+    }
+  for (int g = 0; g < chi; ++g)
+    for (int j = 0; j < d; ++j) {
+      const int lab = LRr[g], nr = lab + c->nc_conserv[j];
+      if (lab < 1000 && minNR <= nr && nr <= maxNR) {
+        if (g >= chir) return fail(c, TEBD_ERR_STATE, "labels: a label past the bond dimension on link " + std::to_string(link + 3));
+        B.cols[tot - nr - minNL].push_back(g + chir * j);
+      }
+    }
+  return TEBD_OK;
+}
+
+// The number-conserving update of nb independent bonds (0-based links): theta of each bond
+// by the ordinary theta kernel, its number blocks gathered and factorised as tasks of the
+// batched SVD (in chunks of ntask_max work slots), the singular values of every block of a
+// bond ordered together (Ordering, JudgePosition), the labels, lambda and truncation error
+// as UpdateLabelLeft / UpdateLabelRight / FormLambdaNC set them, and the Gammas written from
+// the kept states' block vectors.
+int run_bonds_nc(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
+  const int d = c->d, chi = c->chi_max, nt = c->ntask_max;
+  if (nb > nt) return fail(c, TEBD_ERR_ARG, "number-conserving update: batch larger than the workspace");
+  int e = ensure_nc_workspace(c);
+  if (e) return e;
+  const size_t sq = (size_t)c->Rmax * c->Rmax;
+  // 1. theta (m x n, ld m) of every bond into the NC theta store
+  BondTask* TH = c->h_tasks + 4 * nt;
+  BondTask* DTH = c->d_tasks + 4 * nt;
+  int max_chil = 1, max_chir = 1;
+  for (int i = 0; i < nb; ++i) {
+    BondTask& t = TH[i];
+    make_full_task(c, t, links[i], gate_ptrs[i], 0);
+    t.transposed = 0;
+    t.R = d * t.chil;
+    t.C = d * t.chir;
+    t.W = c->nc_theta + (size_t)i * sq;
+    max_chil = std::max(max_chil, t.chil);
+    max_chir = std::max(max_chir, t.chir);
+  }
+  HIPCHK(c, hipMemcpyAsync(DTH, TH, sizeof(BondTask) * nb, hipMemcpyHostToDevice, c->st));
+  {
+    ProfScope ps(c, 0);
+    HIPCHK(c, theta_gate(c, DTH, nb, max_chil, max_chir));
+  }
+  account_theta(c, TH, nb);
+  // 2. the blocks of every bond and their index lists
+  std::vector<NcBondBlocks> bb(nb);
+  for (int i = 0; i < nb; ++i)
+    if ((e = nc_blocks(c, links[i], bb[i]))) return e;
+  std::vector<int> idx;
+  std::vector<NcBlock> blocks;       // blocks with rows and columns
+  std::vector<int> blk_bond, blk_k;  // their bond and block number
+  std::vector<std::vector<int>> blk_of(nb);  // per bond and block number: index in `blocks` or -1
+  std::vector<size_t> row_off, col_off;
+  for (int i = 0; i < nb; ++i) {
+    const int m = d * c->chi[links[i]];
+    size_t voff = (size_t)i * 2 * sq;
+    blk_of[i].assign(bb[i].rows.size(), -1);
+    for (size_t k = 0; k < bb[i].rows.size(); ++k) {
+      const int lgt = (int)bb[i].rows[k].size(), trv = (int)bb[i].cols[k].size();
+      if (lgt == 0 || trv == 0) continue;
+      NcBlock b{};
+      b.theta = c->nc_theta + (size_t)i * sq;
+      b.m = m;
+      row_off.push_back(idx.size());
+      idx.insert(idx.end(), bb[i].rows[k].begin(), bb[i].rows[k].end());
+      col_off.push_back(idx.size());
+      idx.insert(idx.end(), bb[i].cols[k].begin(), bb[i].cols[k].end());
+      b.lgt = lgt;
+      b.trv = trv;
+      b.transposed = lgt < trv ? 1 : 0;
+      b.R = std::max(lgt, trv);
+      const int mins = std::min(lgt, trv);
+      b.U = c->nc_vec + voff;
+      voff += (size_t)lgt * mins;
+      b.V = c->nc_vec + voff;
+      voff += (size_t)trv * mins;
+      blk_of[i][k] = (int)blocks.size();
+      blocks.push_back(b);
+      blk_bond.push_back(i);
+      blk_k.push_back((int)k);
+    }
+  }
+  const int nbk = (int)blocks.size();
+  HIPCHK(c, nc_grow(c, &c->d_nc_idx, &c->nc_idx_cap, std::max<size_t>(idx.size(), 1)));
+  HIPCHK(c, nc_grow(c, &c->d_nc_blk, &c->nc_blk_cap, std::max(nbk, 1)));
+  const int split = nc_gather_split();
+  if ((size_t)nbk * split > c->nc_part_cap) {
+    HIPCHK(c, nc_grow(c, &c->d_nc_part, &c->nc_part_cap, (size_t)nbk * split));
+    (void)hipHostFree(c->h_nc_part);
+    c->h_nc_part = nullptr;
+    HIPCHK(c, hipHostMalloc(&c->h_nc_part, sizeof(double) * c->nc_part_cap));
+  }
+  for (int q = 0; q < nbk; ++q) {
+    blocks[q].rows = c->d_nc_idx + row_off[q];
+    blocks[q].cols = c->d_nc_idx + col_off[q];
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_nc_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, c->st));
+  // 3. the weight of every block (blocks without weight skip the SVD; a non-finite one is
+  // reported)
+  std::vector<double> wgt(nbk, 0.0);
+  if (nbk > 0) {
+    HIPCHK(c, hipMemcpyAsync(c->d_nc_blk, blocks.data(), sizeof(NcBlock) * nbk, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_nc_gather(c->d_nc_blk, nbk, c->d_nc_part, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->h_nc_part, c->d_nc_part, sizeof(double) * nbk * split, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int q = 0; q < nbk; ++q) {
+      for (int x = 0; x < split; ++x) wgt[q] += c->h_nc_part[(size_t)q * split + x];
+      if (!std::isfinite(wgt[q]))
+        return fail(c, TEBD_ERR_NAN, "bond " + std::to_string(links[blk_bond[q]] + 1) + ": theta is not finite");
+    }
+  }
+  // 4. SVDs of the blocks with weight, ntask_max at a time; singular values to the host
+  std::vector<std::vector<double>> sv(nbk);  // per block: min(lgt, trv) values, descending
+  for (int q = 0; q < nbk; ++q) sv[q].assign(std::min(blocks[q].lgt, blocks[q].trv), 0.0);
+  std::vector<int> live;
+  for (int q = 0; q < nbk; ++q)
+    if (wgt[q] > 0.0) live.push_back(q);
+  BondTask* S = c->h_tasks;
+  BondTask* DS = c->d_tasks;
+  std::vector<NcBlock> chunk_blk;
+  std::vector<int> err_links;
+  for (size_t q0 = 0; q0 < live.size(); q0 += nt) {
+    const int n = (int)std::min(live.size() - q0, (size_t)nt);
+    chunk_blk.assign(n, NcBlock{});
+    err_links.assign(n, 0);
+    int Cpm = 1, Pm = 1;
+    for (int q = 0; q < n; ++q) {
+      const NcBlock& b = blocks[live[q0 + q]];
+      BondTask& t = S[q];
+      std::memset(&t, 0, sizeof(t));
+      t.d = d;
+      t.link = links[blk_bond[live[q0 + q]]];
+      t.R = b.R;
+      t.C = std::min(b.lgt, b.trv);
+      t.Cpad = round_up(t.C, JC);
+      t.P = t.Cpad / JC;
+      t.W = c->W + (size_t)q * c->Rmax * c->Cpmax;
+      t.V = c->V + (size_t)q * c->Cpmax * c->Cpmax;
+      t.NV = t.Cpad;
+      t.s = c->s + (size_t)q * c->Cpmax;
+      t.perm = c->perm + (size_t)q * c->Cpmax;
+      attach_jacobi(c, t, q);
+      attach_qr(c, t, q);
+      t.result = c->results + (size_t)RESULT_STRIDE * q;
+      t.lc_out = c->nc_lscratch + (size_t)q * c->Cpmax;
+      // at most chi_max values of a block can be among the chi_max kept ones (and the
+      // vector panels hold chi_max columns)
+      t.chi_max = std::min(t.C, c->chi_max);
+      chunk_blk[q] = b;
+      chunk_blk[q].W = t.W;
+      err_links[q] = t.link;
+      Cpm = std::max(Cpm, t.Cpad);
+      Pm = std::max(Pm, t.P);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_nc_blk, chunk_blk.data(), sizeof(NcBlock) * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(DS, S, sizeof(BondTask) * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_nc_gather(c->d_nc_blk, n, c->d_nc_part, c->st));
+    bool used_qr = false;
+    int max_k = 1;
+    if ((e = svd_factor(c, DS, S, err_links.data(), n, Cpm, Pm, false, &used_qr, &max_k))) return e;
+    HIPCHK(c, launch_nc_vectors(c->d_nc_blk, DS, n, max_k, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->h_nc_s, c->s, sizeof(double) * c->Cpmax * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int q = 0; q < n; ++q) {
+      const int k = (int)c->h_results[RESULT_STRIDE * q];
+      std::vector<double>& v = sv[live[q0 + q]];
+      for (int a = 0; a < k && a < (int)v.size(); ++a) v[a] = c->h_nc_s[(size_t)q * c->Cpmax + a];
+    }
+  }
+  // 5. per bond: all blocks' values in block order (FlattenSS), normalised as RenormThetaNC
+  // normalises theta, ordered (Ordering, JudgePosition), kept above the cut (UpdateLabelLeft,
+  // FormLambdaNC), and the kept states' Gammas
+  std::vector<NcKept> kept;
+  double tsum = 0.0;
+  for (int i = 0; i < nb; ++i) {
+    const int link = links[i];
+    std::vector<double> ssfl;
+    std::vector<int> fl_blk, fl_pos;  // block number and position of each flat value
+    double wsum = 0.0;
+    for (size_t k = 0; k < bb[i].rows.size(); ++k) {
+      const int q = blk_of[i][k];
+      if (q < 0) continue;  // min(lgt, trv) = 0: no values
+      for (size_t a = 0; a < sv[q].size(); ++a) {
+        ssfl.push_back(sv[q][a]);
+        fl_blk.push_back((int)k);
+        fl_pos.push_back((int)a);
+        wsum += sv[q][a] * sv[q][a];
+      }
+    }
+    if (!(wsum > 0.0)) return fail(c, TEBD_ERR_NAN, "bond " + std::to_string(link + 1) + ": theta is zero in every number block");
+    const double inv = 1.0 / std::sqrt(wsum);
+    for (double& x : ssfl) x *= inv;
+    // Ordering: backwards, the first occurrence of the smallest remaining value each time
+    const int sz = (int)ssfl.size();
+    std::vector<int> order(sz);
+    {
+      std::vector<int> ix(sz);
+      for (int a = 0; a < sz; ++a) ix[a] = a;
+      // descending by value, equal values with the higher flat index first
+      std::stable_sort(ix.begin(), ix.end(), [&](int x, int y) {
+        if (ssfl[x] != ssfl[y]) return ssfl[x] > ssfl[y];
+        return x > y;
+      });
+      order = ix;
+    }
+    int knew = 0;
+    while (knew < std::min(chi, sz) && ssfl[order[knew]] > kNcCut) ++knew;
+    if (knew < 1) return fail(c, TEBD_ERR_NAN, "bond " + std::to_string(link + 1) + ": no singular value above the cut");
+    double renorm = 0.0;
+    for (int b = 0; b < knew; ++b) renorm += ssfl[order[b]] * ssfl[order[b]];
+    const double trunc = std::fabs(1.0 - renorm);
+    tsum += trunc;
+    int* LLc = c->nc_LL.data() + (size_t)(link + 1) * chi;
+    int* LRc = c->nc_LR.data() + (size_t)(link + 1) * chi;
+    double* lh = c->h_nc_lam + (size_t)i * chi;
+    for (int b = 0; b < chi; ++b) {
+      if (b < knew) {
+        const int lab = bb[i].minNL + fl_blk[order[b]];
+        LLc[b] = lab;
+        LRc[b] = c->nc_tot - lab;
+        lh[b] = ssfl[order[b]] / std::sqrt(renorm);
+      } else {
+        LLc[b] = 10000;
+        LRc[b] = 10000;
+        lh[b] = 0.0;
+      }
+    }
+    const int chil = c->chi[link], chir = c->chi[link + 2];
+    for (int b = 0; b < knew; ++b) {
+      const int q = blk_of[i][fl_blk[order[b]]];
+      NcKept o{};
+      o.rows = blocks[q].rows;
+      o.cols = blocks[q].cols;
+      o.lgt = blocks[q].lgt;
+      o.trv = blocks[q].trv;
+      o.U = blocks[q].U;
+      o.V = blocks[q].V;
+      o.pos = fl_pos[order[b]];
+      o.gl = c->gam + (size_t)link * c->gslab();
+      o.gr = c->gam + (size_t)(link + 1) * c->gslab();
+      o.ll = c->lam + (size_t)link * c->chi_max;
+      o.lr = c->lam + (size_t)(link + 2) * c->chi_max;
+      o.chil = chil;
+      o.chir = chir;
+      o.d = d;
+      o.knew = knew;
+      o.b = b;
+      kept.push_back(o);
+    }
+    c->chi[link + 1] = knew;
+    c->last_rank[link + 1] = knew;
+    c->last_trunc[link + 1] = trunc;
+  }
+  HIPCHK(c, nc_grow(c, &c->d_nc_kept, &c->nc_kept_cap, kept.size()));
+  HIPCHK(c, hipMemcpyAsync(c->d_nc_kept, kept.data(), sizeof(NcKept) * kept.size(), hipMemcpyHostToDevice, c->st));
+  {
+    ProfScope ps(c, 2);
+    HIPCHK(c, launch_nc_gammas(c->d_nc_kept, (int)kept.size(), kNcCut, c->st));
+  }
+  for (int i = 0; i < nb; ++i)
+    HIPCHK(c, hipMemcpyAsync(c->lam + (size_t)(links[i] + 1) * c->chi_max, c->h_nc_lam + (size_t)i * chi,
+                             sizeof(double) * chi, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  drain_profile(c);
+  c->stats[0] += nb;
+  *trunc_sum = tsum;
+  return TEBD_OK;
+}
+
 int run_bonds(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
+  if (c->nc) return run_bonds_nc(c, links, nb, gate_ptrs, trunc_sum);
   if (c->mode & TEBD_MODE_LEARNED_RANK) return run_bonds_learned(c, links, nb, gate_ptrs, trunc_sum);
   return run_bonds_full(c, links, nb, gate_ptrs, trunc_sum);
 }
@@ -1276,6 +1641,11 @@ tebd_ctx* make_lane(tebd_ctx* c, int lane, int slot0, int nslots) {
   l->cf_sets = 0;
   l->cr_theta[0] = l->cr_theta[1] = l->cr_T = l->cr_K = l->cr_B = l->cr_A = l->cr_rho2 = l->cr_rho1 = nullptr;
   l->cr_s = l->cr_phase = nullptr;
+  l->nc_theta = l->nc_vec = nullptr;   // the number-conserving update runs on the context only
+  l->d_nc_idx = nullptr;
+  l->d_nc_blk = nullptr;
+  l->d_nc_kept = nullptr;
+  l->d_nc_part = l->nc_lscratch = l->h_nc_s = l->h_nc_lam = l->h_nc_part = nullptr;
   l->owns_gemm = false;
   l->owns_stream = lane != 0;
   l->st = lane == 0 ? c->st : nullptr;
@@ -1586,6 +1956,16 @@ void free_all(tebd_ctx* c) {
   (void)hipFree(c->Pb);
   (void)hipFree(c->d_probe);
   (void)hipHostFree(c->h_probe);
+  (void)hipFree(c->nc_theta);
+  (void)hipFree(c->nc_vec);
+  (void)hipFree(c->d_nc_idx);
+  (void)hipFree(c->d_nc_blk);
+  (void)hipFree(c->d_nc_kept);
+  (void)hipFree(c->d_nc_part);
+  (void)hipFree(c->nc_lscratch);
+  (void)hipHostFree(c->h_nc_s);
+  (void)hipHostFree(c->h_nc_lam);
+  (void)hipHostFree(c->h_nc_part);
   (void)hipFree(c->d_gemm);
   (void)hipHostFree(c->h_gemm);
   (void)hipFree(c->d_kprime);
@@ -2082,7 +2462,8 @@ int tebd_trotter_step(tebd_ctx* c, double* truncerr) {
       for (size_t b0 = 0; b0 < links.size(); b0 += c->ntask_max) {
         const int nb = (int)std::min(links.size() - b0, (size_t)c->ntask_max);
         double t = 0.0;
-        e = run_bonds_split(c, links.data() + b0, nb, gp.data() + b0, &t);
+        e = c->nc ? run_bonds_nc(c, links.data() + b0, nb, gp.data() + b0, &t)
+                  : run_bonds_split(c, links.data() + b0, nb, gp.data() + b0, &t);
         if (e) return e;
         total += t;
       }
@@ -2149,6 +2530,7 @@ int canonical_form_chains(tebd_ctx* c, int r0, int dr) {
 int tebd_canonical_form_all(tebd_ctx* c) {
   int e = ensure_state(c, false);
   if (e) return e;
+  if (c->nc) return fail(c, TEBD_ERR_UNSUPPORTED, "canonical_form_all: not with number conservation (CanonicalFormAllNC)");
   // the chains of an ensemble are independent: with lanes, chain r goes to lane r mod n,
   // each lane on its own stream and host thread; each chain sees the same sequence of
   // bond operations either way
@@ -2180,6 +2562,7 @@ int tebd_canonical_form_all(tebd_ctx* c) {
 int tebd_canonical_form(tebd_ctx* c, int link) {
   int e = ensure_state(c, false);
   if (e) return e;
+  if (c->nc) return fail(c, TEBD_ERR_UNSUPPORTED, "canonical_form: not with number conservation");
   if (link < 1 || link > c->L - 1) return fail(c, TEBD_ERR_ARG, "canonical_form: bad link");
   if (is_cut(c, link - 1)) return fail(c, TEBD_ERR_ARG, "canonical_form: bond joins two chains of the ensemble");
   if ((e = ensure_cf_workspace(c, 1))) return e;
@@ -2267,6 +2650,64 @@ int tebd_rank_model_get(tebd_ctx* c, float* params, int n) {
   if (!c || !params || n != TEBD_MLP_PARAMS) return TEBD_ERR_ARG;
   if (!c->have_model) return fail(c, TEBD_ERR_STATE, "no rank model installed");
   HIPCHK(c, hipMemcpy(params, c->mlp, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return TEBD_OK;
+}
+
+int tebd_set_conserved(tebd_ctx* c, const int* conserv, int tot_num) {
+  if (!c) return TEBD_ERR_ARG;
+  if (!conserv && tot_num < 0) {  // off
+    c->nc = false;
+    return TEBD_OK;
+  }
+  if (tot_num < 0) return fail(c, TEBD_ERR_ARG, "set_conserved: negative total");
+  if (c->bc != 'O' || c->nchain > 1)
+    return fail(c, TEBD_ERR_ARG, "set_conserved: number conservation needs one open chain");
+  std::vector<int> cv(c->d);
+  int maxf = 0;
+  for (int i = 0; i < c->d; ++i) {
+    cv[i] = conserv ? conserv[i] : i;
+    if (cv[i] < 0 || cv[i] >= 1000) return fail(c, TEBD_ERR_ARG, "set_conserved: local numbers in [0, 1000)");
+    maxf = std::max(maxf, cv[i]);
+  }
+  c->nc_conserv = cv;
+  c->nc_tot = tot_num;
+  c->nc_maxf = maxf;
+  c->nc = true;
+  if (c->nc_LL.size() != (size_t)(c->L + 1) * c->chi_max) {
+    c->nc_LL.assign((size_t)(c->L + 1) * c->chi_max, 10000);
+    c->nc_LR.assign((size_t)(c->L + 1) * c->chi_max, 10000);
+  }
+  return TEBD_OK;
+}
+
+int tebd_set_labels(tebd_ctx* c, const int* label_left, const int* label_right, int chi_in) {
+  if (!c || !label_left || !label_right || chi_in < 1) return fail(c, TEBD_ERR_ARG, "set_labels: bad arguments");
+  if (!c->nc) return fail(c, TEBD_ERR_STATE, "set_labels: call tebd_set_conserved first");
+  const int chi = c->chi_max, L = c->L;
+  std::vector<int> LL((size_t)(L + 1) * chi, 10000), LR((size_t)(L + 1) * chi, 10000);
+  for (int l = 0; l <= L; ++l)
+    for (int a = 0; a < chi_in; ++a) {
+      const int x = label_left[(size_t)l * chi_in + a], y = label_right[(size_t)l * chi_in + a];
+      if ((x < 1000) != (y < 1000)) return fail(c, TEBD_ERR_ARG, "set_labels: left and right labels disagree on used states");
+      if (x >= 1000) continue;
+      if (a >= chi) return fail(c, TEBD_ERR_SHAPE, "set_labels: a used state past chi_max");
+      LL[(size_t)l * chi + a] = x;
+      LR[(size_t)l * chi + a] = y;
+    }
+  c->nc_LL.swap(LL);
+  c->nc_LR.swap(LR);
+  return TEBD_OK;
+}
+
+int tebd_get_labels(tebd_ctx* c, int* label_left, int* label_right, int chi_out) {
+  if (!c || !label_left || !label_right || chi_out < 1) return fail(c, TEBD_ERR_ARG, "get_labels: bad arguments");
+  if (!c->nc) return fail(c, TEBD_ERR_STATE, "get_labels: no number conservation set");
+  const int chi = c->chi_max;
+  for (int l = 0; l <= c->L; ++l)
+    for (int a = 0; a < chi_out; ++a) {
+      label_left[(size_t)l * chi_out + a] = a < chi ? c->nc_LL[(size_t)l * chi + a] : 10000;
+      label_right[(size_t)l * chi_out + a] = a < chi ? c->nc_LR[(size_t)l * chi + a] : 10000;
+    }
   return TEBD_OK;
 }
 
@@ -2382,6 +2823,7 @@ int tebd_imag_time_prop(tebd_ctx* c, const double* H, int nh, double dtau, int c
                         int steps_for_judge, double conv1, double conv2, int* steps) {
   int e = ensure_state(c, false);
   if (e) return e;
+  if (c->nc) return fail(c, TEBD_ERR_UNSUPPORTED, "imag_time_prop: not with number conservation (ImagTimePropNC)");
   const int Lc = c->chain_len, n = c->d * c->d;
   const int per = c->bc == 'P' ? Lc : Lc - 1;  // bond Hamiltonians per chain
   if (!H || nh != c->nchain * per || !(dtau > 0.0) || chi_in < 1 || chi_in > c->chi_max || max_steps < 0 ||

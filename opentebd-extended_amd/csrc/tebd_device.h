@@ -31,6 +31,37 @@ struct GemmTask {
 
 constexpr int RESULT_STRIDE = 8;
 
+// One block of a number-conserving bond update: theta is block diagonal in the number
+// on the left, and block k gathers the rows (a, i) and columns (g, j) of number k.
+struct NcBlock {
+  const zc* theta;   // the bond's theta, m x n (ld m)
+  int m;
+  const int* rows;   // lgt theta rows a + chi_l i
+  const int* cols;   // trv theta columns g + chi_r j
+  int lgt, trv;
+  int transposed;    // the work matrix is B^dagger (lgt < trv)
+  zc* W;             // SVD work matrix (R x C, ld R); null: only the squared sum is formed
+  int R;
+  zc* U;             // lgt x min(lgt, trv): left singular vectors (the bond's vector store)
+  zc* V;             // trv x min(lgt, trv): right singular vectors
+};
+
+// One kept Schmidt state b of a number-conserving bond update: its block's vectors and
+// position, and where its column of Gamma_l and row of Gamma_r go.
+struct NcKept {
+  const int* rows;
+  const int* cols;
+  int lgt, trv;
+  const zc* U;
+  const zc* V;
+  int pos;
+  zc* gl;            // Gamma_l (chi_l x d x knew)
+  zc* gr;            // Gamma_r (knew x d x chi_r)
+  const double* ll;
+  const double* lr;
+  int chil, chir, d, knew, b;
+};
+
 // A-posteriori check of one sketched bond (learned-rank path), by two sets of p
 // complex Gaussian probes.  theta (m x n), E = theta - Q Q^dagger theta the part of theta
 // outside the sketch's range basis Q (m x r), kept triplets (s_j, u_j, v_j), j < k:

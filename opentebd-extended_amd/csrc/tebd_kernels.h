@@ -63,3 +63,7 @@ hipError_t launch_rho_reduce(const zc* G, const double* s, const zc* T, int chi_
 hipError_t launch_rho1_trace(const zc* Theta, int chi, int d, zc* rho, hipStream_t st);
 hipError_t launch_shrink_slab(const zc* src, int cl, int d, zc* dst, int ncl, int ncr, hipStream_t st);
 hipError_t launch_zero_lambda_tail(double* lam, int nbonds, int chi, int chi_max, hipStream_t st);
+hipError_t launch_nc_gather(const NcBlock* d_blocks, int nblocks, double* d_partial, hipStream_t st);
+int nc_gather_split();
+hipError_t launch_nc_vectors(const NcBlock* d_blocks, const BondTask* d_tasks, int nblocks, int max_k, hipStream_t st);
+hipError_t launch_nc_gammas(const NcKept* d_kept, int nkept, double cut, hipStream_t st);

@@ -86,6 +86,9 @@ def lib():
         L.tebd_rank_log.argtypes = [P, I]
         L.tebd_rank_log_get.argtypes = [P, P, P, I]
         L.tebd_sketch_log_get.argtypes = [P, P, I]
+        L.tebd_set_conserved.argtypes = [P, P, I]
+        L.tebd_set_labels.argtypes = [P, P, P, I]
+        L.tebd_get_labels.argtypes = [P, P, P, I]
         F = ctypes.c_float
         L.tebd_mlp_gradient.argtypes = [P, P, P, I, F, P, P]
         L.tebd_mlp_train.argtypes = [P, P, P, I, I, F, F, P]
@@ -99,6 +102,7 @@ def lib():
                      "tebd_autotune_info",
                      "tebd_set_profiling", "tebd_get_profile", "tebd_expval_two_site", "tebd_rank_model_set",
                      "tebd_set_learned_params", "tebd_rank_model_get", "tebd_rank_log", "tebd_rank_log_get", "tebd_sketch_log_get", "tebd_mlp_gradient",
+                     "tebd_set_conserved", "tebd_set_labels", "tebd_get_labels",
                      "tebd_mlp_train", "tebd_mlp_predict"):
             getattr(L, name).restype = I
         _lib = L

@@ -233,6 +233,33 @@ class Context:
                 self._check(m)
         return X, y
 
+    # -- number conservation (TwoSiteOpNC) ----------------------------------------
+    def set_conserved(self, conserv=None, tot_num=0, labels=None):
+        """Keep the number of every Schmidt state (the reference's number-conserving update):
+        conserv[i] the number of local state i (None: i), tot_num the total; labels =
+        (LabelLeft, LabelRight), (L+1) x chi ints with 10000 for unused states, e.g.
+        models.product_labels.  set_conserved(None, -1) switches it off."""
+        c = None if conserv is None else np.ascontiguousarray(conserv, np.int32)
+        if c is not None and c.shape != (self.d,):
+            raise ValueError("one number per local state")
+        self._check(self._lib.tebd_set_conserved(self._h, None if c is None else _p(c), int(tot_num)))
+        if labels is not None:
+            self.set_labels(*labels)
+
+    def set_labels(self, label_left, label_right):
+        LL = np.ascontiguousarray(label_left, np.int32)
+        LR = np.ascontiguousarray(label_right, np.int32)
+        if LL.shape != LR.shape or LL.ndim != 2 or LL.shape[0] != self.L + 1:
+            raise ValueError("labels: two (L+1) x chi arrays")
+        self._check(self._lib.tebd_set_labels(self._h, _p(LL), _p(LR), LL.shape[1]))
+
+    def labels(self, chi_out=None):
+        chi = int(chi_out or self.chi_max)
+        LL = np.zeros((self.L + 1, chi), np.int32)
+        LR = np.zeros((self.L + 1, chi), np.int32)
+        self._check(self._lib.tebd_get_labels(self._h, _p(LL), _p(LR), chi))
+        return LL, LR
+
     def sketch_log_data(self):
         """Per probe-checked sketch of the record: dict of arrays k, kmax, total (||theta||^2),
         discarded (fraction), smallest (kept s^2), eta (estimate of sum_j ||E v_j||^2),

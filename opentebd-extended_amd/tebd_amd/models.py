@@ -265,3 +265,26 @@ def neel(L, d, up, down):
     for s in range(L):
         c[s, up if s % 2 == 0 else down] = 1.0
     return c
+
+
+def product_labels(coeffs, chi, conserv=None, tot_num=None):
+    """Labels of a product state of number eigenstates for the number-conserving update
+    (ProductStateLabels, Hamiltonian_tools_module.f90:124-178): the number left of each
+    link in its one Schmidt state, from each site's nonzero coefficient, the right labels
+    tot_num - left, 10000 for every other state.  conserv: the number of each local state
+    (default: its index); tot_num defaults to the state's total."""
+    c = np.asarray(coeffs)
+    L, d = c.shape
+    cv = np.arange(d) if conserv is None else np.asarray(conserv)
+    LL = np.full((L + 1, chi), 10000, np.int32)
+    LR = np.full((L + 1, chi), 10000, np.int32)
+    LL[0, 0] = 0
+    for s in range(L):
+        nz = np.flatnonzero(np.abs(c[s]) != 0.0)
+        if nz.size == 0 or np.any(cv[nz] != cv[nz[0]]):
+            raise ValueError(f"site {s + 1}: not a number eigenstate")
+        LL[s + 1, 0] = LL[s, 0] + int(cv[nz[0]])
+    tot = int(LL[L, 0]) if tot_num is None else int(tot_num)
+    LR[:, 0] = tot - LL[:, 0]
+    return LL, LR
+

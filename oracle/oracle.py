@@ -43,6 +43,8 @@ def lib():
         L.orc_chain_entropy.argtypes = [I, I, I, P, P]
         L.orc_set_checker_speed.argtypes = [I]
         L.orc_trotter_step_sched.argtypes = [I, I, I, P, P, P, I, I, ctypes.c_char, P]
+        L.orc_two_site_op_nc.argtypes = [I, I, I, P, P, P, P, I, P, P, I, I, P]
+        L.orc_trotter_step_nc.argtypes = [I, I, I, P, P, P, P, P, P, I, I, P]
         _LIB = L
     return _LIB
 
@@ -115,6 +117,37 @@ class OracleState:
             raise RuntimeError(f"zgesvd info={info} (or a gate table of the wrong size)")
         return t.value
 
+    # -- number conservation (TwoSiteOpNC) --------------------------------------
+    def set_conserved(self, labels_left, labels_right, conserv, tot_num):
+        """Labels (L+1, chi) of the Schmidt states (number left / right of each link,
+        10000 = unused), the number of each local state and the total number."""
+        self.LL = np.ascontiguousarray(labels_left, np.int32)
+        self.LR = np.ascontiguousarray(labels_right, np.int32)
+        self.conserv = np.ascontiguousarray(conserv, np.int32)
+        self.tot = int(tot_num)
+        self.maxf = int(self.conserv.max())
+
+    def _nc(self):
+        return _p(self.LL), _p(self.LR)
+
+    def two_site_op_nc(self, link, gate):
+        g = np.ascontiguousarray(np.asarray(gate, np.complex128).T)
+        t = ctypes.c_double()
+        info = lib().orc_two_site_op_nc(*self._args(), *self._nc(), int(link), _p(g), _p(self.conserv), self.tot,
+                                        self.maxf, ctypes.byref(t))
+        if info != 0:
+            raise RuntimeError(f"zgesvd info={info} (or no block)")
+        return t.value
+
+    def trotter_step_nc(self, gates):
+        g = _gate_stack(gates, self.d)
+        t = ctypes.c_double()
+        info = lib().orc_trotter_step_nc(*self._args(), *self._nc(), _p(g), _p(self.conserv), self.tot, self.maxf,
+                                         ctypes.byref(t))
+        if info != 0:
+            raise RuntimeError(f"zgesvd info={info} (or no block)")
+        return t.value
+
     def canonical_form_all(self):
         info = lib().orc_canonical_form_all(*self._args())
         if info != 0:
@@ -141,6 +174,22 @@ class OracleState:
         out = np.zeros(self.L - 1)
         lib().orc_chain_entropy(self.L, self.d, self.chi, _p(self.lam), _p(out))
         return out
+
+
+def product_labels(coeffs, chi, conserv, tot_num):
+    """ProductStateLabels (Hamiltonian_tools_module.f90:124-178) for a product state of
+    number eigenstates: the number left of each link from the first nonzero coefficient
+    of each site, the right labels tot_num - left, 10000 past the first Schmidt state."""
+    c = np.asarray(coeffs)
+    L = c.shape[0]
+    LL = np.full((L + 1, chi), 10000, np.int32)
+    LR = np.full((L + 1, chi), 10000, np.int32)
+    LL[0, 0] = 0
+    for s in range(L):
+        j = int(np.flatnonzero(np.abs(c[s]) != 0.0)[0])
+        LL[s + 1, 0] = LL[s, 0] + int(conserv[j])
+    LR[:, 0] = tot_num - LL[:, 0]
+    return LL, LR
 
 
 def two_point_rdms(gam, lam, phase=None):

@@ -34,6 +34,11 @@
 !            (observables_module.f90:2286-2526) with local and averaged Sz (spins) or n,
 !            the correlator of the bond two-site operator Sz x Sz (spins) or n x n, for
 !            fermions the fermi-phase correlator c+_up x c_up, and entropies of level 2
+!   mode 6 = number-conserving trajectory: the product state's labels (AllocateLabel,
+!            ProductStateLabels, Hamiltonian_tools_module.f90:124-178), then TrotterStepNC
+!            (propagation_module.f90:890-1017, TwoSiteOpNC local_operations_module.f90:1845-1964);
+!            the next line: totNum intDegFree (0: number = state index - 1, 1: the model's
+!            Conserv); the labels of every link are printed with the final state
 !   [mode 1: link, then Gamma(chi,d,chi) re im for each site, Lambda(chi) for L+1 links]
 !   [mode 2: chiIn dtITP maxITPsteps stepsForJudge convCriterion1 convCriterion2]
 PROGRAM golden_driver
@@ -58,6 +63,8 @@ REAL(KIND=rKind), ALLOCATABLE :: sz(:,:), nop(:,:)
 INTEGER :: model, L, param, chi, nsteps, canon, i, j, k, step, nf, mode, link, chiIn
 INTEGER :: a, b, dd, nH, nfc
 TYPE(measure) :: Meas
+TYPE(vectorInt), POINTER :: LabelLeft(:), LabelRight(:)
+INTEGER :: intdeg
 CHARACTER :: bcc
 CHARACTER(len=256) :: jobfile
 
@@ -213,6 +220,34 @@ IF (mode == 2) THEN
   CALL ImagTimeProp(H, Gammas, Lambdas, chiIn)
   CALL PrintObs(0, 0.0_rKind)
   CALL PrintState()
+  STOP
+END IF
+IF (mode == 6) THEN
+  READ(11,*) totNum, intdeg
+  CALL SVDFinish()  ! the NC update allocates its own SVD workspace per block (SVDInitNC)
+  CALL AllocateLabel(LabelLeft, LabelRight, chi)
+  IF (intdeg == 1) THEN
+    CALL ProductStateLabels(LabelLeft, LabelRight, carray, 1)
+  ELSE
+    CALL ProductStateLabels(LabelLeft, LabelRight, carray)
+  END IF
+  IF (intdeg == 1) PRINT '(A,64I4)', 'CONSERV', (Conserv%vi(k), k=1,dd)
+  CALL PrintObs(0, 0.0_rKind)
+  DO step=1,nsteps
+    IF (intdeg == 1) THEN
+      CALL TrotterStepNC(U, Gammas, Lambdas, LabelLeft, LabelRight, err, 1)
+    ELSE
+      CALL TrotterStepNC(U, Gammas, Lambdas, LabelLeft, LabelRight, err)
+    END IF
+    CALL PrintObs(step, err)
+  END DO
+  CALL PrintState()
+  DO i=1,L+1
+    PRINT '(A,I6)', 'LABELL', i
+    PRINT '(1024I7)', (LabelLeft(i)%vi(k), k=1,chi)
+    PRINT '(A,I6)', 'LABELR', i
+    PRINT '(1024I7)', (LabelRight(i)%vi(k), k=1,chi)
+  END DO
   STOP
 END IF
 totTrunc = 0.0_rKind

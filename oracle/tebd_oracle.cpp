@@ -20,6 +20,10 @@
 //     indices exchanged, SVD without the normalisation of ReshapeTheta),
 //   * CanonicalForm / CanonicalFormAll          (local_operations_module.f90:496-633,
 //                                                propagation_module.f90:617-634),
+//   * the number-conserving update TwoSiteOpNC and TrotterStep2ndOrderNC with the labels of
+//     every Schmidt state (local_operations_module.f90:748-1964; propagation_module.f90:956-1017),
+//     in its general form (intDegFree: the number of local state i is conserv[i]; without
+//     it the reference uses i - 1 for the 1-based state i, i.e. conserv[i] = i here),
 //   * observables: TotalEnergy, OneSiteExpVal, ChainEntropy
 //                                               (observables_module.f90:95-117,134-162,
 //                                                704-832,1432-1466,1583-1602).
@@ -174,6 +178,203 @@ int two_site_op(const View& v, int link, const cplx* Uop, double* truncerr, bool
       for (int i = 0; i < d; ++i)
         v.g(link + 1, a, i, b) =
             std::fabs(l2[b]) > 1e-15 ? VT[a + (size_t)n * (i * chi + b)] / l2[b] : cplx(0.0);
+  return info;
+}
+
+// ---- number-conserving update --------------------------------------------------------
+// Labels: LL[l * chi + a] = LabelLeft(l+1)%vi(a+1), the number of particles left of link l
+// (0-based, l = 0..L) in Schmidt state a; LR likewise for the right; 10000 = unused.
+struct Labels {
+  int chi;
+  int* LL;
+  int* LR;
+  int& left(int l, int a) const { return LL[(size_t)l * chi + a]; }
+  int& right(int l, int a) const { return LR[(size_t)l * chi + a]; }
+};
+
+// The reference's 10.0**(-10): a default-kind (single precision) literal, compared with
+// double precision values after promotion.
+const double kNcCut = (double)std::pow(10.0f, -10.0f);
+
+// Ordering (local_operations_module.f90:1515-1538): order[b] (0-based) of the values from
+// greatest to least; working backwards, each step takes the first occurrence of the
+// smallest remaining value, so equal values end up with the higher index first.
+std::vector<int> nc_ordering(const std::vector<double>& v) {
+  const int n = (int)v.size();
+  std::vector<int> order(n, 0);
+  std::vector<double> w(v);
+  std::vector<char> gone(n, 0);
+  for (int b = n - 1; b >= 0; --b) {
+    int best = -1;
+    for (int k = 0; k < n; ++k)
+      if (!gone[k] && (best < 0 || w[k] < w[best])) best = k;
+    order[b] = best;
+    gone[best] = 1;
+  }
+  return order;
+}
+
+// TwoSiteOpNC_c (local_operations_module.f90:1845-1964), link 0-based.  conserv[i]: the
+// number of local state i (0-based); maxf: maxFilling; tot: totNum.
+int two_site_op_nc(const View& v, const Labels& lb, int link, const cplx* Uop, const int* conserv, int tot, int maxf,
+                   double* truncerr) {
+  const int chi = v.chi, d = v.d;
+  const double* l0 = v.lm(link);
+  const double* l1 = v.lm(link + 1);
+  const double* l2 = v.lm(link + 2);
+  std::vector<cplx> theta((size_t)chi * d * d * chi, cplx(0.0));
+  auto th = [&](int a, int i, int j, int b) -> cplx& {
+    return theta[a + (size_t)chi * (i + (size_t)d * (j + (size_t)d * b))];
+  };
+  // FormThetaNC (:748-850, the intDegFree branch)
+  for (int a = 0; a < chi; ++a)
+    for (int g = 0; g < chi; ++g)
+      for (int bt = 0; bt < chi; ++bt)
+        for (int i = 0; i < d; ++i)
+          for (int j = 0; j < d; ++j) {
+            const int lN1 = lb.left(link, a), lN2 = lb.left(link + 1, bt);
+            const int rN1 = lb.right(link + 2, g), rN2 = lb.right(link + 1, bt);
+            if (std::max(std::max(lN1, lN2), std::max(rN1, rN2)) < 1000 && lN2 - lN1 <= maxf && lN2 - lN1 >= 0 &&
+                rN2 - rN1 <= maxf && rN2 - rN1 >= 0 && lN1 + conserv[i] == lN2 && rN1 + conserv[j] == rN2)
+              th(a, i, j, g) += l0[a] * v.g(link, a, i, bt) * l1[bt] * v.g(link + 1, bt, j, g) * l2[g];
+          }
+  // ThetaOperationNC_c (:950-1044): the gate within the sector of total number
+  // N2 = tot - N_left(a) - N_right(g) on the two sites
+  std::vector<cplx> tmp((size_t)chi * d * d * chi, cplx(0.0));
+  auto tp = [&](int a, int i, int j, int b) -> cplx& {
+    return tmp[a + (size_t)chi * (i + (size_t)d * (j + (size_t)d * b))];
+  };
+  const int d2 = d * d;
+  for (int a = 0; a < chi; ++a)
+    for (int k = 0; k < d; ++k)
+      for (int j = 0; j < d; ++j)
+        for (int g = 0; g < chi; ++g) {
+          const int lN = lb.left(link, a), rN = lb.right(link + 2, g);
+          const int n2 = tot - lN - rN;
+          if (std::max(lN, rN) < 1000 && n2 - conserv[k] >= 0 && n2 - conserv[k] <= maxf && conserv[j] == n2 - conserv[k]) {
+            cplx acc = 0.0;
+            for (int i = 0; i < d; ++i)
+              for (int l = 0; l < d; ++l)
+                if (n2 - conserv[i] >= 0 && n2 - conserv[i] <= maxf && conserv[l] == n2 - conserv[i])
+                  acc += Uop[(k * d + j) + (size_t)d2 * (i * d + l)] * th(a, i, l, g);
+            tp(a, k, j, g) = acc;
+          }
+        }
+  theta.swap(tmp);
+  // RenormThetaNC (:1046-1142)
+  auto in_sector = [&](int a, int i, int j, int g) {
+    const int lN = lb.left(link, a), rN = lb.right(link + 2, g);
+    const int n2 = tot - lN - rN;
+    return std::max(lN, rN) < 1000 && n2 - conserv[i] >= 0 && n2 - conserv[i] <= maxf && conserv[j] == n2 - conserv[i];
+  };
+  cplx nsq = 0.0;
+  for (int a = 0; a < chi; ++a)
+    for (int g = 0; g < chi; ++g)
+      for (int i = 0; i < d; ++i)
+        for (int j = 0; j < d; ++j)
+          if (in_sector(a, i, j, g)) nsq += std::conj(th(a, i, j, g)) * th(a, i, j, g);
+  const double nrm = std::sqrt(nsq.real());
+  for (int a = 0; a < chi; ++a)
+    for (int g = 0; g < chi; ++g)
+      for (int i = 0; i < d; ++i)
+        for (int j = 0; j < d; ++j)
+          if (in_sector(a, i, j, g)) th(a, i, j, g) /= nrm;
+  // minmaxNLR (:1201-1250)
+  int minNL = 1 << 30, maxNL = 0, minNR = 1 << 30, maxNR = 0;
+  for (int a = 0; a < chi; ++a) {
+    minNL = std::min(minNL, lb.left(link, a));
+    maxNL = std::max(maxNL, lb.left(link, a) == 10000 ? 0 : lb.left(link, a));
+    minNR = std::min(minNR, lb.right(link + 2, a));
+    maxNR = std::max(maxNR, lb.right(link + 2, a) == 10000 ? 0 : lb.right(link + 2, a));
+  }
+  maxNL += maxf;
+  maxNR += maxf;
+  if (maxNL > tot - minNR) maxNL = tot - minNR; else minNR = tot - maxNL;
+  if (maxNR > tot - minNL) maxNR = tot - minNL; else minNL = tot - maxNR;
+  const int nblk = maxNL - minNL + 1;
+  if (nblk < 1) return -1;
+  // SizeOfBlocks / IndexLeft / IndexRight (:1252-1443): rows (a, i) and columns (g, j)
+  // of each block of fixed number on the left, in the reference's traversal order
+  std::vector<std::vector<std::pair<int, int>>> rows(nblk), cols(nblk);
+  for (int a = 0; a < chi; ++a)
+    for (int i = 0; i < d; ++i) {
+      const int lab = lb.left(link, a), nl = lab + conserv[i];
+      if (lab < 1000 && minNL <= nl && nl <= maxNL) rows[nl - minNL].push_back({a, i});
+    }
+  for (int g = 0; g < chi; ++g)
+    for (int j = 0; j < d; ++j) {
+      const int lab = lb.right(link + 2, g), nr = lab + conserv[j];
+      if (lab < 1000 && minNR <= nr && nr <= maxNR) cols[tot - nr - minNL].push_back({g, j});
+    }
+  // FormBlockTheta + SVDNC (ZGESVD('A','A') per block, lwork = 5 max(m, n)) + FlattenSS
+  std::vector<std::vector<cplx>> US(nblk), VS(nblk);
+  std::vector<int> mins(nblk);
+  std::vector<double> ssfl;
+  int info = 0;
+  for (int k = 0; k < nblk; ++k) {
+    const int m = (int)rows[k].size(), n = (int)cols[k].size();
+    mins[k] = std::min(m, n);
+    if (mins[k] == 0) continue;
+    std::vector<cplx> B((size_t)m * n);
+    for (int f = 0; f < m; ++f)
+      for (int h = 0; h < n; ++h)
+        B[f + (size_t)m * h] = th(rows[k][f].first, rows[k][f].second, cols[k][h].second, cols[k][h].first);
+    std::vector<double> S(mins[k]);
+    US[k].resize((size_t)m * m);
+    VS[k].resize((size_t)n * n);
+    int lwork = 5 * std::max(m, n), inf = 0;
+    std::vector<cplx> work(lwork);
+    std::vector<double> rwork(5 * mins[k]);
+    const char job = 'A';
+    MKL_Set_Num_Threads_Local(1);
+    zgesvd_(&job, &job, &m, &n, B.data(), &m, S.data(), US[k].data(), &m, VS[k].data(), &n, work.data(), &lwork,
+            rwork.data(), &inf);
+    info |= inf;
+    ssfl.insert(ssfl.end(), S.begin(), S.end());
+  }
+  const int sz = (int)ssfl.size();
+  const std::vector<int> order = nc_ordering(ssfl);
+  // JudgePosition (:1540-1565): block and position (0-based) of each ordered value
+  std::vector<int> pblk(sz), ppos(sz);
+  for (int b = 0; b < sz; ++b) {
+    int rest = order[b] + 1;  // 1-based flat index
+    for (int k = 0; k < nblk; ++k) {
+      if (rest <= 0) break;
+      rest -= mins[k];
+      pblk[b] = k;
+      ppos[b] = rest + mins[k] - 1;
+    }
+  }
+  // UpdateLabelLeft / UpdateLabelRight (:1567-1625)
+  for (int b = 0; b < chi; ++b) {
+    int lab = 10000;
+    if (b < sz && ssfl[order[b]] > kNcCut) lab = minNL + pblk[b];
+    lb.left(link + 1, b) = lab;
+    lb.right(link + 1, b) = lab < 1000 ? tot - lab : 10000;
+  }
+  // FormLambdaNC (:1627-1657)
+  std::vector<double> tl(chi, 0.0);
+  for (int b = 0; b < chi; ++b)
+    if (b < sz && ssfl[order[b]] > kNcCut) tl[b] = ssfl[order[b]];
+  double renorm = 0.0;
+  for (int b = 0; b < chi; ++b) renorm += tl[b] * tl[b];
+  *truncerr = std::fabs(1.0 - renorm);
+  double* lc = v.lm(link + 1);
+  for (int b = 0; b < chi; ++b) lc[b] = tl[b] / std::sqrt(renorm);
+  // FormGamma1NC / FormGamma2NC (:1659-1721)
+  std::fill(v.gam(link), v.gam(link) + v.gsz(), cplx(0.0));
+  std::fill(v.gam(link + 1), v.gam(link + 1) + v.gsz(), cplx(0.0));
+  for (int b = 0; b < std::min(chi, sz); ++b) {
+    const int k = pblk[b], ps = ppos[b], m = (int)rows[k].size(), n = (int)cols[k].size();
+    for (int f = 0; f < m; ++f) {
+      const int a = rows[k][f].first, i = rows[k][f].second;
+      v.g(link, a, i, b) = l0[a] > kNcCut ? US[k][f + (size_t)m * ps] / l0[a] : cplx(0.0);
+    }
+    for (int h = 0; h < n; ++h) {
+      const int g = cols[k][h].first, j = cols[k][h].second;
+      v.g(link + 1, b, j, g) = l2[g] > kNcCut ? VS[k][ps + (size_t)n * h] / l2[g] : cplx(0.0);
+    }
+  }
   return info;
 }
 
@@ -397,6 +598,32 @@ int orc_canonical_form_all(int L, int d, int chi, double* G, double* lam) {
 int orc_canonical_form(int L, int d, int chi, double* G, double* lam, int link) {
   View v{L, d, chi, (cplx*)G, lam};
   return canonical_form(v, link - 1);
+}
+
+// TwoSiteOpNC on bond `link` (1-based); LL, LR: (L+1) x chi labels (updated in place).
+int orc_two_site_op_nc(int L, int d, int chi, double* G, double* lam, int* LL, int* LR, int link, const double* U,
+                       const int* conserv, int tot, int maxf, double* truncerr) {
+  View v{L, d, chi, (cplx*)G, lam};
+  return two_site_op_nc(v, Labels{chi, LL, LR}, link - 1, (const cplx*)U, conserv, tot, maxf, truncerr);
+}
+
+// TrotterStep2ndOrderNC (propagation_module.f90:956-1017): odd, even, odd bonds.
+int orc_trotter_step_nc(int L, int d, int chi, double* G, double* lam, int* LL, int* LR, const double* gates,
+                        const int* conserv, int tot, int maxf, double* total_truncerr) {
+  View v{L, d, chi, (cplx*)G, lam};
+  const Labels lb{chi, LL, LR};
+  const size_t gs = (size_t)d * d * d * d;
+  double total = 0.0;
+  int info = 0;
+  const int starts[3] = {1, 2, 1};
+  for (int sweep = 0; sweep < 3; ++sweep)
+    for (int bond = starts[sweep]; bond <= L - 1; bond += 2) {
+      double t = 0.0;
+      info |= two_site_op_nc(v, lb, bond - 1, (const cplx*)gates + gs * (bond - 1), conserv, tot, maxf, &t);
+      total += t;
+    }
+  *total_truncerr = total;
+  return info;
 }
 
 // TotalEnergy: sum over bonds of Re Tr(H_b rho_b); H: (L-1) d^2 x d^2 complex.

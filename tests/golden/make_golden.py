@@ -55,7 +55,7 @@ def parse(text):
             rest = tok[1:]
             blocks.append((head, nums))
             # integer tag arguments (indices) for H/U/LAMBDA/GAMMA/AOP/DIMS
-            if head in ("H", "U", "LAMBDA", "GAMMA", "AOP", "DIMS", "MTB", "MCORR", "MFCORR"):
+            if head in ("H", "U", "LAMBDA", "GAMMA", "AOP", "DIMS", "MTB", "MCORR", "MFCORR", "LABELL", "LABELR"):
                 blocks[-1] = (head, nums, [int(t) for t in rest])
                 continue
             nums.extend(float(t) for t in rest)
@@ -107,7 +107,7 @@ def run_itp(job):
 def collect(blocks, L, d, chi):
     d2 = d * d
     res = {"H": [], "U": [], "energy": [], "trunc": [], "sz": [], "n": [], "S": [],
-           "lam": [], "gam": [], "aop": []}
+           "lam": [], "gam": [], "aop": [], "labl": [], "labr": []}
     for head, nums, args in blocks:
         a = np.array(nums)
         if head in ("H", "U"):
@@ -132,6 +132,12 @@ def collect(blocks, L, d, chi):
             res["trunc"].append(a[0])
         elif head == "LAMBDA":
             res["lam"].append(a)
+        elif head == "LABELL":
+            res["labl"].append(a.astype(np.int64))
+        elif head == "LABELR":
+            res["labr"].append(a.astype(np.int64))
+        elif head == "CONSERV":
+            res["conserv"] = a.astype(np.int64)
         elif head == "GAMMA":
             z = a[0::2] + 1j * a[1::2]
             # Fortran (a, i, b) column-major -> [b, i, a]
@@ -363,6 +369,38 @@ def checkpoint():
     print("checkpoint files written", os.listdir(dest))
 
 
+def nc():
+    """Number-conserving trajectories (mode 6: ProductStateLabels + TrotterStepNC):
+    Bose-Hubbard d=3 (maxFilling 2) from the unit-filling Mott state, J_i~U[0.8,1.2]
+    (seed 31), U=2, at L=8 chi=16 (truncated) and L=6 chi=32 (full rank); Fermi-Hubbard
+    d=4 (t=1, U=4, number = the model's Conserv, intDegFree) from the Neel state at L=6
+    chi=64 (full rank: its spin-symmetric spectra have degenerate pairs, and a cut through
+    one leaves the kept state to rounding)."""
+    tmp = tempfile.mkdtemp()
+    cases = {
+        "nc_bh_l8": dict(model=2, L=8, param=2, chi=16, consts=[2.0, 0.0, 0.0], up=1, dn=1, nsteps=20, intdeg=0),
+        "nc_bh_l6_full": dict(model=2, L=6, param=2, chi=32, consts=[2.0, 0.0, 0.0], up=1, dn=1, nsteps=15, intdeg=0),
+        "nc_fh_l6": dict(model=3, L=6, param=2, chi=64, consts=[1.0, 4.0, 0.0], up=1, dn=2, nsteps=12, intdeg=1),
+    }
+    for name, cs in cases.items():
+        L, chi, model = cs["L"], cs["chi"], cs["model"]
+        d = 3 if model == 2 else 4
+        field = np.random.default_rng(31).uniform(0.8, 1.2, L - 1) if model == 2 else None
+        c = neel(L, d, cs["up"], cs["dn"])
+        tot = int(sum(np.argmax(np.abs(c[s])) for s in range(L))) if model == 2 else L
+        path = os.path.join(tmp, name + ".txt")
+        write_job(path, model, L, cs["param"], chi, cs["consts"], field, 0.05 + 0j, cs["nsteps"], 0, c, mode=6,
+                  extra=f"{tot} {cs['intdeg']}\n")
+        res = collect(run(path), L, d, chi)
+        meta = dict(model=model, L=L, param=cs["param"], chi=chi, d=d, consts=np.array(cs["consts"]),
+                    dt=np.complex128(0.05), nsteps=cs["nsteps"], canon=0, coeffs=c, tot_num=tot, intdeg=cs["intdeg"])
+        if field is not None:
+            meta["field"] = field
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **meta, **res)
+        print(name, "energy[-1]=", res["energy"][-1], "sum trunc=", np.sum(res["trunc"]),
+              "kept", [(l > 0).sum() for l in res["lam"]])
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "checkpoint":
         checkpoint()
@@ -372,5 +410,7 @@ if __name__ == "__main__":
         itp_stages()
     elif len(sys.argv) > 1 and sys.argv[1] == "schedules":
         schedules()
+    elif len(sys.argv) > 1 and sys.argv[1] == "nc":
+        nc()
     else:
         main()

@@ -1,0 +1,141 @@
+"""The number-conserving update (TwoSiteOpNC, local_operations_module.f90:748-1964, and
+TrotterStep2ndOrderNC, propagation_module.f90:956-1017) as the oracle restates it.
+
+Pinned three ways:
+  * the reference's own NC trajectories (tests/golden/nc_*.npz, golden driver mode 6:
+    ProductStateLabels + TrotterStepNC): Bose-Hubbard d=3 from the Mott state at L=8
+    chi=16 (heavy truncation: the kept states are chosen across number blocks) and L=6
+    chi=32 (full rank), Fermi-Hubbard d=4 with the model's numbers (intDegFree) from the
+    Neel state at full rank -- per-step energy, <n_i>, <Sz_i>, entropy and truncation
+    error, the final lambdas and state vector, and the labels of every Schmidt state,
+    exactly;
+  * at full rank the block-sparse update must equal the dense one (TwoSiteOp) on the
+    same state: the gate conserves number, so theta is block diagonal;
+  * the state keeps no weight outside the sector of the label total under truncation,
+    and every used Schmidt state carries a label consistent with the chain's size.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from helpers import mps_to_vector
+from oracle.oracle import OracleState, product_labels
+
+CASES = ["nc_bh_l8", "nc_bh_l6_full", "nc_fh_l6"]
+
+
+def conserv_of(z):
+    d = int(z["d"])
+    return z["conserv"] if "conserv" in z else np.arange(d)
+
+
+def nc_state(z):
+    chi = int(z["chi"])
+    st = OracleState.product(z["coeffs"], chi)
+    c = conserv_of(z)
+    LL, LR = product_labels(z["coeffs"], chi, c, int(z["tot_num"]))
+    st.set_conserved(LL, LR, c, int(z["tot_num"]))
+    return st
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_nc_reproduces_reference(name):
+    z = load_golden(name)
+    st = nc_state(z)
+    worst = {}
+    for k in range(1, int(z["nsteps"]) + 1):
+        tr = st.trotter_step_nc(z["U"])
+        e = {"trunc": abs(tr - z["trunc"][k]), "E": abs(st.total_energy(z["H"]) - z["energy"][k]),
+             "n": np.max(np.abs(st.one_site_expval(z["nop"]) - z["n"][k])),
+             "sz": np.max(np.abs(st.one_site_expval(z["szop"]) - z["sz"][k])),
+             "S": np.max(np.abs(st.chain_entropy() - z["S"][k]))}
+        for key, v in e.items():
+            worst[key] = max(worst.get(key, 0.0), float(v))
+    worst["lambda"] = float(np.max(np.abs(st.lam - z["lam"])))
+    # the state itself (the Gammas only up to the freedom of degenerate singular vectors:
+    # the Fermi-Hubbard spectra come in spin-degenerate pairs)
+    worst["psi"] = float(np.max(np.abs(mps_to_vector(st.gam, st.lam) - mps_to_vector(z["gam"], z["lam"]))))
+    # the Schmidt spectrum of every number sector of every link (which of two equal
+    # values across sectors comes first is left to rounding)
+    worst["sectors"] = sector_spectra_err(st.LL, st.lam, z["labl"], z["lam"])
+    print(name, worst)
+    assert max(worst.values()) < 1e-10, worst
+    if name.startswith("nc_bh"):  # no degenerate values there: the labels match one for one
+        assert np.array_equal(st.LL, z["labl"]) and np.array_equal(st.LR, z["labr"])
+    assert np.array_equal(np.sort(st.LL, axis=1), np.sort(z["labl"], axis=1))
+
+
+def sector_spectra_err(LL1, lam1, LL2, lam2):
+    worst = 0.0
+    for l in range(LL1.shape[0]):
+        for n in set(LL1[l][LL1[l] < 1000]) | set(LL2[l][LL2[l] < 1000]):
+            a, b = np.sort(lam1[l][LL1[l] == n]), np.sort(lam2[l][LL2[l] == n])
+            if a.size != b.size:
+                return np.inf
+            worst = max(worst, float(np.max(np.abs(a - b))))
+    return worst
+
+
+def test_oracle_nc_equals_dense_update_at_full_rank():
+    """The NC update drops singular values up to 1e-10 (FormLambdaNC, :1627-1657) where
+    the dense one drops them up to 1e-15 (FormLambda1): in the first steps, while the
+    bonds grow, that moves lambda by ~1e-10 (the dropped weight is 1e-20), so the two
+    agree to 1e-9, not to rounding."""
+    z = load_golden("nc_bh_l6_full")
+    nc = nc_state(z)
+    dense = OracleState.product(z["coeffs"], int(z["chi"]))
+    for _ in range(int(z["nsteps"])):
+        t1 = nc.trotter_step_nc(z["U"])
+        t2 = dense.trotter_step(z["U"])
+        assert abs(t1 - t2) < 1e-12
+    for op in (z["nop"], z["szop"]):
+        assert np.max(np.abs(nc.one_site_expval(op) - dense.one_site_expval(op))) < 1e-9
+    assert abs(nc.total_energy(z["H"]) - dense.total_energy(z["H"])) < 1e-9
+    assert np.max(np.abs(nc.chain_entropy() - dense.chain_entropy())) < 1e-9
+    assert np.max(np.abs(nc.lam - dense.lam)) < 1e-9
+    assert np.max(np.abs(mps_to_vector(nc.gam, nc.lam) - mps_to_vector(dense.gam, dense.lam))) < 1e-9
+
+
+@pytest.mark.parametrize("name", ["nc_bh_l8", "nc_fh_l6"])
+def test_oracle_nc_conserves_number_and_labels(name):
+    z = load_golden(name)
+    st = nc_state(z)
+    L, chi, tot = int(z["L"]), int(z["chi"]), int(z["tot_num"])
+    c = conserv_of(z)
+    d = int(z["d"])
+    # number of every basis state of the chain
+    nums = np.zeros([d] * L, np.int64)
+    for s in range(L):
+        shape = [1] * L
+        shape[s] = d
+        nums = nums + np.asarray(c).reshape(shape)
+    for _ in range(int(z["nsteps"])):
+        st.trotter_step_nc(z["U"])
+        psi = mps_to_vector(st.gam, st.lam)
+        # the state has no weight outside the sector of the label total, truncated or not
+        assert np.sum(np.abs(psi[nums != tot]) ** 2) < 1e-28 * np.sum(np.abs(psi) ** 2)
+    # every used Schmidt state of link l carries the number of particles left of the cut:
+    # the one-site number operator summed over the left sites, in that state's sector
+    used = st.LL < 1000
+    assert np.array_equal(used, st.lam > 0.0)
+    assert np.array_equal(st.LR[used], tot - st.LL[used])
+    assert np.all(st.LL[used] >= 0) and np.all(st.LL[used] <= tot)
+    # numbers grow by at most maxFilling per site from the left edge
+    maxf = int(conserv_of(z).max())
+    for l in range(L + 1):
+        labs = st.LL[l][used[l]]
+        assert np.all(labs <= maxf * l) and np.all(tot - labs <= maxf * (L - l))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_host_product_labels_match_oracle(name):
+    from tebd_amd import models as M
+    z = load_golden(name)
+    chi, c, tot = int(z["chi"]), conserv_of(z), int(z["tot_num"])
+    a = M.product_labels(z["coeffs"], chi, c, tot)
+    b = product_labels(z["coeffs"], chi, c, tot)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    with pytest.raises(ValueError):
+        bad = np.array(z["coeffs"]).copy()
+        bad[0, :] = 1.0  # a superposition of different numbers
+        M.product_labels(bad, chi, c, tot)
