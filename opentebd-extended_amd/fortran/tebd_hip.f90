@@ -85,6 +85,24 @@ INTERFACE
     REAL(c_double), VALUE :: dtau, conv1, conv2
   END FUNCTION tebd_imag_time_prop
 
+  INTEGER(c_int) FUNCTION tebd_set_conserved(ctx, conserv, tot_num) BIND(C, name="tebd_set_conserved")
+    IMPORT :: c_ptr, c_int
+    TYPE(c_ptr), VALUE :: ctx, conserv
+    INTEGER(c_int), VALUE :: tot_num
+  END FUNCTION tebd_set_conserved
+
+  INTEGER(c_int) FUNCTION tebd_set_labels(ctx, label_left, label_right, chi) BIND(C, name="tebd_set_labels")
+    IMPORT :: c_ptr, c_int
+    TYPE(c_ptr), VALUE :: ctx, label_left, label_right
+    INTEGER(c_int), VALUE :: chi
+  END FUNCTION tebd_set_labels
+
+  INTEGER(c_int) FUNCTION tebd_get_labels(ctx, label_left, label_right, chi) BIND(C, name="tebd_get_labels")
+    IMPORT :: c_ptr, c_int
+    TYPE(c_ptr), VALUE :: ctx, label_left, label_right
+    INTEGER(c_int), VALUE :: chi
+  END FUNCTION tebd_get_labels
+
   INTEGER(c_int) FUNCTION tebd_canonical_form_all(ctx) BIND(C, name="tebd_canonical_form_all")
     IMPORT :: c_ptr, c_int
     TYPE(c_ptr), VALUE :: ctx

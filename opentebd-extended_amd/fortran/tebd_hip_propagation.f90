@@ -10,10 +10,15 @@
 !   ImagTimeProp(H, Gammas, Lambdas, chiIn[, intDegFree]), ImagTimePropSpin(H, Gammas,
 !   Lambdas, chiIn)                           propagation_module.f90:636-876
 !   ConstructPropagators(H, U, dt)            TEBDtools_module.f90:97-99,1079-1101
+!   TwoSiteOpNC(link, Op2, Gammas, Lambdas, LabelLeft, LabelRight, truncerr[, intDegFree]),
+!   TrotterStepNC(Udt, Gammas, Lambdas, LabelLeft, LabelRight, totalTruncerr[, intDegFree])
+!                                             local_operations_module.f90:1845-1964,
+!                                             propagation_module.f90:890-1241
 ! on the reference's own types (TYPE(tensor), TYPE(vector), TYPE(matrix),
 ! TEBDtools_module.f90:31-45) and globals (systemSize, localSize, trotterOrder,
 ! BoundaryCond, and for imaginary time chiMax, dtITP, maxITPsteps, stepsForJudge,
-! convCriterion1/2 from system_parameters).  Build it after the reference's
+! convCriterion1/2 from system_parameters; for number conservation totNum, the labels'
+! TYPE(vectorInt) and, with intDegFree, Conserv from Hamiltonian_tools_module).  Build it after the reference's
 ! system_parameters and TEBDtools_module and link libtebdhip.so; a program that
 ! also needs the reference's own procedures of the same names renames one set
 ! on USE (e.g. USE tebd_hip_propagation, TrotterStepHIP => TrotterStep).
@@ -29,13 +34,14 @@
 MODULE tebd_hip_propagation
 USE, INTRINSIC :: iso_c_binding
 USE system_parameters, ONLY: rKind, systemSize, localSize, trotterOrder, BoundaryCond, chiMax, dtITP, &
-                             maxITPsteps, stepsForJudge, convCriterion1, convCriterion2
-USE TEBDtools_module, ONLY: tensor, vector, matrix
+                             maxITPsteps, stepsForJudge, convCriterion1, convCriterion2, totNum
+USE TEBDtools_module, ONLY: tensor, vector, matrix, vectorInt
+USE Hamiltonian_tools_module, ONLY: Conserv
 USE tebd_hip
 IMPLICIT NONE
 PRIVATE
 PUBLIC :: SVDInit, SVDFinish, TwoSiteOp, TrotterStep, CanonicalFormAll, ImagTimeProp, ImagTimePropSpin, &
-          ConstructPropagators
+          ConstructPropagators, TwoSiteOpNC, TrotterStepNC
 PUBLIC :: tebd_hip_resident, tebd_hip_mode, tebd_hip_to_host, tebd_hip_to_device, tebd_hip_context
 
 TYPE(c_ptr), SAVE :: ctx = c_null_ptr
@@ -150,6 +156,71 @@ SUBROUTINE TrotterStep(Udt, Gammas, Lambdas, totalTruncerr)
   totalTruncerr = t
   CALL after_call(Gammas, Lambdas)
 END SUBROUTINE TrotterStep
+
+! Number conservation on, with the labels of the call, before a number-conserving update;
+! the number of local state i is Conserv%vi(i) with intDegFree, i - 1 without.
+SUBROUTINE nc_before(LabelLeft, LabelRight, intDegFree)
+  TYPE(vectorInt), POINTER :: LabelLeft(:), LabelRight(:)
+  INTEGER, INTENT(IN), OPTIONAL :: intDegFree
+  INTEGER(c_int), ALLOCATABLE, TARGET :: cv(:), ll(:, :), lr(:, :)
+  INTEGER :: i, chi
+  chi = SIZE(LabelLeft(1)%vi)
+  IF (PRESENT(intDegFree)) THEN
+    ALLOCATE(cv(localSize))
+    cv = Conserv%vi(1:localSize)
+    CALL tebd_check(ctx, tebd_set_conserved(ctx, c_loc(cv), INT(totNum, c_int)), 'number conservation')
+  ELSE
+    CALL tebd_check(ctx, tebd_set_conserved(ctx, c_null_ptr, INT(totNum, c_int)), 'number conservation')
+  END IF
+  ALLOCATE(ll(chi, systemSize + 1), lr(chi, systemSize + 1))
+  DO i = 1, systemSize + 1
+    ll(:, i) = LabelLeft(i)%vi
+    lr(:, i) = LabelRight(i)%vi
+  END DO
+  CALL tebd_check(ctx, tebd_set_labels(ctx, c_loc(ll), c_loc(lr), INT(chi, c_int)), 'labels')
+END SUBROUTINE nc_before
+
+! The updated labels back into LabelLeft / LabelRight, and number conservation off.
+SUBROUTINE nc_after(LabelLeft, LabelRight)
+  TYPE(vectorInt), POINTER :: LabelLeft(:), LabelRight(:)
+  INTEGER(c_int), ALLOCATABLE, TARGET :: ll(:, :), lr(:, :)
+  INTEGER :: i, chi
+  chi = SIZE(LabelLeft(1)%vi)
+  ALLOCATE(ll(chi, systemSize + 1), lr(chi, systemSize + 1))
+  CALL tebd_check(ctx, tebd_get_labels(ctx, c_loc(ll), c_loc(lr), INT(chi, c_int)), 'labels')
+  DO i = 1, systemSize + 1
+    LabelLeft(i)%vi = ll(:, i)
+    LabelRight(i)%vi = lr(:, i)
+  END DO
+  CALL tebd_check(ctx, tebd_set_conserved(ctx, c_null_ptr, -1_c_int), 'number conservation')
+END SUBROUTINE nc_after
+
+SUBROUTINE TwoSiteOpNC(link, Op2, Gammas, Lambdas, LabelLeft, LabelRight, truncerr, intDegFree)
+  INTEGER, INTENT(IN) :: link
+  COMPLEX(KIND=rKind), INTENT(IN) :: Op2(:, :)
+  TYPE(tensor), POINTER :: Gammas(:)
+  TYPE(vector), POINTER :: Lambdas(:)
+  TYPE(vectorInt), POINTER :: LabelLeft(:), LabelRight(:)
+  REAL(KIND=rKind), INTENT(INOUT) :: truncerr
+  INTEGER, INTENT(IN), OPTIONAL :: intDegFree
+  CALL before_call(Gammas, Lambdas)
+  CALL nc_before(LabelLeft, LabelRight, intDegFree)
+  CALL TwoSiteOp(link, Op2, Gammas, Lambdas, truncerr)
+  CALL nc_after(LabelLeft, LabelRight)
+END SUBROUTINE TwoSiteOpNC
+
+SUBROUTINE TrotterStepNC(Udt, Gammas, Lambdas, LabelLeft, LabelRight, totalTruncerr, intDegFree)
+  TYPE(matrix), POINTER :: Udt(:)
+  TYPE(tensor), POINTER :: Gammas(:)
+  TYPE(vector), POINTER :: Lambdas(:)
+  TYPE(vectorInt), POINTER :: LabelLeft(:), LabelRight(:)
+  REAL(KIND=rKind), INTENT(INOUT) :: totalTruncerr
+  INTEGER, INTENT(IN), OPTIONAL :: intDegFree
+  CALL before_call(Gammas, Lambdas)
+  CALL nc_before(LabelLeft, LabelRight, intDegFree)
+  CALL TrotterStep(Udt, Gammas, Lambdas, totalTruncerr)
+  CALL nc_after(LabelLeft, LabelRight)
+END SUBROUTINE TrotterStepNC
 
 SUBROUTINE CanonicalFormAll(Gammas, Lambdas)
   TYPE(tensor), POINTER :: Gammas(:)

@@ -8,15 +8,20 @@
 !       kind 0 (Trotter steps): (L-1) gates of d^2 x d^2 complex(8), column-major;
 !       kind 1 (ImagTimeProp, chi = chiMax): int32 chiIn, maxITPsteps, stepsForJudge,
 !       real(8) dtITP, convCriterion1, convCriterion2, then (L-1) bond Hamiltonians;
+!       kind 2 (TrotterStepNC): int32 totNum, intDegFree (0/1), Conserv(d), then the
+!       LabelLeft and LabelRight of the L+1 links (chi int32 each), then the gates as kind 0;
 !       then L Gammas of chi x d x chi complex(8); L+1 lambdas of chi real(8)
-!   OUT: real(8) total truncation error; L Gammas; L+1 lambdas (same layout)
+!   OUT: real(8) total truncation error; L Gammas; L+1 lambdas (same layout); for kind 2
+!        then the L+1 LabelLeft and L+1 LabelRight (chi int32 each)
 PROGRAM shim_golden
 USE, INTRINSIC :: iso_c_binding
 USE system_parameters, ONLY: rKind, systemSize, localSize, trotterOrder, BoundaryCond, chiMax, dtITP, &
                              maxITPsteps, stepsForJudge, convCriterion1, convCriterion2
-USE TEBDtools_module, ONLY: tensor, vector, matrix
+USE system_parameters, ONLY: totNum
+USE TEBDtools_module, ONLY: tensor, vector, matrix, vectorInt
+USE Hamiltonian_tools_module, ONLY: Conserv
 USE tebd_hip_propagation, ONLY: SVDInit, SVDFinish, TrotterStep, CanonicalFormAll, ImagTimeProp, &
-                                tebd_hip_resident, tebd_hip_to_host, tebd_hip_mode
+                                TrotterStepNC, tebd_hip_resident, tebd_hip_to_host, tebd_hip_mode
 IMPLICIT NONE
 INTEGER(c_int32_t) :: hdr(8), itp(3)
 REAL(c_double) :: itpr(3)
@@ -24,6 +29,8 @@ INTEGER :: L, d, chi, nsteps, i, k, uin, uout, chiIn
 TYPE(tensor), POINTER :: Gammas(:)
 TYPE(vector), POINTER :: Lambdas(:)
 TYPE(matrix), POINTER :: Udt(:)
+TYPE(vectorInt), POINTER :: LabelLeft(:), LabelRight(:)
+INTEGER(c_int32_t) :: nch(2)
 REAL(KIND=rKind) :: truncerr, total
 CHARACTER(len=1024) :: fin, fout
 
@@ -48,6 +55,21 @@ IF (hdr(8) == 1) THEN
   convCriterion2 = itpr(3)
   chiMax = chi
 END IF
+IF (hdr(8) == 2) THEN
+  READ(uin) nch
+  totNum = nch(1)
+  ALLOCATE(Conserv%vi(d))
+  READ(uin) Conserv%vi
+  ALLOCATE(LabelLeft(L + 1), LabelRight(L + 1))
+  DO i = 1, L + 1
+    ALLOCATE(LabelLeft(i)%vi(chi))
+    READ(uin) LabelLeft(i)%vi
+  END DO
+  DO i = 1, L + 1
+    ALLOCATE(LabelRight(i)%vi(chi))
+    READ(uin) LabelRight(i)%vi
+  END DO
+END IF
 ALLOCATE(Udt(L - 1), Gammas(L), Lambdas(L + 1))
 DO i = 1, L - 1
   ALLOCATE(Udt(i)%m(d * d, d * d))
@@ -67,6 +89,17 @@ total = 0.0_rKind
 IF (hdr(8) == 1) THEN
   ! Udt holds the bond Hamiltonians of the job here
   CALL ImagTimeProp(Udt, Gammas, Lambdas, chiIn)
+ELSE IF (hdr(8) == 2) THEN
+  CALL SVDInit(chi)
+  DO k = 1, nsteps
+    IF (nch(2) /= 0) THEN
+      CALL TrotterStepNC(Udt, Gammas, Lambdas, LabelLeft, LabelRight, truncerr, 1)
+    ELSE
+      CALL TrotterStepNC(Udt, Gammas, Lambdas, LabelLeft, LabelRight, truncerr)
+    END IF
+    total = total + truncerr
+  END DO
+  IF (tebd_hip_resident) CALL tebd_hip_to_host(Gammas, Lambdas)
 ELSE
   CALL SVDInit(chi)
   DO k = 1, nsteps
@@ -86,6 +119,14 @@ END DO
 DO i = 1, L + 1
   WRITE(uout) Lambdas(i)%v
 END DO
+IF (hdr(8) == 2) THEN
+  DO i = 1, L + 1
+    WRITE(uout) LabelLeft(i)%vi
+  END DO
+  DO i = 1, L + 1
+    WRITE(uout) LabelRight(i)%vi
+  END DO
+END IF
 CLOSE(uout)
 PRINT '(A,I0,A,I0)', 'shim_golden: steps ', nsteps, ' sites ', L
 END PROGRAM shim_golden

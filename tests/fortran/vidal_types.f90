@@ -2,10 +2,13 @@
 ! (opentebd-extended_amd/fortran/tebd_hip_propagation.f90) binds to, declared with the
 ! reference's names, kinds and components so the shim compiles unchanged without the
 ! reference's sources:
-!   system_parameters: rKind, systemSize, localSize, trotterOrder, BoundaryCond, and the
+!   system_parameters: rKind, systemSize, localSize, trotterOrder, BoundaryCond, the
 !                      imaginary-time settings chiMax, dtITP, maxITPsteps, stepsForJudge,
-!                      convCriterion1/2 (Case_Studies/*/system_parameters.f90:16-58)
-!   TEBDtools_module:  TYPE vector / matrix / tensor (VidalCore_v_2.0/TEBDtools_module.f90:31-45)
+!                      convCriterion1/2, and totNum (Case_Studies/*/system_parameters.f90:16-58)
+!   TEBDtools_module:  TYPE vector / matrix / tensor / vectorInt
+!                      (VidalCore_v_2.0/TEBDtools_module.f90:31-49)
+!   Hamiltonian_tools_module: the on-site numbers Conserv (HamiltonianCore_v_2.0/
+!                      Hamiltonian_tools_module.f90:57)
 ! Only these declarations; none of the reference's procedures.  Host code that uses the
 ! shim in earnest compiles it against the reference's own modules instead.
 MODULE system_parameters
@@ -22,6 +25,7 @@ INTEGER :: stepsForJudge = 1
 INTEGER :: maxITPsteps = 0
 REAL(KIND=rKind) :: convCriterion1 = 0.0_rKind
 REAL(KIND=rKind) :: convCriterion2 = 0.0_rKind
+INTEGER :: totNum = 0
 END MODULE system_parameters
 
 MODULE TEBDtools_module
@@ -36,4 +40,13 @@ END TYPE matrix
 TYPE tensor
   COMPLEX(KIND=rKind), POINTER :: t(:, :, :)
 END TYPE tensor
+TYPE vectorInt
+  INTEGER, POINTER :: vi(:)
+END TYPE vectorInt
 END MODULE TEBDtools_module
+
+MODULE Hamiltonian_tools_module
+USE TEBDtools_module, ONLY: vectorInt
+IMPLICIT NONE
+TYPE(vectorInt) :: Conserv
+END MODULE Hamiltonian_tools_module
