@@ -40,10 +40,11 @@ constexpr int kQrMinCols = 64;    // bond matrices with fewer columns go straigh
 constexpr int kMaxSweeps = 40;
 constexpr double kTolRel = 1e-13;   // relative orthogonality of converged column pairs
 constexpr double kTolAbs = 1e-13;   // rounding floor: |A_pq| <= kTolAbs ||theta|| (|w_p| + |w_q|)
-// The first kEarlySweeps outer sweeps solve each pair's 32 x 32 problem with at most
-// kEarlyInnerSweeps inner sweeps (the pair is revisited anyway); later ones to convergence.
-constexpr int kEarlySweeps = 4;
-constexpr int kEarlyInnerSweeps = 2;
+// Every outer sweep solves each pair's 32 x 32 problem with at most kInnerSweepCap inner
+// sweeps: a pair left coupled is revisited by the next outer sweep, and near convergence
+// two sweeps diagonalise the nearly diagonal problem anyway (one sweep per visit costs
+// almost twice the outer sweeps).
+constexpr int kInnerSweepCap = 2;
 constexpr double kSlotBudget = 48.0 * (1ull << 30);  // bytes of per-bond workspace slots per context
 constexpr size_t kMaxSweepGraphs = 512;  // executable Jacobi sweep graphs kept per context or lane
 constexpr int kGraphRowQuantum = 64;     // = the rotation kernel's slab height (tebd_jacobi.hip)
@@ -342,7 +343,7 @@ int jacobi_sweeps(tebd_ctx* c, BondTask* d_t, const BondTask* h_t, int nb, int P
   int sweeps = 0;
   bool converged = false;
   while (sweeps < kMaxSweeps && !converged) {
-    const int inner = sweeps < kEarlySweeps ? kEarlyInnerSweeps : 0;  // 0: to convergence
+    const int inner = kInnerSweepCap;
     // the kernels read the sweep index only as (sweep > 0) and its parity, so 0, 1 and 2
     // stand for every sweep of their class and one graph serves the whole class
     const int cls = sweeps == 0 ? 0 : ((sweeps & 1) ? 1 : 2);

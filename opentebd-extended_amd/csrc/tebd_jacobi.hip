@@ -69,34 +69,39 @@ __device__ __forceinline__ double rcp_nr(double x) {
 }
 
 // Hermitian 2x2 Jacobi rotation J = [[c, s w], [-s conj(w), c]], w = g/|g|, that
-// zeroes g in [[a, g], [conj(g), b]] via J^dagger H J: with zeta = (b - a) / 2|g|,
-// t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)) (Rutishauser), written as
-// t = sign(b - a) 2|g| / (|b - a| + sqrt((b - a)^2 + 4|g|^2)) so that 1/|g| and the
-// square root are independent (two refined estimates in parallel instead of in
-// sequence on the rotation's critical path) and nothing overflows for tiny |g|;
-// c = 1/sqrt(1 + t^2), s = t c.
-__device__ __forceinline__ void jrot(double a, double b, zc g, double& c, double& s, zc& w) {
+// zeroes g in [[a, g], [conj(g), b]] via J^dagger H J, with the inner angle
+// (|theta| <= pi/4, Rutishauser's smaller root).  With e = sqrt((b - a)^2 + 4|g|^2),
+// cos 2theta = |b - a| / e and sin 2theta = 2|g| / e, and the half-angle formulas give
+// c = sqrt((1 + cos 2theta) / 2) = (1 + cos 2theta) r and |s| = sin 2theta / (2c) =
+// sin 2theta r with r = 1/sqrt(2 (1 + cos 2theta)); s takes the sign of b - a.  Every
+// step is well conditioned (1 + cos 2theta lies in [1, 2]) and nothing overflows for tiny
+// |g|.  The two first reciprocal square roots are independent, so the rotation's
+// critical path holds two refined estimates.  Returns c, s w and t|g|, t = s / c =
+// sign(b - a) sin 2theta / (1 + cos 2theta) = sign(b - a) sin 2theta 2 r^2: the rotated
+// diagonal is (a - t|g|, b + t|g|).
+__device__ __forceinline__ void jrot(double a, double b, zc g, double& c, zc& sw, double& tg) {
   const double g2 = zabs2(g);
-  if (g2 == 0.0) { c = 1.0; s = 0.0; w = zmk(1.0, 0.0); return; }
+  if (g2 == 0.0) { c = 1.0; sw = zmk(0.0, 0.0); tg = 0.0; return; }
   const double dd = b - a;
-  const double e2 = fma(dd, dd, 4.0 * g2);
   const double inv = rsqrt_nr(g2);
-  const double re = rsqrt_nr(e2);
-  w = zmk(g.x * inv, g.y * inv);
-  double tt = (2.0 * g2 * inv) * rcp_nr(fma(e2, re, fabs(dd)));
-  if (dd < 0.0) tt = -tt;
-  c = rsqrt_nr(fma(tt, tt, 1.0));
-  s = tt * c;
+  const double re = rsqrt_nr(fma(dd, dd, 4.0 * g2));
+  const double h = fma(fabs(dd), re, 1.0);  // 1 + cos 2theta
+  const double sin2 = (2.0 * g2 * inv) * re;
+  const double r = rsqrt_nr(2.0 * h);
+  c = h * r;
+  const double s = dd < 0.0 ? -sin2 * r : sin2 * r;
+  sw = zmk(s * (g.x * inv), s * (g.y * inv));
+  tg = (dd < 0.0 ? -sin2 : sin2) * (2.0 * r * r) * (g2 * inv);
 }
 
 // Gram block D = X^H Y (16 x 16) of two 16-column blocks over rows [0, R), as
 // per-wave partial sums of kGramWaves waves: wave w takes rows 8 (w + kGramWaves m)
-// .. +7, m = 0, 1, ..., each in two MFMA k-steps of 4 rows, with the loads of four
+// .. +7, m = 0, 1, ..., each in two MFMA k-steps of 4 rows, with the loads of two
 // consecutive m issued before their MFMAs (the pass is bound by memory latency).  Also
 // accumulates the lane's partial squared norms of X and Y.
 __device__ __forceinline__ void gram_block(const zc* __restrict__ X, const zc* __restrict__ Y, int R, int wave,
                                            int lane, d4& gre, d4& gim, double& nx, double& ny) {
-  constexpr int U = 4;
+  constexpr int U = 2;
   const int lr = lane & 15, lk = lane >> 4;
   gre = d4{0.0, 0.0, 0.0, 0.0};
   gim = d4{0.0, 0.0, 0.0, 0.0};
@@ -129,34 +134,44 @@ __device__ __forceinline__ void gram_block(const zc* __restrict__ X, const zc* _
 // Sums the Gram waves' partial 16x16 blocks in a fixed order (bitwise reproducible)
 // and stores rows [ro, ro+16) x cols [co, co+16) of A.  With `diag` the block is a
 // diagonal block of the Hermitian A: only the upper triangle is taken and mirrored.
-// Called by every thread of the workgroup.
-__device__ __forceinline__ void reduce_block(zc (*A)[JC + 1], double (*red)[2][256], d4 gre, d4 gim, int wave,
-                                             int lane, int ro, int co, bool diag) {
+// The real and the imaginary parts go through the scratch `red` ((kGramWaves - 1) x 256
+// doubles) one after the other, so that it fits in the rotation accumulator's LDS, which
+// is free until the inner solve starts.  Called by every thread of the workgroup.
+__device__ __forceinline__ void reduce_block(zc (*A)[JC + 1], double (*red)[256], d4 gre, d4 gim, int wave, int lane,
+                                             int ro, int co, bool diag) {
+  double xr[4];
   if (wave > 0 && wave < kGramWaves) {
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      red[wave - 1][0][g * 64 + lane] = gre[g];
-      red[wave - 1][1][g * 64 + lane] = gim[g];
-    }
+    for (int g = 0; g < 4; ++g) red[wave - 1][g * 64 + lane] = gre[g];
   }
   __syncthreads();
   if (wave == 0) {
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-      double xr = gre[g], xi = gim[g];
-      for (int w = 0; w < kGramWaves - 1; ++w) {
-        xr += red[w][0][g * 64 + lane];
-        xi += red[w][1][g * 64 + lane];
-      }
+      xr[g] = gre[g];
+      for (int w = 0; w < kGramWaves - 1; ++w) xr[g] += red[w][g * 64 + lane];
+    }
+  }
+  __syncthreads();
+  if (wave > 0 && wave < kGramWaves) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) red[wave - 1][g * 64 + lane] = gim[g];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      double xi = gim[g];
+      for (int w = 0; w < kGramWaves - 1; ++w) xi += red[w][g * 64 + lane];
       const int p = (lane >> 4) + 4 * g, q = lane & 15;
       if (!diag) {
-        A[ro + p][co + q] = zmk(xr, xi);
-        A[co + q][ro + p] = zmk(xr, -xi);
+        A[ro + p][co + q] = zmk(xr[g], xi);
+        A[co + q][ro + p] = zmk(xr[g], -xi);
       } else if (p < q) {
-        A[ro + p][co + q] = zmk(xr, xi);
-        A[co + q][ro + p] = zmk(xr, -xi);
+        A[ro + p][co + q] = zmk(xr[g], xi);
+        A[co + q][ro + p] = zmk(xr[g], -xi);
       } else if (p == q) {
-        A[ro + p][co + p] = zmk(xr, 0.0);
+        A[ro + p][co + p] = zmk(xr[g], 0.0);
       }
     }
   }
@@ -181,92 +196,130 @@ __device__ __forceinline__ void inner_pair(int rd, int t, int& p, int& q) {
 }
 
 // Cyclic Jacobi on A (JC x JC Hermitian, LDS); Q (LDS) accumulates the rotations.
-// A round rotates 16 disjoint index pairs (round-robin order): threads 0..15
-// compute the rotations, then threads 0..255 update one 2x2 block [p_r, q_r] x
-// [p_k, q_k] of A each (J^H A J acts on each such block on its own, so the update is
-// in place) and threads 256..511 two (row, pair) groups of Q each; blocks whose two
-// pairs both stay put are skipped.  A Gram entry is accurate relative to sqrt(A_pp A_qq), so the solve uses
-// a purely relative test (no rotation when |A_pq| <= 1e-15 sqrt(A_pp A_qq)), sets
-// each annihilated entry to exactly zero and ends when a sweep would rotate nothing.
-// That is decided by testing every pair once after a sweep: a sweep that rotates
-// nothing leaves A unchanged, so it rotates nothing iff no pair passes the test now.
+// A round rotates 16 disjoint index pairs (round-robin order) in two phases:
+//   phase 1  lanes 0..15 of wave 0 compute the round's rotations into one of two
+//            buffers and write each rotated pair's own 2x2 block of A, which the
+//            rotation makes diagonal: a' = a - t|g|, b' = b + t|g| (t = s / c), the
+//            annihilated entries exactly zero.  Meanwhile waves 1..7 apply the previous
+//            round's rotations to Q (Q_:p <- c Q_:p - s conj(w) Q_:q,
+//            Q_:q <- s w Q_:p + c Q_:q; 512 (row, pair) groups), which nothing in this
+//            round reads;
+//   phase 2  the 240 threads of waves 0..3 update the off-diagonal blocks
+//            [p_r, q_r] x [p_k, q_k], r < k, of A (J^H A J acts on each such block on
+//            its own, so the update is in place): two threads per block, one per row,
+//            each also writing the mirrored entries of block (k, r) (A is Hermitian).
+// Blocks whose two pairs both stay put are skipped, and so is phase 2 of a round that
+// rotates nothing.  The last round's Q update runs after the loop.  A Gram entry is
+// accurate relative to sqrt(A_pp A_qq), so the solve uses a purely relative test (no
+// rotation when |A_pq| <= 1e-15 sqrt(A_pp A_qq)) and ends when a sweep would rotate
+// nothing.  That is decided by testing every pair once after a sweep: a sweep that
+// rotates nothing leaves A unchanged, so it rotates nothing iff no pair passes the test
+// now.
 __device__ __forceinline__ bool needs_rotation(zc g, double a, double b) {
   const double g2 = zabs2(g);
   return !(g2 <= 1e-30 * fabs(a * b) || g2 == 0.0);
 }
 
-__device__ void inner_jacobi(zc (*A)[JC + 1], zc (*Q)[JC + 1], double* rc_c, double* rc_s, zc* rc_w, int* s_round,
-                             int* s_sweep, int* s_more, int tid, int max_sweeps) {
+static_assert(kRoundThreads == JC * JC / 2, "one thread per group of Q after the last round");
+constexpr int kOffBlocks = (JC / 2) * (JC / 2 - 1) / 2;  // 2x2 blocks above the diagonal
+static_assert(2 * kOffBlocks <= 256, "phase 2 runs on waves 0..3");
+
+// Rotations of one round, pair t: c, s w, and p | q << 8 | (rotates) << 16.
+struct RotBuf {
+  double c[JC / 2];
+  zc sw[JC / 2];
+  int pq[JC / 2];
+};
+
+// Applies the rotations of buffer rb to group grp = 16 k + t of Q: row k, pair t.
+__device__ __forceinline__ void q_rotate(zc (*Q)[JC + 1], const RotBuf& rb, int grp) {
+  const int k = grp >> 4, tp = grp & 15;
+  const int pq = rb.pq[tp];
+  const double c = rb.c[tp];
+  const zc sw = rb.sw[tp];
+  if (!(pq >> 16)) return;
+  const int p = pq & 255, q = (pq >> 8) & 255;
+  const zc xp = Q[k][p], xq = Q[k][q];
+  Q[k][p] = zsub(zscale(xp, c), zmul(zconj(sw), xq));
+  Q[k][q] = zadd(zmul(sw, xp), zscale(xq, c));
+}
+
+__device__ void inner_jacobi(zc (*A)[JC + 1], zc (*Q)[JC + 1], RotBuf* rb, int* s_round, int* s_sweep,
+                             int* s_more, int tid, int max_sweeps) {
   for (int e = tid; e < JC * JC; e += kRoundThreads) {
     const int r = e % JC, c = e / JC;
     Q[r][c] = zmk(r == c ? 1.0 : 0.0, 0.0);
   }
+  // phase 2 owner: block (tr, tk), tr < tk, of the upper triangle, row `half` of it
+  const int ob = tid >> 1, half = tid & 1;
+  int tr = 0, tk = 0;
+  if (ob < kOffBlocks) {
+    int rem = ob;
+    while (rem >= JC / 2 - 1 - tr) {
+      rem -= JC / 2 - 1 - tr;
+      ++tr;
+    }
+    tk = tr + 1 + rem;
+  }
+  int g = 0;  // rounds started, over all sweeps: round g uses buffer g & 1
   for (int isw = 0; isw < max_sweeps; ++isw) {
     if (tid == 0) s_sweep[isw & 1] = 0;
-    for (int rd = 0; rd < JC - 1; ++rd) {
-      if (tid == 0) s_round[rd & 1] = 0;
-      if (tid < JC / 2) {
-        int pp, qq;
-        inner_pair(rd, tid, pp, qq);
-        double c = 1.0, s = 0.0;
-        zc w = zmk(1.0, 0.0);
-        const zc g = A[pp][qq];
-        const double a = A[pp][pp].x, b = A[qq][qq].x;
-        if (needs_rotation(g, a, b)) jrot(a, b, g, c, s, w);
-        rc_c[tid] = c; rc_s[tid] = s; rc_w[tid] = w;
-        if (s != 0.0) {
-          s_round[rd & 1] = 1;
-          s_sweep[isw & 1] = 1;
+    for (int rd = 0; rd < JC - 1; ++rd, ++g) {
+      const int cur = g & 1;
+      if (tid < 64) {
+        if (tid == 0) s_round[cur] = 0;
+        if (tid < JC / 2) {
+          int pp, qq;
+          inner_pair(rd, tid, pp, qq);
+          const zc gv = A[pp][qq];
+          const double a = A[pp][pp].x, b = A[qq][qq].x;
+          double c = 1.0;
+          zc sw = zmk(0.0, 0.0);
+          int rot = 0;
+          if (needs_rotation(gv, a, b)) {
+            double tg;
+            jrot(a, b, gv, c, sw, tg);
+            A[pp][pp] = zmk(a - tg, 0.0);
+            A[qq][qq] = zmk(b + tg, 0.0);
+            A[pp][qq] = zmk(0.0, 0.0);
+            A[qq][pp] = zmk(0.0, 0.0);
+            rot = 1;
+            s_round[cur] = 1;
+            s_sweep[isw & 1] = 1;
+          }
+          rb[cur].c[tid] = c;
+          rb[cur].sw[tid] = sw;
+          rb[cur].pq[tid] = pp | qq << 8 | rot << 16;
         }
+      } else if (g > 0 && s_round[cur ^ 1]) {
+        const int u = tid - 64;  // 448 threads for the 512 groups
+        q_rotate(Q, rb[cur ^ 1], u);
+        if (u < JC * JC / 2 - (kRoundThreads - 64)) q_rotate(Q, rb[cur ^ 1], u + (kRoundThreads - 64));
       }
       __syncthreads();
-      if (!s_round[rd & 1]) continue;
-      if (tid < 256) {
-        const int tr = tid >> 4, tk = tid & 15;
-        const double sr = rc_s[tr], sk = rc_s[tk];
-        if (sr != 0.0 || sk != 0.0) {
-          int pr, qr, pk, qk;
-          inner_pair(rd, tr, pr, qr);
-          inner_pair(rd, tk, pk, qk);
-          zc a00 = A[pr][pk], a01 = A[pr][qk], a10 = A[qr][pk], a11 = A[qr][qk];
-          if (sr != 0.0) {  // rows: A_p <- c A_p - s w A_q ; A_q <- s conj(w) A_p + c A_q
-            const double c = rc_c[tr];
-            const zc w = rc_w[tr];
-            const zc sw = zscale(w, sr), swc = zscale(zconj(w), sr);
-            const zc n00 = zsub(zscale(a00, c), zmul(sw, a10)), n01 = zsub(zscale(a01, c), zmul(sw, a11));
-            const zc n10 = zadd(zmul(swc, a00), zscale(a10, c)), n11 = zadd(zmul(swc, a01), zscale(a11, c));
-            a00 = n00; a01 = n01; a10 = n10; a11 = n11;
-          }
-          if (sk != 0.0) {  // cols: A_:p <- c A_:p - s conj(w) A_:q ; A_:q <- s w A_:p + c A_:q
-            const double c = rc_c[tk];
-            const zc w = rc_w[tk];
-            const zc sw = zscale(w, sk), swc = zscale(zconj(w), sk);
-            const zc n00 = zsub(zscale(a00, c), zmul(swc, a01)), n01 = zadd(zmul(sw, a00), zscale(a01, c));
-            const zc n10 = zsub(zscale(a10, c), zmul(swc, a11)), n11 = zadd(zmul(sw, a10), zscale(a11, c));
-            a00 = n00; a01 = n01; a10 = n10; a11 = n11;
-          }
-          if (tr == tk) {  // the annihilated entry of a rotated pair
-            a01 = zmk(0.0, 0.0);
-            a10 = zmk(0.0, 0.0);
-          }
-          A[pr][pk] = a00; A[pr][qk] = a01; A[qr][pk] = a10; A[qr][qk] = a11;
+      if (!s_round[cur]) continue;
+      if (ob < kOffBlocks) {
+        const RotBuf& R = rb[cur];
+        const int pqr = R.pq[tr], pqk = R.pq[tk];
+        const double cr = R.c[tr], ck = R.c[tk];
+        const zc swr = R.sw[tr], swk = R.sw[tk];
+        if ((pqr | pqk) >> 16) {
+          const int pr = pqr & 255, qr = (pqr >> 8) & 255, pk = pqk & 255, qk = (pqk >> 8) & 255;
+          const int own = half ? qr : pr, oth = half ? pr : qr;
+          const zc own0 = A[own][pk], own1 = A[own][qk], oth0 = A[oth][pk], oth1 = A[oth][qk];
+          // this thread's row after the row rotation (a pair that stays put has c = 1,
+          // s w = 0: the identity): A_p <- c A_p - s w A_q ; A_q <- s conj(w) A_p + c A_q
+          const zc f = half ? zconj(swr) : zmk(-swr.x, -swr.y);
+          const zc x0 = zadd(zscale(own0, cr), zmul(f, oth0));
+          const zc x1 = zadd(zscale(own1, cr), zmul(f, oth1));
+          // then the column rotation: A_:p <- c A_:p - s conj(w) A_:q ; A_:q <- s w A_:p + c A_:q
+          const zc n0 = zsub(zscale(x0, ck), zmul(zconj(swk), x1));
+          const zc n1 = zadd(zmul(swk, x0), zscale(x1, ck));
+          A[own][pk] = n0;
+          A[own][qk] = n1;
+          A[pk][own] = zconj(n0);
+          A[qk][own] = zconj(n1);
         }
-      }
-      if (tid >= 256) {
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {  // Q_:p <- c Q_:p - s conj(w) Q_:q ; Q_:q <- s w Q_:p + c Q_:q
-        const int grp = tid - 256 + 256 * h, k = grp >> 4, tp = grp & 15;
-        const double sn = rc_s[tp];
-        if (sn == 0.0) continue;
-        int pq, qq;
-        inner_pair(rd, tp, pq, qq);
-        const double c = rc_c[tp];
-        const zc w = rc_w[tp];
-        const zc sw = zscale(w, sn), swc = zscale(zconj(w), sn);
-        const zc xp = Q[k][pq], xq = Q[k][qq];
-        Q[k][pq] = zsub(zscale(xp, c), zmul(swc, xq));
-        Q[k][qq] = zadd(zmul(sw, xp), zscale(xq, c));
-      }
       }
       __syncthreads();
     }
@@ -282,6 +335,8 @@ __device__ void inner_jacobi(zc (*A)[JC + 1], zc (*Q)[JC + 1], double* rc_c, dou
       if (!*s_more) break;
     }
   }
+  __syncthreads();
+  if (g > 0 && s_round[(g - 1) & 1]) q_rotate(Q, rb[(g - 1) & 1], tid);
   __syncthreads();
 }
 
@@ -304,11 +359,15 @@ __global__ __launch_bounds__(kRoundThreads) void k_jacobi_round(const BondTask* 
   if (!FULL && *seen == stamp) return;  // unchanged since it was verified orthogonal
 
   __shared__ zc A[JC][JC + 1];
+  // The rotation accumulator Q; until the inner solve starts, its storage holds the
+  // Gram's reduction scratch (red) and the column-norm partials (nred), which keeps
+  // the workgroup at ~35 KB of LDS: registers, not LDS, bound the workgroups per CU.
   __shared__ zc Q[JC][JC + 1];
-  __shared__ double red[kGramWaves - 1][2][256];
-  __shared__ double nred[kGramWaves][2][16];
-  __shared__ double rc_c[JC / 2], rc_s[JC / 2];
-  __shared__ zc rc_w[JC / 2];
+  double (*red)[256] = reinterpret_cast<double (*)[256]>(&Q[0][0]);
+  double (*nred)[2][16] = reinterpret_cast<double (*)[2][16]>(reinterpret_cast<double*>(&Q[0][0]) + (kGramWaves - 1) * 256);
+  static_assert(((kGramWaves - 1) * 256 + kGramWaves * 2 * 16) * sizeof(double) <= sizeof(zc) * JC * (JC + 1),
+                "the Gram scratch must fit in Q's storage");
+  __shared__ RotBuf rb[2];
   __shared__ int s_flag, s_round[2], s_sweep[2], s_more;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -380,7 +439,7 @@ __global__ __launch_bounds__(kRoundThreads) void k_jacobi_round(const BondTask* 
   }
 
   // ---- eigenvectors of A -> Q, to the pair's slot; queue the pair for rotation
-  inner_jacobi(A, Q, rc_c, rc_s, rc_w, s_round, s_sweep, &s_more, tid, inner_sweeps);
+  inner_jacobi(A, Q, rb, s_round, s_sweep, &s_more, tid, inner_sweeps);
   zc* qg = t.Qg + (size_t)p * JC * JC;
   for (int e = tid; e < JC * JC; e += kRoundThreads) qg[e] = Q[e / JC][e % JC];  // qg[k JC + c] = Q[k][c]
   if (tid == 0) {
