@@ -31,6 +31,8 @@
 //                      Rb-side ones, or the identity for the explicit Q
 //   k_qrp_reflect<1,2> Q applied to those columns as the stored panels' block
 //                      reflectors, last panel first: Y <- Y - V_p (T_p (V_p^H Y))
+#include <type_traits>
+
 #include "tebd_device.h"
 #include "tebd_kernels.h"
 
@@ -40,6 +42,20 @@ constexpr int QB = 16;          // panel width
 constexpr int kMaxCols = 4096;  // largest Cb (d chi_max <= 4096)
 constexpr int PT = 512;         // threads of the register-resident panel kernel
 constexpr int PW = PT / 64;     // its waves
+
+// Calls f(std::integral_constant<int, i>) for i = 0 .. N-1 in order (a loop whose index is
+// a compile-time constant in every iteration).
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for_impl(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for_impl<I + 1, N>(f);
+  }
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl<0, N>(f);
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -203,6 +219,62 @@ __device__ __forceinline__ bool householder(zc alpha, double xn2, zc& tau, doubl
   return true;
 }
 
+// Selection of panel j0 by the NT threads of a workgroup: sel[0..nbp) := the nbp
+// candidates (columns j0 + c, c < ncand, squared norms snrm[c] in LDS) of largest norm
+// in descending order, ties to the lowest index; NaN norms are never taken.  *sstop :=
+// 1 when even the largest is not above thr (the numerical rank is j0), 2 when fewer than
+// nbp candidates have a norm, else 0.  Up to NT candidates, every thread ranks one
+// candidate against all the others (one pass, broadcast LDS reads); beyond that, wave 0
+// takes nbp argmax passes (snrm entries taken are overwritten).  snrm has kMaxCols + 1
+// entries: the last receives the largest norm.  Ends with a barrier.
+template <int NT>
+__device__ void select_panel(double* snrm, int ncand, int nbp, double thr, int j0, int* sel, int* sstop, int tid) {
+  const int lane = tid & 63, wave = tid >> 6;
+  if (tid < QB) sel[tid] = -1;
+  __syncthreads();
+  if (ncand <= NT) {
+    if (tid < ncand) {
+      const double v = snrm[tid];
+      if (v == v) {
+        int rank = 0;
+        for (int k = 0; k < ncand; ++k) {
+          const double u = snrm[k];
+          rank += (u > v || (u == v && k < tid)) ? 1 : 0;
+        }
+        if (rank < nbp) sel[rank] = j0 + tid;
+        if (rank == 0) snrm[kMaxCols] = v;
+      }
+    }
+  } else if (wave == 0) {
+    for (int q = 0; q < nbp; ++q) {
+      double bv = -1.0;
+      int bi = -1;
+      for (int k = lane; k < ncand; k += 64) {
+        const double v = snrm[k];
+        if (v > bv) { bv = v; bi = k; }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(bv, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (oi >= 0 && (bi < 0 || ov > bv || (ov == bv && oi < bi))) { bv = ov; bi = oi; }
+      }
+      if (bi < 0) break;  // every lane holds the same winner
+      if (lane == 0) sel[q] = j0 + bi;
+      if (lane == 0 && q == 0) snrm[kMaxCols] = bv;
+      if (lane == (bi & 63)) snrm[bi] = -1.0;  // a lane only ever reads the entries it clears
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int st = 0;
+    if (sel[0] < 0 || !(snrm[kMaxCols] > thr)) st = 1;
+    for (int q = 1; q < nbp && !st; ++q)
+      if (sel[q] < 0) st = 2;
+    *sstop = st;
+  }
+  __syncthreads();
+}
+
 // Column moves of a panel, by one wave (lane q < QB handles panel slot j0 + q and the
 // q-th selected column): sel[0..nbp) (distinct columns >= j0) go to j0..j0+nbp-1 in
 // that order; the columns of j0..j0+nbp-1 not selected go to the selected slots beyond
@@ -289,7 +361,7 @@ __global__ __launch_bounds__(PT) void k_qrp_panel_reg(const BondTask* __restrict
   const int nbp = (kmax - j0) < QB ? (kmax - j0) : QB;
   const int ncand = Cb - j0;
   const size_t ld = Rb;
-  __shared__ double snrm[kMaxCols];
+  __shared__ double snrm[kMaxCols + 1];
   __shared__ int sel[QB], mv_src[QB], mv_dst[QB];
   __shared__ int snmv, sstop;
   __shared__ double red[PW][32];
@@ -301,36 +373,7 @@ __global__ __launch_bounds__(PT) void k_qrp_panel_reg(const BondTask* __restrict
 
   // 1. selection
   for (int k = tid; k < ncand; k += PT) snrm[k] = t.qnrm[j0 + k];
-  if (tid == 0) sstop = 0;
-  if (tid < QB) sel[tid] = j0;
-  __syncthreads();
-  if (wave == 0) {
-    for (int q = 0; q < nbp; ++q) {
-      double bv = -1.0;
-      int bi = -1;
-      for (int k = lane; k < ncand; k += 64) {
-        const double v = snrm[k];
-        if (v > bv) { bv = v; bi = k; }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(bv, off, 64);
-        const int oi = __shfl_xor(bi, off, 64);
-        if (oi >= 0 && (bi < 0 || ov > bv || (ov == bv && oi < bi))) { bv = ov; bi = oi; }
-      }
-      // every lane holds the same winner; a lane only ever reads the entries it clears
-      if (q == 0 && !(bv > t.qstate[1])) {
-        if (lane == 0) sstop = 1;  // numerical rank reached (or the norms are not finite)
-        break;
-      }
-      if (bi < 0) {
-        if (lane == 0) sstop = 2;  // no finite candidate left
-        break;
-      }
-      if (lane == 0) sel[q] = j0 + bi;
-      if (lane == (bi & 63)) snrm[bi] = -1.0;
-    }
-  }
-  __syncthreads();
+  select_panel<PT>(snrm, ncand, nbp, t.qstate[1], j0, sel, &sstop, tid);
   if (sstop) {
     if (tid == 0) t.qstate[0] = sstop == 1 ? (double)j0 : 0.0;
     return;
@@ -352,18 +395,19 @@ __global__ __launch_bounds__(PT) void k_qrp_panel_reg(const BondTask* __restrict
     if (i < Rb) move_rows(t.Wq, ld, i, nmv, mv_src, mv_dst);
   }
 
-  // 3. Householder QR of the panel.  The columns rotate through a[]: at step q the
-  // current column is a[.][0], the later columns k = q + m sit at m = 1 .. 15 - q and
-  // the earlier reflectors p at m = 16 - q + p, so every register index is static.
-  for (int q = 0; q < nbp; ++q) {
+  // 3. Householder QR of the panel, the column loop unrolled so that every register
+  // index is static: column q sits in a[.][q], first as data, then as reflector v_q
+  // (rows > j0 + q) below R(:, q).
+  static_for<QB>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    if (q >= nbp) return;
     const int j = j0 + q;
-    const int nlater = QB - 1 - q;
     double part = 0.0;
 #pragma unroll
     for (int s = 0; s < RPT; ++s) {
       const int i = tid + PT * s;
-      if (i > j && i < Rb) part += zabs2(a[s][0]);
-      if (i == j) salpha = a[s][0];
+      if (i > j && i < Rb) part += zabs2(a[s][q]);
+      if (i == j) salpha = a[s][q];
     }
     part = wave_sum(part);
     if (lane == 0) rnorm[wave] = part;
@@ -379,29 +423,34 @@ __global__ __launch_bounds__(PT) void k_qrp_panel_reg(const BondTask* __restrict
 #pragma unroll
       for (int s = 0; s < RPT; ++s) {
         const int i = tid + PT * s;
-        if (i > j && i < Rb) a[s][0] = zmul(a[s][0], scale);
-        if (i == j) a[s][0] = zmk(beta, 0.0);
+        if (i > j && i < Rb) a[s][q] = zmul(a[s][q], scale);
+        if (i == j) a[s][q] = zmk(beta, 0.0);
       }
     }
-    // slot m <= nlater: conj(v_q) a_(q+m);  slot m > nlater: conj(v_p) v_q
+    // slot m = 1 .. 15 - q: conj(v_q) a_(q+m) (the later columns, for the update);
+    // slot m > 15 - q: conj(v_p) v_q, p = m - 16 + q (the earlier reflectors, for T)
     double x[2 * QB];
 #pragma unroll
     for (int e = 0; e < 2 * QB; ++e) x[e] = 0.0;
+    // v_q on this thread's rows: 1 at row j, zero outside rows j .. Rb - 1, so that the
+    // products and the update below need no row test
+    zc vq[RPT];
 #pragma unroll
     for (int s = 0; s < RPT; ++s) {
       const int i = tid + PT * s;
-      if (i >= j && i < Rb) {
-        const zc vq = (i == j) ? zmk(1.0, 0.0) : a[s][0];
-#pragma unroll
-        for (int m = 1; m < QB; ++m) {
-          const bool later = m <= nlater;  // earlier reflectors: i >= j > j0 + p
-          const zc l = later ? vq : a[s][m];
-          const zc r = later ? a[s][m] : vq;
-          x[2 * m] = fma(l.x, r.x, fma(l.y, r.y, x[2 * m]));
-          x[2 * m + 1] = fma(l.x, r.y, fma(-l.y, r.x, x[2 * m + 1]));
-        }
-      }
+      vq[s] = (i == j) ? zmk(1.0, 0.0) : ((i > j && i < Rb) ? a[s][q] : zmk(0.0, 0.0));
     }
+#pragma unroll
+    for (int s = 0; s < RPT; ++s)
+#pragma unroll
+      for (int m = 1; m < QB; ++m) {
+        const bool later = m <= QB - 1 - q;  // earlier reflectors: rows >= j > j0 + p
+        const zc other = a[s][later ? q + m : m - QB + q];
+        const zc l = later ? vq[s] : other;
+        const zc r = later ? other : vq[s];
+        x[2 * m] = fma(l.x, r.x, fma(l.y, r.y, x[2 * m]));
+        x[2 * m + 1] = fma(l.x, r.y, fma(-l.y, r.x, x[2 * m + 1]));
+      }
     // reduce-scatter of the 32 values over the wave: lane l ends with value
     // 16 b5 + 8 b4 + 4 b3 + 2 b2 + b1 (b = bits of l), summed over all 64 lanes
     {
@@ -442,45 +491,29 @@ __global__ __launch_bounds__(PT) void k_qrp_panel_reg(const BondTask* __restrict
     }
     __syncthreads();
     if (tid == 0) {
-      for (int m = nlater + 1; m < QB; ++m) G[m - QB + q][q] = zmk(wsum[2 * m], wsum[2 * m + 1]);
+      for (int m = QB - q; m < QB; ++m) G[m - QB + q][q] = zmk(wsum[2 * m], wsum[2 * m + 1]);
       stau[q] = tau;
       t.tau[j] = tau;
     }
     if (refl) {
       const zc ctau = zconj(tau);
 #pragma unroll
-      for (int m = 1; m < QB; ++m) {
-        if (m <= nlater) {
-          const zc f = zmul(ctau, zmk(wsum[2 * m], wsum[2 * m + 1]));
+      for (int m = 1; m <= QB - 1 - q; ++m) {
+        const zc f = zmul(ctau, zmk(wsum[2 * m], wsum[2 * m + 1]));
 #pragma unroll
-          for (int s = 0; s < RPT; ++s) {
-            const int i = tid + PT * s;
-            if (i >= j && i < Rb) a[s][m] = zsub(a[s][m], (i == j) ? f : zmul(f, a[s][0]));
-          }
-        }
+        for (int s = 0; s < RPT; ++s) a[s][q + m] = zsub(a[s][q + m], zmul(f, vq[s]));
       }
     }
-    // rotate: the finished reflector goes to the end
-#pragma unroll
-    for (int s = 0; s < RPT; ++s) {
-      const zc c0 = a[s][0];
-#pragma unroll
-      for (int m = 0; m < QB - 1; ++m) a[s][m] = a[s][m + 1];
-      a[s][QB - 1] = c0;
-    }
-  }
+  });
 
-  // 4. store the panel (R above and on the diagonal, reflectors below; column c now sits
-  // at m = (c - nbp) mod QB) and T
+  // 4. store the panel (R above and on the diagonal, reflectors below) and T
 #pragma unroll
   for (int s = 0; s < RPT; ++s) {
     const int i = tid + PT * s;
     if (i < Rb) {
 #pragma unroll
-      for (int m = 0; m < QB; ++m) {
-        const int c = (m + nbp) & (QB - 1);
-        if (c < nbp) t.Wq[i + ld * (j0 + c)] = a[s][m];
-      }
+      for (int c = 0; c < QB; ++c)
+        if (c < nbp) t.Wq[i + ld * (j0 + c)] = a[s][c];
     }
   }
   form_t(G, stau, nbp, t.Tq + (size_t)(j0 / QB) * QB * QB);
@@ -502,7 +535,7 @@ __global__ __launch_bounds__(256) void k_qrp_panel(const BondTask* __restrict__ 
   const int nbp = (kmax - j0) < QB ? (kmax - j0) : QB;
   const int ncand = t.Cb - j0;
   const size_t ld = t.Rb;
-  __shared__ double snrm[kMaxCols];
+  __shared__ double snrm[kMaxCols + 1];
   __shared__ int sel[QB], mv_src[QB], mv_dst[QB];
   __shared__ int snmv, sstop;
   __shared__ double red[4];
@@ -510,35 +543,7 @@ __global__ __launch_bounds__(256) void k_qrp_panel(const BondTask* __restrict__ 
   __shared__ zc stau[QB];
   __shared__ double gred[3][2][256];
   for (int k = tid; k < ncand; k += 256) snrm[k] = t.qnrm[j0 + k];
-  if (tid == 0) sstop = 0;
-  if (tid < QB) sel[tid] = j0;
-  __syncthreads();
-  if (wave == 0) {
-    for (int q = 0; q < nbp; ++q) {
-      double bv = -1.0;
-      int bi = -1;
-      for (int k = lane; k < ncand; k += 64) {
-        const double v = snrm[k];
-        if (v > bv) { bv = v; bi = k; }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(bv, off, 64);
-        const int oi = __shfl_xor(bi, off, 64);
-        if (oi >= 0 && (bi < 0 || ov > bv || (ov == bv && oi < bi))) { bv = ov; bi = oi; }
-      }
-      if (q == 0 && !(bv > t.qstate[1])) {
-        if (lane == 0) sstop = 1;
-        break;
-      }
-      if (bi < 0) {
-        if (lane == 0) sstop = 2;
-        break;
-      }
-      if (lane == 0) sel[q] = j0 + bi;
-      if (lane == (bi & 63)) snrm[bi] = -1.0;
-    }
-  }
-  __syncthreads();
+  select_panel<256>(snrm, ncand, nbp, t.qstate[1], j0, sel, &sstop, tid);
   if (sstop) {
     if (tid == 0) t.qstate[0] = sstop == 1 ? (double)j0 : 0.0;
     return;
