@@ -13,13 +13,16 @@ timed steps; a run whose timed window is not at chi=256 throughout exits with an
 error instead of printing a number.  Each timed step is 191 bond updates of up to
 1024 x 1024 (64 odd, 63 even, 64 odd bonds).
 
-`value` is the batched learned-rank FP64 path (randomised SVD sized per bond,
-full-SVD fallback, the shipped rank model) over the timed steps.  Beside it, on
-the same protocol and window:
+`value` is the mixed-precision batched learned-rank path: theta's contraction over
+the middle bond on the int8 matrix cores from exact power-of-two slices of the FP64
+operands (integer accumulation, FP64 combination), the gate and everything after
+theta in FP64 (randomised SVD sized per bond, full-SVD fallback, the shipped rank
+model), over the timed steps.  Beside it, on the same protocol and window:
   * the exact batched FP64 path (full SVD of every bond) -- its steps/s, and the
-    learned path's observables after the last timed step against it
+    timed path's observables after the last timed step against it
     (`obs_err_vs_exact_gpu`: energy, <n_i>, <Sz_i>, entropy in log_d, truncerr);
-  * the mixed-precision form (theta and gate on the FP32 matrix cores);
+  * the same learned-rank path with theta on the FP64 matrix cores;
+  * the FP32 form (theta and gate on the FP32 matrix cores, FP32 accumulation);
   * the plain per-bond FP64 path (`fp64`, the drop-in denominator), timed over
     --compare-steps steps from the exact path's state at the start of the window.
 `obs_err_vs_oracle` checks each path's bond update against the FP64 CPU oracle:
@@ -503,7 +506,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20, help="timed steps, all at chi=256 in the bulk")
     ap.add_argument("--warmup", type=int, default=5, help="untimed steps after the bulk has saturated")
     ap.add_argument("--compare-steps", type=int, default=3)
-    ap.add_argument("--mode", default="batched,learned")
+    ap.add_argument("--mode", default="batched,learned,split")
     ap.add_argument("--lanes", type=int, default=3, help="streams / host threads per half-sweep of the timed run")
     ap.add_argument("--graphs", type=int, default=1,
                     help="hipGraph replay of the Jacobi sweeps in every run (0 = eager launches)")
@@ -511,7 +514,9 @@ def main():
     ap.add_argument("--roofline-only", action="store_true", help="only the per-kernel event run")
     ap.add_argument("--reference-mode", default="batched", help="exact GPU path of the comparison ('' = skip)")
     ap.add_argument("--mixed-mode", default="batched,learned,mixed",
-                    help="mixed-precision path timed and checked ('' = skip)")
+                    help="FP32-theta path timed and checked ('' = skip)")
+    ap.add_argument("--fp64-theta-mode", default="batched,learned",
+                    help="learned-rank path with FP64 theta, timed and checked ('' = skip)")
     ap.add_argument("--rank-model", default=RANK_MODEL if os.path.exists(RANK_MODEL) else "",
                     help="rank model parameters (.npy) for the learned-rank path ('' = none)")
     ap.add_argument("--compare", default="fp64", help="per-bond FP64 drop-in timed on the window ('' = skip)")
@@ -560,6 +565,10 @@ def main():
         ref = run_mode(args.reference_mode, args.steps, args.warmup, log, pre_steps=n_sat,
                        keep_state=bool(args.compare), lanes=args.lanes, graphs=graphs)
         extra[f"{args.reference_mode}_steps_per_s"] = args.steps / ref["elapsed"]
+    f64t = None
+    if args.fp64_theta_mode:
+        f64t = run_mode(args.fp64_theta_mode, args.steps, args.warmup, log, pre_steps=n_sat, rank_model=model,
+                        lanes=args.lanes, graphs=graphs)
     mixed = None
     if args.mixed_mode:
         mixed = run_mode(args.mixed_mode, args.steps, args.warmup, log, pre_steps=n_sat, rank_model=model,
@@ -578,18 +587,24 @@ def main():
         extra["obs_err_vs_exact_gpu"] = {"max": max(errs.values()), "by_observable": errs,
                                          "against": f"{args.reference_mode} (exact FP64 GPU path), "
                                                     f"after the same {n_pre + args.steps} steps"}
+    if f64t is not None:
+        extra["fp64_theta"] = {"mode": args.fp64_theta_mode, "steps_per_s": args.steps / f64t["elapsed"]}
+        if ref is not None:
+            ferr = obs_errors(f64t)
+            extra["fp64_theta"]["obs_err_vs_exact_gpu"] = {"max": max(ferr.values()), "by_observable": ferr}
     if mixed is not None:
-        extra["mixed"] = {"mode": args.mixed_mode, "steps_per_s": args.steps / mixed["elapsed"]}
+        extra["fp32_theta"] = {"mode": args.mixed_mode, "steps_per_s": args.steps / mixed["elapsed"]}
         if ref is not None:
             merr = obs_errors(mixed)
-            extra["mixed"]["obs_err_vs_exact_gpu"] = {"max": max(merr.values()), "by_observable": merr}
+            extra["fp32_theta"]["obs_err_vs_exact_gpu"] = {"max": max(merr.values()), "by_observable": merr}
     if args.compare and ref is not None:
         cmp_res = run_mode(args.compare, args.compare_steps, 0, log, state=ref["snap"], graphs=graphs, lanes=1)
         extra[f"{args.compare}_steps_per_s"] = args.compare_steps / cmp_res["elapsed"]
         extra[f"{args.compare}_window"] = f"steps {n_pre + 1}-{n_pre + args.compare_steps}"
         extra[f"speedup_vs_{args.compare}"] = sps / extra[f"{args.compare}_steps_per_s"]
     if args.oracle_check:
-        modes = [args.mode] + [m for m in (args.reference_mode, args.mixed_mode, args.compare) if m]
+        modes = [args.mode] + [m for m in (args.fp64_theta_mode, args.reference_mode, args.mixed_mode, args.compare)
+                               if m]
         extra["obs_err_vs_oracle"] = oracle_check(res["snap"], modes, log, rank_model=model)
         extra["obs_err_vs_oracle"]["bonds"] = list(ORACLE_BONDS)
 
@@ -605,8 +620,7 @@ def main():
     st = res["stats"]
     half_steps = 3 * args.steps
     out = {
-        "metric": "Trotter steps/s, L=128 chi=256: batched+learned-rank FP64 path vs FP64 GPU drop-in "
-                  "(mixed FP32-theta form reported beside it), obs err vs FP64 oracle stated",
+        "metric": "Trotter steps/s, L=128 \u03c7=256: mixed+batched vs FP64 GPU at \u22641e-8 obs err",
         "value": sps,
         "unit": "Trotter steps/s",
         "n_gpus": 1,
@@ -616,7 +630,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "complex128 (f64)",
+        "dtype": "i8 slices -> i32 (theta contraction), complex128 f64 (gate, SVD, lambda)" if "split" in args.mode
+                 else "complex128 (f64)",
         "data": "synthetic (Neel product state, Fermi-Hubbard gates)",
         "config": {"workload": f"Fermi-Hubbard quench L={L} d={D} t=1 U=4 dt={DT} chi_max={CHI}, Neel start; "
                                f"timed steps {n_pre + 1}-{n_pre + args.steps} with every bulk bond at chi={CHI}",

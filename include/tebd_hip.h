@@ -43,7 +43,10 @@ enum tebd_mode {
   TEBD_MODE_FP64 = 0u,              /* plain FP64 drop-in: one bond at a time, full SVD */
   TEBD_MODE_MIXED = 1u << 0,        /* FP32 theta/gate on MFMA, FP64 Gram/SVD/lambda */
   TEBD_MODE_BATCHED = 1u << 1,      /* all bonds of one parity per launch */
-  TEBD_MODE_LEARNED_RANK = 1u << 2  /* randomized SVD sized by the rank model, full-SVD fallback */
+  TEBD_MODE_LEARNED_RANK = 1u << 2, /* randomized SVD sized by the rank model, full-SVD fallback */
+  TEBD_MODE_SPLIT = 1u << 3         /* theta on the int8 matrix cores from exact power-of-two
+                                       slices of the FP64 operands (integer accumulation, FP64
+                                       combination); not with TEBD_MODE_MIXED */
 };
 
 /* Allocates the device state for a chain of L sites, local dimension d and

@@ -126,6 +126,7 @@ struct tebd_ctx {
   BondTask* d_tasks = nullptr;   // kStages x ntask_max
   BondTask* h_tasks = nullptr;   // pinned, kStages x ntask_max
   // sketched (learned-rank) path workspace
+  SplitWs split{};               // per task: slices of the split-operand theta (TEBD_MODE_SPLIT)
   zc* Zb = nullptr;              // per task: range basis Q of the sketch (m x l)
   zc* Bb = nullptr;              // per task: theta Omega and its QR, then B^dagger
   double* lam_scratch = nullptr; // per task chi_max: lambda before it is committed
@@ -284,9 +285,14 @@ void drain_profile(tebd_ctx* c) {
 // Bond `link` (0-based, sites link and link + 1) joins two chains of an ensemble.
 bool is_cut(const tebd_ctx* c, int link) { return (link + 1) % c->chain_len == 0; }
 
-// theta + gate of a batch: FP32 matrix cores in the mixed mode, FP64 matrix cores otherwise.
+// theta + gate of a batch: FP32 matrix cores in the mixed mode, int8 matrix cores with exact
+// accumulation in the split mode, FP64 matrix cores otherwise.
 hipError_t theta_gate(tebd_ctx* c, const BondTask* d_t, int nb, int max_chil, int max_chir) {
   if (c->mode & TEBD_MODE_MIXED) return launch_theta_gate_mixed(d_t, nb, c->d, max_chil, max_chir, c->st);
+  if (c->mode & TEBD_MODE_SPLIT) {
+    if (nb > c->ntask_max) return hipErrorInvalidValue;
+    return launch_theta_gate_split(d_t, nb, c->d, max_chil, max_chir, c->split, c->st);
+  }
   return launch_theta_gate_mfma(d_t, nb, c->d, max_chil, max_chir, c->st);
 }
 
@@ -1687,6 +1693,13 @@ tebd_ctx* make_lane(tebd_ctx* c, int lane, int slot0, int nslots) {
   l->h_pred = c->h_pred + s0;
   l->d_tasks = c->d_tasks + kStages * s0;
   l->h_tasks = c->h_tasks + kStages * s0;
+  if (c->split.a) {
+    l->split.a = c->split.a + s0 * c->split.stride;
+    l->split.b = c->split.b + s0 * c->split.stride;
+    l->split.ea = c->split.ea + s0 * c->split.rows;
+    l->split.eb = c->split.eb + s0 * c->split.rows;
+    l->split.eg = c->split.eg + s0 * c->split.pitch;
+  }
   if (c->Zb) {
     l->Zb = c->Zb + s0 * c->Rmax * c->Cpmax;
     l->Bb = c->Bb + s0 * c->Rmax * c->Cpmax;
@@ -1951,6 +1964,11 @@ void free_all(tebd_ctx* c) {
   (void)hipFree(c->cf_s);
   (void)hipFree(c->cf_perm);
   (void)hipFree(c->cf_lam);
+  (void)hipFree(c->split.a);
+  (void)hipFree(c->split.b);
+  (void)hipFree(c->split.ea);
+  (void)hipFree(c->split.eb);
+  (void)hipFree(c->split.eg);
   (void)hipFree(c->Zb);
   (void)hipFree(c->Bb);
   (void)hipFree(c->lam_scratch);
@@ -2239,6 +2257,11 @@ void noalloc_layout(tebd_ctx* c, unsigned char* base) {
 
 // Device bytes of one per-bond workspace slot (every per-slot array tebd_create
 // allocates), for bond matrices of up to Rmax x Cpmax.
+// Split-operand theta workspace per task: row capacity and bytes of one operand's slices.
+size_t split_rows(const tebd_ctx* c) { return (size_t)c->d * round_up(c->chi_max, 16); }
+int split_pitch(const tebd_ctx* c) { return round_up(c->chi_max, 64); }
+size_t split_stride(const tebd_ctx* c) { return (size_t)2 * kSplitSlices * split_rows(c) * split_pitch(c); }
+
 double slot_bytes(const tebd_ctx* c) {
   const double R = c->Rmax, Cp = c->Cpmax, chi = c->chi_max, nblk = c->nblk_max, z = sizeof(zc);
   double b = z * (R * Cp + 2.0 * Cp * Cp + R * chi + Cp * chi + Cp) + 8.0 * Cp * 2 + 4.0 * Cp * 2 + 16.0 +
@@ -2248,6 +2271,7 @@ double slot_bytes(const tebd_ctx* c) {
   if (c->mode & TEBD_MODE_LEARNED_RANK)
     b += 2.0 * z * R * Cp + 8.0 * chi + 2.0 * kGemmRegions * sizeof(GemmTask) + 8.0 + z * (double)probe_stride(c) +
          2.0 * sizeof(ProbeTask);
+  if (c->mode & TEBD_MODE_SPLIT) b += 2.0 * (double)split_stride(c) + 8.0 * split_rows(c) + 4.0 * split_pitch(c);
   return b;
 }
 
@@ -2266,6 +2290,7 @@ int create_impl(tebd_ctx** out, int nchain, int Lc, int d, int chi_max, int orde
   if (bc == 'P' && (Lc < 3 || nchain > 1)) return TEBD_ERR_ARG;  // periodic: one chain of at least 3 sites
   if (!(d == 2 || d == 3 || d == 4 || d == 5 || d == 6 || d == 8)) return TEBD_ERR_ARG;
   if (d * chi_max > 4096) return TEBD_ERR_ARG;
+  if ((mode & TEBD_MODE_MIXED) && (mode & TEBD_MODE_SPLIT)) return TEBD_ERR_ARG;  // one theta precision
   const int L = nchain * Lc;
   tebd_ctx* c = new tebd_ctx();
   c->L = L;
@@ -2349,6 +2374,16 @@ int create_impl(tebd_ctx** out, int nchain, int Lc, int d, int chi_max, int orde
     if (bad(hipHostMalloc(&c->h_gemm, sizeof(GemmTask) * nt * kGemmRegions))) return TEBD_ERR_HIP;
     if (bad(hipMalloc(&c->d_kprime, sizeof(int) * nt))) return TEBD_ERR_HIP;
     if (bad(hipHostMalloc(&c->h_kprime, sizeof(int) * nt))) return TEBD_ERR_HIP;
+  }
+  if (mode & TEBD_MODE_SPLIT) {
+    c->split.rows = split_rows(c);
+    c->split.pitch = split_pitch(c);
+    c->split.stride = split_stride(c);
+    if (bad(hipMalloc(&c->split.a, c->split.stride * nt))) return TEBD_ERR_HIP;
+    if (bad(hipMalloc(&c->split.b, c->split.stride * nt))) return TEBD_ERR_HIP;
+    if (bad(hipMalloc(&c->split.ea, sizeof(int) * c->split.rows * nt))) return TEBD_ERR_HIP;
+    if (bad(hipMalloc(&c->split.eb, sizeof(int) * c->split.rows * nt))) return TEBD_ERR_HIP;
+    if (bad(hipMalloc(&c->split.eg, sizeof(int) * c->split.pitch * nt))) return TEBD_ERR_HIP;
   }
   c->last_rank.assign(L + 1, 0);
   c->last_trunc.assign(L + 1, 0.0);

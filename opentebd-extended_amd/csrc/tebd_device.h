@@ -46,6 +46,23 @@ struct NcBlock {
   zc* V;             // trv x min(lgt, trv): right singular vectors
 };
 
+// Workspace of the split-operand theta (one slot per task of a batch): the int8 slices of
+// A = Gamma_l lambda_c (rows (a, k) -> a d + k) and of B = Gamma_r (columns (l, b) ->
+// b d + l), plane (2 s + part) of `rows` rows x `pitch` bytes each (pitch: the middle
+// bond padded to 64), each row's and column's power-of-two scale, and the power-of-two
+// balance of each middle index g (A's column g times 2^eg, B's row g times 2^-eg).
+constexpr int kSplitSlices = 7;
+struct SplitWs {
+  signed char* a;
+  signed char* b;
+  int* ea;          // rows per task
+  int* eb;
+  int* eg;          // pitch per task
+  size_t rows;      // row capacity: d round_up(chi_max, 16)
+  size_t stride;    // bytes per task: 2 kSplitSlices rows pitch
+  int pitch;        // round_up(chi_max, 64)
+};
+
 // One kept Schmidt state b of a number-conserving bond update: its block's vectors and
 // position, and where its column of Gamma_l and row of Gamma_r go.
 struct NcKept {

@@ -9,6 +9,10 @@ hipError_t launch_theta_gate_mfma(const BondTask* d_tasks, int ntasks, int d, in
                                   hipStream_t st);
 hipError_t launch_theta_gate_mixed(const BondTask* d_tasks, int ntasks, int d, int max_chil, int max_chir,
                                    hipStream_t st);
+// Split-operand theta + gate (int8 matrix cores, exact integer accumulation, FP64
+// combination): slices A and B into ws, then theta into W.
+hipError_t launch_theta_gate_split(const BondTask* d_tasks, int ntasks, int d, int max_chil, int max_chir,
+                                   const SplitWs& ws, hipStream_t st);
 hipError_t launch_prepare_svd(const BondTask* d_tasks, int ntasks, hipStream_t st);
 hipError_t launch_frobenius(const BondTask* d_tasks, int ntasks, int max_cpad, hipStream_t st, int keep_sum);
 hipError_t launch_jacobi_begin(const BondTask* d_tasks, int ntasks, hipStream_t st);

@@ -12,6 +12,7 @@ INTEGER(c_int), PARAMETER :: TEBD_MODE_FP64 = 0
 INTEGER(c_int), PARAMETER :: TEBD_MODE_MIXED = 1
 INTEGER(c_int), PARAMETER :: TEBD_MODE_BATCHED = 2
 INTEGER(c_int), PARAMETER :: TEBD_MODE_LEARNED_RANK = 4
+INTEGER(c_int), PARAMETER :: TEBD_MODE_SPLIT = 8
 
 INTERFACE
   INTEGER(c_int) FUNCTION tebd_create(ctx, L, d, chi_max, order, bc, mode) BIND(C, name="tebd_create")

@@ -22,6 +22,7 @@ MODE_FP64 = 0
 MODE_MIXED = 1 << 0
 MODE_BATCHED = 1 << 1
 MODE_LEARNED_RANK = 1 << 2
+MODE_SPLIT = 1 << 3
 
 MLP_INPUTS = 66
 MLP_HIDDEN = 128

@@ -32,7 +32,7 @@ def _p(a):
 
 
 _MODES = {"fp64": _lib.MODE_FP64, "mixed": _lib.MODE_MIXED, "batched": _lib.MODE_BATCHED,
-          "learned": _lib.MODE_LEARNED_RANK}
+          "learned": _lib.MODE_LEARNED_RANK, "split": _lib.MODE_SPLIT}
 
 
 def parse_mode(mode):
@@ -48,7 +48,7 @@ def parse_mode(mode):
 
 # profiled launch classes of tebd_get_profile, in order, with the kernels each covers
 PROFILE_KINDS = {
-    "theta_gate": "k_theta_gate / k_theta_gate_mixed",
+    "theta_gate": "k_theta_gate / k_theta_gate_mixed / k_split_* + k_theta_split",
     "jacobi_round": "k_jacobi_round",
     "truncate_gamma": "k_truncate, k_write_gammas, k_qrp_vectors_init, k_qrp_reflect<1,2>, k_build_sketch",
     "qr_panel": "k_qrp_panel_reg",

@@ -75,3 +75,21 @@ def test_python_contexts_expose_the_library_surface():
                  "chain_entropy_chains", "expval_two_site_chains"):
         assert callable(getattr(Ensemble, name, None)), name
     assert issubclass(Ensemble, Context)
+
+
+def test_mode_flags_match_the_header():
+    text = open(os.path.join(ROOT, "include", "tebd_hip.h")).read()
+    flags = {m.group(1): eval(m.group(2).replace("u", "")) for m in
+             re.finditer(r"TEBD_MODE_([A-Z_0-9]+)\s*=\s*([0-9u <]+)", text)}
+    assert flags == {"FP64": _lib.MODE_FP64, "MIXED": _lib.MODE_MIXED, "BATCHED": _lib.MODE_BATCHED,
+                     "LEARNED_RANK": _lib.MODE_LEARNED_RANK, "SPLIT": _lib.MODE_SPLIT}
+    from tebd_amd.engine import parse_mode
+    assert parse_mode("batched,learned,split") == _lib.MODE_BATCHED | _lib.MODE_LEARNED_RANK | _lib.MODE_SPLIT
+
+
+def test_fp32_and_split_theta_are_exclusive_without_touching_the_gpu():
+    lib = _lib.lib()
+    h = ctypes.c_void_p()
+    rc = lib.tebd_create(ctypes.byref(h), 8, 2, 4, 2, b"O", _lib.MODE_MIXED | _lib.MODE_SPLIT)
+    assert rc == _lib.TEBD_ERR_ARG
+    assert not h.value

@@ -1,0 +1,97 @@
+"""Split-operand mode: theta's contraction over the middle bond on the int8 matrix cores
+(v_mfma_i32_16x16x64_i8) from exact power-of-two slices of the FP64 operands, with
+integer accumulation and FP64 combination; the gate, the lambda scaling and everything
+after theta in FP64.
+
+Tolerance: the exact (FP64) modes' 1e-9 against the reference's trajectories
+(test_gpu_parity.py).  Five 7-bit slices per operand leave a truncation error of about
+2^-35 of the row and column maxima per product term, far below what the reference's
+own FP64 rounding and the 1e-9 bound resolve; the single-update test pins the theta
+error directly against the FP64 path at 1e-10.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from test_gpu_parity import RTP, run_device
+from tebd_amd import Context, TEBDError
+
+pytestmark = pytest.mark.gpu
+
+SPLIT_TOL = 1e-9
+
+
+@pytest.mark.parametrize("mode", ["split", "batched,split", "batched,learned,split"])
+@pytest.mark.parametrize("name", RTP)
+def test_split_trajectory_matches_reference(name, mode):
+    z = load_golden(name)
+    print(f"[gpu] split trajectory {name} mode={mode}", flush=True)
+    worst, dims = run_device(z, mode)
+    print(f"[gpu] {name} {mode} errors {worst}", flush=True)
+    assert max(worst.values()) < SPLIT_TOL, worst
+    assert dims.max() <= int(z["chi"])
+
+
+@pytest.mark.parametrize("mode", ["split", "batched,learned,split"])
+def test_split_imaginary_time_matches_reference(mode):
+    """Imaginary time with CanonicalFormAll every step against the reference."""
+    z = load_golden("itp_l10_chi8")
+    print(f"[gpu] split itp_l10_chi8 mode={mode}", flush=True)
+    worst, dims = run_device(z, mode, check_every=5)
+    print(f"[gpu] itp {mode} errors {worst}", flush=True)
+    assert max(worst.values()) < SPLIT_TOL, worst
+
+
+def two_site(z, mode, gate, scale=(1.0, 1.0)):
+    """TwoSiteOp on bond `link` of the fixture's state, Gamma_link and Gamma_link+1 scaled."""
+    L, d, chi, link = int(z["L"]), int(z["d"]), int(z["chi"]), int(z["link"])
+    gam = np.array(z["gam_in"], dtype=np.complex128)
+    gam[link - 1] *= scale[0]
+    gam[link] *= scale[1]
+    ctx = Context(L, d, chi, mode=mode, cutoff=0.0)
+    ctx.upload(gam, z["lam_in"])
+    ctx.two_site_op(link, gate)
+    out = ctx.download(chi)
+    ctx.close()
+    return out
+
+
+def block(G, lm, link):
+    return np.einsum("a,bia,b,cjb,c->aijc", lm[link - 1], G[link - 1], lm[link], G[link], lm[link + 1])
+
+
+def test_split_two_site_op_matches_fp64():
+    """One update with a random non-unitary operator on a random state (entries of both
+    signs and a graded spectrum): lambda and the gauge-invariant block lambda Gamma
+    lambda Gamma lambda agree with the FP64 path to 1e-10, and the int8 path really ran
+    (the results are not bit-identical)."""
+    z = load_golden("two_site_op_l4")
+    print("[gpu] split two_site_op_l4", flush=True)
+    link = int(z["link"])
+    g0, l0 = two_site(z, "fp64", z["U"][link - 1])
+    g1, l1 = two_site(z, "split", z["U"][link - 1])
+    r0, r1 = block(g0, l0, link), block(g1, l1, link)
+    err_l = np.abs(l1 - l0).max()
+    err_r = np.abs(r1 - r0).max() / np.abs(r0).max()
+    print(f"[gpu] split two-site op: lambda err {err_l:.2e}, block err {err_r:.2e}", flush=True)
+    assert err_l < 1e-10 and err_r < 1e-10
+    assert err_l > 0.0 or err_r > 0.0
+
+
+def test_split_two_site_op_with_a_wide_range_of_magnitudes():
+    """The same update with Gamma_l scaled by 2^40 and Gamma_r by 2^-45 (and the other way
+    round): the slices' power-of-two scales carry any magnitude, and the new lambda is
+    normalised, so it is unchanged to rounding."""
+    z = load_golden("two_site_op_l4")
+    link = int(z["link"])
+    _, l0 = two_site(z, "fp64", z["U"][link - 1])
+    for sc in ((2.0 ** 40, 2.0 ** -45), (2.0 ** -45, 2.0 ** 40)):
+        _, l1 = two_site(z, "split", z["U"][link - 1], sc)
+        err = np.abs(l1 - l0).max()
+        print(f"[gpu] split Gamma scales {sc}: lambda err {err:.2e}", flush=True)
+        assert err < 1e-10
+
+
+def test_split_and_mixed_are_exclusive():
+    with pytest.raises(TEBDError):
+        Context(4, 2, 4, mode="mixed,split")
