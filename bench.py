@@ -71,8 +71,10 @@ OBS_TOL = 1e-8
 
 RANK_MODEL = os.path.join(ROOT, "opentebd-extended_amd", "tebd_amd", "rank_model_fh.npy")
 WORKLOAD_TAG = "fh_l128_chi256"
-# launch classes timed by the library that are one kernel each: (kernel, roofline bound)
-# (base names as rocprofv3 reports them, without template arguments)
+# launch classes timed by the library, each counted as one kernel: (kernel, roofline bound)
+# (base names as rocprofv3 reports them, without template arguments).  The theta class is
+# k_theta_split with its slicing kernels in the split mode and k_theta_gate_mixed in the
+# FP32 mode (theta_kernel).
 SINGLE_KERNEL = {"theta_gate": ("k_theta_gate_mfma", "mfma"), "jacobi_round": ("k_jacobi_round", "hbm"),
                  "qr_panel": ("k_qrp_panel_reg", "hbm"), "sketch_gemm": ("k_zgemm_mfma", "mfma"),
                  "jacobi_rotate": ("k_jacobi_rotate", "hbm"), "qr_trailing": ("k_qrp_reflect<0>", "hbm")}
@@ -351,9 +353,15 @@ def cpu_baseline(sample_bonds, threads, log):
     return res
 
 
-def roofline_of(prof, lanes):
-    dom = max(SINGLE_KERNEL, key=lambda n: prof[n][1])
-    kernel, bound = SINGLE_KERNEL[dom]
+def theta_kernel(mode):
+    return "k_theta_split" if "split" in mode else ("k_theta_gate_mixed" if "mixed" in mode else "k_theta_gate_mfma")
+
+
+def roofline_of(prof, lanes, mode=""):
+    names = dict(SINGLE_KERNEL)
+    names["theta_gate"] = (theta_kernel(mode), "mfma")
+    dom = max(names, key=lambda n: prof[n][1])
+    kernel, bound = names[dom]
     n_launch, dom_ms, dom_flops, dom_bytes = prof[dom]
     avg_ms = dom_ms / max(n_launch, 1)
     gbs = (dom_bytes / max(n_launch, 1)) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
@@ -370,11 +378,11 @@ def roofline_of(prof, lanes):
                  "flops_per_launch": dom_flops / max(n_launch, 1), "fp64_tflops": tflops,
                  "fp64_frac": tflops / FP64_PEAK_TFLOPS, "hbm_gbs": gbs, "lanes": lanes,
                  "device_ms": {n: prof[n][1] for n in prof},
-                 "per_kernel": {SINGLE_KERNEL[n][0]: {"launches": int(prof[n][0]),
+                 "per_kernel": {names[n][0]: {"launches": int(prof[n][0]),
                                                       "avg_ms": prof[n][1] / max(prof[n][0], 1),
                                                       "bytes_per_launch": prof[n][3] / max(prof[n][0], 1),
                                                       "flops_per_launch": prof[n][2] / max(prof[n][0], 1)}
-                                for n in SINGLE_KERNEL if prof[n][0] > 0}})
+                                for n in names if prof[n][0] > 0}})
     return roof
 
 
@@ -446,6 +454,28 @@ def rocprof_summary(path, window_file=None, traffic_out=None, calib_dir=None, ca
         for name, n, tot, avg in stats[:30]:
             out.append(f"| `{name}` | {n} | {tot / 1e6:.1f} | {avg / 1e3:.2f} | {100 * tot / tot_all:.1f} |")
         out.append("")
+        if window:
+            # every kernel that started inside the window, which opens at the first timed dispatch of
+            # the kernel with the most timed launches (kernels the library does not count are included)
+            ref = max(window, key=lambda k: window[k])
+            if ref in by and window[ref] <= len(by[ref]):
+                t0 = by[ref][-window[ref]][0]
+                allk = []
+                for name, v in by.items():
+                    sel = [x for x in v if x[0] >= t0]
+                    if sel:
+                        tot = sum(x[1] for x in sel)
+                        allk.append((name, len(sel), tot, tot / len(sel)))
+                allk.sort(key=lambda s: -s[2])
+                tot_w = sum(s[2] for s in allk) or 1
+                out.append(f"Every kernel dispatched from the first timed `{ref}` on (the timed steps and the "
+                           "run's closing measurements):")
+                out.append("")
+                out.append("| kernel | dispatches | total ms | average us | % |")
+                out.append("|---|---:|---:|---:|---:|")
+                for name, n, tot, avg in allk[:30]:
+                    out.append(f"| `{name}` | {n} | {tot / 1e6:.1f} | {avg / 1e3:.2f} | {100 * tot / tot_w:.1f} |")
+                out.append("")
     cnt = _read_csv("*counter_collection.csv", path)
     kern = {}
     if cnt:
@@ -550,7 +580,7 @@ def main():
     if args.roofline_only:
         roof_run = run_mode(args.mode, args.steps, args.warmup, log, rank_model=model, lanes=args.roofline_lanes,
                             tag="roofline", profile=True, graphs=graphs)
-        print(json.dumps({"roofline": roofline_of(roof_run["prof"], args.roofline_lanes),
+        print(json.dumps({"roofline": roofline_of(roof_run["prof"], args.roofline_lanes, args.mode),
                           "ms_per_step": 1e3 * roof_run["elapsed"] / args.steps, "steps": args.steps,
                           "pre_steps": roof_run["n_pre"]}), flush=True)
         return
@@ -575,7 +605,7 @@ def main():
                          lanes=args.lanes, graphs=graphs)
     roof_run = run_mode(args.mode, args.steps, args.warmup, log, pre_steps=n_sat, rank_model=model,
                         lanes=args.roofline_lanes, tag="roofline", profile=True, graphs=graphs)
-    roof = roofline_of(roof_run["prof"], args.roofline_lanes)
+    roof = roofline_of(roof_run["prof"], args.roofline_lanes, args.mode)
 
     def obs_errors(run):
         errs = {k: float(np.max(np.abs(np.asarray(run["obs"][k]) - np.asarray(ref["obs"][k])))) for k in run["obs"]}

@@ -95,3 +95,33 @@ def test_split_two_site_op_with_a_wide_range_of_magnitudes():
 def test_split_and_mixed_are_exclusive():
     with pytest.raises(TEBDError):
         Context(4, 2, 4, mode="mixed,split")
+
+
+def test_split_update_on_an_evolved_state_matches_fp64():
+    """One update of bonds 8 and 16 of an XXZ state evolved 60 steps at chi=32 (Gamma
+    entries up to ~1e11 against lambdas down to ~1e-14, the case the middle-bond balance
+    is for): the split path agrees with the FP64 path to 1e-12."""
+    from tebd_amd import models as M
+    z = load_golden("xxz_l32_chi32")
+    L, d, chi = int(z["L"]), int(z["d"]), int(z["chi"])
+    ctx = Context(L, d, chi, mode="batched")
+    ctx.upload(*M.product_state(z["coeffs"], chi))
+    ctx.set_gates(z["U"])
+    for _ in range(60):
+        ctx.trotter_step()
+    gam, lam = ctx.download(chi)
+    ctx.close()
+    for link in (8, 16):
+        out = {}
+        for mode in ("fp64", "split"):
+            c = Context(L, d, chi, mode=mode, cutoff=0.0)
+            c.upload(gam, lam)
+            c.two_site_op(link, z["U"][link - 1])
+            out[mode] = c.download(chi)
+            c.close()
+        (g0, l0), (g1, l1) = out["fp64"], out["split"]
+        r0, r1 = block(g0, l0, link), block(g1, l1, link)
+        err_l = np.abs(l1 - l0).max()
+        err_r = np.abs(r1 - r0).max() / np.abs(r0).max()
+        print(f"[gpu] split evolved state bond {link}: lambda err {err_l:.2e}, block err {err_r:.2e}", flush=True)
+        assert err_l < 1e-12 and err_r < 1e-12
