@@ -103,7 +103,7 @@ def fh_case():
 
 
 @pytest.mark.parametrize("mode,lanes", [("fp64", 1), ("batched", 1), ("batched", 3), ("batched,learned", 1),
-                                        ("batched,learned", 3)])
+                                        ("batched,learned", 3), ("batched,learned,split", 3)])
 def test_fh_trotter_step_at_1024_matches_oracle(fh_case, mode, lanes):
     c = fh_case
     print(f"[gpu] FH L=20 chi=256 step mode={mode} lanes={lanes}", flush=True)
@@ -153,7 +153,7 @@ def big_case(request):
     return p
 
 
-@pytest.mark.parametrize("mode", ["fp64", "batched", "batched,learned"])
+@pytest.mark.parametrize("mode", ["fp64", "batched", "batched,learned", "batched,learned,split"])
 def test_large_bond_updates_match_oracle(big_case, mode):
     c = big_case
     L, d, chi = c["L"], c["d"], c["chi"]

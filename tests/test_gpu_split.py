@@ -125,3 +125,36 @@ def test_split_update_on_an_evolved_state_matches_fp64():
         err_r = np.abs(r1 - r0).max() / np.abs(r0).max()
         print(f"[gpu] split evolved state bond {link}: lambda err {err_l:.2e}, block err {err_r:.2e}", flush=True)
         assert err_l < 1e-12 and err_r < 1e-12
+
+
+@pytest.mark.parametrize("d", [5, 6, 8])
+def test_split_local_dimensions_match_oracle(d):
+    """The tile shapes of d = 5 (16 x 16 (alpha, beta) blocks, 80 rows), 6 and 8 (8 x 8
+    blocks): four Trotter steps with random two-site unitaries from a random product state
+    at chi = 16, against the oracle's state vector."""
+    from scipy.linalg import expm
+    from helpers import mps_to_vector
+    from oracle.oracle import OracleState
+    from tebd_amd import models as M
+    L, chi = 5, 16
+    rng = np.random.default_rng(40 + d)
+    U = []
+    for _ in range(L - 1):
+        X = rng.standard_normal((d * d, d * d)) + 1j * rng.standard_normal((d * d, d * d))
+        U.append(expm(-0.3j * (X + X.conj().T)))
+    U = np.array(U)
+    coeffs = rng.standard_normal((L, d)) + 1j * rng.standard_normal((L, d))
+    coeffs /= np.linalg.norm(coeffs, axis=1, keepdims=True)
+    ref = OracleState.product(coeffs, chi)
+    ctx = Context(L, d, chi, mode="batched,split")
+    ctx.upload(*M.product_state(coeffs, chi))
+    ctx.set_gates(U)
+    for _ in range(4):
+        ctx.trotter_step()
+        ref.trotter_step(U)
+    g, lam = ctx.download(chi)
+    ctx.close()
+    err_l = np.abs(lam - ref.lam).max()
+    err_v = np.abs(mps_to_vector(g, lam) - mps_to_vector(ref.gam, ref.lam)).max()
+    print(f"[gpu] split d={d}: lambda err {err_l:.2e}, state err {err_v:.2e}", flush=True)
+    assert err_l < 1e-10 and err_v < 1e-10
