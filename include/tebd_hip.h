@@ -133,7 +133,9 @@ int tebd_two_site_op(tebd_ctx* ctx, int link, const double* op2, double* truncer
 /* Re-orthogonalises every bipartition (left-to-right, then right-to-left
  * sweep).  Replaces CanonicalFormAll (propagation_module.f90:617-634).  In an
  * ensemble the same bond of every chain is processed as one batch (batched Jacobi
- * SVDs and GEMMs), and the chains are spread over the lanes (tebd_set_lanes). */
+ * SVDs and GEMMs), and the chains are spread over the lanes (tebd_set_lanes).  With
+ * number conservation on it replaces CanonicalFormAllNC (propagation_module.f90:1943-1983):
+ * the number-conserving update with the identity on bonds 1 .. L-1, then L-1 .. 1. */
 int tebd_canonical_form_all(tebd_ctx* ctx);
 
 /* Re-orthogonalises one bipartition, bond `link` (1-based).  Replaces
@@ -176,7 +178,10 @@ int tebd_construct_propagators(const double* H, int L, int d, int order, char bc
  * TrotterStep and CanonicalFormAll.  steps[0..3] = {j at which stage 1 ended (max_steps
  * if it did not converge), 1 if it converged, the same for stage 2}.  The gates of the
  * context are replaced by the imaginary-time ones.  The reference's printed totals
- * (TotalNumber, TotalEnergy, TotalOneSite at each judgement) are not formed. */
+ * (TotalNumber, TotalEnergy, TotalOneSite at each judgement) are not formed.  With number
+ * conservation on it replaces ImagTimePropNC (propagation_module.f90:1986-2121): the same
+ * stages with TrotterStepNC and CanonicalFormAllNC, the labels past a stage's chi reset
+ * to 10000 as CopyLabel does. */
 int tebd_imag_time_prop(tebd_ctx* ctx, const double* H, int nh, double dtau, int chi_in, int max_steps,
                         int steps_for_judge, double conv1, double conv2, int* steps);
 
@@ -321,8 +326,8 @@ const char* tebd_last_error(const tebd_ctx* ctx);
  * conserv = NULL and tot_num < 0 switches conservation off.  One open chain only.
  * tebd_set_labels / tebd_get_labels: LabelLeft / LabelRight ((L+1) x chi ints, row l the
  * reference's link l+1, 10000 = an unused state), e.g. from ProductStateLabels
- * (Hamiltonian_tools_module.f90:124-178).  CanonicalForm and ImagTimeProp are refused while
- * conservation is on. */
+ * (Hamiltonian_tools_module.f90:124-178).  tebd_canonical_form_all and tebd_imag_time_prop
+ * then run their number-conserving forms; tebd_canonical_form (one bond) is refused. */
 int tebd_set_conserved(tebd_ctx* ctx, const int* conserv, int tot_num);
 int tebd_set_labels(tebd_ctx* ctx, const int* label_left, const int* label_right, int chi);
 int tebd_get_labels(tebd_ctx* ctx, int* label_left, int* label_right, int chi);

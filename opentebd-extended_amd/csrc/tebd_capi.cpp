@@ -1553,7 +1553,7 @@ int run_bonds_nc(tebd_ctx* c, const int* links, int nb, const zc* const* gate_pt
       order = ix;
     }
     int knew = 0;
-    while (knew < std::min(chi, sz) && ssfl[order[knew]] > kNcCut) ++knew;
+    while (knew < std::min(c->cap(), sz) && ssfl[order[knew]] > kNcCut) ++knew;
     if (knew < 1) return fail(c, TEBD_ERR_NAN, "bond " + std::to_string(link + 1) + ": no singular value above the cut");
     double renorm = 0.0;
     for (int b = 0; b < knew; ++b) renorm += ssfl[order[b]] * ssfl[order[b]];
@@ -2177,6 +2177,12 @@ int set_stage(tebd_ctx* c, int chi) {
     c->chi = nd;
     for (int l = 0; l <= L; ++l) c->last_rank[l] = std::min(c->last_rank[l], chi);
   }
+  if (c->nc)  // labels past chi reset, as CopyLabel does (TEBDtools_module.f90:623-642)
+    for (int l = 0; l <= L; ++l)
+      for (int a = chi; a < c->chi_max; ++a) {
+        c->nc_LL[(size_t)l * c->chi_max + a] = 10000;
+        c->nc_LR[(size_t)l * c->chi_max + a] = 10000;
+      }
   c->stage_chi = chi;
   return TEBD_OK;
 }
@@ -2563,10 +2569,28 @@ int canonical_form_chains(tebd_ctx* c, int r0, int dr) {
   return TEBD_OK;
 }
 
+// CanonicalFormAllNC (propagation_module.f90:1943-1983): the number-conserving update with
+// the identity on bonds 1 .. L-1, then L-1 .. 1, one bond at a time (each bond's labels and
+// Gammas come from the update before it).
+int canonical_form_all_nc(tebd_ctx* c) {
+  const int n = c->d * c->d;
+  std::vector<double> eye((size_t)2 * n * n, 0.0);
+  for (int k = 0; k < n; ++k) eye[(size_t)2 * (k + (size_t)n * k)] = 1.0;
+  zc* slot = c->gates + c->d4() * (size_t)c->nchain * c->table;
+  HIPCHK(c, hipMemcpyAsync(slot, eye.data(), sizeof(zc) * c->d4(), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  const zc* gp = slot;
+  double t = 0.0;
+  int e = TEBD_OK;
+  for (int l = 0; l < c->L - 1 && !e; ++l) e = run_bonds_nc(c, &l, 1, &gp, &t);
+  for (int l = c->L - 2; l >= 0 && !e; --l) e = run_bonds_nc(c, &l, 1, &gp, &t);
+  return e;
+}
+
 int tebd_canonical_form_all(tebd_ctx* c) {
   int e = ensure_state(c, false);
   if (e) return e;
-  if (c->nc) return fail(c, TEBD_ERR_UNSUPPORTED, "canonical_form_all: not with number conservation (CanonicalFormAllNC)");
+  if (c->nc) return canonical_form_all_nc(c);
   // the chains of an ensemble are independent: with lanes, chain r goes to lane r mod n,
   // each lane on its own stream and host thread; each chain sees the same sequence of
   // bond operations either way
@@ -2859,7 +2883,6 @@ int tebd_imag_time_prop(tebd_ctx* c, const double* H, int nh, double dtau, int c
                         int steps_for_judge, double conv1, double conv2, int* steps) {
   int e = ensure_state(c, false);
   if (e) return e;
-  if (c->nc) return fail(c, TEBD_ERR_UNSUPPORTED, "imag_time_prop: not with number conservation (ImagTimePropNC)");
   const int Lc = c->chain_len, n = c->d * c->d;
   const int per = c->bc == 'P' ? Lc : Lc - 1;  // bond Hamiltonians per chain
   if (!H || nh != c->nchain * per || !(dtau > 0.0) || chi_in < 1 || chi_in > c->chi_max || max_steps < 0 ||

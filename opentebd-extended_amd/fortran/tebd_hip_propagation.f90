@@ -14,6 +14,9 @@
 !   TrotterStepNC(Udt, Gammas, Lambdas, LabelLeft, LabelRight, totalTruncerr[, intDegFree])
 !                                             local_operations_module.f90:1845-1964,
 !                                             propagation_module.f90:890-1241
+!   CanonicalFormAllNC(Gammas, Lambdas, LabelLeft, LabelRight[, intDegFree]),
+!   ImagTimePropNC(H, Gammas, Lambdas, LabelLeft, LabelRight, chiIn[, intDegFree])
+!                                             propagation_module.f90:1943-2121
 ! on the reference's own types (TYPE(tensor), TYPE(vector), TYPE(matrix),
 ! TEBDtools_module.f90:31-45) and globals (systemSize, localSize, trotterOrder,
 ! BoundaryCond, and for imaginary time chiMax, dtITP, maxITPsteps, stepsForJudge,
@@ -41,7 +44,7 @@ USE tebd_hip
 IMPLICIT NONE
 PRIVATE
 PUBLIC :: SVDInit, SVDFinish, TwoSiteOp, TrotterStep, CanonicalFormAll, ImagTimeProp, ImagTimePropSpin, &
-          ConstructPropagators, TwoSiteOpNC, TrotterStepNC
+          ConstructPropagators, TwoSiteOpNC, TrotterStepNC, CanonicalFormAllNC, ImagTimePropNC
 PUBLIC :: tebd_hip_resident, tebd_hip_mode, tebd_hip_to_host, tebd_hip_to_device, tebd_hip_context
 
 TYPE(c_ptr), SAVE :: ctx = c_null_ptr
@@ -230,6 +233,17 @@ SUBROUTINE CanonicalFormAll(Gammas, Lambdas)
   CALL after_call(Gammas, Lambdas)
 END SUBROUTINE CanonicalFormAll
 
+SUBROUTINE CanonicalFormAllNC(Gammas, Lambdas, LabelLeft, LabelRight, intDegFree)
+  TYPE(tensor), POINTER :: Gammas(:)
+  TYPE(vector), POINTER :: Lambdas(:)
+  TYPE(vectorInt), POINTER :: LabelLeft(:), LabelRight(:)
+  INTEGER, INTENT(IN), OPTIONAL :: intDegFree
+  CALL before_call(Gammas, Lambdas)
+  CALL nc_before(LabelLeft, LabelRight, intDegFree)
+  CALL CanonicalFormAll(Gammas, Lambdas)
+  CALL nc_after(LabelLeft, LabelRight)
+END SUBROUTINE CanonicalFormAllNC
+
 ! The propagator table of the reference's ConstructPropagators for trotterOrder and
 ! BoundaryCond; U must be allocated as AllocateProp does (one entry per table slot).
 SUBROUTINE ConstructPropagators(H, U, dt)
@@ -266,11 +280,46 @@ SUBROUTINE ImagTimeProp(H, GammasOuter, LambdasOuter, chiIn, intDegFree)
   TYPE(vector), POINTER :: LambdasOuter(:)
   INTEGER, INTENT(INOUT) :: chiIn
   INTEGER, INTENT(IN), OPTIONAL :: intDegFree
+  IF (.NOT. c_associated(ctx) .OR. ctx_chi /= chiMax) CALL SVDInit(chiMax)
+  CALL tebd_hip_to_device(GammasOuter, LambdasOuter)
+  CALL itp_device(H, chiIn, 'ImagTimeProp')
+  CALL realloc_state(GammasOuter, LambdasOuter)
+  CALL tebd_hip_to_host(GammasOuter, LambdasOuter)
+  IF (PRESENT(intDegFree)) CONTINUE
+END SUBROUTINE ImagTimeProp
+
+! ImagTimePropNC: the same stages with TrotterStepNC and CanonicalFormAllNC on the device.
+! On return the Gammas, Lambdas and labels are reallocated at chiMax and hold the result,
+! as the reference leaves them.  intDegFree selects Conserv as the on-site numbers.
+SUBROUTINE ImagTimePropNC(H, GammasOuter, LambdasOuter, LabelLeftOuter, LabelRightOuter, chiIn, intDegFree)
+  TYPE(matrix), POINTER :: H(:)
+  TYPE(tensor), POINTER :: GammasOuter(:)
+  TYPE(vector), POINTER :: LambdasOuter(:)
+  TYPE(vectorInt), POINTER :: LabelLeftOuter(:), LabelRightOuter(:)
+  INTEGER, INTENT(INOUT) :: chiIn
+  INTEGER, INTENT(IN), OPTIONAL :: intDegFree
+  INTEGER :: i
+  IF (.NOT. c_associated(ctx) .OR. ctx_chi /= chiMax) CALL SVDInit(chiMax)
+  CALL tebd_hip_to_device(GammasOuter, LambdasOuter)
+  CALL nc_before(LabelLeftOuter, LabelRightOuter, intDegFree)
+  CALL itp_device(H, chiIn, 'ImagTimePropNC')
+  CALL realloc_state(GammasOuter, LambdasOuter)
+  DO i = 1, SIZE(LabelLeftOuter)
+    DEALLOCATE(LabelLeftOuter(i)%vi, LabelRightOuter(i)%vi)
+    ALLOCATE(LabelLeftOuter(i)%vi(chiMax), LabelRightOuter(i)%vi(chiMax))
+  END DO
+  CALL tebd_hip_to_host(GammasOuter, LambdasOuter)
+  CALL nc_after(LabelLeftOuter, LabelRightOuter)
+END SUBROUTINE ImagTimePropNC
+
+! tebd_imag_time_prop with the bond Hamiltonians H and the globals of system_parameters.
+SUBROUTINE itp_device(H, chiIn, what)
+  TYPE(matrix), POINTER :: H(:)
+  INTEGER, INTENT(IN) :: chiIn
+  CHARACTER(len=*), INTENT(IN) :: what
   COMPLEX(c_double_complex), ALLOCATABLE, TARGET :: hb(:, :, :)
   INTEGER(c_int), TARGET :: steps(4)
   INTEGER :: i, nb, d2
-  IF (.NOT. c_associated(ctx) .OR. ctx_chi /= chiMax) CALL SVDInit(chiMax)
-  CALL tebd_hip_to_device(GammasOuter, LambdasOuter)
   nb = SIZE(H)
   d2 = SIZE(H(1)%m, 1)
   ALLOCATE(hb(d2, d2, nb))
@@ -279,7 +328,14 @@ SUBROUTINE ImagTimeProp(H, GammasOuter, LambdasOuter, chiIn, intDegFree)
   END DO
   CALL tebd_check(ctx, tebd_imag_time_prop(ctx, c_loc(hb), INT(nb, c_int), REAL(dtITP, c_double), &
                   INT(chiIn, c_int), INT(maxITPsteps, c_int), INT(stepsForJudge, c_int), &
-                  REAL(convCriterion1, c_double), REAL(convCriterion2, c_double), c_loc(steps)), 'ImagTimeProp')
+                  REAL(convCriterion1, c_double), REAL(convCriterion2, c_double), c_loc(steps)), what)
+END SUBROUTINE itp_device
+
+! Gammas and Lambdas reallocated at chiMax (the reference's AllocateGamLam at the last stage).
+SUBROUTINE realloc_state(GammasOuter, LambdasOuter)
+  TYPE(tensor), POINTER :: GammasOuter(:)
+  TYPE(vector), POINTER :: LambdasOuter(:)
+  INTEGER :: i
   DO i = 1, SIZE(GammasOuter)
     DEALLOCATE(GammasOuter(i)%t)
     ALLOCATE(GammasOuter(i)%t(chiMax, localSize, chiMax))
@@ -288,9 +344,7 @@ SUBROUTINE ImagTimeProp(H, GammasOuter, LambdasOuter, chiIn, intDegFree)
     DEALLOCATE(LambdasOuter(i)%v)
     ALLOCATE(LambdasOuter(i)%v(chiMax))
   END DO
-  CALL tebd_hip_to_host(GammasOuter, LambdasOuter)
-  IF (PRESENT(intDegFree)) CONTINUE
-END SUBROUTINE ImagTimeProp
+END SUBROUTINE realloc_state
 
 ! ImagTimePropSpin differs from ImagTimeProp only in what it prints at each judgement.
 SUBROUTINE ImagTimePropSpin(H, GammasOuter, LambdasOuter, chiIn)

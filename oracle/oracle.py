@@ -45,6 +45,7 @@ def lib():
         L.orc_trotter_step_sched.argtypes = [I, I, I, P, P, P, I, I, ctypes.c_char, P]
         L.orc_two_site_op_nc.argtypes = [I, I, I, P, P, P, P, I, P, P, I, I, P]
         L.orc_trotter_step_nc.argtypes = [I, I, I, P, P, P, P, P, P, I, I, P]
+        L.orc_canonical_form_all_nc.argtypes = [I, I, I, P, P, P, P, P, I, I]
         _LIB = L
     return _LIB
 
@@ -148,6 +149,29 @@ class OracleState:
             raise RuntimeError(f"zgesvd info={info} (or no block)")
         return t.value
 
+    def canonical_form_all_nc(self):
+        """CanonicalFormAllNC (propagation_module.f90:1943-1983)."""
+        info = lib().orc_canonical_form_all_nc(*self._args(), *self._nc(), _p(self.conserv), self.tot, self.maxf)
+        if info != 0:
+            raise RuntimeError(f"zgesvd info={info} (or no block)")
+
+    def with_chi(self, chi):
+        """The state in arrays of bond capacity chi, entries past the smaller capacity
+        dropped or zero (CopyGamLam, TEBDtools_module.f90; labels as CopyLabel :623-642)."""
+        m = min(chi, self.chi)
+        gam = np.zeros((self.L, chi, self.d, chi), np.complex128)
+        gam[:, :m, :, :m] = self.gam[:, :m, :, :m]
+        lam = np.zeros((self.L + 1, chi))
+        lam[:, :m] = self.lam[:, :m]
+        out = OracleState(gam, lam)
+        if hasattr(self, "LL"):
+            LL = np.full((self.L + 1, chi), 10000, np.int32)
+            LR = np.full((self.L + 1, chi), 10000, np.int32)
+            LL[:, :m] = self.LL[:, :m]
+            LR[:, :m] = self.LR[:, :m]
+            out.set_conserved(LL, LR, self.conserv, self.tot)
+        return out
+
     def canonical_form_all(self):
         info = lib().orc_canonical_form_all(*self._args())
         if info != 0:
@@ -174,6 +198,36 @@ class OracleState:
         out = np.zeros(self.L - 1)
         lib().orc_chain_entropy(self.L, self.d, self.chi, _p(self.lam), _p(out))
         return out
+
+
+def imag_time_prop_nc(state, gates, chi_in, chi_max, max_steps, steps_for_judge, conv1, conv2):
+    """ImagTimePropNC (propagation_module.f90:1986-2121) from `state` (labels set, any
+    capacity) with the imaginary-time gates (ConstructPropagators with dt = -i dtITP):
+    stages chi = chi_in, chi_in + (chi_max - chi_in) (one stage when they are equal); each
+    stage copies the state into capacity chi (CopyGamLam, CopyLabel), judges every
+    steps_for_judge steps whether the largest relative change of lambda_1 over all links is
+    below its criterion (conv2 on the chi_max stage or when chi_in == chi_max, else conv1),
+    and otherwise takes TrotterStepNC and CanonicalFormAllNC.  Returns ([(steps,
+    converged)] per stage, the final state)."""
+    inc = 1 if chi_in == chi_max else chi_max - chi_in
+    stages = []
+    for chi in range(chi_in, chi_max + 1, inc):
+        conv = conv2 if (chi_in == chi_max or chi == chi_max) else conv1
+        state = state.with_chi(chi)
+        before = state.lam[:, 0].copy()
+        done = None
+        for j in range(1, max_steps + 1):
+            if j % steps_for_judge == 0:
+                after = state.lam[:, 0].copy()
+                gap = float(np.max(np.abs((before - after) / before)))
+                if gap < conv:
+                    done = j
+                    break
+                before = after
+            state.trotter_step_nc(gates)
+            state.canonical_form_all_nc()
+        stages.append((done if done is not None else max_steps, done is not None))
+    return stages, state
 
 
 def product_labels(coeffs, chi, conserv, tot_num):

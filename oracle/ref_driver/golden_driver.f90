@@ -39,6 +39,11 @@
 !            (propagation_module.f90:890-1017, TwoSiteOpNC local_operations_module.f90:1845-1964);
 !            the next line: totNum intDegFree (0: number = state index - 1, 1: the model's
 !            Conserv); the labels of every link are printed with the final state
+!   mode 7 = number-conserving imaginary time: the product state's labels as in mode 6, then
+!            ImagTimePropNC (propagation_module.f90:1986-2121: chi stages, TrotterStepNC and
+!            CanonicalFormAllNC :1943-1983 every step); next lines: totNum intDegFree, then
+!            chiIn dtITP maxITPsteps stepsForJudge convCriterion1 convCriterion2; the final
+!            state and every link's labels are printed
 !   [mode 1: link, then Gamma(chi,d,chi) re im for each site, Lambda(chi) for L+1 links]
 !   [mode 2: chiIn dtITP maxITPsteps stepsForJudge convCriterion1 convCriterion2]
 PROGRAM golden_driver
@@ -220,6 +225,29 @@ IF (mode == 2) THEN
   CALL ImagTimeProp(H, Gammas, Lambdas, chiIn)
   CALL PrintObs(0, 0.0_rKind)
   CALL PrintState()
+  STOP
+END IF
+IF (mode == 7) THEN
+  READ(11,*) totNum, intdeg
+  READ(11,*) chiIn, dtITP, maxITPsteps, stepsForJudge, convCriterion1, convCriterion2
+  CALL SVDFinish()
+  CALL AllocateLabel(LabelLeft, LabelRight, chi)
+  IF (intdeg == 1) THEN
+    CALL ProductStateLabels(LabelLeft, LabelRight, carray, 1)
+    PRINT '(A,64I4)', 'CONSERV', (Conserv%vi(k), k=1,dd)
+    CALL ImagTimePropNC(H, Gammas, Lambdas, LabelLeft, LabelRight, chiIn, 1)
+  ELSE
+    CALL ProductStateLabels(LabelLeft, LabelRight, carray)
+    CALL ImagTimePropNC(H, Gammas, Lambdas, LabelLeft, LabelRight, chiIn)
+  END IF
+  CALL PrintObs(0, 0.0_rKind)
+  CALL PrintState()
+  DO i=1,L+1
+    PRINT '(A,I6)', 'LABELL', i
+    PRINT '(1024I7)', (LabelLeft(i)%vi(k), k=1,chi)
+    PRINT '(A,I6)', 'LABELR', i
+    PRINT '(1024I7)', (LabelRight(i)%vi(k), k=1,chi)
+  END DO
   STOP
 END IF
 IF (mode == 6) THEN

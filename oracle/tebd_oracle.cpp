@@ -626,6 +626,22 @@ int orc_trotter_step_nc(int L, int d, int chi, double* G, double* lam, int* LL, 
   return info;
 }
 
+// CanonicalFormAllNC (propagation_module.f90:1943-1983): TwoSiteOpNC with the identity on
+// bonds 1 .. L-1, then L-1 .. 1.
+int orc_canonical_form_all_nc(int L, int d, int chi, double* G, double* lam, int* LL, int* LR, const int* conserv,
+                              int tot, int maxf) {
+  View v{L, d, chi, (cplx*)G, lam};
+  const Labels lb{chi, LL, LR};
+  const int d2 = d * d;
+  std::vector<cplx> eye((size_t)d2 * d2, cplx(0.0, 0.0));
+  for (int k = 0; k < d2; ++k) eye[k + (size_t)d2 * k] = cplx(1.0, 0.0);
+  int info = 0;
+  double t = 0.0;
+  for (int bond = 1; bond <= L - 1; ++bond) info |= two_site_op_nc(v, lb, bond - 1, eye.data(), conserv, tot, maxf, &t);
+  for (int bond = L - 1; bond >= 1; --bond) info |= two_site_op_nc(v, lb, bond - 1, eye.data(), conserv, tot, maxf, &t);
+  return info;
+}
+
 // TotalEnergy: sum over bonds of Re Tr(H_b rho_b); H: (L-1) d^2 x d^2 complex.
 double orc_total_energy(int L, int d, int chi, const double* G, const double* lam, const double* H) {
   View v{L, d, chi, (cplx*)G, (double*)lam};

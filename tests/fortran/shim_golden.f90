@@ -1,5 +1,6 @@
 ! Runs a golden job through the reference's call surface as the iso_c_binding shim
-! provides it (SVDInit, TrotterStep, CanonicalFormAll, SVDFinish, ImagTimeProp from
+! provides it (SVDInit, TrotterStep, CanonicalFormAll, SVDFinish, ImagTimeProp, and the
+! number-conserving TrotterStepNC and ImagTimePropNC from
 ! tebd_hip_propagation) and writes the final state, for tests/test_fortran_shim.py to
 ! compare with the reference's own outputs (tests/golden).
 !
@@ -10,9 +11,11 @@
 !       real(8) dtITP, convCriterion1, convCriterion2, then (L-1) bond Hamiltonians;
 !       kind 2 (TrotterStepNC): int32 totNum, intDegFree (0/1), Conserv(d), then the
 !       LabelLeft and LabelRight of the L+1 links (chi int32 each), then the gates as kind 0;
+!       kind 3 (ImagTimePropNC, chi = chiMax): the settings of kind 1, then those of kind 2,
+!       then (L-1) bond Hamiltonians;
 !       then L Gammas of chi x d x chi complex(8); L+1 lambdas of chi real(8)
-!   OUT: real(8) total truncation error; L Gammas; L+1 lambdas (same layout); for kind 2
-!        then the L+1 LabelLeft and L+1 LabelRight (chi int32 each)
+!   OUT: real(8) total truncation error; L Gammas; L+1 lambdas (same layout); for kinds 2
+!        and 3 then the L+1 LabelLeft and L+1 LabelRight (chi int32 each)
 PROGRAM shim_golden
 USE, INTRINSIC :: iso_c_binding
 USE system_parameters, ONLY: rKind, systemSize, localSize, trotterOrder, BoundaryCond, chiMax, dtITP, &
@@ -21,7 +24,8 @@ USE system_parameters, ONLY: totNum
 USE TEBDtools_module, ONLY: tensor, vector, matrix, vectorInt
 USE Hamiltonian_tools_module, ONLY: Conserv
 USE tebd_hip_propagation, ONLY: SVDInit, SVDFinish, TrotterStep, CanonicalFormAll, ImagTimeProp, &
-                                TrotterStepNC, tebd_hip_resident, tebd_hip_to_host, tebd_hip_mode
+                                TrotterStepNC, ImagTimePropNC, tebd_hip_resident, tebd_hip_to_host, &
+                                tebd_hip_mode
 IMPLICIT NONE
 INTEGER(c_int32_t) :: hdr(8), itp(3)
 REAL(c_double) :: itpr(3)
@@ -45,7 +49,7 @@ trotterOrder = 2
 BoundaryCond = 'O'
 tebd_hip_resident = hdr(6) /= 0
 tebd_hip_mode = INT(hdr(7), c_int)
-IF (hdr(8) == 1) THEN
+IF (hdr(8) == 1 .OR. hdr(8) == 3) THEN
   READ(uin) itp, itpr
   chiIn = itp(1)
   maxITPsteps = itp(2)
@@ -55,7 +59,7 @@ IF (hdr(8) == 1) THEN
   convCriterion2 = itpr(3)
   chiMax = chi
 END IF
-IF (hdr(8) == 2) THEN
+IF (hdr(8) == 2 .OR. hdr(8) == 3) THEN
   READ(uin) nch
   totNum = nch(1)
   ALLOCATE(Conserv%vi(d))
@@ -89,6 +93,12 @@ total = 0.0_rKind
 IF (hdr(8) == 1) THEN
   ! Udt holds the bond Hamiltonians of the job here
   CALL ImagTimeProp(Udt, Gammas, Lambdas, chiIn)
+ELSE IF (hdr(8) == 3) THEN
+  IF (nch(2) /= 0) THEN
+    CALL ImagTimePropNC(Udt, Gammas, Lambdas, LabelLeft, LabelRight, chiIn, 1)
+  ELSE
+    CALL ImagTimePropNC(Udt, Gammas, Lambdas, LabelLeft, LabelRight, chiIn)
+  END IF
 ELSE IF (hdr(8) == 2) THEN
   CALL SVDInit(chi)
   DO k = 1, nsteps
@@ -119,7 +129,7 @@ END DO
 DO i = 1, L + 1
   WRITE(uout) Lambdas(i)%v
 END DO
-IF (hdr(8) == 2) THEN
+IF (hdr(8) == 2 .OR. hdr(8) == 3) THEN
   DO i = 1, L + 1
     WRITE(uout) LabelLeft(i)%vi
   END DO

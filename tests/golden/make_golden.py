@@ -91,6 +91,8 @@ def run_itp(job):
             in_judgement = False
         elif in_judgement or t.startswith("number") or t.startswith("Sx"):
             continue  # the wrapped tail of a judgement line, and the printed totals
+        elif re.fullmatch(r"\d+", t):
+            continue  # ImagTimePropNC prints each stage's chi (propagation_module.f90:2031)
         else:
             rest.append(line)
     stages, cur = [], []
@@ -401,6 +403,49 @@ def nc():
               "kept", [(l > 0).sum() for l in res["lam"]])
 
 
+def itp_nc():
+    """Number-conserving imaginary time (mode 7: ProductStateLabels + ImagTimePropNC,
+    propagation_module.f90:1986-2121, CanonicalFormAllNC :1943-1983 every step):
+    Bose-Hubbard d=3 (maxFilling 2), J_i~U[0.8,1.2] (seed 31), U=2, from the unit-filling
+    Mott state at L=8 with chi stages 4 -> 12; Fermi-Hubbard d=4 (t=1, U=4, the model's
+    Conserv, intDegFree) from the Neel state at L=6 in one full-rank stage (chi=64: its
+    spin-degenerate spectra leave a truncated cut ambiguous).  Stored as itp_stages: the
+    judgements per stage, the final state, and every link's labels."""
+    tmp = tempfile.mkdtemp()
+    cases = {
+        "itp_nc_bh_l8": dict(model=2, L=8, param=2, chi=12, chi_in=4, consts=[2.0, 0.0, 0.0], up=1, dn=1, intdeg=0,
+                             dtau=0.1, max_steps=500, judge=10, conv1=1e-4, conv2=1e-6),
+        "itp_nc_fh_l6": dict(model=3, L=6, param=2, chi=64, chi_in=64, consts=[1.0, 4.0, 0.0], up=1, dn=2, intdeg=1,
+                             dtau=0.1, max_steps=100, judge=10, conv1=1e-4, conv2=1e-6),
+    }
+    for name, cs in cases.items():
+        L, chi, model = cs["L"], cs["chi"], cs["model"]
+        d = 3 if model == 2 else 4
+        field = np.random.default_rng(31).uniform(0.8, 1.2, L - 1) if model == 2 else None
+        c = neel(L, d, cs["up"], cs["dn"])
+        tot = int(sum(np.argmax(np.abs(c[s])) for s in range(L))) if model == 2 else L
+        params = {k: cs[k] for k in ("chi_in", "dtau", "max_steps", "judge", "conv1", "conv2")}
+        extra = (f"{tot} {cs['intdeg']}\n{cs['chi_in']} {cs['dtau']!r} {cs['max_steps']} {cs['judge']} "
+                 f"{cs['conv1']!r} {cs['conv2']!r}\n")
+        path = os.path.join(tmp, name + ".txt")
+        write_job(path, model, L, cs["param"], chi, cs["consts"], field, 0j, 0, 0, c, mode=7, extra=extra)
+        stages, blocks = run_itp(path)
+        res = collect(blocks, L, d, chi)
+        steps = [st[-1][0] for st in stages]
+        conv = [bool(st[-1][1] < st[-1][2]) for st in stages]
+        for st in stages:  # no judgement of a stage's end sits at the criterion
+            for j, t, cr in st[-2:]:
+                assert abs(t / cr - 1.0) > 1e-4, (name, j, t, cr)
+        meta = dict(model=model, L=L, param=cs["param"], chi=chi, d=d, consts=np.array(cs["consts"]), coeffs=c,
+                    tot_num=tot, intdeg=cs["intdeg"], stage_steps=np.array(steps), stage_converged=np.array(conv),
+                    judgements=np.array([[k, j, t, cr] for k, st in enumerate(stages) for j, t, cr in st]),
+                    **{k: np.array(v) for k, v in params.items()})
+        if field is not None:
+            meta["field"] = field
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **meta, **res)
+        print(name, "stages", steps, conv, "energy", res["energy"][-1], "kept", [(l > 0).sum() for l in res["lam"]])
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "checkpoint":
         checkpoint()
@@ -412,5 +457,7 @@ if __name__ == "__main__":
         schedules()
     elif len(sys.argv) > 1 and sys.argv[1] == "nc":
         nc()
+    elif len(sys.argv) > 1 and sys.argv[1] == "itp_nc":
+        itp_nc()
     else:
         main()

@@ -103,27 +103,34 @@ def write_nc_job(path, z, resident):
     c = np.asarray(z["conserv"]) if "conserv" in z else np.arange(d)
     LL, LR = M.product_labels(z["coeffs"], chi, c, int(z["tot_num"]))
     gam, lam = M.product_state(z["coeffs"], chi)
+    itp = "stage_steps" in z
     with open(path, "wb") as f:
-        np.array([L, d, chi, int(z["nsteps"]), 0, resident, 0, 2], np.int32).tofile(f)
+        np.array([L, d, chi, 0 if itp else int(z["nsteps"]), 0, resident, 0, 3 if itp else 2], np.int32).tofile(f)
+        if itp:
+            np.array([z["chi_in"], z["max_steps"], z["judge"]], np.int32).tofile(f)
+            np.array([z["dtau"], z["conv1"], z["conv2"]], np.float64).tofile(f)
         np.array([int(z["tot_num"]), int(z["intdeg"])], np.int32).tofile(f)
         np.asarray(c, np.int32).tofile(f)
         np.ascontiguousarray(LL, np.int32).tofile(f)
         np.ascontiguousarray(LR, np.int32).tofile(f)
-        np.ascontiguousarray(np.transpose(z["U"], (0, 2, 1))).astype(np.complex128).tofile(f)
+        gates = z["H"] if itp else z["U"]
+        np.ascontiguousarray(np.transpose(gates, (0, 2, 1))).astype(np.complex128).tofile(f)
         np.ascontiguousarray(gam, np.complex128).tofile(f)
         np.ascontiguousarray(lam, np.float64).tofile(f)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name,resident", [("nc_bh_l8", 0), ("nc_bh_l8", 1), ("nc_fh_l6", 0)])
+@pytest.mark.parametrize("name,resident", [("nc_bh_l8", 0), ("nc_bh_l8", 1), ("nc_fh_l6", 0),
+                                           ("itp_nc_bh_l8", 0), ("itp_nc_fh_l6", 0)])
 def test_fortran_shim_nc_reproduces_reference(tmp_path, name, resident):
-    """TrotterStepNC through the shim (with intDegFree and the model's Conserv for the
-    Hubbard chain), against the reference's own NC trajectories: the final observables,
-    lambdas and labels (Hubbard: per number sector)."""
+    """TrotterStepNC and ImagTimePropNC through the shim (with intDegFree and the model's
+    Conserv for the Hubbard chains), against the reference's own NC runs: the final
+    observables, lambdas and labels (Hubbard: per number sector)."""
     from test_oracle_nc import sector_spectra_err
     assert os.path.exists(EXE), "tests/fortran/_build/shim_golden is not built (build() makes it)"
     z = load_golden(name)
-    L, d, chi, n = int(z["L"]), int(z["d"]), int(z["chi"]), int(z["nsteps"])
+    itp = "stage_steps" in z
+    L, d, chi, n = int(z["L"]), int(z["d"]), int(z["chi"]), -1 if itp else int(z["nsteps"])
     job, out = str(tmp_path / "job.bin"), str(tmp_path / "out.bin")
     write_nc_job(job, z, resident)
     print(f"[gpu] fortran shim NC {name} resident={resident}", flush=True)
@@ -134,8 +141,9 @@ def test_fortran_shim_nc_reproduces_reference(tmp_path, name, resident):
     total, gam, lam = read_out_prefix(raw, L, d, chi)
     labs = raw[8 + 16 * ng + 8 * (L + 1) * chi:].view(np.int32).reshape(2, L + 1, chi)
     err = max_obs_err(observables(gam, lam, z["H"], z["szop"], z["nop"]), golden_obs(z, n))
-    err["trunc"] = abs(total - float(np.sum(z["trunc"][1:n + 1])))
-    err["lambda"] = float(np.max(np.abs(lam - z["lam"])))
+    if not itp:
+        err["trunc"] = abs(total - float(np.sum(z["trunc"][1:n + 1])))
+        err["lambda"] = float(np.max(np.abs(lam - z["lam"])))
     err["sectors"] = sector_spectra_err(labs[0], lam, z["labl"], z["lam"])
     print(f"[gpu] {name}: {err}", flush=True)
     assert max(err.values()) < TOL, err

@@ -10,7 +10,12 @@ propagation_module.f90:890-1241).
     batched mode;
   * a larger chain against the oracle: Bose-Hubbard L=16 d=3 chi=48, 8 steps, bond
     matrices up to 144 x 144 in up to 13 blocks -- observables within 1e-9;
-  * one two-site update through tebd_two_site_op, and the refusals.
+  * one two-site update through tebd_two_site_op, and the refusals;
+  * CanonicalFormAllNC against the oracle, and ImagTimePropNC against the reference's own
+    runs (tests/golden/itp_nc_*.npz): Bose-Hubbard L=8 with chi stages 4 -> 12 (both
+    converge, at j=60 and j=100) and Fermi-Hubbard L=6 at full rank (100 steps) -- the same
+    judgement steps, the final observables within 1e-9, every link's sector spectra and
+    labels.
 """
 import numpy as np
 import pytest
@@ -119,7 +124,68 @@ def test_device_two_site_op_nc_and_refusals():
         LL, LR = ctx.labels(chi)
         assert np.array_equal(LL, ref.LL) and np.array_equal(LR, ref.LR)
         with pytest.raises(TEBDError):
-            ctx.canonical_form_all()
+            ctx.canonical_form(3)
     with Context(L, 3, chi, mode="batched", bc="P") as ctx:
         with pytest.raises(TEBDError):
             ctx.set_conserved(c, tot)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "batched"])
+def test_device_canonical_form_all_nc_matches_oracle(mode):
+    """CanonicalFormAllNC on a truncated, non-canonical NC state (three imaginary-time NC
+    steps of Bose-Hubbard L=8 at chi=12): lambdas, state vector and labels as the
+    oracle's."""
+    from test_oracle_nc import itp_nc_start, propagators
+    z = load_golden("itp_nc_bh_l8")
+    L, d, chi = int(z["L"]), int(z["d"]), int(z["chi"])
+    c, tot = conserv_of(z), int(z["tot_num"])
+    gates = propagators(z["H"], -0.3j)
+    ref = itp_nc_start(z, chi)
+    print(f"[gpu] CanonicalFormAllNC mode={mode}", flush=True)
+    with Context(L, d, chi, mode=mode) as ctx:
+        ctx.upload(*M.product_state(z["coeffs"], chi))
+        ctx.set_conserved(c, tot, labels=M.product_labels(z["coeffs"], chi, c, tot))
+        ctx.set_gates(gates)
+        for _ in range(3):
+            ctx.trotter_step()
+            ref.trotter_step_nc(gates)
+        ctx.canonical_form_all()
+        ref.canonical_form_all_nc()
+        g, l = ctx.download(chi)
+        LL, LR = ctx.labels(chi)
+    err_l = float(np.max(np.abs(l - ref.lam)))
+    err_v = float(np.max(np.abs(mps_to_vector(g, l) - mps_to_vector(ref.gam, ref.lam))))
+    print(f"[gpu] lambda err {err_l:.2e}, state err {err_v:.2e}", flush=True)
+    assert err_l < 1e-12 and err_v < 1e-12
+    assert np.array_equal(LL, ref.LL) and np.array_equal(LR, ref.LR)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "batched"])
+@pytest.mark.parametrize("name", ["itp_nc_bh_l8", "itp_nc_fh_l6"])
+def test_device_imag_time_prop_nc_reproduces_reference(name, mode):
+    from tebd_amd import imag_time_prop
+    z = load_golden(name)
+    L, d, chi = int(z["L"]), int(z["d"]), int(z["chi"])
+    c, tot = conserv_of(z), int(z["tot_num"])
+    print(f"[gpu] ImagTimePropNC {name} mode={mode}", flush=True)
+    with Context(L, d, chi, mode=mode) as ctx:
+        ctx.upload(*M.product_state(z["coeffs"], chi))
+        ctx.set_conserved(c, tot, labels=M.product_labels(z["coeffs"], chi, c, tot))
+        stages = imag_time_prop(ctx, z["H"], float(z["dtau"]), chi_in=int(z["chi_in"]),
+                                max_steps=int(z["max_steps"]), steps_for_judge=int(z["judge"]),
+                                conv1=float(z["conv1"]), conv2=float(z["conv2"]))
+        g, l = ctx.download(chi)
+        LL, LR = ctx.labels(chi)
+    print(f"[gpu] stages {stages} (reference {list(z['stage_steps'])})", flush=True)
+    assert [s for s, _ in stages] == list(z["stage_steps"])
+    assert [b for _, b in stages] == list(z["stage_converged"])
+    st = OracleState(g, l)
+    err = {"E": abs(st.total_energy(z["H"]) - z["energy"][-1]),
+           "n": float(np.max(np.abs(st.one_site_expval(z["nop"]) - z["n"][-1]))),
+           "sz": float(np.max(np.abs(st.one_site_expval(z["szop"]) - z["sz"][-1]))),
+           "S": float(np.max(np.abs(st.chain_entropy() - z["S"][-1]))),
+           "sectors": sector_spectra_err(LL, l, z["labl"], z["lam"])}
+    print(f"[gpu] errors {err}", flush=True)
+    assert max(err.values()) < 1e-9, err
+    assert np.array_equal(np.sort(LL, axis=1), np.sort(z["labl"], axis=1))
+    assert np.array_equal(np.sort(LR, axis=1), np.sort(z["labr"], axis=1))
