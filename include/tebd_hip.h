@@ -313,17 +313,18 @@ int tebd_sync(tebd_ctx* ctx);
 const char* tebd_last_error(const tebd_ctx* ctx);
 
 /* ---- number conservation (U(1) block-sparse update) ----------------------------
- * Replaces TwoSiteOpNC / TrotterStepNC (local_operations_module.f90:748-1964,
- * propagation_module.f90:890-1017, 1190-1241).  With conservation on, every two-site
- * update of tebd_two_site_op and tebd_trotter_step (orders 2 and 5, open boundaries) keeps
- * the numbers of the Schmidt states: theta is block diagonal in the number on the left,
+ * Replaces TwoSiteOpNC / TrotterStepNC / SwappingNC (local_operations_module.f90:748-2082,
+ * propagation_module.f90:890-1942).  With conservation on, every two-site update of
+ * tebd_two_site_op and tebd_trotter_step (orders 2 and 5, open or periodic boundaries; the
+ * periodic schedules' swaps as SwappingNC, without normalising theta) keeps the numbers of
+ * the Schmidt states: theta is block diagonal in the number on the left,
  * each block is factorised on its own (tasks of the batched SVD), the values of all blocks
  * are ordered together, states at or below 10.0**(-10) (the reference's single precision
  * literal) are dropped, and the labels of the new states are updated as UpdateLabelLeft /
  * UpdateLabelRight do.
  * tebd_set_conserved: conserv[i] the number of local state i (NULL: i, the reference
  * without intDegFree; the model's Conserv with it), tot_num the total (totNum);
- * conserv = NULL and tot_num < 0 switches conservation off.  One open chain only.
+ * conserv = NULL and tot_num < 0 switches conservation off.  One chain only (no ensemble).
  * tebd_set_labels / tebd_get_labels: LabelLeft / LabelRight ((L+1) x chi ints, row l the
  * reference's link l+1, 10000 = an unused state), e.g. from ProductStateLabels
  * (Hamiltonian_tools_module.f90:124-178).  tebd_canonical_form_all and tebd_imag_time_prop

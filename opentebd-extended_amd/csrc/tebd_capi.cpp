@@ -1364,8 +1364,11 @@ int nc_blocks(tebd_ctx* c, int link, NcBondBlocks& B) {
 // batched SVD (in chunks of ntask_max work slots), the singular values of every block of a
 // bond ordered together (Ordering, JudgePosition), the labels, lambda and truncation error
 // as UpdateLabelLeft / UpdateLabelRight / FormLambdaNC set them, and the Gammas written from
-// the kept states' block vectors.
-int run_bonds_nc(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
+// the kept states' block vectors.  renorm_theta = false skips the normalisation of theta
+// before the cut, as SwappingNC does (local_operations_module.f90:1966-2082: no
+// RenormThetaNC); the swaps of the periodic schedules run that way.
+int run_bonds_nc(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum,
+                 bool renorm_theta) {
   const int d = c->d, chi = c->chi_max, nt = c->ntask_max;
   if (nb > nt) return fail(c, TEBD_ERR_ARG, "number-conserving update: batch larger than the workspace");
   int e = ensure_nc_workspace(c);
@@ -1537,7 +1540,7 @@ int run_bonds_nc(tebd_ctx* c, const int* links, int nb, const zc* const* gate_pt
       }
     }
     if (!(wsum > 0.0)) return fail(c, TEBD_ERR_NAN, "bond " + std::to_string(link + 1) + ": theta is zero in every number block");
-    const double inv = 1.0 / std::sqrt(wsum);
+    const double inv = renorm_theta ? 1.0 / std::sqrt(wsum) : 1.0;
     for (double& x : ssfl) x *= inv;
     // Ordering: backwards, the first occurrence of the smallest remaining value each time
     const int sz = (int)ssfl.size();
@@ -1617,7 +1620,7 @@ int run_bonds_nc(tebd_ctx* c, const int* links, int nb, const zc* const* gate_pt
 }
 
 int run_bonds(tebd_ctx* c, const int* links, int nb, const zc* const* gate_ptrs, double* trunc_sum) {
-  if (c->nc) return run_bonds_nc(c, links, nb, gate_ptrs, trunc_sum);
+  if (c->nc) return run_bonds_nc(c, links, nb, gate_ptrs, trunc_sum, true);
   if (c->mode & TEBD_MODE_LEARNED_RANK) return run_bonds_learned(c, links, nb, gate_ptrs, trunc_sum);
   return run_bonds_full(c, links, nb, gate_ptrs, trunc_sum);
 }
@@ -2482,11 +2485,18 @@ int tebd_trotter_step(tebd_ctx* c, double* truncerr) {
     total += t;
     return r;
   };
+  // a swap: with number conservation SwappingNC, which keeps theta unnormalised
+  auto swp = [&](int link) {
+    double t = 0.0;
+    int r = c->nc ? run_bonds_nc(c, &link, 1, &swap, &t, false) : run_bonds(c, &link, 1, &swap, &t);
+    total += t;
+    return r;
+  };
   for (const Stage& stg : trotter_schedule(Lc, c->order, c->bc)) {
     if (stg.kind == 'E') {  // site L next to site 1, the edge gate on bond 1, and back
-      for (int i = Lc - 1; i >= 2 && !e; --i) e = one(i - 1, swap);
+      for (int i = Lc - 1; i >= 2 && !e; --i) e = swp(i - 1);
       if (!e) e = one(0, c->gates + c->d4() * (size_t)gate_index(Lc, c->order, c->bc, Lc, stg.slot));
-      for (int i = 2; i <= Lc - 1 && !e; ++i) e = one(i - 1, swap);
+      for (int i = 2; i <= Lc - 1 && !e; ++i) e = swp(i - 1);
       if (e) return e;
       continue;
     }
@@ -2504,7 +2514,7 @@ int tebd_trotter_step(tebd_ctx* c, double* truncerr) {
       for (size_t b0 = 0; b0 < links.size(); b0 += c->ntask_max) {
         const int nb = (int)std::min(links.size() - b0, (size_t)c->ntask_max);
         double t = 0.0;
-        e = c->nc ? run_bonds_nc(c, links.data() + b0, nb, gp.data() + b0, &t)
+        e = c->nc ? run_bonds_nc(c, links.data() + b0, nb, gp.data() + b0, &t, true)
                   : run_bonds_split(c, links.data() + b0, nb, gp.data() + b0, &t);
         if (e) return e;
         total += t;
@@ -2582,8 +2592,8 @@ int canonical_form_all_nc(tebd_ctx* c) {
   const zc* gp = slot;
   double t = 0.0;
   int e = TEBD_OK;
-  for (int l = 0; l < c->L - 1 && !e; ++l) e = run_bonds_nc(c, &l, 1, &gp, &t);
-  for (int l = c->L - 2; l >= 0 && !e; --l) e = run_bonds_nc(c, &l, 1, &gp, &t);
+  for (int l = 0; l < c->L - 1 && !e; ++l) e = run_bonds_nc(c, &l, 1, &gp, &t, true);
+  for (int l = c->L - 2; l >= 0 && !e; --l) e = run_bonds_nc(c, &l, 1, &gp, &t, true);
   return e;
 }
 
@@ -2720,8 +2730,7 @@ int tebd_set_conserved(tebd_ctx* c, const int* conserv, int tot_num) {
     return TEBD_OK;
   }
   if (tot_num < 0) return fail(c, TEBD_ERR_ARG, "set_conserved: negative total");
-  if (c->bc != 'O' || c->nchain > 1)
-    return fail(c, TEBD_ERR_ARG, "set_conserved: number conservation needs one open chain");
+  if (c->nchain > 1) return fail(c, TEBD_ERR_ARG, "set_conserved: number conservation needs a single chain");
   std::vector<int> cv(c->d);
   int maxf = 0;
   for (int i = 0; i < c->d; ++i) {

@@ -46,6 +46,7 @@ def lib():
         L.orc_two_site_op_nc.argtypes = [I, I, I, P, P, P, P, I, P, P, I, I, P]
         L.orc_trotter_step_nc.argtypes = [I, I, I, P, P, P, P, P, P, I, I, P]
         L.orc_canonical_form_all_nc.argtypes = [I, I, I, P, P, P, P, P, I, I]
+        L.orc_trotter_step_nc_sched.argtypes = [I, I, I, P, P, P, P, P, I, I, ctypes.c_char, P, I, I, P]
         _LIB = L
     return _LIB
 
@@ -147,6 +148,17 @@ class OracleState:
                                          ctypes.byref(t))
         if info != 0:
             raise RuntimeError(f"zgesvd info={info} (or no block)")
+        return t.value
+
+    def trotter_step_nc_sched(self, gates, order=2, bc="O"):
+        """TrotterStepNC of any of the reference's schedules (order 2 or 5, 'O' or 'P')."""
+        g = _gate_stack(gates, self.d)
+        t = ctypes.c_double()
+        info = lib().orc_trotter_step_nc_sched(*self._args(), *self._nc(), _p(g), g.shape[0], int(order),
+                                               bc.encode()[:1], _p(self.conserv), self.tot, self.maxf,
+                                               ctypes.byref(t))
+        if info != 0:
+            raise RuntimeError(f"zgesvd info={info} (or no block, or a gate table of the wrong size)")
         return t.value
 
     def canonical_form_all_nc(self):

@@ -44,6 +44,11 @@
 !            CanonicalFormAllNC :1943-1983 every step); next lines: totNum intDegFree, then
 !            chiIn dtITP maxITPsteps stepsForJudge convCriterion1 convCriterion2; the final
 !            state and every link's labels are printed
+!   mode 8 = number-conserving trajectory as mode 6 under the Trotter order and boundary of
+!            the next line (as mode 3; TrotterStep5thOrderNC, TrotterStep2ndOrderPBCNC,
+!            TrotterStep5thOrderPBCNC, propagation_module.f90:1019-1942, with SwappingNC
+!            local_operations_module.f90:1966-2082); for a periodic Bose-Hubbard chain a
+!            further line holds the edge tunnelling J_L; then the line totNum intDegFree
 !   [mode 1: link, then Gamma(chi,d,chi) re im for each site, Lambda(chi) for L+1 links]
 !   [mode 2: chiIn dtITP maxITPsteps stepsForJudge convCriterion1 convCriterion2]
 PROGRAM golden_driver
@@ -104,7 +109,7 @@ ELSE
   nf = 0
 END IF
 dd = localSize
-ALLOCATE(field(MAX(nf,1)))
+ALLOCATE(field(MAX(nf,L)))
 field = 0.0_rKind
 IF (nf > 0) READ(11,*) (field(i), i=1,nf)
 READ(11,*) dtre, dtim, nsteps, canon
@@ -117,10 +122,11 @@ DO i=1,L
 END DO
 READ(11,*) mode
 nH = L-1
-IF (mode == 3 .OR. mode == 5) THEN
+IF (mode == 3 .OR. mode == 5 .OR. mode == 8) THEN
   READ(11,*) trotterOrder, bcc
   BoundaryCond = bcc
   IF (BoundaryCond == 'P') nH = L
+  IF (mode == 8 .AND. BoundaryCond == 'P' .AND. model == 2) READ(11,*) field(L)
 END IF
 
 CALL AllocateOps(H, nH, dd*dd)
@@ -250,7 +256,7 @@ IF (mode == 7) THEN
   END DO
   STOP
 END IF
-IF (mode == 6) THEN
+IF (mode == 6 .OR. mode == 8) THEN
   READ(11,*) totNum, intdeg
   CALL SVDFinish()  ! the NC update allocates its own SVD workspace per block (SVDInitNC)
   CALL AllocateLabel(LabelLeft, LabelRight, chi)

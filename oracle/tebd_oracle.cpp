@@ -217,7 +217,7 @@ std::vector<int> nc_ordering(const std::vector<double>& v) {
 // TwoSiteOpNC_c (local_operations_module.f90:1845-1964), link 0-based.  conserv[i]: the
 // number of local state i (0-based); maxf: maxFilling; tot: totNum.
 int two_site_op_nc(const View& v, const Labels& lb, int link, const cplx* Uop, const int* conserv, int tot, int maxf,
-                   double* truncerr) {
+                   double* truncerr, bool renorm_theta = true) {
   const int chi = v.chi, d = v.d;
   const double* l0 = v.lm(link);
   const double* l1 = v.lm(link + 1);
@@ -261,7 +261,7 @@ int two_site_op_nc(const View& v, const Labels& lb, int link, const cplx* Uop, c
           }
         }
   theta.swap(tmp);
-  // RenormThetaNC (:1046-1142)
+  // RenormThetaNC (:1046-1142); SwappingNC (:1966-2082) leaves theta as it is
   auto in_sector = [&](int a, int i, int j, int g) {
     const int lN = lb.left(link, a), rN = lb.right(link + 2, g);
     const int n2 = tot - lN - rN;
@@ -273,7 +273,7 @@ int two_site_op_nc(const View& v, const Labels& lb, int link, const cplx* Uop, c
       for (int i = 0; i < d; ++i)
         for (int j = 0; j < d; ++j)
           if (in_sector(a, i, j, g)) nsq += std::conj(th(a, i, j, g)) * th(a, i, j, g);
-  const double nrm = std::sqrt(nsq.real());
+  const double nrm = renorm_theta ? std::sqrt(nsq.real()) : 1.0;
   for (int a = 0; a < chi; ++a)
     for (int g = 0; g < chi; ++g)
       for (int i = 0; i < d; ++i)
@@ -526,34 +526,25 @@ int orc_trotter_step(int L, int d, int chi, double* G, double* lam, const double
   return info;
 }
 
-// One TrotterStep of any of the reference's schedules (propagation_module.f90:42-614):
-// order 2 or 5, bc 'O' or 'P'; gates = the propagator table of NEWConstructPropagators
-// (TEBDtools_module.f90:1079-1193) for that order and boundary, ngates entries of
-// d^2 x d^2 complex, entry k at gates + k d^4 (1-based index k+1 in the reference).
-int orc_trotter_step_sched(int L, int d, int chi, double* G, double* lam, const double* gates, int ngates,
-                           int order, char bc, double* total_truncerr) {
-  View v{L, d, chi, (cplx*)G, lam};
-  const size_t gs = (size_t)d * d * d * d;
-  const cplx* Ug = (const cplx*)gates;
-  double total = 0.0, t = 0.0;
-  int info = 0;
-  const bool pbc = bc == 'P' || bc == 'p';
-  const int want = order == 2 ? (pbc ? L : L - 1) : (!pbc ? 2 * (L - 1) : (L % 2 == 0 ? 2 * L : 3 * L));
-  if ((order != 2 && order != 5) || ngates != want) return -1;
-  // gate of bond i (1-based) in the reference's 1-based table slot k
-  auto op = [&](int i, int k) {
-    info |= two_site_op(v, i - 1, Ug + gs * (k - 1), &t);
-    total += t;
-  };
+}  // extern "C"
+
+// The stage sequence of one TrotterStep of any of the reference's schedules
+// (propagation_module.f90:42-614; the number-conserving ones :890-1942 follow the same
+// sequence): order 2 or 5, open or periodic.  op(i, k): the gate in the reference's
+// 1-based table slot k on bond i (1-based); swp(i): the swap of the sites of bond i.
+// Returns false for an unknown schedule.
+template <class Op, class Swp>
+bool run_schedule(int L, int order, bool pbc, Op op, Swp swp) {
+  if (order != 2 && order != 5) return false;
   // every odd (first = 1) or even (first = 2) bond, slot k(i)
   auto sweep = [&](int first, auto slot) {
     for (int i = first; i <= L - 1; i += 2) op(i, slot(i));
   };
   // site L next to site 1 by swaps L-1 .. 2, the edge gate on bond 1, swaps 2 .. L-1 back
   auto edge = [&](int k) {
-    for (int i = L - 1; i >= 2; --i) { info |= two_site_op(v, i - 1, nullptr, &t, true); total += t; }
+    for (int i = L - 1; i >= 2; --i) swp(i);
     op(1, k);
-    for (int i = 2; i <= L - 1; ++i) { info |= two_site_op(v, i - 1, nullptr, &t, true); total += t; }
+    for (int i = 2; i <= L - 1; ++i) swp(i);
   };
   auto s1 = [](int i) { return i; };
   if (order == 2 && !pbc) {  // :82-114
@@ -583,6 +574,70 @@ int orc_trotter_step_sched(int L, int d, int chi, double* G, double* lam, const 
     edge(e + 2); sweep(2, ev(3)); sweep(1, o2); sweep(2, ev(1)); edge(e + 1); sweep(2, ev(2)); edge(e + 2);
     sweep(2, ev(2)); edge(e + 1); sweep(2, ev(1)); sweep(1, o1);
   }
+  return true;
+}
+
+int table_size(int L, int order, bool pbc) {
+  return order == 2 ? (pbc ? L : L - 1) : (!pbc ? 2 * (L - 1) : (L % 2 == 0 ? 2 * L : 3 * L));
+}
+
+extern "C" {
+
+// One TrotterStep of any of the reference's schedules: order 2 or 5, bc 'O' or 'P';
+// gates = the propagator table of NEWConstructPropagators (TEBDtools_module.f90:1079-1193)
+// for that order and boundary, ngates entries of d^2 x d^2 complex, entry k at
+// gates + k d^4 (1-based index k+1 in the reference).  Swaps: Swapping (:659-695).
+int orc_trotter_step_sched(int L, int d, int chi, double* G, double* lam, const double* gates, int ngates,
+                           int order, char bc, double* total_truncerr) {
+  View v{L, d, chi, (cplx*)G, lam};
+  const size_t gs = (size_t)d * d * d * d;
+  const cplx* Ug = (const cplx*)gates;
+  double total = 0.0, t = 0.0;
+  int info = 0;
+  const bool pbc = bc == 'P' || bc == 'p';
+  if ((order != 2 && order != 5) || ngates != table_size(L, order, pbc)) return -1;
+  run_schedule(
+      L, order, pbc,
+      [&](int i, int k) {
+        info |= two_site_op(v, i - 1, Ug + gs * (k - 1), &t);
+        total += t;
+      },
+      [&](int i) {
+        info |= two_site_op(v, i - 1, nullptr, &t, true);
+        total += t;
+      });
+  *total_truncerr = total;
+  return info;
+}
+
+// TrotterStepNC under any schedule (TrotterStep5thOrderNC, TrotterStep2ndOrderPBCNC,
+// TrotterStep5thOrderPBCNC, propagation_module.f90:1019-1942): TwoSiteOpNC with the table's
+// gates, and SwappingNC (local_operations_module.f90:1966-2082) for the swaps -- the
+// number-conserving update with the SWAP gate and without RenormThetaNC.
+int orc_trotter_step_nc_sched(int L, int d, int chi, double* G, double* lam, int* LL, int* LR, const double* gates,
+                              int ngates, int order, char bc, const int* conserv, int tot, int maxf,
+                              double* total_truncerr) {
+  View v{L, d, chi, (cplx*)G, lam};
+  const Labels lb{chi, LL, LR};
+  const size_t gs = (size_t)d * d * d * d;
+  const cplx* Ug = (const cplx*)gates;
+  const bool pbc = bc == 'P' || bc == 'p';
+  if ((order != 2 && order != 5) || ngates != table_size(L, order, pbc)) return -1;
+  std::vector<cplx> sw(gs, cplx(0.0));  // SWAP(i d + j, k d + l) = delta(i, l) delta(j, k)
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) sw[(i * d + j) + (size_t)d * d * (j * d + i)] = 1.0;
+  double total = 0.0, t = 0.0;
+  int info = 0;
+  run_schedule(
+      L, order, pbc,
+      [&](int i, int k) {
+        info |= two_site_op_nc(v, lb, i - 1, Ug + gs * (k - 1), conserv, tot, maxf, &t);
+        total += t;
+      },
+      [&](int i) {
+        info |= two_site_op_nc(v, lb, i - 1, sw.data(), conserv, tot, maxf, &t, false);
+        total += t;
+      });
   *total_truncerr = total;
   return info;
 }

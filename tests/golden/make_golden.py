@@ -403,6 +403,33 @@ def nc():
               "kept", [(l > 0).sum() for l in res["lam"]])
 
 
+def nc_sched():
+    """Number-conserving trajectories under the other Trotter schedules (mode 8):
+    TrotterStep5thOrderNC on an open chain, TrotterStep2ndOrderPBCNC and
+    TrotterStep5thOrderPBCNC (SwappingNC moves site L next to site 1 and back) on periodic
+    chains of even and odd L.  Bose-Hubbard d=3 (maxFilling 2), J_i~U[0.8,1.2] (seed 31;
+    the edge J_L the next draw), U=2, from the unit-filling Mott state; chi=12 truncates."""
+    tmp = tempfile.mkdtemp()
+    cases = {"nc_sched_o5_obc_l8": (8, 12, 5, "O", 10), "nc_sched_o2_pbc_l8": (8, 12, 2, "P", 12),
+             "nc_sched_o2_pbc_l7": (7, 12, 2, "P", 12), "nc_sched_o5_pbc_l8": (8, 12, 5, "P", 6),
+             "nc_sched_o5_pbc_l7": (7, 12, 5, "P", 6)}
+    for name, (L, chi, order, bc, nsteps) in cases.items():
+        d = 3
+        field = np.random.default_rng(31).uniform(0.8, 1.2, L)
+        c = neel(L, d, 1, 1)
+        tot = L
+        extra = f"{order} {bc}\n" + (f"{float(field[L - 1])!r}\n" if bc == "P" else "") + f"{tot} 0\n"
+        path = os.path.join(tmp, name + ".txt")
+        write_job(path, 2, L, 2, chi, [2.0, 0.0, 0.0], field[:L - 1], 0.05 + 0j, nsteps, 0, c, mode=8, extra=extra)
+        res = collect(run(path), L, d, chi)
+        meta = dict(model=2, L=L, param=2, chi=chi, d=d, consts=np.array([2.0, 0.0, 0.0]), dt=np.complex128(0.05),
+                    nsteps=nsteps, canon=0, coeffs=c, tot_num=tot, intdeg=0, order=order, bc=bc,
+                    field=field if bc == "P" else field[:L - 1])
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **meta, **res)
+        print(name, "energy[-1]=", res["energy"][-1], "sum trunc=", np.sum(res["trunc"]), "gates", len(res["U"]),
+              "kept", [(l > 0).sum() for l in res["lam"]])
+
+
 def itp_nc():
     """Number-conserving imaginary time (mode 7: ProductStateLabels + ImagTimePropNC,
     propagation_module.f90:1986-2121, CanonicalFormAllNC :1943-1983 every step):
@@ -459,5 +486,7 @@ if __name__ == "__main__":
         nc()
     elif len(sys.argv) > 1 and sys.argv[1] == "itp_nc":
         itp_nc()
+    elif len(sys.argv) > 1 and sys.argv[1] == "nc_sched":
+        nc_sched()
     else:
         main()

@@ -11,6 +11,9 @@ propagation_module.f90:890-1241).
   * a larger chain against the oracle: Bose-Hubbard L=16 d=3 chi=48, 8 steps, bond
     matrices up to 144 x 144 in up to 13 blocks -- observables within 1e-9;
   * one two-site update through tebd_two_site_op, and the refusals;
+  * the other schedules (TrotterStep5thOrderNC, the periodic TrotterStep2ndOrderPBCNC /
+    TrotterStep5thOrderPBCNC with SwappingNC) against the reference's own runs
+    (tests/golden/nc_sched_*.npz) within 1e-10, labels exactly;
   * CanonicalFormAllNC against the oracle, and ImagTimePropNC against the reference's own
     runs (tests/golden/itp_nc_*.npz): Bose-Hubbard L=8 with chi stages 4 -> 12 (both
     converge, at j=60 and j=100) and Fermi-Hubbard L=6 at full rank (100 steps) -- the same
@@ -23,7 +26,7 @@ import pytest
 from conftest import load_golden
 from helpers import mps_to_vector
 from oracle.oracle import OracleState, product_labels
-from tebd_amd import Context, TEBDError, models as M
+from tebd_amd import Context, Ensemble, TEBDError, models as M
 from test_oracle_nc import conserv_of, sector_spectra_err
 
 pytestmark = pytest.mark.gpu
@@ -125,7 +128,7 @@ def test_device_two_site_op_nc_and_refusals():
         assert np.array_equal(LL, ref.LL) and np.array_equal(LR, ref.LR)
         with pytest.raises(TEBDError):
             ctx.canonical_form(3)
-    with Context(L, 3, chi, mode="batched", bc="P") as ctx:
+    with Ensemble(2, L, 3, chi, mode="batched") as ctx:
         with pytest.raises(TEBDError):
             ctx.set_conserved(c, tot)
 
@@ -189,3 +192,30 @@ def test_device_imag_time_prop_nc_reproduces_reference(name, mode):
     assert max(err.values()) < 1e-9, err
     assert np.array_equal(np.sort(LL, axis=1), np.sort(z["labl"], axis=1))
     assert np.array_equal(np.sort(LR, axis=1), np.sort(z["labr"], axis=1))
+
+
+@pytest.mark.parametrize("mode", ["fp64", "batched"])
+@pytest.mark.parametrize("name", ["nc_sched_o5_obc_l8", "nc_sched_o2_pbc_l8", "nc_sched_o2_pbc_l7",
+                                  "nc_sched_o5_pbc_l8", "nc_sched_o5_pbc_l7"])
+def test_device_nc_schedules_reproduce_reference(name, mode):
+    from test_oracle_nc import nc_sched_errors
+    z = load_golden(name)
+    L, d, chi, order, bc = int(z["L"]), int(z["d"]), int(z["chi"]), int(z["order"]), str(z["bc"])
+    c, tot = conserv_of(z), int(z["tot_num"])
+    print(f"[gpu] NC schedule {name} mode={mode}", flush=True)
+    with Context(L, d, chi, mode=mode, order=order, bc=bc) as ctx:
+        ctx.upload(*M.product_state(z["coeffs"], chi))
+        ctx.set_gates(z["U"])
+        ctx.set_conserved(c, tot, labels=M.product_labels(z["coeffs"], chi, c, tot))
+
+        def steps():
+            for _ in range(int(z["nsteps"])):
+                tr = ctx.trotter_step()
+                g, lam = ctx.download(chi)
+                yield tr, g, lam
+
+        worst = nc_sched_errors(z, steps)
+        LL, LR = ctx.labels(chi)
+    print(f"[gpu] {name} {mode}: {worst}", flush=True)
+    assert max(worst.values()) < 1e-10, worst
+    assert np.array_equal(LL, z["labl"]) and np.array_equal(LR, z["labr"])
