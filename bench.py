@@ -676,7 +676,9 @@ def main():
                          "mean_sketch_width": st["sketch_cols"] / max(st["sketch_accepted"] + st["fallbacks"], 1)},
         "jacobi": {"sweeps_per_half_step_per_lane": st["jacobi_sweeps"] / (half_steps * max(args.lanes, 1)),
                    "pair_visits": st["pair_visits"], "grams_formed": st["pair_grams"],
-                   "pairs_rotated": st["pair_rotations"]},
+                   "pairs_rotated": st["pair_rotations"],
+                   "rotations_by_sweep": [int(x) for x in np.trim_zeros(st["sweep_rotations"], "b")],
+                   "solves_by_sweep_count": [int(x) for x in np.trim_zeros(st["solve_sweeps"], "b")]},
         "cpu_baseline": cpu,
     }
     out.update(extra)

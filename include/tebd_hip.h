@@ -208,7 +208,7 @@ int tebd_set_cutoff(tebd_ctx* ctx, double cutoff);
 /* Current bond dimensions chi[l], l = 0..L (chi[0] = chi[L] = 1). */
 int tebd_get_bond_dims(tebd_ctx* ctx, int* chi);
 
-/* Counters since creation (n <= 14): [0] bond updates, [1] Jacobi sweeps,
+/* Counters since creation (n <= 46): [0] bond updates, [1] Jacobi sweeps,
  * [2] sketches of the learned-rank path redone by the full SVD, [3] device
  * milliseconds in trotter steps, [4] sketch columns requested, [5] sketches
  * accepted, [6] sum of the sketches' numerical ranks, [7] rank-model predictions,
@@ -217,7 +217,10 @@ int tebd_get_bond_dims(tebd_ctx* ctx, int* chi);
  * beyond the cap, or too wide to sketch; the fixed width would have sketched
  * them), [12] sketches
  * rejected for their discarded weight (the rest of [2] by the probe check),
- * [13] half-sweeps split over lanes (tebd_set_lanes). */
+ * [13] half-sweeps split over lanes (tebd_set_lanes), [14 + s] (s = 0..15) pairs
+ * rotated in sweep s + 1 of a Jacobi solve (the last entry: sweeps 16 and later), and
+ * [30 + s] Jacobi solves that ran s + 1 sweeps (the last entry: 16 or more; a solve
+ * ends with a sweep that rotates nothing). */
 int tebd_get_stats(tebd_ctx* ctx, double* stats, int n);
 
 /* Per-kernel timing with HIP events recorded on the context's stream (no

@@ -31,7 +31,8 @@ constexpr int kGemmRegions = 8;  // GEMM descriptor arrays per batch (learned pa
 // 1/kProbeSafety of the true value with probability
 // P(Gamma(16, 1) < 1.6) = P(Poisson(1.6) >= 16) < 2e-11.
 constexpr double kProbeSafety = 10.0;
-constexpr int kNumStats = 14;
+constexpr int kSweepHist = 16;                      // sweeps tallied one by one in the histograms
+constexpr int kNumStats = 14 + 2 * kSweepHist;
 constexpr int kNumProf = 28;
 constexpr double kQrTol = 1e-17;  // QR stops when every remaining column norm is <= kQrTol ||theta||_F
 constexpr int kQrPanel = 16;      // panel width of the blocked pivoted QR (tebd_qrp.hip)
@@ -189,7 +190,9 @@ struct tebd_ctx {
   // [11] bonds the rank model sent to the full SVD at once (predicted rank beyond the cap or too wide
   // to sketch), [12] sketches
   // rejected for discarded weight (the others of [2] by the probe check), [13] half-sweeps split
-  // over lanes
+  // over lanes, [14 + s] pairs rotated in sweep s of a Jacobi call (s < kSweepHist - 1; the
+  // last entry sums the later sweeps), [14 + kSweepHist + s] Jacobi calls that ran s + 1
+  // sweeps (the last entry: that many or more)
   double stats[kNumStats] = {};
   // optional per-kernel timing with HIP events on the context's stream:
   // prof[k*4 + 0] launches, [1] milliseconds, [2] algorithmic flops, [3] algorithmic HBM bytes,
@@ -387,9 +390,11 @@ int jacobi_sweeps(tebd_ctx* c, BondTask* d_t, const BondTask* h_t, int nb, int P
     int rot = 0;
     for (int i = 0; i < nb; ++i) rot += c->h_counters[4 * i + (sweeps & 1)];
     converged = rot == 0;
+    c->stats[14 + std::min(sweeps, kSweepHist - 1)] += rot;
     ++sweeps;
   }
   c->stats[1] += sweeps;
+  c->stats[14 + kSweepHist + std::min(sweeps, kSweepHist) - 1] += 1;
   for (int i = 0; i < nb; ++i) {
     const BondTask& t = h_t[i];
     const double visits = c->h_counters[4 * i + 2], rots = c->h_counters[4 * i + 3];

@@ -210,6 +210,18 @@ __device__ __forceinline__ double u01(unsigned long long x) {
 // Z0[(j, b), c] = conj(Gamma_r(c, j, b) lambda_r(b)) for c < chi_c, Gaussian for
 // chi_c <= c < k', zero for k' <= c < kpad.  Z0 has n = d chi_r rows (theta's
 // column index (j, b) -> j chi_r + b).
+// The Gaussian columns are scaled by kSketchNoiseScale (a power of two, so exactly):
+// the range of Y = theta Z0 is unchanged, but the pivoted QR of Y now takes the columns
+// of the old Schmidt vectors first, in the order of their weights, and the random ones
+// last.  Q then starts with the old left Schmidt directions and B^H = theta^H Q has nearly
+// orthogonal columns for the Jacobi SVD that follows.  (Unscaled, a random column is
+// heavier than any Schmidt column and the pivoting took those first, mixing every
+// direction into every column of Q.)  Householder QR errs column by column relative to
+// each column's own norm, so the scale costs no accuracy; the scaled columns stay far
+// above the QR's rank threshold of 1e-17 ||Y||_F unless theta has no weight outside the
+// old Schmidt directions, where they are rightly dropped.
+constexpr double kSketchNoiseScale = 1.0 / 65536.0;
+
 __global__ void k_build_sketch(const BondTask* __restrict__ tasks, const int* __restrict__ kprime,
                                unsigned long long seed) {
   const BondTask t = tasks[blockIdx.y];
@@ -225,7 +237,7 @@ __global__ void k_build_sketch(const BondTask* __restrict__ tasks, const int* __
     } else if (c < kp) {
       const unsigned long long key = seed ^ ((unsigned long long)t.rng_key << 48) ^ ((unsigned long long)e << 1);
       const double u1 = u01(key), u2 = u01(key + 0x632BE59BD9B4E019ull);
-      const double r = sqrt(-2.0 * log(u1));
+      const double r = kSketchNoiseScale * sqrt(-2.0 * log(u1));
       v = zmk(r * cos(6.283185307179586 * u2), r * sin(6.283185307179586 * u2));
     }
     t.V[e] = v;

@@ -301,12 +301,14 @@ class Context:
         return out
 
     def stats(self):
-        out = np.zeros(14)
-        self._check(self._lib.tebd_get_stats(self._h, _p(out), 14))
+        out = np.zeros(46)
+        self._check(self._lib.tebd_get_stats(self._h, _p(out), 46))
         return {"bond_updates": out[0], "jacobi_sweeps": out[1], "fallbacks": out[2], "device_ms": out[3],
                 "sketch_cols": out[4], "sketch_accepted": out[5], "sketch_rank": out[6], "model_predictions": out[7],
                 "pair_grams": out[8], "pair_rotations": out[9], "pair_visits": out[10],
-                "model_full_svd": out[11], "rejected_discarded": out[12], "lane_splits": out[13]}
+                "model_full_svd": out[11], "rejected_discarded": out[12], "lane_splits": out[13],
+                # pairs rotated in sweep s + 1 of a Jacobi solve, and solves that ran s + 1 sweeps
+                "sweep_rotations": out[14:30].copy(), "solve_sweeps": out[30:46].copy()}
 
     def set_profiling(self, on=True):
         self._check(self._lib.tebd_set_profiling(self._h, int(bool(on))))
