@@ -82,8 +82,12 @@ __global__ __launch_bounds__(256) void k_zgemm(const GemmTask* __restrict__ task
 // ---------------------------------------------------------------------------
 // batched complex GEMM on the FP64 matrix cores: TM x TN tiles, 2 NWR waves as NWR x 2
 // (NWR = 2, or 4 for 128 x 96 tiles), each wave (TM/NWR) x (TN/2) in blocks of 16 x 16,
-// complex products as four real v_mfma_f64_16x16x4f64 per k-step, operands staged
-// through LDS in chunks of
+// complex products by the three-multiplication form (three real v_mfma_f64_16x16x4f64
+// per k-step instead of four): T1 = sum Ar Br, T2 = sum Ai Bi, T3 = sum (Ar + Ai)(Br + Bi),
+// Re C = T1 - T2, Im C = T3 - T1 - T2.  It is normwise as accurate as the four-product
+// form (its bound on Im C is (|Ar| + |Ai|)(|Br| + |Bi|) eps per term instead of
+// (|Ar||Bi| + |Ai||Br|) eps, at most twice as large; Higham, Accuracy and Stability of
+// Numerical Algorithms, 2nd ed., sec. 23.2.4).  Operands staged through LDS in chunks of
 // KCH = 16 (or 32: half the barriers per product and twice the time for the next
 // chunk's loads to arrive, for twice the LDS).  Every tile shape and chunk length takes
 // the same operations in the same order per element.
@@ -112,13 +116,14 @@ __global__ __launch_bounds__(128 * NWR) void k_zgemm_mfma(const GemmTask* __rest
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int wr = wave >> 1, wc = wave & 1;  // wave tile (TM / NWR) x (TN / 2)
-  d4 re[RB][CB], im[RB][CB];
+  d4 t1[RB][CB], t2[RB][CB], t3[RB][CB];
 #pragma unroll
   for (int i = 0; i < RB; ++i)
 #pragma unroll
     for (int j = 0; j < CB; ++j) {
-      re[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-      im[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      t1[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      t2[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      t3[i][j] = d4{0.0, 0.0, 0.0, 0.0};
     }
   // chunk k0 of op(A) and B is loaded into registers while the previous one is
   // multiplied from LDS: element e = tid + NT u of the chunk
@@ -164,18 +169,24 @@ __global__ __launch_bounds__(128 * NWR) void k_zgemm_mfma(const GemmTask* __rest
     for (int ks = 0; ks < KCH / 4; ++ks) {
       const int kk = 4 * ks + lk;
       zc a[RB], b[CB];
+      double sa[RB], sb[CB];
 #pragma unroll
-      for (int i = 0; i < RB; ++i) a[i] = As[kk][wr * (TM / NWR) + 16 * i + lr];
+      for (int i = 0; i < RB; ++i) {
+        a[i] = As[kk][wr * (TM / NWR) + 16 * i + lr];
+        sa[i] = a[i].x + a[i].y;
+      }
 #pragma unroll
-      for (int j = 0; j < CB; ++j) b[j] = Bs[kk][wc * (TN / 2) + 16 * j + lr];
+      for (int j = 0; j < CB; ++j) {
+        b[j] = Bs[kk][wc * (TN / 2) + 16 * j + lr];
+        sb[j] = b[j].x + b[j].y;
+      }
 #pragma unroll
       for (int i = 0; i < RB; ++i)
 #pragma unroll
         for (int j = 0; j < CB; ++j) {
-          re[i][j] = mfma4(a[i].x, b[j].x, re[i][j]);
-          re[i][j] = mfma4(-a[i].y, b[j].y, re[i][j]);
-          im[i][j] = mfma4(a[i].x, b[j].y, im[i][j]);
-          im[i][j] = mfma4(a[i].y, b[j].x, im[i][j]);
+          t1[i][j] = mfma4(a[i].x, b[j].x, t1[i][j]);
+          t2[i][j] = mfma4(a[i].y, b[j].y, t2[i][j]);
+          t3[i][j] = mfma4(sa[i], sb[j], t3[i][j]);
         }
     }
     __syncthreads();
@@ -188,7 +199,8 @@ __global__ __launch_bounds__(128 * NWR) void k_zgemm_mfma(const GemmTask* __rest
       for (int g = 0; g < 4; ++g) {
         const int m = m0 + wr * (TM / NWR) + 16 * i + lk + 4 * g, n = n0 + wc * (TN / 2) + 16 * j + lr;
         if (m < t.M && n < t.N) {
-          zc v = zmk(re[i][j][g] * t.alpha, im[i][j][g] * t.alpha);
+          const double re = t1[i][j][g] - t2[i][j][g], im = t3[i][j][g] - t1[i][j][g] - t2[i][j][g];
+          zc v = zmk(re * t.alpha, im * t.alpha);
           if (t.accumulate) v = zadd(v, t.C[m + (size_t)t.ldc * n]);
           t.C[m + (size_t)t.ldc * n] = v;
         }
