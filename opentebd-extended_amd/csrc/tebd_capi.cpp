@@ -455,7 +455,7 @@ int qrp_factor(tebd_ctx* c, BondTask* D, const BondTask* H, int nb, int max_cb, 
     }
     {
       ProfScope ps(c, 6);
-      HIPCHK(c, launch_qrp_trailing(D, nb, max_cb, j0, c->st));
+      HIPCHK(c, launch_qrp_trailing(D, nb, max_rb, max_cb, j0, c->st));
     }
     const bool last = j0 + kQrPanel >= max_cb;
     if ((panel + 1) % kQrCheck == 0 || last) {

@@ -22,7 +22,7 @@ hipError_t launch_jacobi_round(const BondTask* d_tasks, int ntasks, int max_pair
                                int2* wl_items, int wl_cap, int inner_sweeps, hipStream_t st);
 hipError_t launch_jacobi_rotate(const BondTask* d_tasks, int round, int max_rows_w, int max_rows_v, int* wl_count,
                                 int2* wl_items, int wl_cap, hipStream_t st);
-hipError_t launch_qrp_trailing(const BondTask* d_tasks, int ntasks, int max_cb, int j0, hipStream_t st);
+hipError_t launch_qrp_trailing(const BondTask* d_tasks, int ntasks, int max_rb, int max_cb, int j0, hipStream_t st);
 hipError_t launch_qrp_begin(const BondTask* d_tasks, int ntasks, int max_cb, double tol_rel, hipStream_t st);
 hipError_t launch_qrp_panel(const BondTask* d_tasks, int ntasks, int max_rb, int max_cb, int j0, hipStream_t st);
 hipError_t launch_qrp_build_x(const BondTask* d_tasks, int ntasks, hipStream_t st);

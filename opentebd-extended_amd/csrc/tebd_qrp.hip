@@ -759,6 +759,108 @@ __global__ __launch_bounds__(kReflThreads) void k_qrp_reflect(const BondTask* __
   }
 }
 
+// Trailing update of panel j0 (as k_qrp_reflect<0>) on 16 columns per workgroup, with the
+// block held in registers between its two products, so that it is read once and written
+// once instead of read twice: each wave takes RB blocks of 16 rows, and lane (lr, lk) keeps
+// A2(r0 + 4g + lk, lr) of every block r0.  That is both the B operand of Y = V^H A2 (rows
+// as the MFMA's k) and the layout of the result of V Z (rows as its m), so the update
+// A2 -= V (T^H Y) and the remaining column norms come out where the loads went.
+// K = Rb - j0 <= 128 RB rows.
+template <int RB>
+__global__ __launch_bounds__(kReflThreads) void k_qrp_trailing_reg(const BondTask* __restrict__ tasks, int j0) {
+  const BondTask t = tasks[blockIdx.y];
+  const double rank = t.qstate[0];
+  if (rank >= 0.0 && rank <= (double)j0) return;
+  const int kmax = t.Rb < t.Cb ? t.Rb : t.Cb;
+  if (j0 >= kmax) return;
+  const int nbp = (kmax - j0) < QB ? (kmax - j0) : QB;
+  const int kb = j0 + nbp + 16 * blockIdx.x;
+  if (kb >= t.Cb) return;
+  const int nc = (t.Cb - kb) < 16 ? (t.Cb - kb) : 16;
+  const size_t ld = t.Rb;
+  zc* A2 = t.Wq + ld * kb + j0;
+  const int K = t.Rb - j0;
+  const zc* P = t.Wq + ld * j0 + j0;  // P[i + ld q] = A(j0 + i, j0 + q)
+  const zc* Tp = t.Tq + (size_t)(j0 / QB) * QB * QB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  auto vload = [&](int i, int q) -> zc {
+    if (q >= nbp || i < q || i >= K) return zmk(0.0, 0.0);
+    if (i == q) return zmk(1.0, 0.0);
+    return P[i + ld * q];
+  };
+  __shared__ double red[kReflWaves - 1][2][256];
+  __shared__ zc Y[QB][17];
+  __shared__ zc Ts[QB][QB + 1];
+  __shared__ double nsum[kReflWaves][16];
+  for (int e = tid; e < QB * QB; e += kReflThreads) Ts[e % QB][e / QB] = Tp[e];
+  // ---- Y = V^H A2 (16 x 16): A operand conj(V)(q = lr, i), B operand A2(i, c = lr)
+  zc a[RB][4];
+  d4 yre[1][1], yim[1][1];
+  yre[0][0] = d4{0.0, 0.0, 0.0, 0.0};
+  yim[0][0] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int b = 0; b < RB; ++b) {
+    const int r0 = 16 * (wave + kReflWaves * b);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int i = r0 + 4 * g + lk;
+      const zc v = vload(i, lr);
+      const zc x = (lr < nc && i < K) ? A2[i + ld * lr] : zmk(0.0, 0.0);
+      a[b][g] = x;
+      yre[0][0] = mfma4(v.x, x.x, yre[0][0]);
+      yre[0][0] = mfma4(v.y, x.y, yre[0][0]);
+      yim[0][0] = mfma4(v.x, x.y, yim[0][0]);
+      yim[0][0] = mfma4(-v.y, x.x, yim[0][0]);
+    }
+  }
+  gram_reduce<1, 1, kReflWaves>(yre, yim, red, &Y[0][0], 17, wave, lane);
+  // ---- Z = T^H Y, as the B operands Z(4 ks + lk, lr) of the second product
+  zc zb[4];
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) {
+    const int pr = 4 * ks + lk;
+    zc acc = zmk(0.0, 0.0);
+    for (int kk = 0; kk <= pr; ++kk) zfmac(acc, Ts[kk][pr], Y[kk][lr]);
+    zb[ks] = acc;
+  }
+  // ---- A2 -= V Z: A operand V(r0 + lr, q), B operand Z(q, c); element gg of the result
+  // in lane (lr, lk) is row r0 + lk + 4 gg, column lr: the register a[b][gg]
+  double ns = 0.0;
+#pragma unroll
+  for (int b = 0; b < RB; ++b) {
+    const int r0 = 16 * (wave + kReflWaves * b);
+    d4 ore = d4{0.0, 0.0, 0.0, 0.0}, oim = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const zc va = vload(r0 + lr, 4 * ks + lk);
+      ore = mfma4(va.x, zb[ks].x, ore);
+      ore = mfma4(-va.y, zb[ks].y, ore);
+      oim = mfma4(va.x, zb[ks].y, oim);
+      oim = mfma4(va.y, zb[ks].x, oim);
+    }
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+      const int i = r0 + lk + 4 * gg;
+      if (lr < nc && i < K) {
+        const zc x = zsub(a[b][gg], zmk(ore[gg], oim[gg]));
+        A2[i + ld * lr] = x;
+        if (i >= nbp) ns += zabs2(x);
+      }
+    }
+  }
+  // column norms: over the four lanes of equal lr, then over the waves
+  ns += __shfl_xor(ns, 16, 64);
+  ns += __shfl_xor(ns, 32, 64);
+  if (lk == 0) nsum[wave][lr] = ns;
+  __syncthreads();
+  if (tid < nc) {
+    double v = 0.0;
+    for (int w = 0; w < kReflWaves; ++w) v += nsum[w][tid];
+    t.qnrm[kb + tid] = v;
+  }
+}
+
 // Two stored panels applied at once (MODE 1: to the Rb-side vectors Yr, MODE 2: to the
 // explicit factor Qs): Y <- Q_{p-1} Q_p Y = Y - V (T (V^H Y)) with the 32-reflector
 // compact-WY block V = [V_{p-1} V_p] (columns j0 .. j0 + 31 of the factor, j0 the first
@@ -936,10 +1038,15 @@ hipError_t launch_qrp_panel(const BondTask* d_tasks, int ntasks, int max_rb, int
   return hipGetLastError();
 }
 
-hipError_t launch_qrp_trailing(const BondTask* d_tasks, int ntasks, int max_cb, int j0, hipStream_t st) {
+hipError_t launch_qrp_trailing(const BondTask* d_tasks, int ntasks, int max_rb, int max_cb, int j0, hipStream_t st) {
   if (j0 + QB >= max_cb) return hipSuccess;
-  hipLaunchKernelGGL(k_qrp_reflect<0>, dim3((max_cb - j0 - 1) / 32 + 1, ntasks), dim3(kReflThreads), 0, st, d_tasks,
-                     j0);
+  const dim3 g16((max_cb - j0 - 1) / 16 + 1, ntasks);
+  if (max_rb <= 256) hipLaunchKernelGGL(k_qrp_trailing_reg<2>, g16, dim3(kReflThreads), 0, st, d_tasks, j0);
+  else if (max_rb <= 512) hipLaunchKernelGGL(k_qrp_trailing_reg<4>, g16, dim3(kReflThreads), 0, st, d_tasks, j0);
+  else if (max_rb <= 1024) hipLaunchKernelGGL(k_qrp_trailing_reg<8>, g16, dim3(kReflThreads), 0, st, d_tasks, j0);
+  else
+    hipLaunchKernelGGL(k_qrp_reflect<0>, dim3((max_cb - j0 - 1) / 32 + 1, ntasks), dim3(kReflThreads), 0, st,
+                       d_tasks, j0);
   return hipGetLastError();
 }
 
