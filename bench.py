@@ -77,7 +77,7 @@ WORKLOAD_TAG = "fh_l128_chi256"
 # FP32 mode (theta_kernel).
 SINGLE_KERNEL = {"theta_gate": ("k_theta_gate_mfma", "mfma"), "jacobi_round": ("k_jacobi_round", "hbm"),
                  "qr_panel": ("k_qrp_panel_reg", "hbm"), "sketch_gemm": ("k_zgemm_mfma", "mfma"),
-                 "jacobi_rotate": ("k_jacobi_rotate", "hbm"), "qr_trailing": ("k_qrp_reflect<0>", "hbm")}
+                 "jacobi_rotate": ("k_jacobi_rotate", "hbm"), "qr_trailing": ("k_qrp_trailing_reg", "hbm")}
 
 
 class BenchError(RuntimeError):

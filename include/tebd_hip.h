@@ -235,7 +235,7 @@ int tebd_get_stats(tebd_ctx* ctx, double* stats, int n);
  *   3 pivoted-QR panel factorisation (k_qrp_panel_reg / k_qrp_panel)
  *   4 batched complex GEMMs of the learned-rank path (k_zgemm)
  *   5 Jacobi rotation of W and V (k_jacobi_rotate)
- *   6 pivoted-QR trailing update (k_qrp_reflect<0>)
+ *   6 pivoted-QR trailing update (k_qrp_trailing_reg; k_qrp_reflect<0> above 1024 rows)
  * Each launch of classes 1, 3, 4, 5, 6 is one kernel, so its event duration
  * is that kernel's duration. */
 int tebd_set_profiling(tebd_ctx* ctx, int on);

@@ -54,7 +54,7 @@ PROFILE_KINDS = {
     "qr_panel": "k_qrp_panel_reg",
     "sketch_gemm": "k_zgemm",
     "jacobi_rotate": "k_jacobi_rotate",
-    "qr_trailing": "k_qrp_reflect<0>",
+    "qr_trailing": "k_qrp_trailing_reg (k_qrp_reflect<0> above 1024 rows)",
 }
 
 
