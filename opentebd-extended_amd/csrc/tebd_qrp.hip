@@ -893,7 +893,6 @@ __global__ __launch_bounds__(kReflThreads) void k_qrp_reflect_pair(const BondTas
     if (i == q) return zmk(1.0, 0.0);
     return P[i + ld * q];
   };
-  auto v2load = [&](int i, int q) -> zc { return vload(i, QB + q); };
   auto aload = [&](int i, int c) -> zc {
     if (c >= nc || i >= K) return zmk(0.0, 0.0);
     return A2[i + ld * c];
@@ -911,13 +910,44 @@ __global__ __launch_bounds__(kReflThreads) void k_qrp_reflect_pair(const BondTas
     Ts[QB + a][QB + b] = nb2 > 0 ? T2[e] : zmk(0.0, 0.0);
     Ts[QB + a][b] = zmk(0.0, 0.0);
   }
-  // ---- Y = V^H A2 (32 x 32) and G = V_{p-1}^H V_p (16 x 16), rows split over the waves
+  // ---- Y = V^H A2 (32 x 32) and G = V_{p-1}^H V_p (16 x 16), rows split over the waves;
+  // G from the same loads of V as Y (its columns 0-15 and 16-31 are V_{p-1} and V_p)
   {
-    d4 yre[2][2], yim[2][2];
-    gram_partial<2, 2>(vload, aload, K, wave, kReflWaves, lane, yre, yim);
+    d4 yre[2][2], yim[2][2], gre[1][1], gim[1][1];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        yre[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+        yim[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+      }
+    gre[0][0] = d4{0.0, 0.0, 0.0, 0.0};
+    gim[0][0] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int r0 = 8 * wave; r0 < K; r0 += 8 * kReflWaves) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = r0 + 4 * h + lk;
+        zc l[2], rr[2];
+#pragma unroll
+        for (int pb = 0; pb < 2; ++pb) l[pb] = vload(i, 16 * pb + lr);
+#pragma unroll
+        for (int qb = 0; qb < 2; ++qb) rr[qb] = aload(i, 16 * qb + lr);
+#pragma unroll
+        for (int pb = 0; pb < 2; ++pb)
+#pragma unroll
+          for (int qb = 0; qb < 2; ++qb) {
+            yre[pb][qb] = mfma4(l[pb].x, rr[qb].x, yre[pb][qb]);
+            yre[pb][qb] = mfma4(l[pb].y, rr[qb].y, yre[pb][qb]);
+            yim[pb][qb] = mfma4(l[pb].x, rr[qb].y, yim[pb][qb]);
+            yim[pb][qb] = mfma4(-l[pb].y, rr[qb].x, yim[pb][qb]);
+          }
+        gre[0][0] = mfma4(l[0].x, l[1].x, gre[0][0]);
+        gre[0][0] = mfma4(l[0].y, l[1].y, gre[0][0]);
+        gim[0][0] = mfma4(l[0].x, l[1].y, gim[0][0]);
+        gim[0][0] = mfma4(-l[0].y, l[1].x, gim[0][0]);
+      }
+    }
     gram_reduce<2, 2, kReflWaves>(yre, yim, red, &Y[0][0], 33, wave, lane);
-    d4 gre[1][1], gim[1][1];
-    gram_partial<1, 1>(vload, v2load, K, wave, kReflWaves, lane, gre, gim);
     gram_reduce<1, 1, kReflWaves>(gre, gim, red, &G[0][0], QB + 1, wave, lane);
   }
   // ---- T12 = -T_{p-1} (G T_p), one entry per thread of the first 256
