@@ -78,3 +78,19 @@ def test_autotuned_gemm_tiles_do_not_change_results():
     np.testing.assert_array_equal(out[False][0], out[True][0])
     assert info["shape_classes"] > 0 and info["timing_launches"] >= 2 * info["shape_classes"]
     assert sum(info["chosen"].values()) == info["shape_classes"]
+
+
+@pytest.mark.parametrize("mode,lanes", [("batched", 1), ("batched,learned", 3)])
+def test_jacobi_statistics_are_consistent(mode, lanes):
+    """The per-sweep histograms of tebd_get_stats add up to the totals: pairs rotated per
+    sweep sum to the pairs rotated, and the solves by sweep count to the sweeps run (no
+    solve needs 16 sweeps here, so the last bucket is empty); every solve ends with a
+    sweep that rotates nothing."""
+    print(f"[gpu] Jacobi statistics {mode} lanes={lanes}", flush=True)
+    _, _, _, _, st = evolve(mode, lanes, steps=6)
+    rot, solves = st["sweep_rotations"], st["solve_sweeps"]
+    assert rot.sum() == st["pair_rotations"] > 0
+    assert solves[-1] == 0
+    assert (solves * np.arange(1, 17)).sum() == st["jacobi_sweeps"]
+    # a solve of s sweeps rotated in its first s - 1: the first sweep is the busiest
+    assert rot[0] == rot.max()
