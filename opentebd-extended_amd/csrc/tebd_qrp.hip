@@ -24,13 +24,16 @@
 //                      the displaced columns moved to the vacated slots; the
 //                      compact-WY factor T of the panel is kept (panel p at Tq + p QB^2)
 //   k_qrp_panel        the same for any Rb, through global memory (256 threads)
-//   k_qrp_reflect<0>   trailing update A2 <- Q_panel^H A2 on FP64 MFMA and the exact
-//                      remaining norms for the next panel's choice
+//   k_qrp_trailing_reg trailing update A2 <- Q_panel^H A2 on FP64 MFMA and the exact
+//                      remaining norms for the next panel's choice, 16 columns per
+//                      workgroup held in registers between the two products (Rb <= 1024)
+//   k_qrp_reflect<0>   the same for any Rb, reading the trailing block twice
 //   k_qrp_build_x      X = R(0:r, :)^dagger, zero padding columns
 //   k_qrp_vectors_init Cb-side vectors P X(:, kept)/s and the start [V(:, kept); 0] of the
 //                      Rb-side ones, or the identity for the explicit Q
 //   k_qrp_reflect<1,2> Q applied to those columns as the stored panels' block
-//                      reflectors, last panel first: Y <- Y - V_p (T_p (V_p^H Y))
+//                      reflectors, last panel first: Y <- Y - V_p (T_p (V_p^H Y));
+//                      k_qrp_reflect_pair<1,2> two panels per pass
 #include <type_traits>
 
 #include "tebd_device.h"
