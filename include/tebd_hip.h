@@ -250,8 +250,8 @@ int tebd_get_profile(tebd_ctx* ctx, double* out, int n);
 int tebd_set_graphs(tebd_ctx* ctx, int on);
 
 /* Host concurrency of a batched context (no counterpart in the reference): with
- * n lanes (1 to 4, default 3) the bonds of each half-sweep are split into n contiguous
- * parts that run on n HIP streams (the context's own and n - 1 more), each driven by
+ * n lanes (1 to 4, default 3) the bonds of each half-sweep are dealt out to n parts in
+ * turn (bond i of the batch to part n - 1 - i mod n), which run on n HIP streams (the context's own and n - 1 more), each driven by
  * its own host thread, so that one part's latency-bound phases (QR panels, inner
  * Jacobi solves) overlap the others' work; n = 1 runs a half-sweep as one batch on the
  * context's stream.  HIP runs a process's streams on a few hardware queues (four

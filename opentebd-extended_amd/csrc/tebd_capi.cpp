@@ -1790,7 +1790,7 @@ bool ensure_lanes(tebd_ctx* c, int n) {
   return true;
 }
 
-// The bonds of one half-sweep: with n lanes, contiguous parts of the batch run on n
+// The bonds of one half-sweep: with n lanes, interleaved parts of the batch run on n
 // streams, part 0 from this thread and the others from their own threads, so that one
 // part's latency-bound phases (QR panels, inner Jacobi solves) overlap the others'
 // work.  Every bond's result is independent of which bonds share its batch, so the
@@ -1803,10 +1803,19 @@ int run_bonds_split(tebd_ctx* c, const int* links, int nb, const zc* const* gate
   const int n = std::min(c->nlanes, nb / 4);
   if (n < 2 || c->ntask_max < 2 * kMaxLanes || side < kLaneMinSide) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
   if (!ensure_lanes(c, n)) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
-  int first[kMaxLanes + 1];
-  for (int k = 0; k <= n; ++k) first[k] = (int)((long long)nb * k / n);
+  // bond i of the batch goes to lane n - 1 - (i mod n): every lane gets every n-th bond,
+  // so the chain's cheap edge bonds are spread over the lanes instead of making the lanes
+  // that hold them finish early; the last lane, which has the spare workspace slots, takes
+  // the largest share
+  std::vector<int> lk[kMaxLanes];
+  std::vector<const zc*> gk[kMaxLanes];
+  for (int i = 0; i < nb; ++i) {
+    const int k = n - 1 - i % n;
+    lk[k].push_back(links[i]);
+    gk[k].push_back(gate_ptrs[i]);
+  }
   for (int k = 0; k < n; ++k)
-    if (first[k + 1] - first[k] > c->lanes[k]->ntask_max) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
+    if (lk[k].empty() || (int)lk[k].size() > c->lanes[k]->ntask_max) return run_bonds(c, links, nb, gate_ptrs, trunc_sum);
   // the lanes start after everything already queued on the context's stream
   HIPCHK(c, hipStreamSynchronize(c->st));
   for (int k = 0; k < n; ++k) lane_pull(c->lanes[k], c);
@@ -1815,15 +1824,15 @@ int run_bonds_split(tebd_ctx* c, const int* links, int nb, const zc* const* gate
   std::vector<std::thread> workers;
   for (int k = 1; k < n; ++k)
     workers.emplace_back([&, k] {
-      err[k] = run_bonds(c->lanes[k], links + first[k], first[k + 1] - first[k], gate_ptrs + first[k], &tr[k]);
+      err[k] = run_bonds(c->lanes[k], lk[k].data(), (int)lk[k].size(), gk[k].data(), &tr[k]);
     });
-  err[0] = run_bonds(c->lanes[0], links, first[1], gate_ptrs, &tr[0]);
+  err[0] = run_bonds(c->lanes[0], lk[0].data(), (int)lk[0].size(), gk[0].data(), &tr[0]);
   for (auto& w : workers) w.join();
   for (int k = 0; k < n; ++k)
     if (err[k]) return fail(c, err[k], c->lanes[k]->err);
   double tsum = 0.0;
   for (int k = 0; k < n; ++k) {
-    lane_push(c, c->lanes[k], links + first[k], first[k + 1] - first[k]);
+    lane_push(c, c->lanes[k], lk[k].data(), (int)lk[k].size());
     tsum += tr[k];
   }
   c->stats[13] += 1;
