@@ -13,9 +13,11 @@
 !       LabelLeft and LabelRight of the L+1 links (chi int32 each), then the gates as kind 0;
 !       kind 3 (ImagTimePropNC, chi = chiMax): the settings of kind 1, then those of kind 2,
 !       then (L-1) bond Hamiltonians;
+!       kind 4 (TrotterStepNC under another schedule): int32 trotterOrder, periodic (0/1) and
+!       the number of gates of the table, then kind 2 with that many gates;
 !       then L Gammas of chi x d x chi complex(8); L+1 lambdas of chi real(8)
-!   OUT: real(8) total truncation error; L Gammas; L+1 lambdas (same layout); for kinds 2
-!        and 3 then the L+1 LabelLeft and L+1 LabelRight (chi int32 each)
+!   OUT: real(8) total truncation error; L Gammas; L+1 lambdas (same layout); for kinds 2,
+!        3 and 4 then the L+1 LabelLeft and L+1 LabelRight (chi int32 each)
 PROGRAM shim_golden
 USE, INTRINSIC :: iso_c_binding
 USE system_parameters, ONLY: rKind, systemSize, localSize, trotterOrder, BoundaryCond, chiMax, dtITP, &
@@ -34,7 +36,8 @@ TYPE(tensor), POINTER :: Gammas(:)
 TYPE(vector), POINTER :: Lambdas(:)
 TYPE(matrix), POINTER :: Udt(:)
 TYPE(vectorInt), POINTER :: LabelLeft(:), LabelRight(:)
-INTEGER(c_int32_t) :: nch(2)
+INTEGER(c_int32_t) :: nch(2), sch(3)
+INTEGER :: ng
 REAL(KIND=rKind) :: truncerr, total
 CHARACTER(len=1024) :: fin, fout
 
@@ -59,7 +62,14 @@ IF (hdr(8) == 1 .OR. hdr(8) == 3) THEN
   convCriterion2 = itpr(3)
   chiMax = chi
 END IF
-IF (hdr(8) == 2 .OR. hdr(8) == 3) THEN
+ng = L - 1
+IF (hdr(8) == 4) THEN
+  READ(uin) sch
+  trotterOrder = sch(1)
+  IF (sch(2) /= 0) BoundaryCond = 'P'
+  ng = sch(3)
+END IF
+IF (hdr(8) == 2 .OR. hdr(8) == 3 .OR. hdr(8) == 4) THEN
   READ(uin) nch
   totNum = nch(1)
   ALLOCATE(Conserv%vi(d))
@@ -74,8 +84,8 @@ IF (hdr(8) == 2 .OR. hdr(8) == 3) THEN
     READ(uin) LabelRight(i)%vi
   END DO
 END IF
-ALLOCATE(Udt(L - 1), Gammas(L), Lambdas(L + 1))
-DO i = 1, L - 1
+ALLOCATE(Udt(ng), Gammas(L), Lambdas(L + 1))
+DO i = 1, ng
   ALLOCATE(Udt(i)%m(d * d, d * d))
   READ(uin) Udt(i)%m
 END DO
@@ -99,7 +109,7 @@ ELSE IF (hdr(8) == 3) THEN
   ELSE
     CALL ImagTimePropNC(Udt, Gammas, Lambdas, LabelLeft, LabelRight, chiIn)
   END IF
-ELSE IF (hdr(8) == 2) THEN
+ELSE IF (hdr(8) == 2 .OR. hdr(8) == 4) THEN
   CALL SVDInit(chi)
   DO k = 1, nsteps
     IF (nch(2) /= 0) THEN
@@ -129,7 +139,7 @@ END DO
 DO i = 1, L + 1
   WRITE(uout) Lambdas(i)%v
 END DO
-IF (hdr(8) == 2 .OR. hdr(8) == 3) THEN
+IF (hdr(8) == 2 .OR. hdr(8) == 3 .OR. hdr(8) == 4) THEN
   DO i = 1, L + 1
     WRITE(uout) LabelLeft(i)%vi
   END DO
