@@ -46,7 +46,7 @@ def lib():
         L.orc_two_site_op_nc.argtypes = [I, I, I, P, P, P, P, I, P, P, I, I, P]
         L.orc_trotter_step_nc.argtypes = [I, I, I, P, P, P, P, P, P, I, I, P]
         L.orc_canonical_form_all_nc.argtypes = [I, I, I, P, P, P, P, P, I, I]
-        L.orc_trotter_step_nc_sched.argtypes = [I, I, I, P, P, P, P, P, I, I, ctypes.c_char, P, I, I, P]
+        L.orc_trotter_step_nc_sched.argtypes = [I, I, I, P, P, P, P, P, I, I, ctypes.c_char, P, I, I, P, I]
         _LIB = L
     return _LIB
 
@@ -150,13 +150,15 @@ class OracleState:
             raise RuntimeError(f"zgesvd info={info} (or no block)")
         return t.value
 
-    def trotter_step_nc_sched(self, gates, order=2, bc="O"):
-        """TrotterStepNC of any of the reference's schedules (order 2 or 5, 'O' or 'P')."""
+    def trotter_step_nc_sched(self, gates, order=2, bc="O", swap_renorm=False):
+        """TrotterStepNC of any of the reference's schedules (order 2 or 5, 'O' or 'P').
+        swap_renorm=True plants a deviation (theta normalised in the swaps, which the
+        reference's SwappingNC does not do) for tests of the fixtures' resolution."""
         g = _gate_stack(gates, self.d)
         t = ctypes.c_double()
         info = lib().orc_trotter_step_nc_sched(*self._args(), *self._nc(), _p(g), g.shape[0], int(order),
                                                bc.encode()[:1], _p(self.conserv), self.tot, self.maxf,
-                                               ctypes.byref(t))
+                                               ctypes.byref(t), int(bool(swap_renorm)))
         if info != 0:
             raise RuntimeError(f"zgesvd info={info} (or no block, or a gate table of the wrong size)")
         return t.value

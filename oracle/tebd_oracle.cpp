@@ -614,9 +614,11 @@ int orc_trotter_step_sched(int L, int d, int chi, double* G, double* lam, const 
 // TrotterStep5thOrderPBCNC, propagation_module.f90:1019-1942): TwoSiteOpNC with the table's
 // gates, and SwappingNC (local_operations_module.f90:1966-2082) for the swaps -- the
 // number-conserving update with the SWAP gate and without RenormThetaNC.
+// swap_renorm = 1 normalises theta in the swaps too (not the reference's SwappingNC): a
+// planted deviation for the tests that show the fixtures resolve it.
 int orc_trotter_step_nc_sched(int L, int d, int chi, double* G, double* lam, int* LL, int* LR, const double* gates,
                               int ngates, int order, char bc, const int* conserv, int tot, int maxf,
-                              double* total_truncerr) {
+                              double* total_truncerr, int swap_renorm) {
   View v{L, d, chi, (cplx*)G, lam};
   const Labels lb{chi, LL, LR};
   const size_t gs = (size_t)d * d * d * d;
@@ -635,7 +637,7 @@ int orc_trotter_step_nc_sched(int L, int d, int chi, double* G, double* lam, int
         total += t;
       },
       [&](int i) {
-        info |= two_site_op_nc(v, lb, i - 1, sw.data(), conserv, tot, maxf, &t, false);
+        info |= two_site_op_nc(v, lb, i - 1, sw.data(), conserv, tot, maxf, &t, swap_renorm != 0);
         total += t;
       });
   *total_truncerr = total;

@@ -262,3 +262,16 @@ def test_oracle_nc_schedule_open_second_order_is_trotter_step_nc():
         tb = b.trotter_step_nc_sched(z["U"], 2, "O")
         assert ta == tb
     assert np.array_equal(a.lam, b.lam) and np.array_equal(a.gam, b.gam) and np.array_equal(a.LL, b.LL)
+
+
+def test_nc_schedule_fixtures_resolve_the_swap_normalisation():
+    """Planted in the oracle: normalising theta before the cut in the swaps (RenormThetaNC,
+    which the reference's SwappingNC skips) moves the truncation error of the even
+    periodic chain by about 2e-6 against the reference's run, so the fixture pins it."""
+    z = load_golden("nc_sched_o2_pbc_l8")
+    st = nc_state(z)
+    worst = 0.0
+    for k in range(1, int(z["nsteps"]) + 1):
+        tr = st.trotter_step_nc_sched(z["U"], 2, "P", swap_renorm=True)
+        worst = max(worst, abs(tr - z["trunc"][k]))
+    assert worst > 1e-7, worst
