@@ -68,6 +68,7 @@ BULK = slice(4, L - 3)   # bonds 4 .. L-4 of bond_dims(): cap min(4^b, 4^(L-b), 
 MAX_SATURATION_STEPS = 200
 ORACLE_BONDS = (32, 64, 97)  # 1-based bulk bonds of the oracle check
 OBS_TOL = 1e-8
+NC_CONSERV = np.array([0, 1, 1, 2], np.int32)  # particle number of [0, up, down, up+down]
 
 RANK_MODEL = os.path.join(ROOT, "opentebd-extended_amd", "tebd_amd", "rank_model_fh.npy")
 WORKLOAD_TAG = "fh_l128_chi256"
@@ -255,6 +256,121 @@ def oracle_check(snap, modes, log, rank_model=None):
                      "sketches_accepted": st["sketch_accepted"], "full_svd_fallbacks": st["fallbacks"]}
         log(f"[bench] oracle check {mode}: max err {out[mode]['max_err']:.2e}")
     return out
+
+
+def sector_spectra_err(LL, lam, LL_ref, lam_ref):
+    """Largest difference between the Schmidt values of each number sector of each link
+    (which of two equal values of one sector comes first is left to rounding)."""
+    worst = 0.0
+    for link in range(LL.shape[0]):
+        for n in set(LL[link][lam[link] > 0].tolist()) | set(LL_ref[link][lam_ref[link] > 0].tolist()):
+            a = np.sort(lam[link][(LL[link] == n) & (lam[link] > 0)])[::-1]
+            b = np.sort(lam_ref[link][(LL_ref[link] == n) & (lam_ref[link] > 0)])[::-1]
+            k = max(a.size, b.size)
+            a, b = np.pad(a, (0, k - a.size)), np.pad(b, (0, k - b.size))
+            worst = max(worst, float(np.max(np.abs(a - b))) if k else 0.0)
+    return worst
+
+
+def run_nc(steps, warmup, log, graphs=True, sample_bonds=ORACLE_BONDS):
+    """The number-conserving form of the same workload (TrotterStepNC,
+    propagation_module.f90:890-1018; TwoSiteOpNC_c, local_operations_module.f90:1845-1964):
+    the Hubbard chain with the particle number of the basis [0, up, down, up+down] =
+    [0, 1, 1, 2] conserved, total L from the Neel state, every bond's theta factorised
+    block by block.  Saturation, warm-up and the timed window as in run_mode (exact FP64,
+    batched).  Then from the state at the start of the window the gates of `sample_bonds`
+    are applied by the device and by the oracle (one thread, timed: the CPU figure); their
+    lambdas, the local observables, every link's number-sector spectra and the labels are
+    compared."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle.oracle import OracleState
+    from tebd_amd import Context, models as M
+
+    H, U = gates_and_h()
+    _, _, nop, _, _ = M.fermion_ops(2)
+    coeffs = M.neel(L, D, 1, 2)
+    tot = L
+    mode = "batched"
+
+    def nc_context(state, labels):
+        ctx = Context(L, D, CHI, mode=mode)
+        ctx.upload(*state)
+        ctx.set_gates(U)
+        ctx.set_graphs(graphs)
+        ctx.set_conserved(NC_CONSERV, tot, labels=labels)
+        return ctx
+
+    ctx = nc_context(M.product_state(coeffs, CHI), M.product_labels(coeffs, CHI, NC_CONSERV, tot))
+    tw = time.perf_counter()
+    n_pre = 0
+    while bulk_min(ctx) < CHI:
+        ctx.trotter_step()
+        n_pre += 1
+        if n_pre % 10 == 0:
+            log(f"[bench] NC saturation step {n_pre}: bulk min chi={bulk_min(ctx)} ({time.perf_counter() - tw:.1f}s)")
+        if n_pre > MAX_SATURATION_STEPS:
+            raise BenchError(f"NC: bulk bonds not saturated after {n_pre} steps")
+    n_sat = n_pre
+    for _ in range(warmup):
+        ctx.trotter_step()
+        n_pre += 1
+    ctx.sync()
+    log(f"[bench] NC: saturated after {n_sat} steps, {warmup} warm-up steps ({time.perf_counter() - tw:.1f}s)")
+    snap, snap_labels = ctx.download(), ctx.labels(CHI)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ctx.trotter_step()
+        log(f"[bench] NC timed step {k + 1}/{steps} ({time.perf_counter() - t0:.2f}s)")
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    dims = ctx.bond_dims()
+    if int(dims[BULK].min()) < CHI:
+        raise BenchError(f"NC: timed window left chi={CHI} (bulk min {dims[BULK].min()})")
+    number_err = float(abs(np.sum(ctx.one_site_expval(nop)) - tot))
+    ctx.close()
+
+    ref = OracleState(snap[0].copy(), snap[1].copy())
+    ref.set_conserved(snap_labels[0].copy(), snap_labels[1].copy(), NC_CONSERV, tot)
+    times = []
+    with threadpool_limits(limits=1):
+        for b in sample_bonds:
+            tc = time.perf_counter()
+            ref.two_site_op_nc(b, U[b - 1])
+            times.append(time.perf_counter() - tc)
+    log(f"[bench] NC oracle updates of bonds {tuple(sample_bonds)}: {sum(times):.1f}s")
+    dev = nc_context(snap, snap_labels)
+    for b in sample_bonds:
+        dev.two_site_op(b, U[b - 1])
+    g, lam = dev.download()
+    LL, LR = dev.labels(CHI)
+    dev.close()
+    with Context(L, D, CHI, mode="fp64") as ev:
+        ev.upload(g, lam)
+        o = observables(ev, H)
+        ev.upload(ref.gam, ref.lam)
+        r = observables(ev, H)
+    errs = {"lambda": float(max(np.max(np.abs(lam[b] - ref.lam[b])) for b in sample_bonds)),
+            "E_total": abs(o["E"] - r["E"]),
+            "sectors": sector_spectra_err(LL, lam, ref.LL, ref.lam)}
+    for k in ("n", "sz", "S"):
+        errs[k] = float(np.max(np.abs(np.asarray(o[k]) - np.asarray(r[k]))))
+    t_bond = float(np.mean(times))
+    model, _ = host_cpu()
+    return {"workload": f"Fermi-Hubbard quench L={L} d={D} t=1 U=4 dt={DT} chi_max={CHI}, Neel start, particle "
+                        f"number conserved (numbers [0,1,1,2], total {tot}); timed steps {n_pre + 1}-"
+                        f"{n_pre + steps} with every bulk bond at chi={CHI}",
+            "mode": mode, "graphs": graphs, "steps": steps, "steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+            "saturation_steps": n_sat, "mean_bond_dim": float(dims.mean()),
+            "particle_number_err": number_err,
+            "obs_err_vs_oracle": {"max_err": max(errs.values()), "by_observable": errs,
+                                  "labels_equal": bool(np.array_equal(LL, ref.LL) and np.array_equal(LR, ref.LR)),
+                                  "meets_1e-8": bool(max(errs.values()) <= OBS_TOL), "bonds": list(sample_bonds)},
+            "cpu_baseline": {"value": 1.0 / (t_bond * N_UPDATES), "unit": "Trotter steps/s", "cores": 1,
+                             "kind": "port", "cpu_model": model,
+                             "sample": f"the NC updates of bonds {tuple(sample_bonds)} above by the FP64 oracle "
+                                       f"(TwoSiteOpNC on padded chi={CHI}) on one thread, {t_bond:.2f}s each, "
+                                       f"x{N_UPDATES} bond updates per step"}}
 
 
 def copy_calibration(mib):
@@ -552,6 +668,9 @@ def main():
     ap.add_argument("--compare", default="fp64", help="per-bond FP64 drop-in timed on the window ('' = skip)")
     ap.add_argument("--oracle-check", type=int, default=1, help="bond updates of every path vs the CPU oracle")
     ap.add_argument("--cpu-sample-bonds", type=int, default=3)
+    ap.add_argument("--nc-steps", type=int, default=10,
+                    help="timed steps of the number-conserving form of the workload (0 = skip)")
+    ap.add_argument("--nc-only", action="store_true", help="only the number-conserving run")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the all-cores CPU figure (0 = auto)")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -575,6 +694,9 @@ def main():
         copy_calibration(args.copy_calibration)
         return
 
+    if args.nc_only:
+        print(json.dumps({"number_conserving": run_nc(args.nc_steps, args.warmup, log, bool(args.graphs))}), flush=True)
+        return
     model = np.load(args.rank_model) if args.rank_model else None
     graphs = bool(args.graphs)
     if args.roofline_only:
@@ -637,6 +759,9 @@ def main():
                                if m]
         extra["obs_err_vs_oracle"] = oracle_check(res["snap"], modes, log, rank_model=model)
         extra["obs_err_vs_oracle"]["bonds"] = list(ORACLE_BONDS)
+
+    if args.nc_steps > 0:
+        extra["number_conserving"] = run_nc(args.nc_steps, args.warmup, log, graphs)
 
     cpu = None
     if args.cpu_sample_bonds > 0:
