@@ -318,12 +318,14 @@ def run_nc(steps, warmup, log, graphs=True, sample_bonds=ORACLE_BONDS):
     ctx.sync()
     log(f"[bench] NC: saturated after {n_sat} steps, {warmup} warm-up steps ({time.perf_counter() - tw:.1f}s)")
     snap, snap_labels = ctx.download(), ctx.labels(CHI)
+    st0 = ctx.stats()
     t0 = time.perf_counter()
     for k in range(steps):
         ctx.trotter_step()
         log(f"[bench] NC timed step {k + 1}/{steps} ({time.perf_counter() - t0:.2f}s)")
     ctx.sync()
     elapsed = time.perf_counter() - t0
+    st1 = ctx.stats()
     dims = ctx.bond_dims()
     if int(dims[BULK].min()) < CHI:
         raise BenchError(f"NC: timed window left chi={CHI} (bulk min {dims[BULK].min()})")
@@ -362,6 +364,8 @@ def run_nc(steps, warmup, log, graphs=True, sample_bonds=ORACLE_BONDS):
                         f"{n_pre + steps} with every bulk bond at chi={CHI}",
             "mode": mode, "graphs": graphs, "steps": steps, "steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
             "saturation_steps": n_sat, "mean_bond_dim": float(dims.mean()),
+            "block_svd_batches_per_step": float(np.sum(st1["solve_sweeps"] - st0["solve_sweeps"])) / steps,
+            "jacobi_sweeps_per_step": (st1["jacobi_sweeps"] - st0["jacobi_sweeps"]) / steps,
             "particle_number_err": number_err,
             "obs_err_vs_oracle": {"max_err": max(errs.values()), "by_observable": errs,
                                   "labels_equal": bool(np.array_equal(LL, ref.LL) and np.array_equal(LR, ref.LR)),

@@ -50,6 +50,12 @@ constexpr int kInnerSweepCap = 3;
 constexpr double kSlotBudget = 48.0 * (1ull << 30);  // bytes of per-bond workspace slots per context
 constexpr size_t kMaxSweepGraphs = 512;  // executable Jacobi sweep graphs kept per context or lane
 constexpr int kGraphRowQuantum = 64;     // = the rotation kernel's slab height (tebd_jacobi.hip)
+// The number-conserving update's block SVDs in the batched mode: blocks of up to
+// kNcSmallSide rows run kNcSmallSlots at a time in a workspace of that many small slots
+// (a half-sweep of Fermi-Hubbard bonds at chi = 256 has about a dozen blocks per bond,
+// which the bond-sized slots would take ntask_max = L / 2 at a time).
+constexpr int kNcSmallSlots = 512;
+constexpr int kNcSmallSide = 512;
 
 int round_up(int x, int m) { return ((x + m - 1) / m) * m; }
 
@@ -167,6 +173,10 @@ struct tebd_ctx {
   zc* nc_theta = nullptr;
   zc* nc_vec = nullptr;
   int nc_bonds = 0;
+  // the small-block SVD workspace (batched mode, made on first use): a context of its own
+  // of which only the task slots, the Jacobi / QR state, the stream and the pinned
+  // staging of singular values are used
+  tebd_ctx* nc_small = nullptr;
   int* d_nc_idx = nullptr;
   size_t nc_idx_cap = 0;
   NcBlock* d_nc_blk = nullptr;
@@ -1317,6 +1327,33 @@ int ensure_nc_workspace(tebd_ctx* c) {
   return TEBD_OK;
 }
 
+void free_all(tebd_ctx* c);
+extern "C" int create_impl(tebd_ctx** out, int nchain, int Lc, int d, int chi_max, int order, char bc,
+                           unsigned mode);
+
+// The small-block SVD workspace: a batched context with d = 2 and chi = side / 2, whose
+// kNcSmallSlots slots hold bond matrices of side = min(kNcSmallSide, Rmax) rows and
+// columns with side / 2 singular vectors each.  Its state arrays are never used (its
+// Gammas are freed at once).
+int ensure_nc_small(tebd_ctx* c) {
+  if (c->nc_small) return TEBD_OK;
+  const int side = std::min(kNcSmallSide, round_up(c->Rmax, 2));
+  tebd_ctx* sm = nullptr;
+  if (create_impl(&sm, 1, 2 * kNcSmallSlots, 2, side / 2, 2, 'O', TEBD_MODE_BATCHED) != TEBD_OK || !sm)
+    return fail(c, TEBD_ERR_HIP, "number-conserving update: cannot allocate the small-block workspace");
+  (void)hipFree(sm->gam);
+  sm->gam = nullptr;
+  const size_t nt = sm->ntask_max;
+  if (hipMalloc(&sm->nc_lscratch, sizeof(double) * sm->Cpmax * nt) != hipSuccess ||
+      hipHostMalloc(&sm->h_nc_s, sizeof(double) * sm->Cpmax * nt) != hipSuccess) {
+    free_all(sm);
+    delete sm;
+    return fail(c, TEBD_ERR_HIP, "number-conserving update: cannot allocate the small-block workspace");
+  }
+  c->nc_small = sm;
+  return TEBD_OK;
+}
+
 // One bond's number blocks (minmaxNLR, SizeOfBlocks, IndexLeft, IndexRight,
 // local_operations_module.f90:1201-1443), rows as theta rows a + chi_l i and columns as
 // theta columns g + chi_r j, in the reference's traversal order.
@@ -1467,64 +1504,104 @@ int run_bonds_nc(tebd_ctx* c, const int* links, int nb, const zc* const* gate_pt
         return fail(c, TEBD_ERR_NAN, "bond " + std::to_string(links[blk_bond[q]] + 1) + ": theta is not finite");
     }
   }
-  // 4. SVDs of the blocks with weight, ntask_max at a time; singular values to the host
+  // 4. SVDs of the blocks with weight; singular values to the host.  In the batched mode the
+  // blocks that fit the small-block workspace run there, kNcSmallSlots at a time, the
+  // others ntask_max at a time in the context's slots.  A block's result does not depend on
+  // the slots or the batch it runs in.
   std::vector<std::vector<double>> sv(nbk);  // per block: min(lgt, trv) values, descending
   for (int q = 0; q < nbk; ++q) sv[q].assign(std::min(blocks[q].lgt, blocks[q].trv), 0.0);
-  std::vector<int> live;
-  for (int q = 0; q < nbk; ++q)
-    if (wgt[q] > 0.0) live.push_back(q);
-  BondTask* S = c->h_tasks;
-  BondTask* DS = c->d_tasks;
+  tebd_ctx* sm = nullptr;
+  if (c->mode & TEBD_MODE_BATCHED) {
+    if ((e = ensure_nc_small(c))) return e;
+    sm = c->nc_small;
+  }
+  std::vector<int> live_ctx, live_small;
+  for (int q = 0; q < nbk; ++q) {
+    if (!(wgt[q] > 0.0)) continue;
+    const NcBlock& b = blocks[q];
+    const int mins = std::min(b.lgt, b.trv);
+    const bool fits = sm && b.R <= sm->Rmax && round_up(mins, JC) <= sm->Cpmax &&
+                      std::min(mins, c->chi_max) <= sm->chi_max;
+    (fits ? live_small : live_ctx).push_back(q);
+  }
+  // largest first, so that each batch's sweeps run about as many rounds as its blocks need
+  std::stable_sort(live_small.begin(), live_small.end(), [&](int x, int y) {
+    const int cx = std::min(blocks[x].lgt, blocks[x].trv), cy = std::min(blocks[y].lgt, blocks[y].trv);
+    return cx != cy ? cx > cy : blocks[x].R > blocks[y].R;
+  });
   std::vector<NcBlock> chunk_blk;
   std::vector<int> err_links;
-  for (size_t q0 = 0; q0 < live.size(); q0 += nt) {
-    const int n = (int)std::min(live.size() - q0, (size_t)nt);
-    chunk_blk.assign(n, NcBlock{});
-    err_links.assign(n, 0);
-    int Cpm = 1, Pm = 1;
-    for (int q = 0; q < n; ++q) {
-      const NcBlock& b = blocks[live[q0 + q]];
-      BondTask& t = S[q];
-      std::memset(&t, 0, sizeof(t));
-      t.d = d;
-      t.link = links[blk_bond[live[q0 + q]]];
-      t.R = b.R;
-      t.C = std::min(b.lgt, b.trv);
-      t.Cpad = round_up(t.C, JC);
-      t.P = t.Cpad / JC;
-      t.W = c->W + (size_t)q * c->Rmax * c->Cpmax;
-      t.V = c->V + (size_t)q * c->Cpmax * c->Cpmax;
-      t.NV = t.Cpad;
-      t.s = c->s + (size_t)q * c->Cpmax;
-      t.perm = c->perm + (size_t)q * c->Cpmax;
-      attach_jacobi(c, t, q);
-      attach_qr(c, t, q);
-      t.result = c->results + (size_t)RESULT_STRIDE * q;
-      t.lc_out = c->nc_lscratch + (size_t)q * c->Cpmax;
-      // at most chi_max values of a block can be among the chi_max kept ones (and the
-      // vector panels hold chi_max columns)
-      t.chi_max = std::min(t.C, c->chi_max);
-      chunk_blk[q] = b;
-      chunk_blk[q].W = t.W;
-      err_links[q] = t.link;
-      Cpm = std::max(Cpm, t.Cpad);
-      Pm = std::max(Pm, t.P);
+  // the blocks `live` in the slots of w (c or its small-block workspace), on w's stream
+  auto factor_blocks = [&](tebd_ctx* w, const std::vector<int>& live) -> int {
+    const int wn = w->ntask_max;
+    BondTask* S = w->h_tasks;
+    BondTask* DS = w->d_tasks;
+    if (w != c) {
+      w->cutoff = c->cutoff;
+      w->graphs = c->graphs;
+      w->profiling = c->profiling;
     }
-    HIPCHK(c, hipMemcpyAsync(c->d_nc_blk, chunk_blk.data(), sizeof(NcBlock) * n, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(DS, S, sizeof(BondTask) * n, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, launch_nc_gather(c->d_nc_blk, n, c->d_nc_part, c->st));
-    bool used_qr = false;
-    int max_k = 1;
-    if ((e = svd_factor(c, DS, S, err_links.data(), n, Cpm, Pm, false, &used_qr, &max_k))) return e;
-    HIPCHK(c, launch_nc_vectors(c->d_nc_blk, DS, n, max_k, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->h_nc_s, c->s, sizeof(double) * c->Cpmax * n, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    for (int q = 0; q < n; ++q) {
-      const int k = (int)c->h_results[RESULT_STRIDE * q];
-      std::vector<double>& v = sv[live[q0 + q]];
-      for (int a = 0; a < k && a < (int)v.size(); ++a) v[a] = c->h_nc_s[(size_t)q * c->Cpmax + a];
+    for (size_t q0 = 0; q0 < live.size(); q0 += wn) {
+      const int n = (int)std::min(live.size() - q0, (size_t)wn);
+      chunk_blk.assign(n, NcBlock{});
+      err_links.assign(n, 0);
+      int Cpm = 1, Pm = 1;
+      for (int q = 0; q < n; ++q) {
+        const NcBlock& b = blocks[live[q0 + q]];
+        BondTask& t = S[q];
+        std::memset(&t, 0, sizeof(t));
+        t.d = d;
+        t.link = links[blk_bond[live[q0 + q]]];
+        t.R = b.R;
+        t.C = std::min(b.lgt, b.trv);
+        t.Cpad = round_up(t.C, JC);
+        t.P = t.Cpad / JC;
+        t.W = w->W + (size_t)q * w->Rmax * w->Cpmax;
+        t.V = w->V + (size_t)q * w->Cpmax * w->Cpmax;
+        t.NV = t.Cpad;
+        t.s = w->s + (size_t)q * w->Cpmax;
+        t.perm = w->perm + (size_t)q * w->Cpmax;
+        attach_jacobi(w, t, q);
+        attach_qr(w, t, q);
+        t.result = w->results + (size_t)RESULT_STRIDE * q;
+        t.lc_out = w->nc_lscratch + (size_t)q * w->Cpmax;
+        // at most chi_max values of a block can be among the chi_max kept ones (and the
+        // vector panels hold chi_max columns)
+        t.chi_max = std::min(t.C, c->chi_max);
+        chunk_blk[q] = b;
+        chunk_blk[q].W = t.W;
+        err_links[q] = t.link;
+        Cpm = std::max(Cpm, t.Cpad);
+        Pm = std::max(Pm, t.P);
+      }
+      HIPCHK(c, hipMemcpyAsync(c->d_nc_blk, chunk_blk.data(), sizeof(NcBlock) * n, hipMemcpyHostToDevice, w->st));
+      HIPCHK(c, hipMemcpyAsync(DS, S, sizeof(BondTask) * n, hipMemcpyHostToDevice, w->st));
+      HIPCHK(c, launch_nc_gather(c->d_nc_blk, n, c->d_nc_part, w->st));
+      bool used_qr = false;
+      int max_k = 1;
+      int ef = svd_factor(w, DS, S, err_links.data(), n, Cpm, Pm, false, &used_qr, &max_k);
+      if (!ef) {
+        HIPCHK(c, launch_nc_vectors(c->d_nc_blk, DS, n, max_k, w->st));
+        HIPCHK(c, hipMemcpyAsync(w->h_nc_s, w->s, sizeof(double) * w->Cpmax * n, hipMemcpyDeviceToHost, w->st));
+      }
+      HIPCHK(c, hipStreamSynchronize(w->st));
+      if (w != c) {  // the workspace's counters and timings belong to the context
+        drain_profile(w);
+        for (int k = 0; k < kNumStats; ++k) c->stats[k] += w->stats[k], w->stats[k] = 0.0;
+        for (int k = 0; k < kNumProf; ++k) c->prof[k] += w->prof[k], w->prof[k] = 0.0;
+        if (ef) return fail(c, ef, w->err);
+      }
+      if (ef) return ef;
+      for (int q = 0; q < n; ++q) {
+        const int k = (int)w->h_results[RESULT_STRIDE * q];
+        std::vector<double>& v = sv[live[q0 + q]];
+        for (int a = 0; a < k && a < (int)v.size(); ++a) v[a] = w->h_nc_s[(size_t)q * w->Cpmax + a];
+      }
     }
-  }
+    return TEBD_OK;
+  };
+  if ((e = factor_blocks(c, live_ctx))) return e;
+  if (sm && (e = factor_blocks(sm, live_small))) return e;
   // 5. per bond: all blocks' values in block order (FlattenSS), normalised as RenormThetaNC
   // normalises theta, ordered (Ordering, JudgePosition), kept above the cut (UpdateLabelLeft,
   // FormLambdaNC), and the kept states' Gammas
@@ -1658,6 +1735,7 @@ tebd_ctx* make_lane(tebd_ctx* c, int lane, int slot0, int nslots) {
   l->cr_theta[0] = l->cr_theta[1] = l->cr_T = l->cr_K = l->cr_B = l->cr_A = l->cr_rho2 = l->cr_rho1 = nullptr;
   l->cr_s = l->cr_phase = nullptr;
   l->nc_theta = l->nc_vec = nullptr;   // the number-conserving update runs on the context only
+  l->nc_small = nullptr;
   l->d_nc_idx = nullptr;
   l->d_nc_blk = nullptr;
   l->d_nc_kept = nullptr;
@@ -1937,6 +2015,12 @@ void free_all(tebd_ctx* c) {
   for (tebd_ctx*& l : c->lanes) {
     free_lane(l);
     l = nullptr;
+  }
+  if (tebd_ctx* sm = c->nc_small) {
+    c->nc_small = nullptr;
+    if (sm->st) (void)hipStreamSynchronize(sm->st);
+    free_all(sm);
+    delete sm;
   }
   c->lanes_made = 0;
   clear_graphs(c);
