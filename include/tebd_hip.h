@@ -331,7 +331,10 @@ const char* tebd_last_error(const tebd_ctx* ctx);
  * tebd_set_labels / tebd_get_labels: LabelLeft / LabelRight ((L+1) x chi ints, row l the
  * reference's link l+1, 10000 = an unused state), e.g. from ProductStateLabels
  * (Hamiltonian_tools_module.f90:124-178).  tebd_canonical_form_all and tebd_imag_time_prop
- * then run their number-conserving forms; tebd_canonical_form (one bond) is refused. */
+ * then run their number-conserving forms; tebd_canonical_form (one bond) is refused.
+ * In the batched mode the first number-conserving update also allocates a second SVD
+ * workspace of 512 slots for the number blocks of up to min(512, d chi_max) rows (about
+ * 8.8 GiB when d chi_max >= 512), held until tebd_destroy. */
 int tebd_set_conserved(tebd_ctx* ctx, const int* conserv, int tot_num);
 int tebd_set_labels(tebd_ctx* ctx, const int* label_left, const int* label_right, int chi);
 int tebd_get_labels(tebd_ctx* ctx, int* label_left, int* label_right, int chi);
