@@ -39,6 +39,12 @@ counter summary (profiles/*_traffic.json).  `--roofline-only` runs just that run
 (the command the committed rocprofv3 kernel trace is taken of); `--rocprof-summary
 DIR` turns rocprofv3 CSV output into the tables kept under profiles/.
 
+`number_conserving` is the same chain with the particle number conserved
+(TrotterStepNC, numbers [0, 1, 1, 2], total L): exact batched FP64 block SVDs, the
+same protocol (its 1e-10 cut of the values saturates the bulk later), steps/s, a
+three-bond check against the oracle's TwoSiteOpNC, and that oracle's one-thread time
+as its own CPU figure.  `--nc-only` runs just this part, `--nc-steps 0` skips it.
+
 `cpu_baseline` times the FP64 CPU oracle (the reference algorithm: padded chi,
 full ZGESVD of every (chi d) x (chi d) bond matrix with the reference's minimum
 workspace) on this host: one thread (the reference is a sequential program) on a
